@@ -1,0 +1,260 @@
+"""Benchmark of the wall-integral hot path on one MI355X.
+
+Workload (the 2-D at-scale configuration): one star polygon of 4,096 edges, 1,048,576 particles (30 % within one
+support of the wall), h = 0.004, kernels cubic, w2, w4, w6.  One step = the adjacency rebuild (bit-exact cell-list
+slots + winding indicator) followed by all twelve moments for each of the four kernels (4 x 12 x N particle-moment
+evaluations), inputs resident in HBM.  ``value`` is the mixed form's throughput; the all-FP64 plain path (plan
+interpreter, one thread per pair) and the FP64 closed form are timed beside it, the mixed form's error is measured
+against the FP64 closed form on the device, and the CPU oracle is timed on a bounded subsample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12          # B/s
+PEAK_FP64 = 78.6e12        # flop/s (MI355X FP64 vector; the matrix FP64 peak is the same figure)
+METRIC = "particle-moment evals/s @≤1e-6 rel err vs FP64; served p50/p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def active_block_counts(torch, verts, edges, slots, counts, lpos, lsup, K, kname):
+    """number of active (pair, block) chords of every block radius, from the pair geometry (for the op model)."""
+    from warpsphboundaries_amd.plan import block_profiles
+    rows = torch.repeat_interleave(torch.arange(len(counts), device=counts.device), counts.long())
+    off = torch.cumsum(counts.long(), 0) - counts.long()
+    k = torch.arange(len(rows), device=rows.device) - off[rows]
+    e = slots.view(-1, K)[rows, k].long()
+    h = lsup[rows]
+    p = (verts[edges[e, 0]] - lpos[rows]) / h[:, None]
+    q = (verts[edges[e, 1]] - lpos[rows]) / h[:, None]
+    d = q - p
+    ell = torch.sqrt((d * d).sum(1))
+    z = (p[:, 0] * q[:, 1] - p[:, 1] * q[:, 0]) / ell
+    s0 = (d * p).sum(1) / ell
+    s1 = (d * q).sum(1) / ell
+    out = []
+    for b in block_profiles(kname):
+        R = float(b.R)
+        L = torch.sqrt(torch.clamp((R - z.abs()) * (R + z.abs()), min=0))
+        act = (z.abs() < R) & (torch.maximum(s0, -L) < torch.minimum(s1, L))
+        out.append(int(act.sum()))
+    return len(rows), out
+
+
+def cpu_baseline(work, n_sample=8192, seed=7):
+    """the CPU oracle (numpy FP64 restatement of the reference's edge reductions) on a stratified subsample:
+    pair list from the oracle's cell-list slot fill, twelve channels per pair, row sums; single thread."""
+    from oracle import adjacency as oadj
+    from oracle import edge2d
+    from warpsphboundaries_amd import workloads as WL
+    from warpsphboundaries_amd.scene import SurfaceRep
+    sub = WL.subsample(work, n_sample, seed)
+    loop, solid = sub.loops[0]
+    rep = SurfaceRep.polygon(loop, solid)
+    V, E = rep.vertices.numpy(), rep.edges.numpy()
+    st = oadj.Static(V, E, rep.background, sub.h)
+    pts, sup = sub.positions, sub.supports
+    # slots without the per-row indicator loop (the indicator is a table lookup for most rows on every path)
+    cl = st.cl
+    nx, ny = cl.dims
+
+    def run():
+        cx = np.floor((pts[:, 0] - cl.lo[0]) / cl.cell).astype(np.int64)
+        cy = np.floor((pts[:, 1] - cl.lo[1]) / cl.cell).astype(np.int64)
+        inside = (cx >= 0) & (cx < nx) & (cy >= 0) & (cy < ny)
+        cid = np.where(inside, cx * ny + cy, 0)
+        nc = np.where(inside, cl.start[cid + 1] - cl.start[cid], 0)
+        rows = np.repeat(np.arange(len(pts)), nc)
+        off = np.arange(int(nc.sum())) - np.repeat(np.cumsum(nc) - nc, nc)
+        e = cl.items[cl.start[cid[rows]] + off]
+        ax, ay = V[E[e, 0], 0], V[E[e, 0], 1]
+        dx, dy = V[E[e, 1], 0] - ax, V[E[e, 1], 1] - ay
+        qx, qy = pts[rows, 0], pts[rows, 1]
+        t = np.clip(((qx - ax) * dx + (qy - ay) * dy) / np.maximum(dx * dx + dy * dy, 1e-300), 0, 1)
+        keep = np.sqrt((qx - (ax + t * dx)) ** 2 + (qy - (ay + t * dy)) ** 2) < sup[rows]
+        r, e = rows[keep], e[keep]
+        P = (V[E[e, 0]] - pts[r]) / sup[r, None]
+        Q = (V[E[e, 1]] - pts[r]) / sup[r, None]
+        for k in sub.kernels:
+            c = edge2d.edge_channels12(P, Q, k)
+            out = np.zeros((len(pts), 12))
+            np.add.at(out, r, c)
+        return len(r)
+
+    run()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        npairs = run()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    evals = 12 * len(sub.kernels) * len(pts)
+    return {"value": evals / t, "unit": "particle-moment evals/s", "cores": 1, "kind": "port",
+            "sample": f"{len(pts)}-particle stratified subsample of the star workload ({int(sub.near.sum())} near-wall, "
+                      f"{npairs} pairs), slot fill + twelve channels + row sums for 4 kernels, numpy FP64 oracle, "
+                      f"median of 3 runs of {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--n", type=int, default=1048576)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the wall-integral path shards by particle ranges with replicated geometry "
+              "(no collective during evaluation), and this build runs on a single MI355X; use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from warpsphboundaries_amd import workloads as WL
+    from warpsphboundaries_amd import opmodel
+    from warpsphboundaries_amd.scene import SurfaceRep
+    from warpsphboundaries_amd import _lib as L
+
+    dev = torch.device("cuda:0")
+    log("generating the star workload ...")
+    work = WL.config2(n=args.n)
+    loop, solid = work.loops[0]
+    rep = SurfaceRep.polygon(loop, solid)
+    lpos = torch.tensor(work.positions, dtype=torch.float64, device=dev)
+    lsup = torch.tensor(work.supports, dtype=torch.float64, device=dev)
+    N = len(work.positions)
+    geo = rep.geom(work.h, dev)
+    K = geo.K
+    log(f"N={N} edges={len(loop)} K={K} cells={geo.info['nx']}x{geo.info['ny']}")
+    slots = torch.empty(N * K, dtype=torch.int32, device=dev)
+    counts = torch.empty(N, dtype=torch.int32, device=dev)
+    ind = torch.empty(N, dtype=torch.float64, device=dev)
+    outs = {k: torch.empty((N, 12), dtype=torch.float64, device=dev) for k in work.kernels}
+
+    def step(form, scratch=None):
+        geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind)
+        for k in work.kernels:
+            geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k], scratch=scratch)
+
+    def timed(form, steps, warmup, scratch=None):
+        for _ in range(warmup):
+            step(form, scratch)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(form, scratch)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    # ---- headline: mixed form, adjacency rebuild + evaluate
+    t_mixed = timed("mixed", args.steps, args.warmup)
+    log(f"mixed: {t_mixed * 1e3:.3f} ms/step")
+    res_mixed = {k: outs[k].clone() for k in work.kernels}
+    t_fp64 = timed("fp64", args.steps, args.warmup)
+    log(f"fp64: {t_fp64 * 1e3:.3f} ms/step")
+    res_fp64 = {k: outs[k].clone() for k in work.kernels}
+    nbytes = int(L.load().wsb_moments2d_scratch_bytes(N, K))
+    scratch = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    t_plain = timed("plain", args.plain_steps, 1, scratch)
+    log(f"plain: {t_plain * 1e3:.3f} ms/step")
+    res_plain = {k: outs[k].clone() for k in work.kernels}
+    del scratch
+
+    # ---- error of the mixed form (and the plain path) against the FP64 closed form, signal-relative per channel
+    err = {}
+    for k in work.kernels:
+        ref = res_fp64[k]
+        scale = ref.abs().amax(0).clamp(min=1e-300)
+        err[k] = {"mixed": float(((res_mixed[k] - ref).abs().amax(0) / scale).max()),
+                  "plain": float(((res_plain[k] - ref).abs().amax(0) / scale).max())}
+    log("signal-relative error vs fp64:", err)
+
+    # ---- evaluate-only timing of each kernel's launch (events on the launch stream), the dominant kernel
+    geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    per_kernel = {}
+    for form in ("mixed", "fp64"):
+        for k in work.kernels:
+            for _ in range(2):
+                geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k])
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = max(5, args.steps)
+            e0.record(stream)
+            for _ in range(reps):
+                geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k])
+            e1.record(stream)
+            torch.cuda.synchronize()
+            per_kernel[(form, k)] = e0.elapsed_time(e1) / 1e3 / reps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_adj = e0.elapsed_time(e1) / 1e3 / args.steps
+    log("per-launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()}, "adjacency", t_adj)
+
+    # ---- roofline of the dominant kernel (the mixed-form tile kernel, w6 is the heaviest profile)
+    kdom = max(work.kernels, key=lambda k: per_kernel[("mixed", k)])
+    npairs, active = active_block_counts(torch, rep.vertices.to(dev), rep.edges.to(dev).long(), slots, counts, lpos, lsup, K, kdom)
+    from warpsphboundaries_amd.plan import block_profiles
+    flops = npairs * (opmodel.pair_geometry_flops() + 12) + sum(a * opmodel.block_flops(b) for a, b in
+                                                                  zip(active, block_profiles(kdom))) + N * opmodel.row_flops()
+    bytes_alg = N * opmodel.BYTES_PER_PARTICLE_2D
+    t_dom = per_kernel[("mixed", kdom)]
+    t_roof = max(bytes_alg / PEAK_HBM, flops / PEAK_FP64)
+    compute_bound = flops / PEAK_FP64 >= bytes_alg / PEAK_HBM
+    roof = {"bound": "mfma" if compute_bound else "hbm",
+            "pipe": "FP64 VALU (no MFMA: transcendental gather-reduce; the MI355X FP64 vector peak equals its FP64 matrix peak)",
+            "kernel": f"tile12_kernel<{kdom}, mixed>",
+            "achieved": (flops / t_dom / 1e12) if compute_bound else (bytes_alg / t_dom / 1e9),
+            "peak": PEAK_FP64 / 1e12 if compute_bound else PEAK_HBM / 1e9,
+            "unit": "TFLOP/s" if compute_bound else "GB/s",
+            "frac": t_roof / t_dom, "traffic": None,
+            "algorithmic_flops": flops, "algorithmic_bytes": bytes_alg, "pairs": npairs, "active_chords": active,
+            "launch_seconds": t_dom, "hbm_GBps_algorithmic": bytes_alg / t_dom / 1e9}
+
+    evals = 12 * len(work.kernels) * N
+    value = evals / t_mixed
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle-moment evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_mixed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 (mixed form: FP64 arithmetic, FP32 hardware angle)", "data": "synthetic",
+        "config": {"workload": "star polygon 4096 edges, 1,048,576 particles (30% within one support), h=0.004, "
+                               "kernels cubic/w2/w4/w6; step = adjacency rebuild + twelve moments x 4 kernels",
+                   "N": N, "edges": len(loop), "K": K, "pairs": npairs},
+        "forms": {"mixed": {"evals_per_s": evals / t_mixed, "ms_per_step": t_mixed * 1e3},
+                  "fp64": {"evals_per_s": evals / t_fp64, "ms_per_step": t_fp64 * 1e3},
+                  "plain_fp64": {"evals_per_s": evals / t_plain, "ms_per_step": t_plain * 1e3}},
+        "speedup_mixed_vs_plain": t_plain / t_mixed,
+        "evaluate_only_evals_per_s": {"mixed": evals / sum(per_kernel[("mixed", k)] for k in work.kernels),
+                                      "fp64": evals / sum(per_kernel[("fp64", k)] for k in work.kernels)},
+        "adjacency_ms": t_adj * 1e3,
+        "max_signal_relative_error_vs_fp64": err,
+        "served_latency": None,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        log("timing the CPU oracle ...")
+        line["cpu_baseline"] = cpu_baseline(work)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+"""Parity of the HIP 2-D path against the CPU oracle (adjacency bit-exact; moments 1e-12 for the FP64 forms,
+1e-6 signal-relative for the mixed form) and against the reference's golden fixtures."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjacency as oadj
+from oracle import edge2d
+from tests.golden_data import ELEMENTS, KERNELS, rel_units_h, twelve
+from warpsphboundaries_amd import workloads as WL
+from warpsphboundaries_amd.device import edge_channels12
+from warpsphboundaries_amd.scene import SurfaceRep
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+FORMS = ["plain", "fp64", "mixed"]
+
+
+def _rep(loop, solid):
+    return SurfaceRep.polygon(loop, solid)
+
+
+def _geometries():
+    g = WL.rng(11)
+    star = WL.star_polygon(256, g=g)
+    notch = np.array([[0.1, 0.1], [0.9, 0.1], [0.9, 0.9], [0.52, 0.9], [0.5, 0.3], [0.48, 0.9], [0.1, 0.9]])
+    return {
+        "box-outside": (np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]]), "outside"),
+        "64gon": (WL.regular_polygon((0.5, 0.5), 0.15, 64), "inside"),
+        "star256": (star, "inside"),
+        "notch": (notch, "inside"),
+    }
+
+
+def _particles(loop, h, n, seed):
+    """uniform points plus adversarial ones: on edges, at vertices, on cell and bin boundaries, near endpoints."""
+    g = WL.rng(seed)
+    lo, hi = loop.min(0) - 2 * h, loop.max(0) + 2 * h
+    pts = [g.uniform(lo, hi, (n, 2))]
+    p, q = loop, np.roll(loop, -1, 0)
+    k = g.integers(0, len(loop), n // 4)
+    t = g.uniform(0, 1, (n // 4, 1))
+    pts.append(p[k] + t * (q[k] - p[k]))                                   # on the edges (up to rounding)
+    pts.append(loop.copy())                                                # exactly at the vertices
+    pts.append(p[k] + g.normal(0, 1e-9, (n // 4, 2)))                     # near the endpoints
+    mid = 0.5 * (p + q)
+    pts.append(mid)                                                        # edge midpoints
+    return np.concatenate(pts)
+
+
+def _oracle_rows(st, kname, lpos, lsup, e, cnt, ind, plane):
+    K = st.K
+    rows = np.repeat(np.arange(len(lpos)), cnt)
+    slots = np.concatenate([e[r * K:r * K + c] for r, c in enumerate(cnt)]) if len(rows) else np.zeros(0, int)
+    P = (st.verts[st.edges[slots, 0]] - lpos[rows]) / lsup[rows, None]
+    Q = (st.verts[st.edges[slots, 1]] - lpos[rows]) / lsup[rows, None]
+    c = edge2d.edge_channels12(P, Q, kname) if len(rows) else np.zeros((0, 12))
+    out = np.zeros((len(lpos), 12))
+    np.add.at(out, rows, c)
+    return out + ind[:, None] * plane[None]
+
+
+@pytest.mark.parametrize("name", ["box-outside", "64gon", "star256", "notch"])
+def test_adjacency_bit_exact(name):
+    loop, solid = _geometries()[name]
+    rep = _rep(loop, solid)
+    h = 0.02 if name != "star256" else 0.01
+    pts = _particles(rep.vertices.numpy(), h, 3000, 5)
+    g = WL.rng(6)
+    sup = np.where(g.uniform(0, 1, len(pts)) < 0.3, 0.6 * h, h)
+    smax = h
+    # put a few particles exactly on cell corners of the static grid
+    st = oadj.Static(rep.vertices.numpy(), rep.edges.numpy(), rep.background, smax)
+    corners = st.cl.lo + st.cl.cell * g.integers(0, min(st.cl.dims), (200, 2)).astype(np.float64)
+    pts = np.concatenate([pts, corners])
+    sup = np.concatenate([sup, np.full(len(corners), h)])
+    geo = rep.geom(smax, DEV)
+    assert geo.K == st.K
+    lp = torch.tensor(pts, dtype=torch.float64, device=DEV)
+    ls = torch.tensor(sup, dtype=torch.float64, device=DEV)
+    valid = torch.tensor(g.uniform(0, 1, len(pts)) < 0.95, device=DEV)
+    slots, counts, ind = geo.adjacency(lp, ls, valid=valid, pad=True)
+    e_ref, c_ref, i_ref = oadj.fixed_topology(st, pts, sup, valid.cpu().numpy())
+    print(name, "rows", len(pts), "K", st.K, "pairs", int(c_ref.sum()))
+    assert np.array_equal(slots.cpu().numpy(), e_ref)
+    assert np.array_equal(counts.cpu().numpy(), c_ref)
+    got = ind.cpu().numpy()
+    # the boundary angle sums go through atan2 (device vs host libm): equal to rounding, every other row bit-exact
+    frac = (i_ref != np.round(i_ref))
+    assert np.array_equal(got[~frac], i_ref[~frac])
+    np.testing.assert_allclose(got[frac], i_ref[frac], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("kname", KERNELS)
+def test_moments_match_oracle(form, kname):
+    loop, solid = _geometries()["star256"]
+    rep = _rep(loop, solid)
+    h = 0.01
+    pts = _particles(rep.vertices.numpy(), h, 2000, 9)
+    sup = np.full(len(pts), h)
+    st = oadj.Static(rep.vertices.numpy(), rep.edges.numpy(), rep.background, h)
+    e_ref, c_ref, i_ref = oadj.fixed_topology(st, pts, sup)
+    ref = _oracle_rows(st, kname, pts, sup, e_ref, c_ref, i_ref, edge2d.plane_vector(kname))
+    geo = rep.geom(h, DEV)
+    lp = torch.tensor(pts, dtype=torch.float64, device=DEV)
+    ls = torch.tensor(sup, dtype=torch.float64, device=DEV)
+    slots, counts, ind = geo.adjacency(lp, ls)
+    out = geo.moments(kname, slots, counts, ind, lp, ls, form=form).cpu().numpy()
+    err = np.abs(out - ref)
+    sig = err.max(0) / np.maximum(np.abs(ref).max(0), 1e-300)
+    print(form, kname, "max abs", err.max(), "signal-relative per channel", sig.max())
+    if form == "mixed":
+        assert sig.max() <= 1e-6
+    else:
+        assert err.max() <= 1e-12
+    # relaunch is bitwise identical
+    again = geo.moments(kname, slots, counts, ind, lp, ls, form=form).cpu().numpy()
+    assert np.array_equal(out, again)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_pair_channels_match_oracle_on_degenerate_edges(form):
+    g = WL.rng(3)
+    n = 3000
+    p = g.uniform(-1.2, 1.2, (n, 2))
+    q = p + g.normal(0, 0.5, (n, 2))
+    q[:50] = p[:50] * g.uniform(-2, -0.1, (50, 1))
+    p[50:100] = g.normal(0, 1e-9, (50, 2))
+    q[100:150] = p[100:150] + g.normal(0, 1e-6, (50, 2))
+    verts = np.concatenate([p, q])
+    edges = np.stack([np.arange(n), np.arange(n) + n], 1)
+    pos = np.zeros((1, 2))
+    sup = np.ones(1)
+    for kname in KERNELS:
+        ref = edge2d.edge_channels12(p, q, kname)
+        t = lambda a, dt: torch.tensor(a, dtype=dt, device=DEV)
+        got = edge_channels12(t(np.zeros(n), torch.int32), t(np.arange(n), torch.int32), t(pos, torch.float64),
+                              t(sup, torch.float64), t(verts, torch.float64), t(edges, torch.int32), kname, form).cpu().numpy()
+        err = np.abs(got - ref)
+        tol = 1e-6 * np.abs(ref).max() if form == "mixed" else 1e-12
+        assert err.max() <= tol, (kname, form, err.max())
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("kname", KERNELS)
+def test_golden_fixture_elements_on_device(form, kname):
+    """every golden element: device edge channels summed over the polygon + indicator x plane vector."""
+    verts, edges, owner, inds, refs = [], [], [], [], []
+    base = 0
+    for i, rec in enumerate(ELEMENTS):
+        rel = rel_units_h(rec["polygon"], rec["x"], rec["h"])
+        n = len(rel)
+        verts.append(rel)
+        edges.append(np.stack([np.arange(n), (np.arange(n) + 1) % n], 1) + base)
+        owner.append(np.full(n, i))
+        inds.append(oadj.indicator_points(rel, np.stack([np.arange(n), (np.arange(n) + 1) % n], 1), 0, np.zeros((1, 2)),
+                                          np.ones(1))[0])
+        refs.append(twelve(rec, kname))
+        base += n
+    V = np.concatenate(verts)
+    E = np.concatenate(edges)
+    t = lambda a, dt: torch.tensor(a, dtype=dt, device=DEV)
+    P = len(E)
+    c = edge_channels12(t(np.zeros(P), torch.int32), t(np.arange(P), torch.int32), t(np.zeros((1, 2)), torch.float64),
+                        t(np.ones(1), torch.float64), t(V, torch.float64), t(E, torch.int32), kname, form).cpu().numpy()
+    tot = np.zeros((len(ELEMENTS), 12))
+    np.add.at(tot, np.concatenate(owner), c)
+    tot += np.array(inds)[:, None] * edge2d.plane_vector(kname)[None]
+    ref = np.array(refs)
+    err = np.abs(tot - ref)
+    if form == "mixed":
+        assert (err.max(0) / np.abs(ref).max(0)).max() <= 1e-6
+    else:
+        tol = np.array([1e-12] * 3 + [1e-11] * 6 + [1e-12] * 3)
+        assert (err <= tol).all(), err.max(0)

@@ -1,0 +1,73 @@
+"""Host logic on the CPU: the product's exact plan compiler and the algebra of the closed-form kernel, both against
+the oracle's independent block recursion (numpy models of the device algorithms in tests/cpu_models.py)."""
+from fractions import Fraction as F
+
+import numpy as np
+import pytest
+
+from oracle import edge2d
+from tests import cpu_models as M
+from warpsphboundaries_amd import kernels as K
+from warpsphboundaries_amd import plan as PL
+
+KERNELS = ["cubic", "w2", "w4", "w6"]
+
+
+def random_edges(n, seed, scale=1.2):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-scale, scale, (n, 2))
+    q = p + rng.normal(0, 0.5, (n, 2))
+    # near-degenerate cases: the query on the edge line, near an endpoint, tiny edges
+    q[:20] = p[:20] * rng.uniform(-2, -0.1, (20, 1))             # origin on the segment line
+    p[20:40] = rng.normal(0, 1e-9, (20, 2))                        # endpoint at the query
+    q[40:60] = p[40:60] + rng.normal(0, 1e-6, (20, 2))            # tiny edges
+    return p, q
+
+
+def test_kernel_table_exact_pins():
+    """mass 1 and the whole-support second moments, computed from the blocks (exact rationals)."""
+    assert {n: K.plane_mass(n) for n in KERNELS} == {n: F(1) for n in KERNELS}
+    assert K.plane_second_moment("w2") == F(5, 72) and K.plane_second_moment("w4") == F(7, 132)
+    assert K.plane_second_moment("w6") == F(3, 70) and K.plane_second_moment("cubic") == F(31, 392)
+    # the indicator weight of m00 is exactly the mass
+    for n in KERNELS:
+        assert PL.indicator_weights(n) == 1
+
+
+def test_phi_vanishes_at_R_and_differentiates_back():
+    for n in KERNELS:
+        for b in PL.block_profiles(n):
+            R = b.R
+            assert sum(c * R ** k for k, c in enumerate(b.Phi)) == 0
+            # d Phi / dr = r P(r)
+            dphi = [k * c for k, c in enumerate(b.Phi)][1:]
+            rp = [F(0)] + list(b.P)
+            assert all(a == bb for a, bb in zip(dphi + [F(0)] * 5, rp + [F(0)] * 5))
+
+
+@pytest.mark.parametrize("k", KERNELS)
+def test_flattened_plan_matches_oracle_recursion(k):
+    p, q = random_edges(400, 1)
+    got = M.plan_interpreter(PL.surface_plan(k), p, q)
+    ref = edge2d.edge_channels12(p, q, k)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("k", KERNELS)
+def test_closed_form_algebra_matches_oracle(k):
+    p, q = random_edges(2000, 2)
+    got = M.closed_form(k, p, q)
+    ref = edge2d.edge_channels12(p, q, k)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-12)
+
+
+@pytest.mark.parametrize("alpha", [(3, 0), (2, 1), (1, 3), (0, 4), (2, 2)])
+def test_general_plan_matches_oracle(alpha):
+    p, q = random_edges(300, 3)
+    for k in ["cubic", "w4"]:
+        got = M.plan_interpreter(PL.general_plan(k, alpha), p, q)[:, 0]
+        ref = edge2d.moment_edges(p, q, k, alpha)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
+        got = M.plan_interpreter(PL.general_plan(k, alpha, grad=True), p, q)[:, :2]
+        ref = edge2d.moment_grad_edges(p, q, k, alpha)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11)

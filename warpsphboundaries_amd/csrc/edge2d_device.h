@@ -1,0 +1,337 @@
+// Device arithmetic of the 2-D edge reductions: the plan interpreter of the plain FP64 form and the
+// closed form with shared primitive tables of the optimized forms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tables.inc"
+#include "wsb_internal.h"
+
+#define WSB_INV_PI 0.31830988618379067154
+
+// ================================================================================================= plain form
+// A term-by-term interpreter of the flattened plan, one (query, edge) pair per thread: every term recomputes
+// its primitives from scratch (reference edge/warp2d.py:28-149 and edge/warpbc.py:578-638).
+namespace plain {
+
+__device__ inline double ipow(double x, int k) {
+    double r = 1.0;
+    for (int i = 0; i < k; ++i) r *= x;
+    return r;
+}
+
+__device__ inline double asinh_ref(double u) {
+    const double a = fabs(u);
+    const double v = log(a + sqrt(a * a + 1.0));
+    return u < 0.0 ? -v : v;
+}
+
+// odd antiderivative of r^m, m >= 0 (parity-chain recurrence, z = 0 guarded)
+__device__ inline double Im(int m, double s, double z) {
+    const double zz = z * z;
+    const double r2 = s * s + zz;
+    const double r = r2 > 0.0 ? sqrt(r2) : 0.0;
+    double val = s;
+    int idx = 0;
+    if (m % 2 == 1) {
+        const double tail = z != 0.0 ? zz * asinh_ref(s / fabs(z)) : 0.0;
+        val = (s * r + tail) / 2.0;
+        idx = 1;
+    }
+    while (idx < m) {
+        idx += 2;
+        const double rp = (idx % 2 == 0) ? ipow(r2, idx / 2) : r * ipow(r2, (idx - 1) / 2);
+        val = (s * rp + (double)idx * zz * val) / (double)(idx + 1);
+    }
+    return val;
+}
+
+__device__ inline double binom(int n, int k) {
+    double v = 1.0;
+    for (int i = 0; i < k; ++i) v = v * (double)(n - i) / (double)(i + 1);
+    return v;
+}
+
+// int s^j r^m ds, m >= 0
+__device__ inline double S(int j, int m, double s, double z) {
+    const double zz = z * z;
+    double tot = 0.0;
+    if (j % 2 == 0) {
+        const int h = j / 2;
+        for (int i = 0; i <= h; ++i) {
+            const double sg = ((h - i) % 2 == 1) ? -1.0 : 1.0;
+            tot += binom(h, i) * sg * ipow(zz, h - i) * Im(m + 2 * i, s, z);
+        }
+    } else {
+        const int h = (j - 1) / 2;
+        const double r2 = s * s + zz;
+        const double r = r2 > 0.0 ? sqrt(r2) : 0.0;
+        for (int i = 0; i <= h; ++i) {
+            const int mm = m + 2 * i;
+            const double rp = (mm % 2 == 0) ? ipow(r2, (mm + 2) / 2) : r * ipow(r2, (mm + 1) / 2);
+            const double sg = ((h - i) % 2 == 1) ? -1.0 : 1.0;
+            tot += binom(h, i) * sg * ipow(zz, h - i) * rp / (double)(mm + 2);
+        }
+    }
+    return tot;
+}
+
+__device__ inline double sdiff(int j, int m, double lo, double hi, double z) { return S(j, m, hi, z) - S(j, m, lo, z); }
+
+__device__ inline double dangle(double z, double lo, double hi) {
+    if (z == 0.0) return 0.0;
+    const double y = z * (hi - lo);
+    const double x = z * z + lo * hi;
+    if (y == 0.0 && x == 0.0) return 0.0;
+    return atan2(y, x);
+}
+
+// coefficient of s^j in (z n0 + s t0)^a (z n1 + s t1)^b
+__device__ inline double ypoly_coef(int a, int b, int j, double z, double n0, double n1, double t0, double t1) {
+    double tot = 0.0;
+    for (int ja = 0; ja <= a; ++ja) {
+        const int jb = j - ja;
+        if (jb >= 0 && jb <= b) {
+            const double ca = binom(a, ja) * ipow(t0, ja) * ipow(z * n0, a - ja);
+            const double cb = binom(b, jb) * ipow(t1, jb) * ipow(z * n1, b - jb);
+            tot += ca * cb;
+        }
+    }
+    return tot;
+}
+
+__device__ inline double edge_integral(int a, int b, double lo, double hi, double z, double n0, double n1, double t0,
+                                       double t1, const int32_t* cn, const double* cc, int c0, int c1) {
+    double tot = 0.0;
+    for (int j = 0; j <= a + b; ++j) {
+        const double yc = ypoly_coef(a, b, j, z, n0, n1, t0, t1);
+        if (yc != 0.0) {
+            double inner = 0.0;
+            for (int k = c0; k < c1; ++k) inner += cc[k] * sdiff(j, cn[k], lo, hi, z);
+            tot += yc * inner;
+        }
+    }
+    return tot;
+}
+
+struct Plan {
+    const double* radii;
+    const int32_t* cn;
+    const double* cc;
+    const int32_t* E;   // [nE*7] ch, i, a, b, R, c0, c1
+    const int32_t* V;   // [nV*4] ch, R, c0, c1
+    const double* mR;
+    int nE, nV;
+};
+
+// the edge-local channels of one pair (units of h, x 1/pi), p and q relative to the query in units of h
+__device__ inline void pair_channels(const Plan& pl, double px, double py, double qx, double qy, double ch[wsb::NCH]) {
+    const double dx = qx - px, dy = qy - py;
+    const double ell = sqrt(dx * dx + dy * dy);
+    if (ell == 0.0) return;
+    const double t0 = dx / ell, t1 = dy / ell;
+    const double n0 = t1, n1 = -t0;
+    const double z = wsb_cross2(px, py, qx, qy) / ell;
+    const double s0 = (dx * px + dy * py) / ell;
+    const double s1 = (dx * qx + dy * qy) / ell;
+    const double az = fabs(z);
+    for (int t = 0; t < pl.nE; ++t) {
+        const int32_t* e = pl.E + 7 * t;
+        const double R = pl.radii[e[4]];
+        if (az < R) {
+            const double L = sqrt((R - az) * (R + az));
+            const double lo = fmax(s0, -L), hi = fmin(s1, L);
+            if (lo < hi) {
+                const double ni = e[1] == 1 ? n1 : n0;
+                ch[e[0]] += ni * edge_integral(e[2], e[3], lo, hi, z, n0, n1, t0, t1, pl.cn, pl.cc, e[5], e[6]);
+            }
+        }
+    }
+    for (int t = 0; t < pl.nV; ++t) {
+        const int32_t* v = pl.V + 4 * t;
+        const double R = pl.radii[v[1]];
+        if (az < R) {
+            const double L = sqrt((R - az) * (R + az));
+            const double lo = fmax(s0, -L), hi = fmin(s1, L);
+            if (lo < hi) {
+                double poly = 0.0;
+                for (int k = v[2]; k < v[3]; ++k) poly += pl.cc[k] / (double)(pl.cn[k] + 2) * sdiff(0, pl.cn[k], lo, hi, z);
+                ch[v[0]] += z * poly - pl.mR[t] * dangle(z, lo, hi);
+            }
+        }
+    }
+    for (int k = 0; k < wsb::NCH; ++k) ch[k] *= WSB_INV_PI;
+}
+
+}  // namespace plain
+
+// ================================================================================================= closed form
+// One pair, all twelve channels from ONE set of primitives per kernel block: the chord, the endpoint radii,
+// one logarithm (the asinh difference), one angle, and the differences dI_m = I_m(hi) - I_m(lo) of the odd
+// antiderivatives of r^m by the linear recurrence  dI_m = (d[s r^m] + m z^2 dI_{m-2}) / (m + 1).  With
+//     A[f] = int f(r) ds,  V[f] = int sum f_n r^n/(n+2) ds,  B[f] = int s f(r) ds = sum f_n d[r^(n+2)]/(n+2)
+// for the block profile P and its compact potential Phi (exact tables of tables.inc) the channels are
+//     lam += z V[P] - mR[P] theta                   m10 += n_x A[Phi]       m01 += n_y A[Phi]
+//     G_j -= n_j A[P]                               g1_dj += -n_j (z n_d A[P] + t_d B[P]) + delta_dj lam
+//     m20 += n_x (z n_x A[Phi] + t_x B[Phi]) - (z V[Phi] - mR[Phi] theta)
+//     m11 += n_x (z n_y A[Phi] + t_y B[Phi])        m02 += n_y (z n_y A[Phi] + t_y B[Phi]) - (z V[Phi] - mR[Phi] theta)
+// which is the compact-potential recursion of the reference (edge/np2d.py:46-101, edge/warpbc.py:119-136) with
+// the line integrals of every channel sharing their primitives.
+namespace closed {
+
+// the two arithmetic forms
+enum { FP64 = 1, MIXED = 2 };
+
+__device__ __forceinline__ double dangle_fp64(double z, double lo, double hi) {
+    if (z == 0.0) return 0.0;
+    const double y = z * (hi - lo);
+    const double x = z * z + lo * hi;
+    if (y == 0.0 && x == 0.0) return 0.0;
+    return atan2(y, x);
+}
+
+// the angle through the FP32 hardware path: an exact power-of-two rescale keeps both arguments in the
+// normal FP32 range; the angle enters with an O(1) weight, so its FP32 rounding is not amplified
+__device__ __forceinline__ double dangle_mixed(double z, double lo, double hi) {
+    if (z == 0.0) return 0.0;
+    double y = z * (hi - lo);
+    double x = z * z + lo * hi;
+    if (y == 0.0 && x == 0.0) return 0.0;
+    int e;
+    (void)frexp(fmax(fabs(x), fabs(y)), &e);
+    y = ldexp(y, -e);
+    x = ldexp(x, -e);
+    return (double)atan2f((float)y, (float)x);
+}
+
+template <int KID, int FORM>
+__device__ __forceinline__ void pair_channels(double px, double py, double qx, double qy, double c[wsb::NCH]) {
+    using T = KTab<KID>;
+    constexpr int NM = T::NMAXP;
+    const double dx = qx - px, dy = qy - py;
+    const double ell2 = dx * dx + dy * dy;
+    if (!(ell2 > 0.0)) return;
+    const double ell = sqrt(ell2);
+    const double il = 1.0 / ell;
+    const double t0 = dx * il, t1 = dy * il;
+    const double n0 = t1, n1 = -t0;
+    const double z = wsb_cross2(px, py, qx, qy) * il;     // zero exactly when the rounded predicate is zero
+    const double s0 = (dx * px + dy * py) * il;
+    const double s1 = (dx * qx + dy * qy) * il;
+    const double az = fabs(z), zz = z * z;
+#pragma unroll
+    for (int b = 0; b < T::NB; ++b) {
+        const double R = T::R[b];
+        if (az < R) {
+            const double L = sqrt((R - az) * (R + az));
+            const double lo = fmax(s0, -L), hi = fmin(s1, L);
+            if (lo < hi) {
+                const double rh = sqrt(hi * hi + zz), rl = sqrt(lo * lo + zz);
+                double dA = 0.0;
+                if (z != 0.0) {
+                    // asinh(hi/|z|) - asinh(lo/|z|) = log(F(hi)/F(lo)),  F(s) = s + r  (= z^2/(r - s) for s < 0)
+                    const double Fh = hi >= 0.0 ? hi + rh : zz / (rh - hi);
+                    const double Fl = lo >= 0.0 ? lo + rl : zz / (rl - lo);
+                    dA = log(Fh / Fl);
+                }
+                double dI[NM + 1];
+                dI[0] = hi - lo;
+                if (NM >= 1) dI[1] = 0.5 * (hi * rh - lo * rl + zz * dA);
+                double AP = 0.0, VP = 0.0, AF = 0.0, VF = 0.0, BP = 0.0, BF = 0.0;
+                double ph = rh, pl = rl;
+#pragma unroll
+                for (int m = 0; m <= NM + 2; ++m) {
+                    if (m >= 2) {
+                        ph *= rh;
+                        pl *= rl;
+                    }
+                    if (m >= 2 && m <= NM) dI[m] = ((hi * ph - lo * pl) + (double)m * zz * dI[m - 2]) * (1.0 / (double)(m + 1));
+                    if (m <= NM) {
+                        if (T::P[b][m] != 0.0) AP += T::P[b][m] * dI[m];
+                        if (T::QP[b][m] != 0.0) VP += T::QP[b][m] * dI[m];
+                        if (T::F[b][m] != 0.0) AF += T::F[b][m] * dI[m];
+                        if (T::QF[b][m] != 0.0) VF += T::QF[b][m] * dI[m];
+                    }
+                    if (m >= 2) {
+                        const double dR = ph - pl;
+                        if (T::QP[b][m - 2] != 0.0) BP += T::QP[b][m - 2] * dR;
+                        if (T::QF[b][m - 2] != 0.0) BF += T::QF[b][m - 2] * dR;
+                    }
+                }
+                const double th = FORM == MIXED ? dangle_mixed(z, lo, hi) : dangle_fp64(z, lo, hi);
+                const double lam = z * VP - T::MRP[b] * th;
+                const double valF = z * VF - T::MRF[b] * th;
+                const double zn0 = z * n0, zn1 = z * n1;
+                const double YxP = zn0 * AP + t0 * BP, YyP = zn1 * AP + t1 * BP;
+                const double YxF = zn0 * AF + t0 * BF, YyF = zn1 * AF + t1 * BF;
+                c[0] += lam;
+                c[1] += n0 * AF;
+                c[2] += n1 * AF;
+                c[3] -= n0 * AP;
+                c[4] -= n1 * AP;
+                c[5] += lam - n0 * YxP;
+                c[6] -= n1 * YxP;
+                c[7] -= n0 * YyP;
+                c[8] += lam - n1 * YyP;
+                c[9] += n0 * YxF - valF;
+                c[10] += n0 * YyF;
+                c[11] += n1 * YyF - valF;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < wsb::NCH; ++k) c[k] *= WSB_INV_PI;
+}
+
+}  // namespace closed
+
+// ================================================================================================= row output
+template <int KID>
+__device__ __forceinline__ const double* plane_of() { return KTab<KID>::PLANE; }
+
+// indicator x plane vector, physical scaling, world rotation, store / accumulate one row of twelve channels
+__device__ __forceinline__ void finalize_row(double c[wsb::NCH], double ind, double h, const double* plane, const double* rot4,
+                                             int flags, double* out) {
+    if (!(flags & WSB_OUT_NO_INDICATOR)) {
+#pragma unroll
+        for (int k = 0; k < wsb::NCH; ++k) c[k] += ind * plane[k];
+    }
+    if (flags & WSB_OUT_PHYSICAL) {
+        const double hh = h * h, ih = 1.0 / h;
+        c[1] *= h;
+        c[2] *= h;
+        c[3] *= ih;
+        c[4] *= ih;
+        c[9] *= hh;
+        c[10] *= hh;
+        c[11] *= hh;
+    }
+    if ((flags & WSB_OUT_WORLD) && rot4) {
+        const double r00 = rot4[0], r01 = rot4[1], r10 = rot4[2], r11 = rot4[3];
+        // vectors v' = R v
+        double a0 = c[1], a1 = c[2];
+        c[1] = r00 * a0 + r01 * a1;
+        c[2] = r10 * a0 + r11 * a1;
+        a0 = c[3];
+        a1 = c[4];
+        c[3] = r00 * a0 + r01 * a1;
+        c[4] = r10 * a0 + r11 * a1;
+        // tensors M' = R M R^T, M = [[c5, c6], [c7, c8]]
+        const double m00 = c[5], m01 = c[6], m10 = c[7], m11 = c[8];
+        c[5] = r00 * (m00 * r00 + m01 * r01) + r01 * (m10 * r00 + m11 * r01);
+        c[6] = r00 * (m00 * r10 + m01 * r11) + r01 * (m10 * r10 + m11 * r11);
+        c[7] = r10 * (m00 * r00 + m01 * r01) + r11 * (m10 * r00 + m11 * r01);
+        c[8] = r10 * (m00 * r10 + m01 * r11) + r11 * (m10 * r10 + m11 * r11);
+        const double xx = c[9], xy = c[10], yy = c[11];
+        c[9] = r00 * r00 * xx + 2.0 * r00 * r01 * xy + r01 * r01 * yy;
+        c[10] = r00 * r10 * xx + (r00 * r11 + r01 * r10) * xy + r01 * r11 * yy;
+        c[11] = r10 * r10 * xx + 2.0 * r10 * r11 * xy + r11 * r11 * yy;
+    }
+    if (flags & WSB_OUT_ACCUMULATE) {
+#pragma unroll
+        for (int k = 0; k < wsb::NCH; ++k) out[k] += c[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < wsb::NCH; ++k) out[k] = c[k];
+    }
+}

@@ -1,0 +1,63 @@
+"""Write ``tables.inc``: the exact kernel-block profiles rounded once to float64 constants, as compile-time
+tables for the optimized HIP kernels (one template specialisation per kernel, fully unrolled loops).
+
+Run by the build (``__graft_entry__.build`` / ``warpsphboundaries_amd.build``); output next to this file.
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from warpsphboundaries_amd import plan as PL          # noqa: E402
+from warpsphboundaries_amd.kernels import NAMES, plane_channels   # noqa: E402
+
+NMAX = 16          # fixed array extent (>= max power + 1 of every profile)
+
+
+def _arr(vals):
+    vals = list(vals) + [0] * (NMAX - len(vals))
+    return "{" + ", ".join(repr(float(v)) for v in vals) + "}"
+
+
+def render():
+    out = ["// generated by gen_tables.py from the exact kernel table -- do not edit",
+           "#pragma once", "", "template <int KID> struct KTab;", ""]
+    for kid, name in enumerate(NAMES):
+        bl = PL.block_profiles(name)
+        nb = len(bl)
+        degP = [len(b.P) - 1 for b in bl]
+        degF = [len(b.Phi) - 1 for b in bl]
+        QP = [[c / (n + 2) for n, c in enumerate(b.P)] for b in bl]
+        QF = [[c / (n + 2) for n, c in enumerate(b.Phi)] for b in bl]
+        out.append(f"// {name}")
+        out.append(f"template <> struct KTab<{kid}> {{")
+        out.append(f"  static constexpr int NB = {nb};")
+        out.append(f"  static constexpr int NMAXP = {max(max(degP), max(degF))};")
+        out.append(f"  static constexpr double R[NB] = {{{', '.join(repr(float(b.R)) for b in bl)}}};")
+        out.append(f"  static constexpr int DEGP[NB] = {{{', '.join(map(str, degP))}}};")
+        out.append(f"  static constexpr int DEGF[NB] = {{{', '.join(map(str, degF))}}};")
+        out.append(f"  static constexpr double P[NB][{NMAX}] = {{{', '.join(_arr(b.P) for b in bl)}}};")
+        out.append(f"  static constexpr double F[NB][{NMAX}] = {{{', '.join(_arr(b.Phi) for b in bl)}}};")
+        out.append(f"  static constexpr double QP[NB][{NMAX}] = {{{', '.join(_arr(q) for q in QP)}}};")
+        out.append(f"  static constexpr double QF[NB][{NMAX}] = {{{', '.join(_arr(q) for q in QF)}}};")
+        out.append(f"  static constexpr double MRP[NB] = {{{', '.join(repr(float(b.mR_P)) for b in bl)}}};")
+        out.append(f"  static constexpr double MRF[NB] = {{{', '.join(repr(float(b.mR_Phi)) for b in bl)}}};")
+        out.append(f"  static constexpr double PLANE[12] = {{{', '.join(repr(v) for v in plane_channels(name))}}};")
+        out.append("};")
+        out.append("")
+    return "\n".join(out) + "\n"
+
+
+def main(path=None):
+    path = path or os.path.join(HERE, "tables.inc")
+    text = render()
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as f:
+            f.write(text)
+    return path
+
+
+if __name__ == "__main__":
+    print(main())

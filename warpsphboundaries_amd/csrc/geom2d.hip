@@ -1,0 +1,457 @@
+// Static 2-D geometry (edge cell list, cell indicator, y-bins) and the fixed-capacity adjacency kernel.
+//
+// Bit-exactness contract: every floating-point operation of the slot predicate and of the cell / bin
+// index arithmetic is written in the reference's order with no fused multiply-add (contract(off) for this
+// whole file), so the slots (set and order) equal those of scene/fixedadj.py:67-97 and of the cell list
+// built by scene/scene.py:100-119,273-279.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "wsb_internal.h"
+
+namespace {
+
+// ------------------------------------------------------------------------------------------------ host build
+struct HostCells {
+    double lo0, lo1, cell;
+    int nx, ny;
+    std::vector<int32_t> start, items;
+};
+
+// every edge into every cell overlapped by its support-inflated box, items of a cell in ascending edge id
+// (reference scene/scene.py:100-119 buildCellList with the boxes of :273-279)
+HostCells build_cells(const double* V, const int32_t* E, int ne, double smax) {
+    std::vector<double> lo(2 * (size_t)ne), hi(2 * (size_t)ne);
+    double g0 = INFINITY, g1 = INFINITY, G0 = -INFINITY, G1 = -INFINITY;
+    for (int e = 0; e < ne; ++e) {
+        const double ax = V[2 * E[2 * e]], ay = V[2 * E[2 * e] + 1];
+        const double bx = V[2 * E[2 * e + 1]], by = V[2 * E[2 * e + 1] + 1];
+        lo[2 * e] = std::min(ax, bx) - smax;
+        lo[2 * e + 1] = std::min(ay, by) - smax;
+        hi[2 * e] = std::max(ax, bx) + smax;
+        hi[2 * e + 1] = std::max(ay, by) + smax;
+        g0 = std::min(g0, lo[2 * e]);
+        g1 = std::min(g1, lo[2 * e + 1]);
+        G0 = std::max(G0, hi[2 * e]);
+        G1 = std::max(G1, hi[2 * e + 1]);
+    }
+    HostCells c;
+    c.lo0 = g0;
+    c.lo1 = g1;
+    c.cell = smax;
+    const int64_t nx = std::max<int64_t>((int64_t)std::ceil((G0 - g0) / smax), 1);
+    const int64_t ny = std::max<int64_t>((int64_t)std::ceil((G1 - g1) / smax), 1);
+    c.nx = (int)nx;
+    c.ny = (int)ny;
+    std::vector<int64_t> cid;
+    std::vector<int32_t> item;
+    for (int e = 0; e < ne; ++e) {
+        const int64_t i0x = std::max<int64_t>((int64_t)std::floor((lo[2 * e] - g0) / smax), 0);
+        const int64_t i0y = std::max<int64_t>((int64_t)std::floor((lo[2 * e + 1] - g1) / smax), 0);
+        const int64_t i1x = std::min<int64_t>((int64_t)std::floor((hi[2 * e] - g0) / smax), nx - 1);
+        const int64_t i1y = std::min<int64_t>((int64_t)std::floor((hi[2 * e + 1] - g1) / smax), ny - 1);
+        const int64_t cx = std::max<int64_t>(i1x - i0x + 1, 0), cy = std::max<int64_t>(i1y - i0y + 1, 0);
+        for (int64_t l = 0; l < cx * cy; ++l) {
+            cid.push_back((i0x + l % cx) * ny + (i0y + l / cx));
+            item.push_back(e);
+        }
+    }
+    // stable counting sort by cell id
+    c.start.assign((size_t)(nx * ny + 1), 0);
+    for (int64_t k : cid) c.start[k + 1]++;
+    for (int64_t k = 0; k < nx * ny; ++k) c.start[k + 1] += c.start[k];
+    c.items.resize(item.size());
+    std::vector<int32_t> fill(c.start.begin(), c.start.end() - 1);
+    for (size_t i = 0; i < item.size(); ++i) c.items[fill[cid[i]]++] = item[i];
+    return c;
+}
+
+struct HostBins {
+    double y0, dy;
+    int nb;
+    std::vector<int32_t> start, items;
+};
+
+// reference scene/scene.py:314-331 (SurfaceRep._ybins)
+HostBins build_bins(const double* V, const int32_t* E, int ne) {
+    HostBins b;
+    std::vector<double> ylo(ne), yhi(ne);
+    double ymin = INFINITY, ymax = -INFINITY;
+    for (int e = 0; e < ne; ++e) {
+        const double ay = V[2 * E[2 * e] + 1], by = V[2 * E[2 * e + 1] + 1];
+        ylo[e] = std::min(ay, by);
+        yhi[e] = std::max(ay, by);
+        ymin = std::min(ymin, ylo[e]);
+        ymax = std::max(ymax, yhi[e]);
+    }
+    b.nb = std::max(1, (int)std::floor(std::sqrt((double)ne)) * 2);
+    b.y0 = ymin;
+    b.dy = std::max(ymax - b.y0, 1e-300) / b.nb;
+    std::vector<int64_t> bin;
+    std::vector<int32_t> item;
+    for (int e = 0; e < ne; ++e) {
+        const int64_t i0 = std::clamp<int64_t>((int64_t)std::floor((ylo[e] - b.y0) / b.dy), 0, b.nb - 1);
+        const int64_t i1 = std::clamp<int64_t>((int64_t)std::floor((yhi[e] - b.y0) / b.dy), 0, b.nb - 1);
+        for (int64_t k = i0; k <= i1; ++k) {
+            bin.push_back(k);
+            item.push_back(e);
+        }
+    }
+    b.start.assign((size_t)b.nb + 1, 0);
+    for (int64_t k : bin) b.start[k + 1]++;
+    for (int k = 0; k < b.nb; ++k) b.start[k + 1] += b.start[k];
+    b.items.resize(item.size());
+    std::vector<int32_t> fill(b.start.begin(), b.start.end() - 1);
+    for (size_t i = 0; i < item.size(); ++i) b.items[fill[bin[i]]++] = item[i];
+    return b;
+}
+
+// winding number + background at one point (reference scene/scene.py:333-371 and scene/fixedadj.py:98-146);
+// `h` normalises the relative coordinates (h = 0: unnormalised, the cell-centre convention)
+__host__ __device__ inline double winding_point(double px, double py, double h, const double* V, const int32_t* E, int ne,
+                                                const int32_t* bstart, const int32_t* bitems, double y0, double dy, int nb,
+                                                double bg) {
+    const double iyf = floor((py - y0) / dy);
+    if (!(iyf >= 0.0 && iyf <= (double)nb)) return bg;
+    const int iy = (int)iyf < nb - 1 ? (int)iyf : nb - 1;
+    double w = 0.0;
+    bool boundary = false;
+    for (int i = bstart[iy]; i < bstart[iy + 1]; ++i) {
+        const int e = bitems[i];
+        const int v0 = E[2 * e], v1 = E[2 * e + 1];
+        const double ax = V[2 * v0], ay = V[2 * v0 + 1];
+        const double cxx = V[2 * v1], cyy = V[2 * v1 + 1];
+        double arx, ary, crx, cry;
+        if (h > 0.0) {
+            arx = (ax - px) / h;
+            ary = (ay - py) / h;
+            crx = (cxx - px) / h;
+            cry = (cyy - py) / h;
+        } else {
+            arx = ax - px;
+            ary = ay - py;
+            crx = cxx - px;
+            cry = cyy - py;
+        }
+        const double cross = wsb_cross2(arx, ary, crx, cry);
+        if (ay <= py && py < cyy && cross > 0.0) w += 1.0;
+        if (cyy <= py && py < ay && cross < 0.0) w -= 1.0;
+        if (cross == 0.0 && arx * crx + ary * cry <= 0.0) boundary = true;
+    }
+    if (boundary) {
+        double angle = 0.0;
+        for (int e = 0; e < ne; ++e) {
+            const int v0 = E[2 * e], v1 = E[2 * e + 1];
+            double arx, ary, crx, cry;
+            if (h > 0.0) {
+                arx = (V[2 * v0] - px) / h;
+                ary = (V[2 * v0 + 1] - py) / h;
+                crx = (V[2 * v1] - px) / h;
+                cry = (V[2 * v1 + 1] - py) / h;
+            } else {
+                arx = V[2 * v0] - px;
+                ary = V[2 * v0 + 1] - py;
+                crx = V[2 * v1] - px;
+                cry = V[2 * v1 + 1] - py;
+            }
+            const double cross = wsb_cross2(arx, ary, crx, cry);
+            if (cross != 0.0) angle += atan2(cross, arx * crx + ary * cry);
+        }
+        w = angle / 6.283185307179586;
+    }
+    return w + bg;
+}
+
+// ------------------------------------------------------------------------------------------------ device kernels
+struct AdjArgs {
+    const double* lpos;
+    const double* lsup;
+    const uint8_t* valid;
+    int n;
+    const double* V;
+    const int32_t* E;
+    int ne;
+    double clo0, clo1, cell;
+    int nx, ny;
+    const int32_t* cstart;
+    const int32_t* citems;
+    int K;
+    const double* cind;
+    const int32_t* cempty;
+    double bg, y0, dy;
+    int nb;
+    const int32_t* bstart;
+    const int32_t* bitems;
+    int32_t* slots;
+    int32_t* counts;
+    double* ind;
+    int pad;
+};
+
+// one thread per row (reference scene/fixedadj.py:67-147)
+__global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.n) return;
+    int n = 0;
+    double ind = 0.0;
+    if (a.valid == nullptr || a.valid[r] != 0) {
+        const double px = a.lpos[2 * r], py = a.lpos[2 * r + 1];
+        const double h = a.lsup[r];
+        const double fx = floor((px - a.clo0) / a.cell);
+        const double fy = floor((py - a.clo1) / a.cell);
+        const bool inside = fx >= 0.0 && fx < (double)a.nx && fy >= 0.0 && fy < (double)a.ny;
+        if (inside) {
+            const int cid = (int)fx * a.ny + (int)fy;
+            for (int i = a.cstart[cid]; i < a.cstart[cid + 1]; ++i) {
+                const int e = a.citems[i];
+                const int v0 = a.E[2 * e], v1 = a.E[2 * e + 1];
+                const double ax = a.V[2 * v0], ay = a.V[2 * v0 + 1];
+                const double dx = a.V[2 * v1] - ax;
+                const double dyy = a.V[2 * v1 + 1] - ay;
+                double t = ((px - ax) * dx + (py - ay) * dyy) / fmax(dx * dx + dyy * dyy, 1.0e-300);
+                t = fmin(fmax(t, 0.0), 1.0);
+                const double ex = px - (ax + t * dx);
+                const double ey = py - (ay + t * dyy);
+                if (sqrt(ex * ex + ey * ey) < h) {
+                    a.slots[(int64_t)r * a.K + n] = e;
+                    ++n;
+                }
+            }
+            if (a.cempty[cid] != 0) {
+                ind = a.cind[cid];
+            } else {
+                ind = winding_point(px, py, h, a.V, a.E, a.ne, a.bstart, a.bitems, a.y0, a.dy, a.nb, a.bg);
+            }
+        } else {
+            ind = a.bg;
+        }
+    }
+    if (a.pad)
+        for (int s = n; s < a.K; ++s) a.slots[(int64_t)r * a.K + s] = -1;
+    if (a.counts) a.counts[r] = n;
+    a.ind[r] = ind;
+}
+
+struct SdArgs {
+    const double* lpos;
+    int m;
+    double smax;
+    const double* V;
+    const int32_t* E;
+    int ne;
+    double clo0, clo1, cell;
+    int nx, ny;
+    const int32_t* cstart;
+    const int32_t* citems;
+    const double* cind;
+    const int32_t* cempty;
+    double bg, y0, dy;
+    int nb;
+    const int32_t* bstart;
+    const int32_t* bitems;
+    double* d;
+    double* nrm;
+    int32_t* edge;
+};
+
+__device__ inline double seg_dist(double px, double py, const double* V, const int32_t* E, int e, double* cpx, double* cpy) {
+    const double ax = V[2 * E[2 * e]], ay = V[2 * E[2 * e] + 1];
+    const double ex = V[2 * E[2 * e + 1]] - ax, ey = V[2 * E[2 * e + 1] + 1] - ay;
+    double t = ((px - ax) * ex + (py - ay) * ey) / (ex * ex + ey * ey);
+    t = fmin(fmax(t, 0.0), 1.0);
+    *cpx = ax + t * ex;
+    *cpy = ay + t * ey;
+    const double dx = px - *cpx, dy = py - *cpy;
+    return sqrt(dx * dx + dy * dy);
+}
+
+// nearest edge, sign from the winding number at support normalisation `smax` (reference scene/scene.py:1312-1372
+// with the sync-free indicator of scene/fixedadj.py:273-276); the cell list gives the exact nearest edge whenever
+// it is closer than smax, else every edge is scanned
+__global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.m) return;
+    const double px = a.lpos[2 * r], py = a.lpos[2 * r + 1];
+    const double fx = floor((px - a.clo0) / a.cell);
+    const double fy = floor((py - a.clo1) / a.cell);
+    const bool inside = fx >= 0.0 && fx < (double)a.nx && fy >= 0.0 && fy < (double)a.ny;
+    const int cid = inside ? (int)fx * a.ny + (int)fy : 0;
+    double best = INFINITY, bx = 0, by = 0;
+    int k = -1;
+    if (inside) {
+        for (int i = a.cstart[cid]; i < a.cstart[cid + 1]; ++i) {
+            double qx, qy;
+            const int e = a.citems[i];
+            const double dd = seg_dist(px, py, a.V, a.E, e, &qx, &qy);
+            if (dd < best) { best = dd; k = e; bx = qx; by = qy; }
+        }
+    }
+    if (!(best < a.cell)) {             // cell = the support the static list was inflated by
+        best = INFINITY;
+        k = -1;
+        for (int e = 0; e < a.ne; ++e) {
+            double qx, qy;
+            const double dd = seg_dist(px, py, a.V, a.E, e, &qx, &qy);
+            if (dd < best) { best = dd; k = e; bx = qx; by = qy; }
+        }
+    }
+    double ind;
+    if (inside) {
+        ind = a.cempty[cid] ? a.cind[cid]
+                            : winding_point(px, py, a.smax, a.V, a.E, a.ne, a.bstart, a.bitems, a.y0, a.dy, a.nb, a.bg);
+    } else {
+        ind = a.bg;
+    }
+    const bool ins = ind > 0.5;
+    a.d[r] = ins ? -best : best;
+    double n0 = 0, n1 = 0;
+    if (k >= 0) {
+        const double ex = a.V[2 * a.E[2 * k + 1]] - a.V[2 * a.E[2 * k]];
+        const double ey = a.V[2 * a.E[2 * k + 1] + 1] - a.V[2 * a.E[2 * k] + 1];
+        const double el = sqrt(ex * ex + ey * ey);
+        n0 = ey / el;
+        n1 = -ex / el;
+        const double vx = px - bx, vy = py - by;
+        const double dn = sqrt(vx * vx + vy * vy);
+        if (dn > 1e-12 && !ins) {
+            n0 = vx / fmax(dn, 1e-300);
+            n1 = vy / fmax(dn, 1e-300);
+        }
+    }
+    a.nrm[2 * r] = n0;
+    a.nrm[2 * r + 1] = n1;
+    if (a.edge) a.edge[r] = k;
+}
+
+template <class T>
+int upload(T** dst, const T* src, size_t count, int64_t* bytes) {
+    const size_t nb = std::max<size_t>(count, 1) * sizeof(T);
+    WSB_HIP(hipMalloc(dst, nb));
+    if (count) WSB_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    *bytes += (int64_t)nb;
+    return WSB_OK;
+}
+
+}  // namespace
+
+extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edges, int ne, int background,
+                                 double support_max, wsb_geom2d** out, int* K) {
+    WSB_REQUIRE(verts && edges && out, WSB_EINVAL, "null argument");
+    WSB_REQUIRE(nv >= 2 && ne >= 1, WSB_EINVAL, "geometry needs at least one edge (nv=%d ne=%d)", nv, ne);
+    WSB_REQUIRE(support_max > 0 && std::isfinite(support_max), WSB_EINVAL, "support_max must be positive and finite");
+    for (int i = 0; i < 2 * ne; ++i)
+        WSB_REQUIRE(edges[i] >= 0 && edges[i] < nv, WSB_EINVAL, "edge vertex index %d out of range", edges[i]);
+    for (int i = 0; i < 2 * nv; ++i) WSB_REQUIRE(std::isfinite(verts[i]), WSB_EINVAL, "non-finite vertex");
+    HostCells c = build_cells(verts, edges, ne, support_max);
+    HostBins b = build_bins(verts, edges, ne);
+    const int ncells = c.nx * c.ny;
+    std::vector<double> cind(ncells, 0.0);
+    std::vector<int32_t> cempty(ncells, 0);
+    int Kmax = 1;
+    for (int k = 0; k < ncells; ++k) {
+        const int cnt = c.start[k + 1] - c.start[k];
+        Kmax = std::max(Kmax, cnt);
+        if (cnt == 0) {
+            cempty[k] = 1;
+            const int cx = k / c.ny, cy = k % c.ny;
+            const double px = c.lo0 + ((double)cx + 0.5) * c.cell;
+            const double py = c.lo1 + ((double)cy + 0.5) * c.cell;
+            cind[k] = winding_point(px, py, 0.0, verts, edges, ne, b.start.data(), b.items.data(), b.y0, b.dy, b.nb,
+                                    (double)background);
+        }
+    }
+    auto* g = new wsb_geom2d();
+    WSB_HIP(hipGetDevice(&g->device));
+    g->nv = nv;
+    g->ne = ne;
+    g->background = background;
+    g->support_max = support_max;
+    g->clo0 = c.lo0;
+    g->clo1 = c.lo1;
+    g->cell = c.cell;
+    g->nx = c.nx;
+    g->ny = c.ny;
+    g->K = Kmax;
+    g->y0 = b.y0;
+    g->dy = b.dy;
+    g->nb = b.nb;
+    g->ncell_items = (int64_t)c.items.size();
+    g->nbin_items = (int64_t)b.items.size();
+    int st = WSB_OK;
+    if (!st) st = upload(&g->verts, verts, 2 * (size_t)nv, &g->bytes);
+    if (!st) st = upload(&g->edges, edges, 2 * (size_t)ne, &g->bytes);
+    if (!st) st = upload(&g->cstart, c.start.data(), c.start.size(), &g->bytes);
+    if (!st) st = upload(&g->citems, c.items.data(), c.items.size(), &g->bytes);
+    if (!st) st = upload(&g->cind, cind.data(), cind.size(), &g->bytes);
+    if (!st) st = upload(&g->cempty, cempty.data(), cempty.size(), &g->bytes);
+    if (!st) st = upload(&g->bstart, b.start.data(), b.start.size(), &g->bytes);
+    if (!st) st = upload(&g->bitems, b.items.data(), b.items.size(), &g->bytes);
+    if (st) {
+        wsb_geom2d_destroy(g);
+        return st;
+    }
+    *out = g;
+    if (K) *K = Kmax;
+    return WSB_OK;
+}
+
+extern "C" int wsb_geom2d_destroy(wsb_geom2d* g) {
+    if (!g) return WSB_OK;
+    for (void* p : {(void*)g->verts, (void*)g->edges, (void*)g->cstart, (void*)g->citems, (void*)g->cind,
+                    (void*)g->cempty, (void*)g->bstart, (void*)g->bitems})
+        if (p) (void)hipFree(p);
+    delete g;
+    return WSB_OK;
+}
+
+extern "C" int wsb_geom2d_info(const wsb_geom2d* g, int64_t* info) {
+    WSB_REQUIRE(g && info, WSB_EINVAL, "null argument");
+    info[0] = g->K;
+    info[1] = g->nx;
+    info[2] = g->ny;
+    info[3] = g->nb;
+    info[4] = g->ne;
+    info[5] = g->nv;
+    info[6] = g->ncell_items;
+    info[7] = g->bytes;
+    return WSB_OK;
+}
+
+extern "C" int wsb_geom2d_arrays(const wsb_geom2d* g, const double** verts, const int32_t** edges) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
+    if (verts) *verts = g->verts;
+    if (edges) *edges = g->edges;
+    return WSB_OK;
+}
+
+extern "C" int wsb_adjacency2d(const wsb_geom2d* g, const double* lpos, const double* lsup, const uint8_t* valid, int n,
+                               int32_t* slots, int32_t* counts, double* ind, int pad, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(lpos && lsup && slots && ind, WSB_EINVAL, "null buffer");
+    AdjArgs a{lpos, lsup, valid, n, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny,
+              g->cstart, g->citems, g->K, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb,
+              g->bstart, g->bitems, slots, counts, ind, pad};
+    hipLaunchKernelGGL(adjacency2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+extern "C" int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max, double* d,
+                                     double* nrm, int32_t* edge, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
+    WSB_REQUIRE(m >= 0, WSB_EINVAL, "negative point count");
+    if (m == 0) return WSB_OK;
+    WSB_REQUIRE(lpos && d && nrm, WSB_EINVAL, "null buffer");
+    WSB_REQUIRE(support_max > 0, WSB_EINVAL, "support_max must be positive");
+    SdArgs a{lpos, m, support_max, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny, g->cstart,
+             g->citems, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb, g->bstart, g->bitems,
+             d, nrm, edge};
+    hipLaunchKernelGGL(signed_distance2d_kernel, dim3((m + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}

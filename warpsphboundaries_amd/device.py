@@ -1,0 +1,175 @@
+"""Device-resident objects of the library: uploaded plans and static 2-D geometries (one per device)."""
+from __future__ import annotations
+
+import ctypes
+from functools import lru_cache
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .kernels import KERNEL_ID
+from .plan import surface_plan, general_plan
+
+
+class DevicePlan:
+    """an uploaded flattened plan (the plain form's term table)."""
+
+    def __init__(self, pl, kernel_id, device_index):
+        a = pl.arrays()
+        self.kernel_id = kernel_id
+        self.device_index = device_index
+        self.nE, self.nV = a["nE"], a["nV"]
+        h = ctypes.c_void_p()
+        with torch.cuda.device(device_index):
+            L.check(L.load().wsb_plan_create(kernel_id, a["radii"].ctypes.data, len(a["radii"]), a["cn"].ctypes.data,
+                                             a["cc"].ctypes.data, len(a["cc"]), a["E"].ctypes.data, a["nE"],
+                                             a["V"].ctypes.data, a["mR"].ctypes.data, a["nV"], ctypes.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                L.load().wsb_plan_destroy(self.handle)
+        except Exception:
+            pass
+
+
+@lru_cache(maxsize=None)
+def surface_device_plan(kernel: str, device_index: int) -> DevicePlan:
+    return DevicePlan(surface_plan(kernel), KERNEL_ID[kernel], device_index)
+
+
+@lru_cache(maxsize=None)
+def general_device_plan(kernel: str, alpha, grad: bool, device_index: int) -> DevicePlan:
+    return DevicePlan(general_plan(kernel, tuple(alpha), grad), KERNEL_ID[kernel], device_index)
+
+
+class Geom2D:
+    """static device structures of one closed-loop surface for one support_max (cell list, cell indicator, y-bins)."""
+
+    def __init__(self, vertices: np.ndarray, edges: np.ndarray, background: int, support_max: float, device_index: int):
+        v = np.ascontiguousarray(vertices, dtype=np.float64)
+        e = np.ascontiguousarray(edges, dtype=np.int32)
+        self.device_index = device_index
+        self.support_max = float(support_max)
+        self.background = int(background)
+        h = ctypes.c_void_p()
+        K = ctypes.c_int()
+        with torch.cuda.device(device_index):
+            L.check(L.load().wsb_geom2d_create(v.ctypes.data, len(v), e.ctypes.data, len(e), int(background),
+                                               float(support_max), ctypes.byref(h), ctypes.byref(K)))
+        self.handle = h
+        self.K = int(K.value)
+        info = (ctypes.c_int64 * 8)()
+        L.check(L.load().wsb_geom2d_info(self.handle, info))
+        self.info = dict(K=info[0], nx=info[1], ny=info[2], nbins=info[3], ne=info[4], nv=info[5], cell_items=info[6],
+                         device_bytes=info[7])
+        pv, pe = ctypes.c_void_p(), ctypes.c_void_p()
+        L.check(L.load().wsb_geom2d_arrays(self.handle, ctypes.byref(pv), ctypes.byref(pe)))
+        self.verts_ptr, self.edges_ptr = pv, pe
+        self.ne, self.nv = len(e), len(v)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                L.load().wsb_geom2d_destroy(self.handle)
+        except Exception:
+            pass
+
+    # ---- hot path entry points (device tensors, current stream, no host synchronisation)
+    def adjacency(self, lpos, lsup, valid=None, pad=True, slots=None, counts=None, ind=None):
+        """(slots [N*K] int32, counts [N] int32, ind [N] f64) -- the bit-exact fixed-capacity adjacency."""
+        L.require_device(lpos, lsup)
+        n = lpos.shape[0]
+        dev = lpos.device
+        lpos = lpos.to(torch.float64).contiguous()
+        lsup = lsup.to(torch.float64).contiguous()
+        if slots is None:
+            slots = torch.empty(n * self.K, dtype=torch.int32, device=dev)      # the kernel writes every slot when pad
+        if counts is None:
+            counts = torch.empty(n, dtype=torch.int32, device=dev)
+        if ind is None:
+            ind = torch.empty(n, dtype=torch.float64, device=dev)
+        v = None if valid is None else valid.to(torch.uint8).contiguous()
+        L.check(L.load().wsb_adjacency2d(self.handle, L.ptr(lpos), L.ptr(lsup), L.ptr(v), n, L.ptr(slots), L.ptr(counts),
+                                         L.ptr(ind), 1 if pad else 0, L.stream_ptr(dev)))
+        return slots, counts, ind
+
+    def moments(self, kernel, slots, counts, ind, lpos, lsup, form="fp64", rot4=None, flags=0, out=None, scratch=None):
+        """[N, 12] twelve channels per row (device order), see include/wsb.h for the flags."""
+        L.require_device(lpos, lsup, slots, counts, ind)
+        n = lpos.shape[0]
+        dev = lpos.device
+        f = L.form_id(form)
+        kid = KERNEL_ID[kernel]
+        if out is None:
+            out = torch.zeros((n, 12), dtype=torch.float64, device=dev) if flags & L.OUT_ACCUMULATE \
+                else torch.empty((n, 12), dtype=torch.float64, device=dev)
+        plan = None
+        nbytes = 0
+        if f == 0:
+            plan = surface_device_plan(kernel, dev.index or 0).handle
+            nbytes = int(L.load().wsb_moments2d_scratch_bytes(n, self.K))
+            if scratch is None or scratch.numel() < nbytes:
+                scratch = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+        lpos = lpos.to(torch.float64).contiguous()
+        lsup = lsup.to(torch.float64).contiguous()
+        L.check(L.load().wsb_moments2d(self.handle, plan, kid, f, L.ptr(slots), L.ptr(counts), self.K, L.ptr(ind), L.ptr(lpos),
+                                       L.ptr(lsup), n, L.ptr(rot4), int(flags), L.ptr(out), L.ptr(scratch), nbytes,
+                                       L.stream_ptr(dev)))
+        return out
+
+    def signed_distance(self, lpos, support_max=None):
+        L.require_device(lpos)
+        m = lpos.shape[0]
+        dev = lpos.device
+        lpos = lpos.to(torch.float64).contiguous()
+        d = torch.empty(m, dtype=torch.float64, device=dev)
+        nrm = torch.empty((m, 2), dtype=torch.float64, device=dev)
+        edge = torch.empty(m, dtype=torch.int32, device=dev)
+        L.check(L.load().wsb_signed_distance2d(self.handle, L.ptr(lpos), m, float(support_max or self.support_max), L.ptr(d),
+                                               L.ptr(nrm), L.ptr(edge), L.stream_ptr(dev)))
+        return d, nrm, edge
+
+
+def edge_channels12(pair_q, pair_e, positions, supports, vertices, edges, kernel, form="fp64"):
+    """[P, 12] edge-local channels (units of h) of an explicit pair list on the device."""
+    L.require_device(pair_q, pair_e, positions, supports, vertices, edges)
+    dev = positions.device
+    P = pair_q.shape[0]
+    out = torch.empty((P, 12), dtype=torch.float64, device=dev)
+    if P == 0:
+        return out
+    f = L.form_id(form)
+    plan = surface_device_plan(kernel, dev.index or 0).handle if f == 0 else None
+    pq = pair_q.to(torch.int32).contiguous()
+    pe = pair_e.to(torch.int32).contiguous()
+    pos = positions.to(torch.float64).contiguous()
+    sup = supports.to(torch.float64).contiguous()
+    V = vertices.to(torch.float64).contiguous()
+    E = edges.to(torch.int32).contiguous()
+    L.check(L.load().wsb_edge_channels2d(plan, KERNEL_ID[kernel], f, L.ptr(pq), L.ptr(pe), P, L.ptr(pos), L.ptr(sup),
+                                         pos.shape[0], L.ptr(V), L.ptr(E), E.shape[0], L.ptr(out), L.stream_ptr(dev)))
+    return out
+
+
+def plan_channels(pair_q, pair_e, positions, supports, vertices, edges, kernel, alpha, grad=False):
+    """edge-local m_alpha (channel 0) or grad_x m_alpha (channels 0, 1) of arbitrary alpha through the plan
+    interpreter: [P, 12] with the unused channels zero."""
+    L.require_device(pair_q, pair_e, positions, supports, vertices, edges)
+    dev = positions.device
+    P = pair_q.shape[0]
+    out = torch.empty((P, 12), dtype=torch.float64, device=dev)
+    if P == 0:
+        return out
+    plan = general_device_plan(kernel, tuple(int(a) for a in alpha), bool(grad), dev.index or 0).handle
+    pq = pair_q.to(torch.int32).contiguous()
+    pe = pair_e.to(torch.int32).contiguous()
+    pos = positions.to(torch.float64).contiguous()
+    sup = supports.to(torch.float64).contiguous()
+    V = vertices.to(torch.float64).contiguous()
+    E = edges.to(torch.int32).contiguous()
+    L.check(L.load().wsb_edge_channels2d(plan, KERNEL_ID[kernel], 0, L.ptr(pq), L.ptr(pe), P, L.ptr(pos), L.ptr(sup),
+                                         pos.shape[0], L.ptr(V), L.ptr(E), E.shape[0], L.ptr(out), L.stream_ptr(dev)))
+    return out

@@ -1,0 +1,119 @@
+"""Fused wall evaluation at one position set (reference ``scene/fused.py`` and ``edge/warpfused.py``).
+
+    fw = FusedWall(scene, adjacency, groups)       # stage 1: one tile-kernel launch per (group, body, representation)
+    out = fw.evaluate(outputs, a1=None)            # stage 2: {name: float64 tensor [B, N, ...]}, world frame, physical units
+
+Stage 1 evaluates all twelve channels of every row with the row sums, the indicator x plane vector, the physical
+scaling and the world rotation fused into the kernel (``WSB_OUT_PHYSICAL | WSB_OUT_WORLD``); several
+representations of one body accumulate into the same rows.  Stage 2 only slices:
+
+    "lam"  [B, N]        int_solid W                  "g0"   [B, N, 2]     int_solid grad_x W
+    "cov"  [B, N, 2, 2]  int_solid y (x) grad_x W      "m1"   [B, N, 2]     int_solid y W
+    "m2"   [B, N, 2, 2]  int_solid y (x) y W           "lap"  [B, N]        2 lam - tr cov
+    "a1g1" [B, N, 2]     sum_d a1_d cov_dj for a per-query world vector a1
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import _lib as L
+from .scene import FixedTopology, SurfaceRep, kernelName
+
+F64 = torch.float64
+KINDS = {"lam": (0, 1), "g0": (1, 2), "cov": (2, 4), "a1g1": (3, 2), "lap": (4, 1), "m1": (5, 2), "m2": (6, 4)}
+
+
+@dataclass(frozen=True)
+class WallOutput:
+    name: str
+    group: int
+    kind: str
+
+
+@dataclass(frozen=True)
+class FusedGroup:
+    """one kernel of the fused evaluation; ``channels`` None = all twelve; ``form`` the arithmetic form
+    ('fp64' closed form, 'mixed', or 'plain' = the plan interpreter)."""
+    kernel: str
+    channels: Optional[tuple] = None
+    form: str = "fp64"
+
+    def resolved(self):
+        ch = None if self.channels is None else tuple(sorted(set(int(c) for c in self.channels)))
+        return FusedGroup(kernelName(self.kernel), ch, self.form)
+
+
+class WallAggregate:
+    """marker base of the wall aggregates a scheme consumes (``.out``, ``evaluate``)."""
+
+
+class FusedWall(WallAggregate):
+    def __init__(self, scene, adjacency, groups):
+        self.scene = scene
+        self.adj = adjacency
+        self.groups = tuple(g.resolved() if isinstance(g, FusedGroup) else FusedGroup(kernelName(g)) for g in groups)
+        self.N = adjacency.numQueries
+        dev = torch.device(scene.device)
+        self.dev = dev
+        B = len(adjacency.bodies)
+        # per group: [B, N, 12] world-frame physical channels (device order)
+        self.channels = torch.zeros((len(self.groups), B, self.N, 12), dtype=F64, device=dev)
+        self._scratch = None
+        for gi, grp in enumerate(self.groups):
+            for bi, ba in enumerate(adjacency.bodies):
+                R = ba.body.pose.Rflat().to(dev, F64).contiguous()
+                reps = ba.repList if ba.repList is not None else ba.body.reps
+                for rep, topo in zip(reps, ba.reps):
+                    if not isinstance(rep, SurfaceRep) or not isinstance(topo, FixedTopology):
+                        raise NotImplementedError("FusedWall: surface representations with the fixed-capacity adjacency only")
+                    flags = L.OUT_PHYSICAL | L.OUT_WORLD | L.OUT_ACCUMULATE
+                    topo.geom.moments(grp.kernel, topo.e, topo.counts, topo.ind, ba.lpos, ba.lsup, form=grp.form, rot4=R,
+                                      flags=flags, out=self.channels[gi, bi])
+            if grp.channels is not None:
+                keep = torch.zeros(12, dtype=F64, device=dev)
+                keep[list(grp.channels)] = 1.0
+                self.channels[gi] *= keep
+
+    def evaluate(self, outputs, a1=None):
+        """{name: tensor [B, N, ...]} float64, world frame, physical units."""
+        res = {}
+        for o in outputs:
+            c = self.channels[o.group]
+            B, N = c.shape[0], c.shape[1]
+            if o.kind == "lam":
+                res[o.name] = c[..., 0].clone()
+            elif o.kind == "g0":
+                res[o.name] = c[..., 3:5].clone()
+            elif o.kind == "m1":
+                res[o.name] = c[..., 1:3].clone()
+            elif o.kind == "cov":
+                res[o.name] = c[..., 5:9].reshape(B, N, 2, 2).clone()
+            elif o.kind == "m2":
+                res[o.name] = torch.stack([c[..., 9], c[..., 10], c[..., 10], c[..., 11]], -1).reshape(B, N, 2, 2)
+            elif o.kind == "lap":
+                res[o.name] = 2.0 * c[..., 0] - c[..., 5] - c[..., 8]
+            elif o.kind == "a1g1":
+                if a1 is None:
+                    raise ValueError("FusedWall.evaluate: a1 [B, N, 2] is required for 'a1g1' outputs")
+                cov = c[..., 5:9].reshape(B, N, 2, 2)
+                res[o.name] = torch.einsum("bnd,bndj->bnj", a1.to(F64), cov)
+            else:
+                raise KeyError(f"unknown output kind {o.kind!r}; expected one of {sorted(KINDS)}")
+        return res
+
+    def copy_from(self, other):
+        """overwrite this wall's channels with those of ``other`` (same scene, groups and query count) in place."""
+        if other.channels.shape != self.channels.shape:
+            raise ValueError("FusedWall.copy_from: different structure")
+        self.channels.copy_(other.channels)
+
+    def reset_derived(self):
+        """nothing is derived lazily: the channels are complete after stage 1."""
+        return None
+
+    @staticmethod
+    def supported(scene, lowering=True):
+        return all(isinstance(r, SurfaceRep) for b in scene.bodies for r in b.reps)

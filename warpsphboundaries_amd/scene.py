@@ -1,0 +1,354 @@
+"""Scene layer (2-D): bodies with a pose, closed-loop surface representations, the fixed-capacity adjacency.
+
+Mirrors the reference's ``scene/scene.py`` (``Pose`` :47-87, ``SurfaceRep`` :213-416, ``Body`` :913-1067,
+``Scene`` :1227-1372) and ``scene/fixedadj.py`` (``FixedTopology``, ``fixed_adjacency`` :249-270) for the
+surface representations of the wall-integral path.  Representations are stored in the body frame; every
+device computation (adjacency, indicator, moments, signed distance) runs in the HIP library.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from fractions import Fraction
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .device import Geom2D
+from .kernels import KERNEL_ID, plane_channels
+
+F64 = torch.float64
+
+KERNEL_NAMES = {"CubicSpline": "cubic", "cubic": "cubic", "Wendland2": "w2", "Wendland4": "w4", "Wendland6": "w6",
+                "w2": "w2", "w4": "w4", "w6": "w6"}
+
+
+def kernelName(kernel) -> str:
+    """registered kernel name of a string or an enum-like kernel (``.name`` Wendland2/4/6, CubicSpline)
+    (reference scene/boundaryOps.py:35-44)."""
+    key = kernel if isinstance(kernel, str) else getattr(kernel, "name", str(kernel))
+    try:
+        return KERNEL_NAMES[key]
+    except KeyError:
+        raise NotImplementedError(f"{kernel}: not one of the exact piecewise-polynomial kernels {sorted(set(KERNEL_NAMES.values()))}") from None
+
+
+@dataclass
+class ParticleState:
+    """the duck type the provider reads (warpSPHCore ParticleState): positions [N,2], supports [N], kinds [N] (0 fluid)."""
+    positions: torch.Tensor
+    supports: torch.Tensor
+    masses: Optional[torch.Tensor] = None
+    densities: Optional[torch.Tensor] = None
+    kinds: Optional[torch.Tensor] = None
+
+
+# ----------------------------------------------------------------------------------------------------------------------- frames
+@dataclass
+class Pose:
+    center: torch.Tensor
+    angle: float = 0.0
+
+    def _cs(self):
+        a = float(self.angle)
+        return math.cos(a), math.sin(a)
+
+    @property
+    def R(self):
+        c, s = self._cs()
+        return torch.tensor([[c, -s], [s, c]], dtype=F64, device=self.center.device)
+
+    def Rflat(self):
+        return self.R.reshape(-1)
+
+    def toLocal(self, p):
+        c, s = self._cs()
+        d = p - self.center
+        return torch.stack([d[:, 0] * c + d[:, 1] * s, d[:, 1] * c - d[:, 0] * s], 1)
+
+    def toWorld(self, p):
+        c, s = self._cs()
+        return torch.stack([p[:, 0] * c - p[:, 1] * s + self.center[0], p[:, 0] * s + p[:, 1] * c + self.center[1]], 1)
+
+    def vecToWorld(self, v):
+        c, s = self._cs()
+        return torch.stack([v[:, 0] * c - v[:, 1] * s, v[:, 0] * s + v[:, 1] * c], 1)
+
+
+# ----------------------------------------------------------------------------------------------------------------------- surfaces
+def _orientation_sign(P: np.ndarray) -> float:
+    """sign of the signed area of a closed loop, robust for tiny / translated / nearly collinear loops
+    (anchor at the first vertex, long double; exact rationals when that is not conclusive -- reference
+    scene/scene.py:229-244)."""
+    local = P.astype(np.longdouble) - P[0].astype(np.longdouble)
+    terms = local[:, 0] * np.roll(local[:, 1], -1) - np.roll(local[:, 0], -1) * local[:, 1]
+    a = np.sum(terms)
+    if len(P) == 3 or abs(a) <= 64 * np.finfo(np.longdouble).eps * np.sum(abs(terms)):
+        v = [[Fraction(float(c)) for c in p] for p in P]
+        a = sum((v[i][0] - v[0][0]) * (v[(i + 1) % len(v)][1] - v[0][1]) - (v[(i + 1) % len(v)][0] - v[0][0]) * (v[i][1] - v[0][1])
+                for i in range(len(v)))
+    return float(np.sign(a))
+
+
+class SurfaceRep:
+    """closed polylines in the body frame, counter-clockwise around the SOLID (solid on the left of every edge);
+    ``background = 1`` when the solid is the unbounded outside (a tank wall / cavity)."""
+    kind = "surface"
+
+    def __init__(self, vertices, edges, background: int = 0, conditioned: bool = True):
+        self.vertices = torch.as_tensor(np.asarray(vertices, dtype=np.float64), dtype=F64)
+        self.edges = torch.as_tensor(np.asarray(edges, dtype=np.int32), dtype=torch.int32)
+        self.background = int(background)
+        self.conditioned = bool(conditioned)
+        self._geoms = {}
+
+    @staticmethod
+    def polygon(points, solid: str = "inside", conditioned: bool = True):
+        """one closed loop; ``solid='inside'``: the polygon is the body (made counter-clockwise), ``'outside'``: a hole
+        in an infinite solid (clockwise, background 1)."""
+        P = np.asarray(points, dtype=np.float64)
+        if (_orientation_sign(P) > 0) != (solid == "inside"):
+            P = P[::-1].copy()
+        n = len(P)
+        E = np.stack([np.arange(n), (np.arange(n) + 1) % n], 1)
+        return SurfaceRep(np.ascontiguousarray(P), E, background=0 if solid == "inside" else 1, conditioned=conditioned)
+
+    @staticmethod
+    def box(lo, hi, solid: str = "inside"):
+        (x0, y0), (x1, y1) = lo, hi
+        return SurfaceRep.polygon([(x0, y0), (x1, y0), (x1, y1), (x0, y1)], solid)
+
+    @staticmethod
+    def regularPolygon(center, radius, n, solid="inside", areaPreserving=True):
+        r = radius * np.sqrt(2 * np.pi / (n * np.sin(2 * np.pi / n))) if areaPreserving else radius
+        t = 2 * np.pi * np.arange(n) / n
+        return SurfaceRep.polygon(np.stack([center[0] + r * np.cos(t), center[1] + r * np.sin(t)], 1), solid)
+
+    def to(self, device):
+        self.vertices = self.vertices.to(device)
+        self.edges = self.edges.to(device)
+        return self
+
+    def bounds(self):
+        return self.vertices.amin(0), self.vertices.amax(0)
+
+    # ---- device structures
+    def geom(self, support_max: float, device) -> Geom2D:
+        """static device structures for ``support_max`` (built once per (support_max, device))."""
+        dev = torch.device(device)
+        idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        key = (float(support_max), idx)
+        g = self._geoms.get(key)
+        if g is None:
+            g = Geom2D(self.vertices.detach().cpu().numpy(), self.edges.detach().cpu().numpy(), self.background,
+                       float(support_max), idx)
+            self._geoms[key] = g
+        return g
+
+    def indicator(self, p, supports=None):
+        """winding number + background (1 inside the solid); on an edge 1/2, at a vertex the angle fraction."""
+        L.require_device(p)
+        sup = torch.ones(len(p), dtype=F64, device=p.device) if supports is None else supports.to(F64)
+        smax = float(sup.max()) if len(p) else 1.0
+        return self.geom(smax, p.device).adjacency(p, sup, pad=False)[2]
+
+    def topology(self, lpos, lsup):
+        """`SurfaceTopology`: the (query, edge) pairs with the segment closer than the support, from the fixed slots
+        (pair order = slot order, as the reference's sorted pair list)."""
+        if len(lpos) == 0:
+            return None
+        smax = float(lsup.max())
+        slots, counts, ind = self.geom(smax, lpos.device).adjacency(lpos, lsup, pad=True)
+        K = self.geom(smax, lpos.device).K
+        e = slots.view(-1, K)
+        keep = e >= 0
+        qi = torch.arange(len(lpos), device=lpos.device, dtype=torch.int32)[:, None].expand(-1, K)[keep]
+        return SurfaceTopology(qi.contiguous(), e[keep].contiguous(), smax, ind)
+
+
+@dataclass
+class SurfaceTopology:
+    qi: torch.Tensor
+    e: torch.Tensor
+    supportMax: float
+    ind: torch.Tensor
+
+    def indicator(self, rep=None, lpos=None, lsup=None):
+        return self.ind
+
+
+@dataclass
+class FixedTopology:
+    """K slots per row: ``e[r*K + s]`` (-1 = empty), ``counts`` [N], ``ind`` [N] (reference scene/fixedadj.py:39-64)."""
+    e: torch.Tensor
+    counts: torch.Tensor
+    K: int
+    ind: torch.Tensor
+    geom: Geom2D = None
+
+    @property
+    def qi(self):
+        return torch.arange(len(self.counts) * self.K, device=self.e.device, dtype=torch.int32) // self.K
+
+    def indicator(self, rep=None, lpos=None, lsup=None):
+        return self.ind
+
+
+# ----------------------------------------------------------------------------------------------------------------------- bodies
+@dataclass
+class Body:
+    bodyId: int = 0
+    center: tuple = (0.0, 0.0)
+    angle: float = 0.0
+    linearVelocity: tuple = (0.0, 0.0)
+    angularVelocity: float = 0.0
+    reps: list = field(default_factory=list)
+
+    def __post_init__(self):
+        self.center = torch.as_tensor(self.center, dtype=F64)
+        self.linearVelocity = torch.as_tensor(self.linearVelocity, dtype=F64)
+
+    @property
+    def pose(self):
+        return Pose(self.center, float(self.angle))
+
+    def toLocal(self, world):
+        return self.pose.toLocal(world)
+
+    def unbounded(self):
+        return any(isinstance(r, SurfaceRep) and r.background for r in self.reps)
+
+    def obb(self):
+        los, his = zip(*[r.bounds() for r in self.reps])
+        dev = self.center.device
+        return torch.stack([a.to(dev) for a in los]).amin(0), torch.stack([a.to(dev) for a in his]).amax(0)
+
+    def velocityAt(self, world):
+        w = float(self.angularVelocity)
+        r = world - self.center
+        return self.linearVelocity + w * torch.stack([-r[:, 1], r[:, 0]], 1)
+
+
+@dataclass
+class BodyAdjacency:
+    body: object
+    cand: torch.Tensor
+    lpos: torch.Tensor
+    lsup: torch.Tensor
+    reps: list
+    valid: Optional[torch.Tensor] = None
+    repList: Optional[list] = None
+
+
+@dataclass
+class SceneAdjacency:
+    numQueries: int
+    bodies: list
+    stats: dict
+    queryParticles: object = None
+    supportMax: float = 0.0
+
+
+class Scene:
+    def __init__(self, bodies: List[Body], device="cuda:0"):
+        self.bodies = bodies
+        self.device = device
+        for b in bodies:
+            b.center = b.center.to(device)
+            b.linearVelocity = b.linearVelocity.to(device)
+            for r in b.reps:
+                r.to(device)
+
+    def inside(self, points, body=None):
+        pts = points.to(self.device, F64)
+        out = torch.zeros(len(pts), dtype=torch.bool, device=pts.device)
+        for bi, bd in enumerate(self.bodies):
+            if body is not None and bi != body:
+                continue
+            lp = bd.toLocal(pts)
+            for rep in bd.reps:
+                out |= rep.indicator(lp) > 0.5
+        return out
+
+    def signed_distance(self, points, body=None, supportMax=None, want_body=False):
+        """(d [M], n [M,2], hit [M][, body [M]]): d > 0 in the fluid, n the unit normal into the fluid (world frame);
+        the nearest edge of every surface loop, sign by the winding number at ``supportMax`` normalisation
+        (reference scene/scene.py:1312-1372)."""
+        pts = points.to(self.device, F64)
+        M = len(pts)
+        bidx = torch.full((M,), -1, dtype=torch.long, device=pts.device)
+        best = torch.full((M,), float("inf"), dtype=F64, device=pts.device)
+        normal = torch.zeros((M, 2), dtype=F64, device=pts.device)
+        hit = torch.zeros(M, dtype=torch.bool, device=pts.device)
+        for bi, bd in enumerate(self.bodies):
+            if body is not None and bi != body:
+                continue
+            lp = bd.toLocal(pts)
+            for rep in bd.reps:
+                if not isinstance(rep, SurfaceRep):
+                    continue
+                smax = float(supportMax) if supportMax is not None else _default_support(rep)
+                d, n_loc, _ = rep.geom(smax, pts.device).signed_distance(lp, smax)
+                n = bd.pose.vecToWorld(n_loc)
+                better = d < best
+                best = torch.where(better, d, best)
+                normal = torch.where(better[:, None], n, normal)
+                bidx = torch.where(better, torch.full_like(bidx, bi), bidx)
+                hit |= True
+        if want_body:
+            return best, normal, hit, bidx
+        return best, normal, hit
+
+
+def _default_support(rep: SurfaceRep) -> float:
+    """support normalisation of the winding number when none is given: the largest edge box extent (any positive
+    value gives the same winding number away from the boundary)."""
+    lo, hi = rep.bounds()
+    return float(max((hi - lo).max().item(), 1e-12))
+
+
+def queryAllowed(queryParticles, dev):
+    """fluid queries only (BoundaryToFluid): kinds == 0, or every query when kinds are absent."""
+    kinds = getattr(queryParticles, "kinds", None)
+    n = queryParticles.positions.shape[0]
+    if kinds is None:
+        return torch.ones(n, dtype=torch.bool, device=dev)
+    return kinds.to(dev) == 0
+
+
+def fixed_adjacency(scene: Scene, queryParticles, operationProperties=None, supportMax: float = None):
+    """`SceneAdjacency` of every query against every body with fixed shapes: rows = queries, `valid` = broadphase
+    (support circle against the body box in the body frame; every row for an exterior solid) and fluid kind;
+    one adjacency launch per (body, representation), no host synchronisation (reference scene/fixedadj.py:249-270)."""
+    dev = torch.device(scene.device)
+    pos = queryParticles.positions.to(dev, F64)
+    sup = queryParticles.supports.to(dev, F64)
+    if pos.shape[1] != 2:
+        raise NotImplementedError("2D only")
+    if supportMax is None:
+        supportMax = float(sup.max())
+    allowed = queryAllowed(queryParticles, dev)
+    N = len(pos)
+    cand = torch.arange(N, device=dev)
+    bodies = []
+    for body in scene.bodies:
+        lpos = body.toLocal(pos)
+        if body.unbounded():
+            valid = allowed
+        else:
+            lo, hi = body.obb()
+            d = (lo - lpos).clamp(min=0) + (lpos - hi).clamp(min=0)
+            valid = (d.norm(dim=1) < sup) & allowed
+        reps = []
+        for rep in body.reps:
+            g = rep.geom(float(supportMax), dev)
+            slots, counts, ind = g.adjacency(lpos, sup, valid=valid, pad=True)
+            reps.append(FixedTopology(slots, counts, g.K, ind, g))
+        bodies.append(BodyAdjacency(body, cand, lpos, sup, reps, valid, list(body.reps)))
+    return SceneAdjacency(N, bodies, {"candidates": None}, queryParticles, float(supportMax))
+
+
+def plane_vector(kernel: str, device) -> torch.Tensor:
+    return torch.tensor(plane_channels(kernel), dtype=F64, device=device)
