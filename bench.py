@@ -142,10 +142,13 @@ def main():
     slots = torch.empty(N * K, dtype=torch.int32, device=dev)
     counts = torch.empty(N, dtype=torch.int32, device=dev)
     ind = torch.empty(N, dtype=torch.float64, device=dev)
+    order = torch.empty(N, dtype=torch.int32, device=dev)
+    sort_scratch = torch.empty(int(L.load().wsb_sort2d_scratch_bytes(geo.handle, N)), dtype=torch.uint8, device=dev)
     outs = {k: torch.empty((N, 12), dtype=torch.float64, device=dev) for k in work.kernels}
 
     def step(form, scratch=None):
-        geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind)
+        geo.sort(lpos, order=order, scratch=sort_scratch)
+        geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind, order=order)
         for k in work.kernels:
             geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k], scratch=scratch)
 
@@ -183,7 +186,8 @@ def main():
     log("signal-relative error vs fp64:", err)
 
     # ---- evaluate-only timing of each kernel's launch (events on the launch stream), the dominant kernel
-    geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind)
+    geo.sort(lpos, order=order, scratch=sort_scratch)
+    geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind, order=order)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     per_kernel = {}
@@ -202,11 +206,17 @@ def main():
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(args.steps):
-        geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind)
+        geo.sort(lpos, order=order, scratch=sort_scratch)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_sort = e0.elapsed_time(e1) / 1e3 / args.steps
+    e0.record(stream)
+    for _ in range(args.steps):
+        geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind, order=order)
     e1.record(stream)
     torch.cuda.synchronize()
     t_adj = e0.elapsed_time(e1) / 1e3 / args.steps
-    log("per-launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()}, "adjacency", t_adj)
+    log("per-launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()}, "adjacency", t_adj, "sort", t_sort)
 
     # ---- roofline of the dominant kernel (the mixed-form tile kernel, w6 is the heaviest profile)
     kdom = max(work.kernels, key=lambda k: per_kernel[("mixed", k)])
@@ -243,7 +253,7 @@ def main():
         "speedup_mixed_vs_plain": t_plain / t_mixed,
         "evaluate_only_evals_per_s": {"mixed": evals / sum(per_kernel[("mixed", k)] for k in work.kernels),
                                       "fp64": evals / sum(per_kernel[("fp64", k)] for k in work.kernels)},
-        "adjacency_ms": t_adj * 1e3,
+        "adjacency_ms": t_adj * 1e3, "cell_sort_ms": t_sort * 1e3,
         "max_signal_relative_error_vs_fp64": err,
         "served_latency": None,
         "roofline": roof,

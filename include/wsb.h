@@ -76,19 +76,28 @@ int wsb_geom2d_info(const wsb_geom2d* g, int64_t* info);
 /* device pointers of the uploaded vertices [nv*2] f64 and edges [ne*2] i32 */
 int wsb_geom2d_arrays(const wsb_geom2d* g, const double** verts, const int32_t** edges);
 
+/* ---- particle rows sorted by static cell (device radix sort of the cell keys; no counterpart in the reference,
+ * which walks rows in input order): order[n] is a permutation of 0..n-1 with the rows of one cell adjacent.
+ * Passing it to the adjacency / moment entry points changes which thread serves a row, never a result. */
+size_t wsb_sort2d_scratch_bytes(const wsb_geom2d* g, int n);
+int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_t* order, void* scratch, size_t scratch_bytes,
+               void* stream);
+
 /* ---- fixed-capacity adjacency (reference scene/fixedadj.py:67-147), bit-exact FP64 predicate:
  * slots[n*K]: edges of the particle's cell with point-segment distance < lsup, ascending edge id;
  * counts[n] (nullable); ind[n] winding number + background (0 for invalid rows).  valid[n] nullable (all).
- * pad != 0 writes -1 into the unused slots (the reference layout); pad == 0 leaves them untouched. */
+ * order[n] nullable (wsb_sort2d).  pad != 0 writes -1 into the unused slots (the reference layout); pad == 0
+ * leaves them untouched. */
 int wsb_adjacency2d(const wsb_geom2d* g, const double* lpos, const double* lsup, const uint8_t* valid, int n,
-                    int32_t* slots, int32_t* counts, double* ind, int pad, void* stream);
+                    const int32_t* order, int32_t* slots, int32_t* counts, double* ind, int pad, void* stream);
 
 /* ---- twelve moments per particle from the adjacency (reference scene/scene.py:384-416 and
  * scene/fused.py:56-154): out[n*12] = sum over the particle's slots (slot order) + ind * plane vector.
+ * order[n] (nullable, wsb_sort2d): the tile kernel groups the rows in that order (same results).
  * rot4 (device, row-major R, nullable) for WSB_OUT_WORLD.  WSB_FORM_PLAIN needs `scratch` of
  * wsb_moments2d_scratch_bytes(n, K) bytes (pair list and per-pair channels); the other forms need none. */
 size_t wsb_moments2d_scratch_bytes(int n, int K);
-int wsb_moments2d(const wsb_geom2d* g, const wsb_plan* plan, int kernel_id, int form,
+int wsb_moments2d(const wsb_geom2d* g, const wsb_plan* plan, int kernel_id, int form, const int32_t* order,
                   const int32_t* slots, const int32_t* counts, int K, const double* ind,
                   const double* lpos, const double* lsup, int n, const double* rot4, int flags,
                   double* out, void* scratch, size_t scratch_bytes, void* stream);

@@ -84,6 +84,11 @@ def test_adjacency_bit_exact(name):
     print(name, "rows", len(pts), "K", st.K, "pairs", int(c_ref.sum()))
     assert np.array_equal(slots.cpu().numpy(), e_ref)
     assert np.array_equal(counts.cpu().numpy(), c_ref)
+    # rows served in cell order give the same slots and indicator
+    order = geo.sort(lp)
+    assert sorted(order.cpu().numpy().tolist()) == list(range(len(pts)))
+    s2, c2, i2 = geo.adjacency(lp, ls, valid=valid, pad=True, order=order)
+    assert torch.equal(s2, slots) and torch.equal(c2, counts) and torch.equal(i2, ind)
     got = ind.cpu().numpy()
     # the boundary angle sums go through atan2 (device vs host libm): equal to rounding, every other row bit-exact
     frac = (i_ref != np.round(i_ref))
@@ -113,10 +118,14 @@ def test_moments_match_oracle(form, kname):
     if form == "mixed":
         assert sig.max() <= 1e-6
     else:
-        assert err.max() <= 1e-12
-    # relaunch is bitwise identical
+        # the monomial plan of w6 carries coefficients up to 1.3e4: the reference's own FP64 gate for it is 5e-12
+        assert err.max() <= (5e-12 if kname == "w6" else 1e-12)
+    # relaunch is bitwise identical, and so is the launch over rows grouped by cell
     again = geo.moments(kname, slots, counts, ind, lp, ls, form=form).cpu().numpy()
     assert np.array_equal(out, again)
+    order = geo.sort(lp)
+    grouped = geo.moments(kname, slots, counts, ind, lp, ls, form=form, order=order).cpu().numpy()
+    assert np.array_equal(out, grouped)
 
 
 @pytest.mark.parametrize("form", FORMS)
@@ -138,7 +147,7 @@ def test_pair_channels_match_oracle_on_degenerate_edges(form):
         got = edge_channels12(t(np.zeros(n), torch.int32), t(np.arange(n), torch.int32), t(pos, torch.float64),
                               t(sup, torch.float64), t(verts, torch.float64), t(edges, torch.int32), kname, form).cpu().numpy()
         err = np.abs(got - ref)
-        tol = 1e-6 * np.abs(ref).max() if form == "mixed" else 1e-12
+        tol = 1e-6 * np.abs(ref).max() if form == "mixed" else (5e-12 if kname == "w6" else 1e-12)
         assert err.max() <= tol, (kname, form, err.max())
 
 
@@ -172,5 +181,5 @@ def test_golden_fixture_elements_on_device(form, kname):
     if form == "mixed":
         assert (err.max(0) / np.abs(ref).max(0)).max() <= 1e-6
     else:
-        tol = np.array([1e-12] * 3 + [1e-11] * 6 + [1e-12] * 3)
+        tol = np.array([1e-12] * 3 + [1e-11] * 6 + [1e-12] * 3) * (5 if kname == "w6" else 1)
         assert (err <= tol).all(), err.max(0)

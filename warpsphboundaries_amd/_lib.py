@@ -27,10 +27,13 @@ SIGNATURES = {
     "wsb_geom2d_destroy": (c_int, [c_void_p]),
     "wsb_geom2d_info": (c_int, [c_void_p, POINTER(c_int64)]),
     "wsb_geom2d_arrays": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p)]),
-    "wsb_adjacency2d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "wsb_sort2d_scratch_bytes": (c_size_t, [c_void_p, c_int]),
+    "wsb_sort2d": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "wsb_adjacency2d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                c_void_p]),
     "wsb_moments2d_scratch_bytes": (c_size_t, [c_int, c_int]),
-    "wsb_moments2d": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
-                              c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "wsb_moments2d": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                              c_int, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_edge_channels2d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                     c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_signed_distance2d": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hipcub/hipcub.hpp>
+
 #include "wsb_internal.h"
 
 namespace {
@@ -110,7 +112,15 @@ HostBins build_bins(const double* V, const int32_t* E, int ne) {
 }
 
 // winding number + background at one point (reference scene/scene.py:333-371 and scene/fixedadj.py:98-146);
-// `h` normalises the relative coordinates (h = 0: unnormalised, the cell-centre convention)
+// `h` normalises the relative coordinates (h = 0: unnormalised, the cell-centre convention).
+//
+// The reference tests the sign of the separately rounded cross product of the support-normalised relative
+// coordinates.  Each bin edge is first tested on the unnormalised differences: when |cu| exceeds 16 u times the
+// magnitude of its two products (u = 2^-53) the exact cross product of the rounded differences, and hence the
+// normalised rounded one, has the sign of cu and is nonzero (rounding of the products, the subtraction and the
+// divisions by h each perturb it by at most 5 u of that magnitude).  Otherwise the reference's own arithmetic is
+// evaluated.  Edges wholly left of the point neither cross the ray nor contain the point: their cross product is
+// strictly negative for an upward and strictly positive for a downward edge, so they change nothing.
 __host__ __device__ inline double winding_point(double px, double py, double h, const double* V, const int32_t* E, int ne,
                                                 const int32_t* bstart, const int32_t* bitems, double y0, double dy, int nb,
                                                 double bg) {
@@ -124,21 +134,33 @@ __host__ __device__ inline double winding_point(double px, double py, double h, 
         const int v0 = E[2 * e], v1 = E[2 * e + 1];
         const double ax = V[2 * v0], ay = V[2 * v0 + 1];
         const double cxx = V[2 * v1], cyy = V[2 * v1 + 1];
+        if (ax < px && cxx < px) continue;
+        const bool up = ay <= py && py < cyy;
+        const bool down = cyy <= py && py < ay;
+        const double ux = ax - px, uy = ay - py, vx = cxx - px, vy = cyy - py;
+        const double l = ux * vy, r = uy * vx;
+        const double mag = fabs(l) + fabs(r);
+        const double cu = l - r;
+        if (h > 0.0 && mag > 1e-280 && fabs(cu) > 1.7763568394002505e-15 * mag) {   // 16 u
+            if (up && cu > 0.0) w += 1.0;
+            if (down && cu < 0.0) w -= 1.0;
+            continue;
+        }
         double arx, ary, crx, cry;
         if (h > 0.0) {
-            arx = (ax - px) / h;
-            ary = (ay - py) / h;
-            crx = (cxx - px) / h;
-            cry = (cyy - py) / h;
+            arx = ux / h;
+            ary = uy / h;
+            crx = vx / h;
+            cry = vy / h;
         } else {
-            arx = ax - px;
-            ary = ay - py;
-            crx = cxx - px;
-            cry = cyy - py;
+            arx = ux;
+            ary = uy;
+            crx = vx;
+            cry = vy;
         }
         const double cross = wsb_cross2(arx, ary, crx, cry);
-        if (ay <= py && py < cyy && cross > 0.0) w += 1.0;
-        if (cyy <= py && py < ay && cross < 0.0) w -= 1.0;
+        if (up && cross > 0.0) w += 1.0;
+        if (down && cross < 0.0) w -= 1.0;
         if (cross == 0.0 && arx * crx + ary * cry <= 0.0) boundary = true;
     }
     if (boundary) {
@@ -170,6 +192,7 @@ struct AdjArgs {
     const double* lpos;
     const double* lsup;
     const uint8_t* valid;
+    const int32_t* order;
     int n;
     const double* V;
     const int32_t* E;
@@ -191,10 +214,14 @@ struct AdjArgs {
     int pad;
 };
 
-// one thread per row (reference scene/fixedadj.py:67-147)
+// one thread per row (reference scene/fixedadj.py:67-147); with `order` (rows sorted by cell) thread i serves
+// row order[i], so the rows of one wavefront share their cell's item list.  A candidate whose segment box lies
+// farther than h (1 + 1e-9) from the point on either axis cannot pass the reference predicate (the rounded
+// distance then exceeds h), so it is rejected before the division.
 __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.n) return;
+    const int i_ = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i_ >= a.n) return;
+    const int r = a.order ? a.order[i_] : i_;
     int n = 0;
     double ind = 0.0;
     if (a.valid == nullptr || a.valid[r] != 0) {
@@ -205,12 +232,17 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
         const bool inside = fx >= 0.0 && fx < (double)a.nx && fy >= 0.0 && fy < (double)a.ny;
         if (inside) {
             const int cid = (int)fx * a.ny + (int)fy;
+            const double reach = h * (1.0 + 1e-9) + 1e-14 * (1.0 + fabs(px) + fabs(py));
             for (int i = a.cstart[cid]; i < a.cstart[cid + 1]; ++i) {
                 const int e = a.citems[i];
                 const int v0 = a.E[2 * e], v1 = a.E[2 * e + 1];
                 const double ax = a.V[2 * v0], ay = a.V[2 * v0 + 1];
-                const double dx = a.V[2 * v1] - ax;
-                const double dyy = a.V[2 * v1 + 1] - ay;
+                const double bx = a.V[2 * v1], by = a.V[2 * v1 + 1];
+                if (fmin(ax, bx) - px > reach || px - fmax(ax, bx) > reach || fmin(ay, by) - py > reach ||
+                    py - fmax(ay, by) > reach)
+                    continue;
+                const double dx = bx - ax;
+                const double dyy = by - ay;
                 double t = ((px - ax) * dx + (py - ay) * dyy) / fmax(dx * dx + dyy * dyy, 1.0e-300);
                 t = fmin(fmax(t, 0.0), 1.0);
                 const double ex = px - (ax + t * dx);
@@ -326,6 +358,44 @@ __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
     if (a.edge) a.edge[r] = k;
 }
 
+// cell key of every particle (outside the grid: nx*ny), for the radix sort of the rows
+__global__ __launch_bounds__(256) void cell_keys_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
+                                                        int ny, uint32_t* keys, int32_t* idx) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double fx = floor((lpos[2 * r] - clo0) / cell);
+    const double fy = floor((lpos[2 * r + 1] - clo1) / cell);
+    const bool inside = fx >= 0.0 && fx < (double)nx && fy >= 0.0 && fy < (double)ny;
+    keys[r] = inside ? (uint32_t)((int)fx * ny + (int)fy) : (uint32_t)(nx * ny);
+    idx[r] = r;
+}
+
+struct SortLayout {
+    size_t kin, kout, vin, tmp, tmp_bytes, total;
+};
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+SortLayout sort_layout(int n, int bits) {
+    SortLayout L{};
+    size_t tmp = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (const int32_t*)nullptr, (int32_t*)nullptr, n > 0 ? n : 1, 0, bits);
+    L.kin = 0;
+    L.kout = al256(L.kin + sizeof(uint32_t) * (size_t)n);
+    L.vin = al256(L.kout + sizeof(uint32_t) * (size_t)n);
+    L.tmp = al256(L.vin + sizeof(int32_t) * (size_t)n);
+    L.tmp_bytes = tmp;
+    L.total = al256(L.tmp + tmp);
+    return L;
+}
+
+int key_bits(const wsb_geom2d* g) {
+    int bits = 1;
+    while ((1ll << bits) <= (long long)g->nx * g->ny) ++bits;
+    return bits;
+}
+
 template <class T>
 int upload(T** dst, const T* src, size_t count, int64_t* bytes) {
     const size_t nb = std::max<size_t>(count, 1) * sizeof(T);
@@ -427,13 +497,40 @@ extern "C" int wsb_geom2d_arrays(const wsb_geom2d* g, const double** verts, cons
     return WSB_OK;
 }
 
+extern "C" size_t wsb_sort2d_scratch_bytes(const wsb_geom2d* g, int n) {
+    if (!g || n <= 0) return 0;
+    return sort_layout(n, key_bits(g)).total;
+}
+
+extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_t* order, void* scratch, size_t scratch_bytes,
+                          void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(lpos && order, WSB_EINVAL, "null buffer");
+    const int bits = key_bits(g);
+    const SortLayout L = sort_layout(n, bits);
+    WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", L.total, scratch_bytes);
+    char* base = static_cast<char*>(scratch);
+    uint32_t* kin = reinterpret_cast<uint32_t*>(base + L.kin);
+    uint32_t* kout = reinterpret_cast<uint32_t*>(base + L.kout);
+    int32_t* vin = reinterpret_cast<int32_t*>(base + L.vin);
+    hipStream_t s = wsb::as_stream(stream);
+    hipLaunchKernelGGL(cell_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx,
+                       g->ny, kin, vin);
+    WSB_HIP(hipGetLastError());
+    size_t tmp = L.tmp_bytes;
+    WSB_HIP(hipcub::DeviceRadixSort::SortPairs(base + L.tmp, tmp, kin, kout, vin, order, n, 0, bits, s));
+    return WSB_OK;
+}
+
 extern "C" int wsb_adjacency2d(const wsb_geom2d* g, const double* lpos, const double* lsup, const uint8_t* valid, int n,
-                               int32_t* slots, int32_t* counts, double* ind, int pad, void* stream) {
+                               const int32_t* order, int32_t* slots, int32_t* counts, double* ind, int pad, void* stream) {
     WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
     WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(lpos && lsup && slots && ind, WSB_EINVAL, "null buffer");
-    AdjArgs a{lpos, lsup, valid, n, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny,
+    AdjArgs a{lpos, lsup, valid, order, n, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny,
               g->cstart, g->citems, g->K, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb,
               g->bstart, g->bitems, slots, counts, ind, pad};
     hipLaunchKernelGGL(adjacency2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);

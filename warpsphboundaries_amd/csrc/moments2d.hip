@@ -105,6 +105,7 @@ __global__ __launch_bounds__(256) void plain_rows_kernel(RowsArgs a) {
 
 // ------------------------------------------------------------------------------------------------ tile path
 struct TileArgs {
+    const int32_t* order;       // nullable: tile position i serves row order[i] (rows sorted by cell)
     const int32_t* slots;
     const int32_t* counts;
     int K;
@@ -131,10 +132,12 @@ __global__ __launch_bounds__(TILE) void tile12_kernel(TileArgs a) {
     __shared__ int s_off[TILE + 1];
     __shared__ double s_px[TILE], s_py[TILE], s_h[TILE];
     __shared__ double s_res[wsb::NCH][TILE];
+    __shared__ int s_row[TILE];
     const int tid = threadIdx.x;
-    const int row0 = blockIdx.x * TILE;
-    const int row = row0 + tid;
-    const bool live = row < a.n;
+    const int pos = blockIdx.x * TILE + tid;
+    const bool live = pos < a.n;
+    const int row = live ? (a.order ? a.order[pos] : pos) : 0;
+    s_row[tid] = row;
     const int cnt = live ? min(max(a.counts[row], 0), a.K) : 0;
     double h = 1.0;
     if (live) {
@@ -164,7 +167,7 @@ __global__ __launch_bounds__(TILE) void tile12_kernel(TileArgs a) {
                 else hi = mid;
             }
             const int r = lo;
-            const int e = a.slots[(int64_t)(row0 + r) * a.K + (j - s_off[r])];
+            const int e = a.slots[(int64_t)s_row[r] * a.K + (j - s_off[r])];
             if (e >= 0 && e < a.ne) {
                 const double x = s_px[r], y = s_py[r], hr = s_h[r];
                 const int v0 = a.E[2 * e], v1 = a.E[2 * e + 1];
@@ -274,9 +277,10 @@ extern "C" size_t wsb_moments2d_scratch_bytes(int n, int K) {
     return plain_layout(n, K).total;
 }
 
-extern "C" int wsb_moments2d(const wsb_geom2d* g, const wsb_plan* plan, int kernel_id, int form, const int32_t* slots,
-                             const int32_t* counts, int K, const double* ind, const double* lpos, const double* lsup, int n,
-                             const double* rot4, int flags, double* out, void* scratch, size_t scratch_bytes, void* stream) {
+extern "C" int wsb_moments2d(const wsb_geom2d* g, const wsb_plan* plan, int kernel_id, int form, const int32_t* order,
+                             const int32_t* slots, const int32_t* counts, int K, const double* ind, const double* lpos,
+                             const double* lsup, int n, const double* rot4, int flags, double* out, void* scratch,
+                             size_t scratch_bytes, void* stream) {
     WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
     WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes n=%d K=%d", n, K);
     WSB_REQUIRE(kernel_id >= 0 && kernel_id <= 3, WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
@@ -308,7 +312,7 @@ extern "C" int wsb_moments2d(const wsb_geom2d* g, const wsb_plan* plan, int kern
         WSB_HIP(hipGetLastError());
         return WSB_OK;
     }
-    TileArgs ta{slots, counts, K, ind, lpos, lsup, n, g->verts, g->edges, g->ne, rot4, flags, out};
+    TileArgs ta{order, slots, counts, K, ind, lpos, lsup, n, g->verts, g->edges, g->ne, rot4, flags, out};
     if (form == WSB_FORM_FP64) return dispatch_tile<closed::FP64>(kernel_id, ta, s);
     if (form == WSB_FORM_MIXED) return dispatch_tile<closed::MIXED>(kernel_id, ta, s);
     return wsb::set_error(WSB_ENOTSUP, "unknown form %d", form);

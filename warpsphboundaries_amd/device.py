@@ -78,7 +78,21 @@ class Geom2D:
             pass
 
     # ---- hot path entry points (device tensors, current stream, no host synchronisation)
-    def adjacency(self, lpos, lsup, valid=None, pad=True, slots=None, counts=None, ind=None):
+    def sort(self, lpos, order=None, scratch=None):
+        """order [N] int32: the rows grouped by static cell (device radix sort of the cell keys)."""
+        L.require_device(lpos)
+        n = lpos.shape[0]
+        dev = lpos.device
+        lpos = lpos.to(torch.float64).contiguous()
+        if order is None:
+            order = torch.empty(n, dtype=torch.int32, device=dev)
+        nbytes = int(L.load().wsb_sort2d_scratch_bytes(self.handle, n))
+        if scratch is None or scratch.numel() < nbytes:
+            scratch = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+        L.check(L.load().wsb_sort2d(self.handle, L.ptr(lpos), n, L.ptr(order), L.ptr(scratch), nbytes, L.stream_ptr(dev)))
+        return order
+
+    def adjacency(self, lpos, lsup, valid=None, pad=True, slots=None, counts=None, ind=None, order=None):
         """(slots [N*K] int32, counts [N] int32, ind [N] f64) -- the bit-exact fixed-capacity adjacency."""
         L.require_device(lpos, lsup)
         n = lpos.shape[0]
@@ -92,11 +106,11 @@ class Geom2D:
         if ind is None:
             ind = torch.empty(n, dtype=torch.float64, device=dev)
         v = None if valid is None else valid.to(torch.uint8).contiguous()
-        L.check(L.load().wsb_adjacency2d(self.handle, L.ptr(lpos), L.ptr(lsup), L.ptr(v), n, L.ptr(slots), L.ptr(counts),
+        L.check(L.load().wsb_adjacency2d(self.handle, L.ptr(lpos), L.ptr(lsup), L.ptr(v), n, L.ptr(order), L.ptr(slots), L.ptr(counts),
                                          L.ptr(ind), 1 if pad else 0, L.stream_ptr(dev)))
         return slots, counts, ind
 
-    def moments(self, kernel, slots, counts, ind, lpos, lsup, form="fp64", rot4=None, flags=0, out=None, scratch=None):
+    def moments(self, kernel, slots, counts, ind, lpos, lsup, form="fp64", rot4=None, flags=0, out=None, scratch=None, order=None):
         """[N, 12] twelve channels per row (device order), see include/wsb.h for the flags."""
         L.require_device(lpos, lsup, slots, counts, ind)
         n = lpos.shape[0]
@@ -115,7 +129,7 @@ class Geom2D:
                 scratch = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
         lpos = lpos.to(torch.float64).contiguous()
         lsup = lsup.to(torch.float64).contiguous()
-        L.check(L.load().wsb_moments2d(self.handle, plan, kid, f, L.ptr(slots), L.ptr(counts), self.K, L.ptr(ind), L.ptr(lpos),
+        L.check(L.load().wsb_moments2d(self.handle, plan, kid, f, L.ptr(order), L.ptr(slots), L.ptr(counts), self.K, L.ptr(ind), L.ptr(lpos),
                                        L.ptr(lsup), n, L.ptr(rot4), int(flags), L.ptr(out), L.ptr(scratch), nbytes,
                                        L.stream_ptr(dev)))
         return out
