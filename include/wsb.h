@@ -114,6 +114,11 @@ int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form, const int
 int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max,
                           double* d, double* nrm, int32_t* edge, void* stream);
 
+/* ---- solid-sphere closed form (3-D, FP64; reference curvbound/symbolic.py:166-178 outside the ball,
+ * mesh3d/radial.py:102-140 on both sides): out[i] = int_ball W dV for a ball of radius R[i] and a query at signed
+ * distance d[i] from its surface (d > 0 outside), units of h; NaN when R <= 0 or d < -R. */
+int wsb_sphere_closed_form(int kernel_id, const double* R, const double* d, int n, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

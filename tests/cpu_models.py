@@ -98,10 +98,25 @@ def plan_interpreter(pl: PL.SurfacePlan, p, q):
     return out * INV_PI
 
 
-def closed_form(kname, p, q):
-    """the shared-primitive closed form of the optimized device forms (same recurrences, same algebra): [P, 12]."""
-    t0, t1, n0, n1, z, s0, s1 = _geo(p, q)
+def closed_form(kname, p, q, h=1.0):
+    """the shared-primitive closed form of the optimized device forms (same geometry path, recurrences and algebra),
+    for edges p -> q relative to the query (physical units; support h): [P, 12]."""
+    ux, uy, vx, vy = p[:, 0], p[:, 1], q[:, 0], q[:, 1]
+    dx, dy = vx - ux, vy - uy
+    ln = np.sqrt(dx * dx + dy * dy)
+    tx, ty, iell = dx / ln, dy / ln, 1.0 / ln
+    ih = 1.0 / h
+    cr = ux * vy - uy * vx
+    mag = np.abs(ux * vy) + np.abs(uy * vx)
+    slow = ~((mag > 1e-280) & (np.abs(cr) > 1.7763568394002505e-15 * mag))
+    cn = (ux / h) * (vy / h) - (uy / h) * (vx / h)
+    cr = np.where(slow, cn * h * h, cr)
+    z = cr * ih * iell
+    px, py, qx, qy = ux * ih, uy * ih, vx * ih, vy * ih
+    s0, s1 = tx * px + ty * py, tx * qx + ty * qy
+    n0, n1 = ty, -tx
     az, zz = np.abs(z), z * z
+    r0sq, r1sq = px * px + py * py, qx * qx + qy * qy
     c = np.zeros((len(p), 12))
     for b in PL.block_profiles(kname):
         R = float(b.R)
@@ -110,15 +125,19 @@ def closed_form(kname, p, q):
         QP = [float(x) / (n + 2) for n, x in enumerate(b.P)]
         QF = [float(x) / (n + 2) for n, x in enumerate(b.Phi)]
         NM = max(len(P), len(Fp)) - 1
+        clo = (s0 < 0) & (r0sq > R * R)
+        chi = (s1 > 0) & (r1sq > R * R)
         L = np.sqrt(np.maximum((R - az) * (R + az), 0))
-        lo, hi = np.maximum(s0, -L), np.minimum(s1, L)
+        lo, hi = np.where(clo, -L, s0), np.where(chi, L, s1)
         m = (az < R) & (lo < hi)
         lo, hi = np.where(m, lo, 0.0), np.where(m, hi, 1.0)
-        rh, rl = np.sqrt(hi * hi + zz), np.sqrt(lo * lo + zz)
+        rl = np.where(clo, R, np.sqrt(r0sq))
+        rh = np.where(chi, R, np.sqrt(r1sq))
+        rl, rh = np.where(m, rl, 1.0), np.where(m, rh, 1.0)
         with np.errstate(divide="ignore", invalid="ignore"):
-            Fh = np.where(hi >= 0, hi + rh, zz / (rh - hi))
-            Fl = np.where(lo >= 0, lo + rl, zz / (rl - lo))
-            dA = np.where(z != 0, np.log(Fh / Fl), 0.0)
+            Nh, Dh = np.where(hi >= 0, hi + rh, zz), np.where(hi >= 0, 1.0, rh - hi)
+            Nl, Dl = np.where(lo >= 0, lo + rl, zz), np.where(lo >= 0, 1.0, rl - lo)
+            dA = np.where(z != 0, np.log((Nh * Dl) / (Dh * Nl)), 0.0)
         dI = [hi - lo, 0.5 * (hi * rh - lo * rl + zz * dA)]
         ph, pl = rh.copy(), rl.copy()
         AP = VP = AF = VF = BP = BF = 0.0
@@ -138,8 +157,8 @@ def closed_form(kname, p, q):
         th = _dangle(z, lo, hi)
         lam = z * VP - float(b.mR_P) * th
         valF = z * VF - float(b.mR_Phi) * th
-        YxP, YyP = z * n0 * AP + t0 * BP, z * n1 * AP + t1 * BP
-        YxF, YyF = z * n0 * AF + t0 * BF, z * n1 * AF + t1 * BF
+        YxP, YyP = z * n0 * AP + tx * BP, z * n1 * AP + ty * BP
+        YxF, YyF = z * n0 * AF + tx * BF, z * n1 * AF + ty * BF
         blk = np.stack([lam, n0 * AF, n1 * AF, -n0 * AP, -n1 * AP, lam - n0 * YxP, -n1 * YxP, -n0 * YyP, lam - n1 * YyP,
                         n0 * YxF - valF, n0 * YyF, n1 * YyF - valF], 1)
         c += np.where(m[:, None], blk, 0.0)

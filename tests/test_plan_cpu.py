@@ -59,6 +59,13 @@ def test_closed_form_algebra_matches_oracle(k):
     got = M.closed_form(k, p, q)
     ref = edge2d.edge_channels12(p, q, k)
     np.testing.assert_allclose(got, ref, rtol=0, atol=2e-12)
+    # physical units with a support other than 1: the predicate's sign and zero follow the reference's normalised
+    # coordinates (v - x)/h exactly, also for queries on the edge lines (the first 20 edges)
+    h = 0.0037
+    P, Q = p * h, q * h
+    got = M.closed_form(k, P, Q, h)
+    ref = edge2d.edge_channels12(P / h, Q / h, k)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=5e-12)
 
 
 @pytest.mark.parametrize("alpha", [(3, 0), (2, 1), (1, 3), (0, 4), (2, 2)])

@@ -12,7 +12,7 @@ from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_size_t, c_void
 import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwsb.so")
+LIB_PATH = os.environ.get("WSB_LIBRARY", os.path.join(HERE, "libwsb.so"))
 
 _ERRORS = {1: ValueError, 2: NotImplementedError, 3: ValueError, 4: MemoryError, 5: RuntimeError}
 
@@ -37,6 +37,7 @@ SIGNATURES = {
     "wsb_edge_channels2d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                     c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_signed_distance2d": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "wsb_sphere_closed_form": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
 }
 
 # C enums

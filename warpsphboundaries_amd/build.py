@@ -5,7 +5,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["api.hip", "geom2d.hip", "moments2d.hip"]
+SOURCES = ["api.hip", "geom2d.hip", "moments2d.hip", "sphere3d.hip"]
 OUT = os.path.join(HERE, "libwsb.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -204,35 +204,61 @@ __device__ __forceinline__ double dangle_mixed(double z, double lo, double hi) {
     return (double)atan2f((float)y, (float)x);
 }
 
+template <int FORM>
+__device__ __forceinline__ double xsqrt(double a) {
+    if (FORM == MIXED) {
+        // FP32 reciprocal square root seed, one FP64 Newton step (relative error ~1e-14)
+        if (a > 1e-30 && a < 1e30) {
+            const double g = (double)__builtin_amdgcn_rsqf((float)a);
+            const double y = a * g;
+            return fma(fma(-y, y, a), 0.5 * g, y);
+        }
+    }
+    return sqrt(a);
+}
+
+// One (query, edge) pair, all twelve channels.  Inputs: the rounded differences u = a - x, v = b - x of the edge
+// endpoints and the query (the numerators of the reference's relative coordinates), the support h and 1/h, and
+// the edge's unit tangent t and 1/length in the local frame.
+//
+// The sign of z (and whether it is zero) must be the sign of the reference's rounded predicate
+// cross2((a-x)/h, (b-x)/h) that the indicator uses: the cross product of u and v decides it when it exceeds
+// 16 u_r times the magnitude of its products (u_r = 2^-53; the divisions by h and the roundings move the
+// predicate by at most 5 u_r of that magnitude), else the reference's predicate itself is evaluated.
 template <int KID, int FORM>
-__device__ __forceinline__ void pair_channels(double px, double py, double qx, double qy, double c[wsb::NCH]) {
+__device__ __forceinline__ void pair_channels(double ux, double uy, double vx, double vy, double h, double ih, double tx,
+                                              double ty, double iell, double c[wsb::NCH]) {
     using T = KTab<KID>;
     constexpr int NM = T::NMAXP;
-    const double dx = qx - px, dy = qy - py;
-    const double ell2 = dx * dx + dy * dy;
-    if (!(ell2 > 0.0)) return;
-    const double ell = sqrt(ell2);
-    const double il = 1.0 / ell;
-    const double t0 = dx * il, t1 = dy * il;
-    const double n0 = t1, n1 = -t0;
-    const double z = wsb_cross2(px, py, qx, qy) * il;     // zero exactly when the rounded predicate is zero
-    const double s0 = (dx * px + dy * py) * il;
-    const double s1 = (dx * qx + dy * qy) * il;
+    if (iell == 0.0) return;
+    double cr = wsb_cross2(ux, uy, vx, vy);
+    const double mag = fabs(ux * vy) + fabs(uy * vx);
+    if (!(mag > 1e-280 && fabs(cr) > 1.7763568394002505e-15 * mag)) cr = wsb_cross2(ux / h, uy / h, vx / h, vy / h) * h * h;
+    const double z = cr * ih * iell;                     // cross(p', q') / |q' - p'|
+    const double px = ux * ih, py = uy * ih, qx = vx * ih, qy = vy * ih;
+    const double s0 = tx * px + ty * py, s1 = tx * qx + ty * qy;
+    const double n0 = ty, n1 = -tx;
     const double az = fabs(z), zz = z * z;
+    const double r0sq = px * px + py * py, r1sq = qx * qx + qy * qy;
+    double r0 = -1.0, r1 = -1.0;                         // endpoint radii, computed once when an unclipped end needs them
 #pragma unroll
     for (int b = 0; b < T::NB; ++b) {
         const double R = T::R[b];
         if (az < R) {
-            const double L = sqrt((R - az) * (R + az));
-            const double lo = fmax(s0, -L), hi = fmin(s1, L);
+            const bool clo = s0 < 0.0 && r0sq > R * R;
+            const bool chi = s1 > 0.0 && r1sq > R * R;
+            const double L = (clo || chi) ? xsqrt<FORM>((R - az) * (R + az)) : 0.0;
+            const double lo = clo ? -L : s0, hi = chi ? L : s1;
             if (lo < hi) {
-                const double rh = sqrt(hi * hi + zz), rl = sqrt(lo * lo + zz);
+                if (!clo && r0 < 0.0) r0 = xsqrt<FORM>(r0sq);
+                if (!chi && r1 < 0.0) r1 = xsqrt<FORM>(r1sq);
+                const double rl = clo ? R : r0, rh = chi ? R : r1;
                 double dA = 0.0;
                 if (z != 0.0) {
                     // asinh(hi/|z|) - asinh(lo/|z|) = log(F(hi)/F(lo)),  F(s) = s + r  (= z^2/(r - s) for s < 0)
-                    const double Fh = hi >= 0.0 ? hi + rh : zz / (rh - hi);
-                    const double Fl = lo >= 0.0 ? lo + rl : zz / (rl - lo);
-                    dA = log(Fh / Fl);
+                    const double Nh = hi >= 0.0 ? hi + rh : zz, Dh = hi >= 0.0 ? 1.0 : rh - hi;
+                    const double Nl = lo >= 0.0 ? lo + rl : zz, Dl = lo >= 0.0 ? 1.0 : rl - lo;
+                    dA = log((Nh * Dl) / (Dh * Nl));
                 }
                 double dI[NM + 1];
                 dI[0] = hi - lo;
@@ -262,8 +288,8 @@ __device__ __forceinline__ void pair_channels(double px, double py, double qx, d
                 const double lam = z * VP - T::MRP[b] * th;
                 const double valF = z * VF - T::MRF[b] * th;
                 const double zn0 = z * n0, zn1 = z * n1;
-                const double YxP = zn0 * AP + t0 * BP, YyP = zn1 * AP + t1 * BP;
-                const double YxF = zn0 * AF + t0 * BF, YyF = zn1 * AF + t1 * BF;
+                const double YxP = zn0 * AP + tx * BP, YyP = zn1 * AP + ty * BP;
+                const double YxF = zn0 * AF + tx * BF, YyF = zn1 * AF + ty * BF;
                 c[0] += lam;
                 c[1] += n0 * AF;
                 c[2] += n1 * AF;

@@ -46,6 +46,21 @@ def render():
         out.append(f"  static constexpr double PLANE[12] = {{{', '.join(repr(v) for v in plane_channels(name))}}};")
         out.append("};")
         out.append("")
+    # 3-D sphere: per block the moment polynomial coefficients c_n/(n+k), k = 2, 3, 4
+    from warpsphboundaries_amd.sphere import device_tables
+    out.append("template <int KID> struct STab;")
+    for kid, name in enumerate(NAMES):
+        rows = device_tables()[name]
+        nb = len(rows)
+        deg = max(len(r[1]) for r in rows) - 1
+        out.append(f"template <> struct STab<{kid}> {{")
+        out.append(f"  static constexpr int NB = {nb};")
+        out.append(f"  static constexpr int DEG = {deg};")
+        out.append(f"  static constexpr double R[NB] = {{{', '.join(repr(r[0]) for r in rows)}}};")
+        for k, idx in ((2, 1), (3, 2), (4, 3)):
+            out.append(f"  static constexpr double M{k}[NB][{NMAX}] = {{{', '.join(_arr(r[idx]) for r in rows)}}};")
+        out.append("};")
+        out.append("")
     return "\n".join(out) + "\n"
 
 

@@ -122,7 +122,7 @@ HostBins build_bins(const double* V, const int32_t* E, int ne) {
 // evaluated.  Edges wholly left of the point neither cross the ray nor contain the point: their cross product is
 // strictly negative for an upward and strictly positive for a downward edge, so they change nothing.
 __host__ __device__ inline double winding_point(double px, double py, double h, const double* V, const int32_t* E, int ne,
-                                                const int32_t* bstart, const int32_t* bitems, double y0, double dy, int nb,
+                                                const int32_t* bstart, const double4* bxy, double y0, double dy, int nb,
                                                 double bg) {
     const double iyf = floor((py - y0) / dy);
     if (!(iyf >= 0.0 && iyf <= (double)nb)) return bg;
@@ -130,10 +130,9 @@ __host__ __device__ inline double winding_point(double px, double py, double h, 
     double w = 0.0;
     bool boundary = false;
     for (int i = bstart[iy]; i < bstart[iy + 1]; ++i) {
-        const int e = bitems[i];
-        const int v0 = E[2 * e], v1 = E[2 * e + 1];
-        const double ax = V[2 * v0], ay = V[2 * v0 + 1];
-        const double cxx = V[2 * v1], cyy = V[2 * v1 + 1];
+        const double4 rec = bxy[i];
+        const double ax = rec.x, ay = rec.y;
+        const double cxx = rec.z, cyy = rec.w;
         if (ax < px && cxx < px) continue;
         const bool up = ay <= py && py < cyy;
         const bool down = cyy <= py && py < ay;
@@ -207,7 +206,8 @@ struct AdjArgs {
     double bg, y0, dy;
     int nb;
     const int32_t* bstart;
-    const int32_t* bitems;
+    const double4* bxy;
+    const double4* cxy;
     int32_t* slots;
     int32_t* counts;
     double* ind;
@@ -234,10 +234,9 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
             const int cid = (int)fx * a.ny + (int)fy;
             const double reach = h * (1.0 + 1e-9) + 1e-14 * (1.0 + fabs(px) + fabs(py));
             for (int i = a.cstart[cid]; i < a.cstart[cid + 1]; ++i) {
-                const int e = a.citems[i];
-                const int v0 = a.E[2 * e], v1 = a.E[2 * e + 1];
-                const double ax = a.V[2 * v0], ay = a.V[2 * v0 + 1];
-                const double bx = a.V[2 * v1], by = a.V[2 * v1 + 1];
+                const double4 rec = a.cxy[i];
+                const double ax = rec.x, ay = rec.y;
+                const double bx = rec.z, by = rec.w;
                 if (fmin(ax, bx) - px > reach || px - fmax(ax, bx) > reach || fmin(ay, by) - py > reach ||
                     py - fmax(ay, by) > reach)
                     continue;
@@ -248,14 +247,14 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
                 const double ex = px - (ax + t * dx);
                 const double ey = py - (ay + t * dyy);
                 if (sqrt(ex * ex + ey * ey) < h) {
-                    a.slots[(int64_t)r * a.K + n] = e;
+                    a.slots[(int64_t)r * a.K + n] = a.citems[i];
                     ++n;
                 }
             }
             if (a.cempty[cid] != 0) {
                 ind = a.cind[cid];
             } else {
-                ind = winding_point(px, py, h, a.V, a.E, a.ne, a.bstart, a.bitems, a.y0, a.dy, a.nb, a.bg);
+                ind = winding_point(px, py, h, a.V, a.E, a.ne, a.bstart, a.bxy, a.y0, a.dy, a.nb, a.bg);
             }
         } else {
             ind = a.bg;
@@ -283,7 +282,7 @@ struct SdArgs {
     double bg, y0, dy;
     int nb;
     const int32_t* bstart;
-    const int32_t* bitems;
+    const double4* bxy;
     double* d;
     double* nrm;
     int32_t* edge;
@@ -333,7 +332,7 @@ __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
     double ind;
     if (inside) {
         ind = a.cempty[cid] ? a.cind[cid]
-                            : winding_point(px, py, a.smax, a.V, a.E, a.ne, a.bstart, a.bitems, a.y0, a.dy, a.nb, a.bg);
+                            : winding_point(px, py, a.smax, a.V, a.E, a.ne, a.bstart, a.bxy, a.y0, a.dy, a.nb, a.bg);
     } else {
         ind = a.bg;
     }
@@ -358,42 +357,48 @@ __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
     if (a.edge) a.edge[r] = k;
 }
 
-// cell key of every particle (outside the grid: nx*ny), for the radix sort of the rows
-__global__ __launch_bounds__(256) void cell_keys_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
-                                                        int ny, uint32_t* keys, int32_t* idx) {
+// counting sort of the rows by static cell: histogram, exclusive scan, scatter.  The order of the rows inside one
+// cell depends on the atomics' timing; it only decides which thread serves a row, never a result.
+__global__ __launch_bounds__(256) void cell_count_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
+                                                         int ny, int32_t* key, int32_t* hist) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const double fx = floor((lpos[2 * r] - clo0) / cell);
     const double fy = floor((lpos[2 * r + 1] - clo1) / cell);
     const bool inside = fx >= 0.0 && fx < (double)nx && fy >= 0.0 && fy < (double)ny;
-    keys[r] = inside ? (uint32_t)((int)fx * ny + (int)fy) : (uint32_t)(nx * ny);
-    idx[r] = r;
+    const int k = inside ? (int)fx * ny + (int)fy : nx * ny;
+    key[r] = k;
+    atomicAdd(hist + k, 1);
+}
+
+__global__ __launch_bounds__(256) void cell_scatter_kernel(const int32_t* key, int n, int32_t* cursor, int32_t* order) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    order[atomicAdd(cursor + key[r], 1)] = r;
+}
+
+__global__ void zero_kernel(int32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 struct SortLayout {
-    size_t kin, kout, vin, tmp, tmp_bytes, total;
+    size_t key, hist, cur, tmp, tmp_bytes, total;
 };
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-SortLayout sort_layout(int n, int bits) {
+SortLayout sort_layout(int n, int nbins) {
     SortLayout L{};
     size_t tmp = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const int32_t*)nullptr, (int32_t*)nullptr, n > 0 ? n : 1, 0, bits);
-    L.kin = 0;
-    L.kout = al256(L.kin + sizeof(uint32_t) * (size_t)n);
-    L.vin = al256(L.kout + sizeof(uint32_t) * (size_t)n);
-    L.tmp = al256(L.vin + sizeof(int32_t) * (size_t)n);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, nbins);
+    L.key = 0;
+    L.hist = al256(L.key + sizeof(int32_t) * (size_t)n);
+    L.cur = al256(L.hist + sizeof(int32_t) * (size_t)nbins);
+    L.tmp = al256(L.cur + sizeof(int32_t) * (size_t)nbins);
     L.tmp_bytes = tmp;
     L.total = al256(L.tmp + tmp);
     return L;
-}
-
-int key_bits(const wsb_geom2d* g) {
-    int bits = 1;
-    while ((1ll << bits) <= (long long)g->nx * g->ny) ++bits;
-    return bits;
 }
 
 template <class T>
@@ -417,6 +422,19 @@ extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edg
     for (int i = 0; i < 2 * nv; ++i) WSB_REQUIRE(std::isfinite(verts[i]), WSB_EINVAL, "non-finite vertex");
     HostCells c = build_cells(verts, edges, ne, support_max);
     HostBins b = build_bins(verts, edges, ne);
+    auto rec_of = [&](int e) {
+        return double4{verts[2 * edges[2 * e]], verts[2 * edges[2 * e] + 1], verts[2 * edges[2 * e + 1]],
+                       verts[2 * edges[2 * e + 1] + 1]};
+    };
+    std::vector<double4> cxy(c.items.size()), bxy(b.items.size()), erec(ne), etan(ne);
+    for (size_t i = 0; i < c.items.size(); ++i) cxy[i] = rec_of(c.items[i]);
+    for (size_t i = 0; i < b.items.size(); ++i) bxy[i] = rec_of(b.items[i]);
+    for (int e = 0; e < ne; ++e) {
+        erec[e] = rec_of(e);
+        const double dx = erec[e].z - erec[e].x, dy = erec[e].w - erec[e].y;
+        const double len = std::sqrt(dx * dx + dy * dy);
+        etan[e] = len > 0 ? double4{dx / len, dy / len, 1.0 / len, len} : double4{0, 0, 0, 0};
+    }
     const int ncells = c.nx * c.ny;
     std::vector<double> cind(ncells, 0.0);
     std::vector<int32_t> cempty(ncells, 0);
@@ -429,7 +447,7 @@ extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edg
             const int cx = k / c.ny, cy = k % c.ny;
             const double px = c.lo0 + ((double)cx + 0.5) * c.cell;
             const double py = c.lo1 + ((double)cy + 0.5) * c.cell;
-            cind[k] = winding_point(px, py, 0.0, verts, edges, ne, b.start.data(), b.items.data(), b.y0, b.dy, b.nb,
+            cind[k] = winding_point(px, py, 0.0, verts, edges, ne, b.start.data(), bxy.data(), b.y0, b.dy, b.nb,
                                     (double)background);
         }
     }
@@ -459,6 +477,10 @@ extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edg
     if (!st) st = upload(&g->cempty, cempty.data(), cempty.size(), &g->bytes);
     if (!st) st = upload(&g->bstart, b.start.data(), b.start.size(), &g->bytes);
     if (!st) st = upload(&g->bitems, b.items.data(), b.items.size(), &g->bytes);
+    if (!st) st = upload(&g->cxy, cxy.data(), cxy.size(), &g->bytes);
+    if (!st) st = upload(&g->bxy, bxy.data(), bxy.size(), &g->bytes);
+    if (!st) st = upload(&g->erec, erec.data(), erec.size(), &g->bytes);
+    if (!st) st = upload(&g->etan, etan.data(), etan.size(), &g->bytes);
     if (st) {
         wsb_geom2d_destroy(g);
         return st;
@@ -471,7 +493,8 @@ extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edg
 extern "C" int wsb_geom2d_destroy(wsb_geom2d* g) {
     if (!g) return WSB_OK;
     for (void* p : {(void*)g->verts, (void*)g->edges, (void*)g->cstart, (void*)g->citems, (void*)g->cind,
-                    (void*)g->cempty, (void*)g->bstart, (void*)g->bitems})
+                    (void*)g->cempty, (void*)g->bstart, (void*)g->bitems, (void*)g->cxy, (void*)g->bxy, (void*)g->erec,
+                    (void*)g->etan})
         if (p) (void)hipFree(p);
     delete g;
     return WSB_OK;
@@ -499,7 +522,7 @@ extern "C" int wsb_geom2d_arrays(const wsb_geom2d* g, const double** verts, cons
 
 extern "C" size_t wsb_sort2d_scratch_bytes(const wsb_geom2d* g, int n) {
     if (!g || n <= 0) return 0;
-    return sort_layout(n, key_bits(g)).total;
+    return sort_layout(n, g->nx * g->ny + 1).total;
 }
 
 extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_t* order, void* scratch, size_t scratch_bytes,
@@ -508,19 +531,22 @@ extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_
     WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(lpos && order, WSB_EINVAL, "null buffer");
-    const int bits = key_bits(g);
-    const SortLayout L = sort_layout(n, bits);
+    const int nbins = g->nx * g->ny + 1;
+    const SortLayout L = sort_layout(n, nbins);
     WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", L.total, scratch_bytes);
     char* base = static_cast<char*>(scratch);
-    uint32_t* kin = reinterpret_cast<uint32_t*>(base + L.kin);
-    uint32_t* kout = reinterpret_cast<uint32_t*>(base + L.kout);
-    int32_t* vin = reinterpret_cast<int32_t*>(base + L.vin);
+    int32_t* key = reinterpret_cast<int32_t*>(base + L.key);
+    int32_t* hist = reinterpret_cast<int32_t*>(base + L.hist);
+    int32_t* cur = reinterpret_cast<int32_t*>(base + L.cur);
     hipStream_t s = wsb::as_stream(stream);
-    hipLaunchKernelGGL(cell_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx,
-                       g->ny, kin, vin);
+    hipLaunchKernelGGL(zero_kernel, dim3((nbins + 255) / 256), dim3(256), 0, s, hist, nbins);
+    hipLaunchKernelGGL(cell_count_kernel, dim3((n + 255) / 256), dim3(256), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx,
+                       g->ny, key, hist);
     WSB_HIP(hipGetLastError());
     size_t tmp = L.tmp_bytes;
-    WSB_HIP(hipcub::DeviceRadixSort::SortPairs(base + L.tmp, tmp, kin, kout, vin, order, n, 0, bits, s));
+    WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, hist, cur, nbins, s));
+    hipLaunchKernelGGL(cell_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key, n, cur, order);
+    WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
 
@@ -532,7 +558,7 @@ extern "C" int wsb_adjacency2d(const wsb_geom2d* g, const double* lpos, const do
     WSB_REQUIRE(lpos && lsup && slots && ind, WSB_EINVAL, "null buffer");
     AdjArgs a{lpos, lsup, valid, order, n, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny,
               g->cstart, g->citems, g->K, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb,
-              g->bstart, g->bitems, slots, counts, ind, pad};
+              g->bstart, g->bxy, g->cxy, slots, counts, ind, pad};
     hipLaunchKernelGGL(adjacency2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
@@ -546,7 +572,7 @@ extern "C" int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, in
     WSB_REQUIRE(lpos && d && nrm, WSB_EINVAL, "null buffer");
     WSB_REQUIRE(support_max > 0, WSB_EINVAL, "support_max must be positive");
     SdArgs a{lpos, m, support_max, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny, g->cstart,
-             g->citems, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb, g->bstart, g->bitems,
+             g->citems, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb, g->bstart, g->bxy,
              d, nrm, edge};
     hipLaunchKernelGGL(signed_distance2d_kernel, dim3((m + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
     WSB_HIP(hipGetLastError());

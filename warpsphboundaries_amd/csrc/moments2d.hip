@@ -113,8 +113,8 @@ struct TileArgs {
     const double* lpos;
     const double* lsup;
     int n;
-    const double* V;
-    const int32_t* E;
+    const double4* erec;
+    const double4* etan;
     int ne;
     const double* rot4;
     int flags;
@@ -126,11 +126,11 @@ struct TileArgs {
 // form, the twelve channels go through LDS and each row's lane adds its own pairs in slot order: the row sums
 // are deterministic and independent of how pairs fall into chunks.
 template <int KID, int FORM>
-__global__ __launch_bounds__(TILE) void tile12_kernel(TileArgs a) {
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4, 8))) void tile12_kernel(TileArgs a) {
     using BlockScan = hipcub::BlockScan<int, TILE>;
     __shared__ typename BlockScan::TempStorage scan_tmp;
     __shared__ int s_off[TILE + 1];
-    __shared__ double s_px[TILE], s_py[TILE], s_h[TILE];
+    __shared__ double s_px[TILE], s_py[TILE], s_h[TILE], s_ih[TILE];
     __shared__ double s_res[wsb::NCH][TILE];
     __shared__ int s_row[TILE];
     const int tid = threadIdx.x;
@@ -145,6 +145,7 @@ __global__ __launch_bounds__(TILE) void tile12_kernel(TileArgs a) {
         s_py[tid] = a.lpos[2 * row + 1];
         h = a.lsup[row];
         s_h[tid] = h;
+        s_ih[tid] = 1.0 / h;
     }
     int off, total;
     BlockScan(scan_tmp).ExclusiveSum(cnt, off, total);
@@ -169,11 +170,10 @@ __global__ __launch_bounds__(TILE) void tile12_kernel(TileArgs a) {
             const int r = lo;
             const int e = a.slots[(int64_t)s_row[r] * a.K + (j - s_off[r])];
             if (e >= 0 && e < a.ne) {
-                const double x = s_px[r], y = s_py[r], hr = s_h[r];
-                const int v0 = a.E[2 * e], v1 = a.E[2 * e + 1];
-                const double px = (a.V[2 * v0] - x) / hr, py = (a.V[2 * v0 + 1] - y) / hr;
-                const double qx = (a.V[2 * v1] - x) / hr, qy = (a.V[2 * v1 + 1] - y) / hr;
-                closed::pair_channels<KID, FORM>(px, py, qx, qy, c);
+                const double x = s_px[r], y = s_py[r];
+                const double4 ab = a.erec[e];
+                const double4 tn = a.etan[e];
+                closed::pair_channels<KID, FORM>(ab.x - x, ab.y - y, ab.z - x, ab.w - y, s_h[r], s_ih[r], tn.x, tn.y, tn.z, c);
             }
         }
 #pragma unroll
@@ -230,9 +230,10 @@ __global__ __launch_bounds__(256) void pairs12_kernel(PairArgs a) {
     if (e >= 0 && e < a.ne && q >= 0 && q < a.nq) {
         const double h = a.sup[q], x = a.pos[2 * q], y = a.pos[2 * q + 1];
         const int v0 = a.E[2 * e], v1 = a.E[2 * e + 1];
-        const double px = (a.V[2 * v0] - x) / h, py = (a.V[2 * v0 + 1] - y) / h;
-        const double qx = (a.V[2 * v1] - x) / h, qy = (a.V[2 * v1 + 1] - y) / h;
-        closed::pair_channels<KID, FORM>(px, py, qx, qy, c);
+        const double ax = a.V[2 * v0], ay = a.V[2 * v0 + 1], bx = a.V[2 * v1], by = a.V[2 * v1 + 1];
+        const double dx = bx - ax, dy = by - ay;
+        const double len = sqrt(dx * dx + dy * dy);
+        if (len > 0.0) closed::pair_channels<KID, FORM>(ax - x, ay - y, bx - x, by - y, h, 1.0 / h, dx / len, dy / len, 1.0 / len, c);
     }
 #pragma unroll
     for (int k = 0; k < wsb::NCH; ++k) a.out[(int64_t)j * wsb::NCH + k] = c[k];
@@ -312,7 +313,7 @@ extern "C" int wsb_moments2d(const wsb_geom2d* g, const wsb_plan* plan, int kern
         WSB_HIP(hipGetLastError());
         return WSB_OK;
     }
-    TileArgs ta{order, slots, counts, K, ind, lpos, lsup, n, g->verts, g->edges, g->ne, rot4, flags, out};
+    TileArgs ta{order, slots, counts, K, ind, lpos, lsup, n, g->erec, g->etan, g->ne, rot4, flags, out};
     if (form == WSB_FORM_FP64) return dispatch_tile<closed::FP64>(kernel_id, ta, s);
     if (form == WSB_FORM_MIXED) return dispatch_tile<closed::MIXED>(kernel_id, ta, s);
     return wsb::set_error(WSB_ENOTSUP, "unknown form %d", form);

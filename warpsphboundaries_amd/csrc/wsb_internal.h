@@ -62,6 +62,11 @@ struct wsb_geom2d {
     int32_t* cempty = nullptr;    // [nx*ny]
     int32_t* bstart = nullptr;    // [nb+1]
     int32_t* bitems = nullptr;    // [nbin_items]
+    // coordinate records (one load instead of edge -> vertex indirection)
+    double4* cxy = nullptr;       // [ncell_items] (ax, ay, bx, by) of each cell item
+    double4* bxy = nullptr;       // [nbin_items]  (ax, ay, bx, by) of each bin item
+    double4* erec = nullptr;      // [ne] (ax, ay, bx, by)
+    double4* etan = nullptr;      // [ne] (tx, ty, 1/len, len): unit tangent and length in the local frame
 };
 
 struct wsb_plan {
