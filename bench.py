@@ -216,7 +216,14 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     t_adj = e0.elapsed_time(e1) / 1e3 / args.steps
-    log("per-launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()}, "adjacency", t_adj, "sort", t_sort)
+    e0.record(stream)
+    for _ in range(args.steps):
+        geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_adj_unsorted = e0.elapsed_time(e1) / 1e3 / args.steps
+    log("per-launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()}, "adjacency", t_adj, "sort", t_sort,
+        "adjacency in input order", t_adj_unsorted)
 
     # ---- roofline of the dominant kernel (the mixed-form tile kernel, w6 is the heaviest profile)
     kdom = max(work.kernels, key=lambda k: per_kernel[("mixed", k)])
@@ -253,7 +260,7 @@ def main():
         "speedup_mixed_vs_plain": t_plain / t_mixed,
         "evaluate_only_evals_per_s": {"mixed": evals / sum(per_kernel[("mixed", k)] for k in work.kernels),
                                       "fp64": evals / sum(per_kernel[("fp64", k)] for k in work.kernels)},
-        "adjacency_ms": t_adj * 1e3, "cell_sort_ms": t_sort * 1e3,
+        "adjacency_ms": t_adj * 1e3, "cell_sort_ms": t_sort * 1e3, "adjacency_input_order_ms": t_adj_unsorted * 1e3,
         "max_signal_relative_error_vs_fp64": err,
         "served_latency": None,
         "roofline": roof,

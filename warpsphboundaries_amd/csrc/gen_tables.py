@@ -59,6 +59,19 @@ def render():
         out.append(f"  static constexpr double R[NB] = {{{', '.join(repr(r[0]) for r in rows)}}};")
         for k, idx in ((2, 1), (3, 2), (4, 3)):
             out.append(f"  static constexpr double M{k}[NB][{NMAX}] = {{{', '.join(_arr(r[idx]) for r in rows)}}};")
+        # the same coefficients split exactly: c = hi + lo (lo carries the rounding of the exact rational)
+        from warpsphboundaries_amd.sphere import blocks3d
+        from fractions import Fraction
+        for k in (2, 3, 4):
+            his, los = [], []
+            for Rj, c in blocks3d(name):
+                exact = [x / (n + k) for n, x in enumerate(c)]
+                hi = [float(x) for x in exact]
+                lo = [float(x - Fraction(h)) for x, h in zip(exact, hi)]
+                his.append(hi)
+                los.append(lo)
+            out.append(f"  static constexpr double M{k}H[NB][{NMAX}] = {{{', '.join(_arr(x) for x in his)}}};")
+            out.append(f"  static constexpr double M{k}L[NB][{NMAX}] = {{{', '.join(_arr(x) for x in los)}}};")
         out.append("};")
         out.append("")
     return "\n".join(out) + "\n"

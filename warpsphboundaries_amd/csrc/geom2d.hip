@@ -359,22 +359,32 @@ __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
 
 // counting sort of the rows by static cell: histogram, exclusive scan, scatter.  The order of the rows inside one
 // cell depends on the atomics' timing; it only decides which thread serves a row, never a result.
+// Rows inside the grid are placed by per-cell atomics (a cell holds a few dozen rows); the rows outside the grid,
+// which share one key and may be most of the particles, are placed after them by an exclusive scan of their flags.
 __global__ __launch_bounds__(256) void cell_count_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
-                                                         int ny, int32_t* key, int32_t* hist) {
+                                                         int ny, int32_t* key, int32_t* outflag, int32_t* hist) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const double fx = floor((lpos[2 * r] - clo0) / cell);
     const double fy = floor((lpos[2 * r + 1] - clo1) / cell);
     const bool inside = fx >= 0.0 && fx < (double)nx && fy >= 0.0 && fy < (double)ny;
-    const int k = inside ? (int)fx * ny + (int)fy : nx * ny;
+    const int k = inside ? (int)fx * ny + (int)fy : -1;
     key[r] = k;
-    atomicAdd(hist + k, 1);
+    outflag[r] = inside ? 0 : 1;
+    if (inside) atomicAdd(hist + k, 1);
 }
 
-__global__ __launch_bounds__(256) void cell_scatter_kernel(const int32_t* key, int n, int32_t* cursor, int32_t* order) {
+__global__ __launch_bounds__(256) void cell_scatter_kernel(const int32_t* key, const int32_t* outflag, const int32_t* outpos, int n,
+                                                           int32_t* cursor, int32_t* order) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    order[atomicAdd(cursor + key[r], 1)] = r;
+    const int k = key[r];
+    if (k >= 0) {
+        order[atomicAdd(cursor + k, 1)] = r;
+    } else {
+        const int n_out = outpos[n - 1] + outflag[n - 1];
+        order[(n - n_out) + outpos[r]] = r;
+    }
 }
 
 __global__ void zero_kernel(int32_t* p, int n) {
@@ -383,17 +393,21 @@ __global__ void zero_kernel(int32_t* p, int n) {
 }
 
 struct SortLayout {
-    size_t key, hist, cur, tmp, tmp_bytes, total;
+    size_t key, flag, opos, hist, cur, tmp, tmp_bytes, total;
 };
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 SortLayout sort_layout(int n, int nbins) {
     SortLayout L{};
-    size_t tmp = 0;
+    size_t tmp = 0, tmp2 = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, nbins);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, (const int32_t*)nullptr, (int32_t*)nullptr, n);
+    tmp = std::max(tmp, tmp2);
     L.key = 0;
-    L.hist = al256(L.key + sizeof(int32_t) * (size_t)n);
+    L.flag = al256(L.key + sizeof(int32_t) * (size_t)n);
+    L.opos = al256(L.flag + sizeof(int32_t) * (size_t)n);
+    L.hist = al256(L.opos + sizeof(int32_t) * (size_t)n);
     L.cur = al256(L.hist + sizeof(int32_t) * (size_t)nbins);
     L.tmp = al256(L.cur + sizeof(int32_t) * (size_t)nbins);
     L.tmp_bytes = tmp;
@@ -522,7 +536,7 @@ extern "C" int wsb_geom2d_arrays(const wsb_geom2d* g, const double** verts, cons
 
 extern "C" size_t wsb_sort2d_scratch_bytes(const wsb_geom2d* g, int n) {
     if (!g || n <= 0) return 0;
-    return sort_layout(n, g->nx * g->ny + 1).total;
+    return sort_layout(n, g->nx * g->ny).total;
 }
 
 extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_t* order, void* scratch, size_t scratch_bytes,
@@ -531,21 +545,25 @@ extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_
     WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(lpos && order, WSB_EINVAL, "null buffer");
-    const int nbins = g->nx * g->ny + 1;
+    const int nbins = g->nx * g->ny;
     const SortLayout L = sort_layout(n, nbins);
     WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", L.total, scratch_bytes);
     char* base = static_cast<char*>(scratch);
     int32_t* key = reinterpret_cast<int32_t*>(base + L.key);
+    int32_t* flag = reinterpret_cast<int32_t*>(base + L.flag);
+    int32_t* opos = reinterpret_cast<int32_t*>(base + L.opos);
     int32_t* hist = reinterpret_cast<int32_t*>(base + L.hist);
     int32_t* cur = reinterpret_cast<int32_t*>(base + L.cur);
     hipStream_t s = wsb::as_stream(stream);
     hipLaunchKernelGGL(zero_kernel, dim3((nbins + 255) / 256), dim3(256), 0, s, hist, nbins);
     hipLaunchKernelGGL(cell_count_kernel, dim3((n + 255) / 256), dim3(256), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx,
-                       g->ny, key, hist);
+                       g->ny, key, flag, hist);
     WSB_HIP(hipGetLastError());
     size_t tmp = L.tmp_bytes;
     WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, hist, cur, nbins, s));
-    hipLaunchKernelGGL(cell_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key, n, cur, order);
+    tmp = L.tmp_bytes;
+    WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, flag, opos, n, s));
+    hipLaunchKernelGGL(cell_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key, flag, opos, n, cur, order);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
 }

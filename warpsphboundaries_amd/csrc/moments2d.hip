@@ -126,7 +126,7 @@ struct TileArgs {
 // form, the twelve channels go through LDS and each row's lane adds its own pairs in slot order: the row sums
 // are deterministic and independent of how pairs fall into chunks.
 template <int KID, int FORM>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4, 8))) void tile12_kernel(TileArgs a) {
+__global__ __launch_bounds__(TILE) void tile12_kernel(TileArgs a) {
     using BlockScan = hipcub::BlockScan<int, TILE>;
     __shared__ typename BlockScan::TempStorage scan_tmp;
     __shared__ int s_off[TILE + 1];
