@@ -1,0 +1,320 @@
+"""Continuous-batching server of boundary-moment queries from many concurrent simulations (no reference
+counterpart: the reference evaluates one position set per call).
+
+    srv = BoundaryServer(device="cuda:0", hbm_budget=8 << 30, streams=4, form="mixed")
+    srv.add_scene("tank", [(loop, "outside"), (obstacle, "inside")], support_max=h)    # resident geometry
+    req = srv.submit("tank", positions, supports, kernel="w4")                        # any time, from any client
+    done = srv.poll()                                                                  # retire finished, admit, launch
+
+* Geometry, cell lists and their coordinate records stay resident on the device (``add_scene``); every resident
+  byte is charged to the HBM budget.
+* Requests are admitted by HBM footprint: a batch is launched only when its buffers (inputs, slots, indicator,
+  outputs, sort scratch) fit in the budget left after the resident scenes and the batches in flight.  A request
+  larger than the whole budget is rejected with ``MemoryError``; otherwise it waits.
+* Continuous batching: pending requests of one (scene, kernel) are merged into a batch (up to ``max_batch_rows``);
+  each of ``streams`` HIP streams carries one batch at a time, so batches of different scenes overlap on the device;
+  as soon as a stream's batch retires, the oldest pending work is admitted into it.
+* Captured launches: with ``graphs=True`` the adjacency + moment launches of a (scene, kernel, stream, row bucket)
+  are captured once in a HIP graph over fixed buffers and replayed (rows beyond the request count are parked outside
+  the geometry's grid, where they cost one cell lookup).
+* Latency = arrival (``submit``) to retirement (the poll that sees the batch's event complete), host clock.
+Results are bitwise independent of batching: every row's channels are a fixed-order sum over its own slots.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .scene import SurfaceRep, kernelName
+
+F64 = torch.float64
+PARK = 1.0e30          # coordinate of padded rows: outside every grid
+
+
+@dataclass
+class Request:
+    rid: int
+    scene: str
+    kernel: str
+    positions: np.ndarray
+    supports: np.ndarray
+    arrival: float
+    done: Optional[float] = None
+    result: Optional[np.ndarray] = None       # [n, 12] float64, physical units, device channel order
+    error: Optional[BaseException] = None
+    cancelled: bool = False
+
+    @property
+    def n(self):
+        return len(self.positions)
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+@dataclass
+class ResidentScene:
+    name: str
+    reps: list            # SurfaceRep
+    support_max: float
+    geoms: list
+    K: int
+    bytes: int
+
+
+@dataclass
+class _Slot:
+    """per-stream fixed buffers for one row capacity (reused by every batch that fits)."""
+    cap: int
+    Kcap: int
+    lpos: torch.Tensor
+    lsup: torch.Tensor
+    slots: torch.Tensor
+    counts: torch.Tensor
+    ind: torch.Tensor
+    order: torch.Tensor
+    out: torch.Tensor
+    host_pos: torch.Tensor
+    host_sup: torch.Tensor
+    host_out: torch.Tensor
+    sort_scratch: torch.Tensor
+    bytes: int
+    graphs: dict = field(default_factory=dict)
+    filled: int = 0           # rows of lpos / lsup holding request data (the rest is parked)
+
+
+@dataclass
+class _Batch:
+    stream_index: int
+    scene: ResidentScene
+    kernel: str
+    requests: List[Request]
+    rows: int
+    slot: _Slot
+    event: torch.cuda.Event
+    bytes: int
+
+
+class BoundaryServer:
+    def __init__(self, device="cuda:0", hbm_budget: int = 8 << 30, streams: int = 4, form: str = "mixed",
+                 max_batch_rows: int = 1 << 18, sort_threshold: int = 1 << 16, graphs: bool = False):
+        self.device = torch.device(device)
+        self.budget = int(hbm_budget)
+        self.form = form
+        self.max_batch_rows = int(max_batch_rows)
+        self.sort_threshold = int(sort_threshold)
+        self.use_graphs = bool(graphs)
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(int(streams))]
+        self.busy = [None] * len(self.streams)
+        self.slots: List[Dict[int, _Slot]] = [dict() for _ in self.streams]
+        self.scenes: Dict[str, ResidentScene] = {}
+        self.pending: deque = deque()
+        self.resident_bytes = 0
+        self.slot_bytes = 0
+        self._rid = 0
+        self.completed: List[Request] = []
+        self.launched_batches = 0
+        self.launched_rows = 0
+
+    # ------------------------------------------------------------------------------------------------ scenes
+    def add_scene(self, name: str, loops, support_max: float):
+        """make a scene resident: ``loops`` = [(vertices [n,2], 'inside' | 'outside')] or SurfaceReps."""
+        if name in self.scenes:
+            raise ValueError(f"scene {name!r} exists")
+        reps = [l if isinstance(l, SurfaceRep) else SurfaceRep.polygon(l[0], l[1]) for l in loops]
+        geoms = [r.geom(float(support_max), self.device) for r in reps]
+        nbytes = sum(int(g.info["device_bytes"]) for g in geoms)
+        if self.resident_bytes + nbytes > self.budget:
+            raise MemoryError(f"scene {name!r} needs {nbytes} resident bytes; {self.budget - self.resident_bytes} left")
+        self.resident_bytes += nbytes
+        sc = ResidentScene(name, reps, float(support_max), geoms, max(g.K for g in geoms), nbytes)
+        self.scenes[name] = sc
+        return sc
+
+    def remove_scene(self, name: str):
+        sc = self.scenes.pop(name)
+        self.resident_bytes -= sc.bytes
+
+    # ------------------------------------------------------------------------------------------------ requests
+    def submit(self, scene: str, positions, supports, kernel="w4") -> Request:
+        sc = self.scenes.get(scene)
+        if sc is None:
+            raise KeyError(f"unknown scene {scene!r}")
+        k = kernelName(kernel)
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 2))
+        sup = np.ascontiguousarray(np.broadcast_to(np.asarray(supports, dtype=np.float64), (len(pos),)))
+        if len(pos) and not (np.all(sup > 0) and np.all(sup <= sc.support_max)):
+            raise ValueError("supports must lie in (0, support_max] of the scene (the resident cell list is built for it)")
+        req = Request(self._rid, scene, k, pos, sup, time.perf_counter())
+        self._rid += 1
+        if self._footprint(sc, max(len(pos), 1)) + self.resident_bytes > self.budget:
+            req.error = MemoryError(f"request of {len(pos)} rows exceeds the HBM budget")
+            req.done = time.perf_counter()
+            self.completed.append(req)
+            return req
+        self.pending.append(req)
+        return req
+
+    def cancel(self, req: Request):
+        req.cancelled = True
+        if req in self.pending:
+            self.pending.remove(req)
+
+    # ------------------------------------------------------------------------------------------------ scheduling
+    def _footprint(self, sc: ResidentScene, rows: int) -> int:
+        cap = _bucket(rows)
+        per_row = 16 + 8 + 4 * sc.K + 4 + 8 + 4 + 96 + 16 + 16 + 96
+        return cap * per_row + (1 << 20)
+
+    def _in_flight_bytes(self):
+        return sum(b.bytes for b in self.busy if b is not None)
+
+    def poll(self) -> List[Request]:
+        """retire finished batches, admit and launch pending work; returns the requests completed now."""
+        now_done = []
+        for i, b in enumerate(self.busy):
+            if b is not None and b.event.query():
+                now = time.perf_counter()
+                res = b.slot.host_out[:b.rows].numpy()
+                off = 0
+                for r in b.requests:
+                    if not r.cancelled:
+                        r.result = res[off:off + r.n].copy()
+                    r.done = now
+                    off += r.n
+                    now_done.append(r)
+                self.busy[i] = None
+        while self.pending:
+            free = [i for i, b in enumerate(self.busy) if b is None]
+            if not free:
+                break
+            head = self.pending[0]
+            sc = self.scenes[head.scene]
+            group, rows = [], 0
+            for r in list(self.pending):
+                if r.scene == head.scene and r.kernel == head.kernel and (not group or rows + r.n <= self.max_batch_rows):
+                    group.append(r)
+                    rows += r.n
+                    if rows >= self.max_batch_rows:
+                        break
+            fp = self._footprint(sc, max(rows, 1))
+            if self.resident_bytes + self._in_flight_bytes() + fp > self.budget:
+                if len(group) > 1:          # admit less: the head alone
+                    group, rows = [head], head.n
+                    fp = self._footprint(sc, max(rows, 1))
+                if self.resident_bytes + self._in_flight_bytes() + fp > self.budget:
+                    break                   # wait for memory to free up
+            for r in group:
+                self.pending.remove(r)
+            si = free[0]
+            if self.use_graphs:
+                # stream affinity: a free stream whose buffers already hold this batch's captured graph
+                cap = _bucket(rows)
+                for i in free:
+                    sl = self.slots[i].get(cap)
+                    if sl is not None and (sc.name, head.kernel) in sl.graphs:
+                        si = i
+                        break
+            self._launch(si, sc, head.kernel, group, rows, fp)
+        self.completed.extend(now_done)
+        return now_done
+
+    def run(self, until_idle=True, sleep=0.0):
+        while self.pending or any(b is not None for b in self.busy):
+            self.poll()
+            if sleep:
+                time.sleep(sleep)
+
+    # ------------------------------------------------------------------------------------------------ launches
+    def _slot(self, si: int, sc: ResidentScene, rows: int) -> _Slot:
+        cap = _bucket(rows)
+        pool = self.slots[si]
+        key = cap
+        s = pool.get(key)
+        if s is None or s.Kcap < sc.K:
+            dev = self.device
+            Kcap = max(sc.K, 1 if s is None else s.Kcap)
+            sort_bytes = max(int(L.load().wsb_sort2d_scratch_bytes(g.handle, cap)) for g in sc.geoms)
+            s = _Slot(cap, Kcap,
+                      torch.full((cap, 2), PARK, dtype=F64, device=dev), torch.ones(cap, dtype=F64, device=dev),
+                      torch.empty(cap * Kcap, dtype=torch.int32, device=dev), torch.empty(cap, dtype=torch.int32, device=dev),
+                      torch.empty(cap, dtype=F64, device=dev), torch.empty(cap, dtype=torch.int32, device=dev),
+                      torch.empty((cap, 12), dtype=F64, device=dev),
+                      torch.empty((cap, 2), dtype=F64).pin_memory(), torch.empty(cap, dtype=F64).pin_memory(),
+                      torch.empty((cap, 12), dtype=F64).pin_memory(),
+                      torch.empty(max(sort_bytes, 1), dtype=torch.uint8, device=dev), cap * (16 + 8 + 4 * Kcap + 4 + 8 + 4 + 96))
+            pool[key] = s
+        return s
+
+    def _evaluate(self, sc: ResidentScene, kernel: str, s: _Slot, n: int, sort: bool):
+        """the launches of one batch over the slot's buffers (current stream)."""
+        for j, g in enumerate(sc.geoms):
+            order = None
+            if sort:
+                order = g.sort(s.lpos[:n], order=s.order[:n], scratch=s.sort_scratch)
+            g.adjacency(s.lpos[:n], s.lsup[:n], pad=False, slots=s.slots[:n * g.K], counts=s.counts[:n], ind=s.ind[:n],
+                        order=order)
+            flags = L.OUT_PHYSICAL | (L.OUT_ACCUMULATE if j else 0)
+            g.moments(kernel, s.slots[:n * g.K], s.counts[:n], s.ind[:n], s.lpos[:n], s.lsup[:n], form=self.form,
+                      flags=flags, out=s.out[:n])
+
+    def _launch(self, si: int, sc: ResidentScene, kernel: str, group: List[Request], rows: int, fp: int):
+        s = self._slot(si, sc, rows)
+        st = self.streams[si]
+        hp, hs = s.host_pos.numpy(), s.host_sup.numpy()
+        off = 0
+        for r in group:
+            hp[off:off + r.n] = r.positions
+            hs[off:off + r.n] = r.supports
+            off += r.n
+        use_graph = self.use_graphs
+        with torch.cuda.stream(st):
+            s.lpos[:rows].copy_(s.host_pos[:rows], non_blocking=True)
+            s.lsup[:rows].copy_(s.host_sup[:rows], non_blocking=True)
+            if use_graph:
+                if s.filled > rows:                                   # park the rows of a larger previous batch
+                    s.lpos[rows:s.filled].fill_(PARK)
+                    s.lsup[rows:s.filled].fill_(sc.support_max)
+                s.filled = rows
+                key = (sc.name, kernel)
+                gr = s.graphs.get(key)
+                if gr is None:
+                    self._evaluate(sc, kernel, s, s.cap, False)      # warm-up outside the capture
+                    gr = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(gr, stream=st):
+                        self._evaluate(sc, kernel, s, s.cap, False)
+                    s.graphs[key] = gr
+                gr.replay()
+            else:
+                s.filled = max(s.filled, rows)
+                self._evaluate(sc, kernel, s, rows, rows >= self.sort_threshold)
+            s.host_out[:rows].copy_(s.out[:rows], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        self.busy[si] = _Batch(si, sc, kernel, group, rows, s, ev, fp)
+        self.launched_batches += 1
+        self.launched_rows += rows
+
+    # ------------------------------------------------------------------------------------------------ stats
+    def latency_stats(self, requests=None):
+        reqs = [r for r in (requests or self.completed) if r.latency is not None and r.error is None]
+        lat = np.array([r.latency for r in reqs])
+        if not len(lat):
+            return {}
+        return {"requests": len(lat), "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3)}
+
+
+def _bucket(rows: int) -> int:
+    """row capacity bucket: the next power of two, at least 1024."""
+    cap = 1024
+    while cap < rows:
+        cap *= 2
+    return cap

@@ -128,3 +128,43 @@ def subsample(w: Workload, n: int, seed=7) -> Workload:
     idx = np.concatenate([g.choice(near, k, replace=False) if k else near[:0], g.choice(far, n - k, replace=False)])
     idx.sort()
     return Workload(w.name + f"-sub{n}", w.loops, w.positions[idx], w.supports[idx], w.kernels, w.h, w.near[idx])
+
+
+@dataclass
+class ServedScene:
+    name: str
+    loops: list
+    h: float
+    kernel: str
+    positions: np.ndarray
+
+
+def config5(n_scenes=256, seed=5, n_min=1000, n_max=200000, e_min=16, e_max=8192):
+    """the served mixed-size workload (2-D scenes): star polygons with E = 10^U(log 16, log 8192) edges, particle
+    counts 10^U(3, log 2e5) uniform in [0,1]^2 outside the star, kernel in {w2, w4, w6}, h = dx sqrt(40/pi)
+    (about 40 neighbours, dx = 1/sqrt(n))."""
+    g = rng(seed)
+    scenes = []
+    for i in range(n_scenes):
+        E = int(10 ** g.uniform(np.log10(e_min), np.log10(e_max)))
+        n = int(10 ** g.uniform(np.log10(n_min), np.log10(n_max)))
+        star = star_polygon(max(E, 3), g=g)
+        pts = np.empty((0, 2))
+        while len(pts) < n:
+            c = g.uniform(0.0, 1.0, (int(1.5 * (n - len(pts))) + 64, 2))
+            c = c[~inside_star(c, star)]
+            pts = np.concatenate([pts, c])
+        h = float(np.sqrt(40.0 / np.pi) / np.sqrt(n))
+        scenes.append(ServedScene(f"scene{i:03d}", [(star, "inside")], h, ["w2", "w4", "w6"][int(g.integers(0, 3))], pts[:n]))
+    return scenes
+
+
+def jitter(positions, h, seed):
+    """one timestep: positions moved by N(0, (0.01 h)^2)."""
+    g = rng(seed)
+    return positions + g.normal(0.0, 0.01 * h, positions.shape)
+
+
+def poisson_arrivals(n, qps, seed=6):
+    g = rng(seed)
+    return np.cumsum(g.exponential(1.0 / qps, n))
