@@ -119,6 +119,59 @@ int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double
  * distance d[i] from its surface (d > 0 outside), units of h; NaN when R <= 0 or d < -R. */
 int wsb_sphere_closed_form(int kernel_id, const double* R, const double* d, int n, double* out, void* stream);
 
+/* ================================================================ 3-D: closed oriented triangle surfaces
+ * The 22 stored moments of baseline(3)[:22] (reference fluxbridge/contract.py:41-46), in request order
+ *     m000 m100 m010 m001 m200 m110 m101 m020 m011 m002  Gx Gy Gz  g1xx g1xy g1xz g1yx g1yy g1yz g1zx g1zy g1zz
+ * (moments of y^alpha W, kernel gradients of y^alpha, g1[d][j] = int y_d d_j W), y = source - query. */
+typedef struct wsb_mesh3d wsb_mesh3d;
+
+enum wsb_mesh_flags {
+    WSB_MESH_HOST_ONLY = 1,   /* build the static tables on the host only (host evaluation entry points)            */
+    WSB_MESH_SOUP = 2         /* open triangle set: no closedness check, no winding number (adjacency only)          */
+};
+
+/* ---- static geometry of one closed surface (reference mesh3d/reference.py:219-250 ClosedSurface checks;
+ * fluxbridge/geometry.py:74-80 Geometry.closed_surface).  verts[nv*3], faces[nf*3] (outward orientation: a cavity's
+ * faces point into the cavity).  Builds the face records, a grid of cells of support_max (coarser above 2^24 cells)
+ * with each cell's candidate faces (vertex box inflated by support_max), the winding number of every face-free cell
+ * and the axis walk of every other cell.  info[9] (nullable): K (largest candidate list), nx, ny, nz, nf, cell items,
+ * device bytes, face-free components, cells with faces. */
+int wsb_mesh3d_create(const double* verts, int nv, const int32_t* faces, int nf, double support_max, int flags,
+                      wsb_mesh3d** out, int64_t* info);
+int wsb_mesh3d_destroy(wsb_mesh3d* m);
+int wsb_mesh3d_info(const wsb_mesh3d* m, int64_t* info);
+/* lo_cell[4]: grid origin (x, y, z) and cell size */
+int wsb_mesh3d_grid(const wsb_mesh3d* m, double* lo_cell);
+
+/* ---- face adjacency + winding number (reference fluxbridge/device_search.py:68-195 count / fill / sort with
+ * each triangle an element; winding number = the solid angle sum of mesh3d/reference.py:255-258 / 4 pi).
+ * pos[n*3], sup[n] (<= support_max).  offsets[n+1] (CSR, int64) is always written; faces[capacity] receives the
+ * kept faces of each particle in ascending order (entries past `capacity` are dropped: the caller compares
+ * offsets[n] with it); chi[n] (nullable) the winding number.  scratch: wsb_adjacency3d_scratch_bytes(n) bytes. */
+size_t wsb_adjacency3d_scratch_bytes(int n);
+int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, int n, int64_t* offsets, int32_t* faces,
+                    int64_t capacity, double* chi, void* scratch, size_t scratch_bytes, void* stream);
+
+/* ---- the 22 stored moments per particle from the adjacency (reference fluxbridge/evaluate.py:158-175 batch +
+ * reduce, fluxbridge/wp_func.py:260-351 element in compact mode): out[n*22] = sum over the particle's faces of
+ * the compact face terms + chi * whole-ball vector; flags WSB_OUT_PHYSICAL (h^(|alpha| - |derivative|)),
+ * WSB_OUT_NO_INDICATOR, WSB_OUT_ACCUMULATE.  form WSB_FORM_FP64 or WSB_FORM_MIXED.  Pairs at or past
+ * `capacity` are ignored. */
+int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets, const int32_t* faces,
+                  int64_t capacity, const double* chi, const double* pos, const double* sup, int n, int flags,
+                  double* out, void* stream);
+
+/* ---- host evaluations of the same arithmetic (validation; every pointer is host memory) */
+int wsb_adjacency3d_host(const wsb_mesh3d* m, const double* pos, const double* sup, int n, int64_t* offsets,
+                         int32_t* faces, int64_t capacity, double* chi);
+int wsb_winding3d_exact_host(const wsb_mesh3d* m, const double* pos, int n, double* chi);
+/* out[P*23]: face accumulators (M0, M1[3], V_Phi, M2[6], G[3], g1[9]) of each (query, face) pair, units of h; form
+ * WSB_FORM_MIXED evaluates the single-precision angles and square-root seeds of the mixed form on the host */
+int wsb_face_terms3d_host(const wsb_mesh3d* m, int kernel_id, int form, const int32_t* pair_face, const double* pair_pos,
+                          const double* pair_sup, int P, double* out);
+int wsb_finalize22_host(int kernel_id, const double* acc, const double* chi, const double* sup, int n, int flags,
+                        double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,20 @@ def w_pi(name, q):
     return sp.Integer(0)
 
 
+@lru_cache(maxsize=None)
+def c3_pi(name):
+    """pi C3 from the 3-D normalisation 4 pi C3 int_0^1 q^2 w(q) dq = 1 (curvbound/kernels.py:10-14 lists the values)."""
+    _, pieces = _SPEC[name]
+    m = sum(sp.integrate(e * _q ** 2, (_q, lo, hi)) for (lo, hi), e in pieces)
+    v = sp.nsimplify(1 / (4 * m))
+    return Fraction(int(sp.fraction(v)[0]), int(sp.fraction(v)[1]))
+
+
+def c3_over_c2(name):
+    c2 = _SPEC[name][0]
+    return c3_pi(name) / Fraction(int(sp.fraction(c2)[0]), int(sp.fraction(c2)[1]))
+
+
 def radial_integral(name, power):
     """exact int_0^1 r^power pi W dr."""
     c2pi, pieces = _SPEC[name]

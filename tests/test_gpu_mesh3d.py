@@ -1,0 +1,189 @@
+"""3-D closed-surface path on the device: face adjacency (bit-exact), winding numbers, the 22 stored moments.
+
+References: the reference's 70-digit fixtures (tests/golden/fluxbridge-*.json, copied data), the brute-force element
+test of the oracle, the host evaluation of the same arithmetic, and the compact-form FP64 oracle with the snapped
+winding number.  Tolerances: |device - fixture| <= 1e-12 (1 + |fixture|) (FP64 form); device vs host evaluation of
+the same arithmetic (FMA contraction differs) 1e-11 (1 + |value|) per channel in units of h; device vs FP64 oracle 1e-12 (w6: 1e-11, the
+oracle's own FP64 error at that size) in units of h; mixed vs FP64 1e-6 signal-relative per channel.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mesh3d as OM
+from warpsphboundaries_amd import mesh3d as MM
+from warpsphboundaries_amd.plan3d import H_POWER3, REQUESTS22
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+KERNELS = ["cubic", "w2", "w4", "w6"]
+
+
+def _fixture_cases():
+    out = []
+    for fn, lim in (("fluxbridge-v1.json", 44), ("fluxbridge-nearplane-v1.json", 10 ** 6)):
+        d = json.load(open(os.path.join(GOLD, fn)))
+        for ci, c in enumerate(d["cases"]):
+            if ci < lim and c.get("dimension", 3) == 3 and c.get("measure", "volume") == "volume":
+                out.append(c)
+    return out
+
+
+def _cavity(level=2, h=0.08):
+    Vb, Fb = MM.box_surface((0, 0, 0), (1, 1, 1))
+    Vs, Fs = MM.icosphere(level, 0.25, (0.5, 0.5, 0.5))
+    V = np.vstack([Vb, Vs])
+    F = np.vstack([Fb, (Fs + len(Vb))[:, ::-1]])
+    return V, F
+
+
+def _points(n, seed=1):
+    rng = np.random.default_rng(seed)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    p1 = 0.5 + dirs * (0.25 + rng.uniform(-0.06, 0.06, n))[:, None]
+    p2 = rng.uniform(0, 1, (n, 3))
+    p2[:, 0] = rng.uniform(-0.03, 0.05, n)
+    p3 = rng.uniform(-0.05, 1.05, (n, 3))
+    return np.vstack([p1, p2, p3])
+
+
+def _units(h):
+    return np.asarray(h, dtype=np.float64)[..., None] ** np.array(H_POWER3, dtype=np.float64)
+
+
+def test_fixtures_fp64_and_mixed():
+    worst = 0.0
+    for c in _fixture_cases():
+        V = np.array([[float(x) for x in v] for v in c["vertices"]])
+        q = np.array([[float(x) for x in c["query"]]])
+        h = float(c["h"])
+        m = MM.Mesh3D(V, OM.tetrahedron_faces(V), h, device=DEV)
+        Q = torch.tensor(q, device=DEV)
+        H = torch.full((1,), h, dtype=torch.float64, device=DEV)
+        adj = m.adjacency(Q, H)
+        ref = np.array([float(v) for v in c["values"]])[:22]
+        got = m.moments(adj, Q, H, c["kernel"], "fp64").cpu().numpy()[0]
+        err = np.abs(got - ref) / (1 + np.abs(ref))
+        worst = max(worst, err.max())
+        assert err.max() <= 1e-12, (c.get("label", c["query"]), c["kernel"], err.max(), int(err.argmax()))
+        mix = m.moments(adj, Q, H, c["kernel"], "mixed").cpu().numpy()[0]
+        scale = np.abs(ref).max()
+        assert (np.abs(mix - ref) <= 1e-6 * scale + 1e-30).all(), (c.get("label"), c["kernel"])
+    print("fixture worst", worst)
+
+
+def test_adjacency_winding_bitexact_and_host_equal():
+    V, F = _cavity(2)
+    h = 0.08
+    P = _points(400)
+    H = np.full(len(P), h)
+    H[::3] *= 0.6
+    m = MM.Mesh3D(V, F, h, device=DEV)
+    adj = m.adjacency(torch.tensor(P, device=DEV), torch.tensor(H, device=DEV))
+    q, el, off = adj.numpy()
+    cols, offs = OM.face_adjacency(V[F], P, H)
+    assert np.array_equal(off, offs) and np.array_equal(el, cols)
+    hoff, hfaces, hchi = MM.Mesh3D(V, F, h, host_only=True).adjacency_host(P, H)
+    assert np.array_equal(hoff, off) and np.array_equal(hfaces, el)
+    chi = adj.chi.cpu().numpy()
+    np.testing.assert_allclose(chi, hchi, rtol=0, atol=1e-14)
+    assert np.abs(chi - OM.winding(V, F, P)).max() < 1e-12
+
+
+def test_capacity_retry():
+    V, F = _cavity(2)
+    m = MM.Mesh3D(V, F, 0.08, device=DEV)
+    P = torch.tensor(_points(200), device=DEV)
+    H = torch.full((len(P),), 0.08, dtype=torch.float64, device=DEV)
+    a = m.adjacency(P, H, capacity=3)
+    b = m.adjacency(P, H)
+    assert a.count == b.count
+    assert torch.equal(a.faces[: a.count], b.faces[: b.count])
+
+
+@pytest.mark.parametrize("k", KERNELS)
+def test_moments_match_host_and_oracle(k):
+    V, F = _cavity(2)
+    h = 0.08
+    P = _points(120, seed=7)
+    H = np.full(len(P), h)
+    m = MM.Mesh3D(V, F, h, device=DEV)
+    Q, HH = torch.tensor(P, device=DEV), torch.tensor(H, device=DEV)
+    got = m.evaluate(Q, HH, k, "fp64").cpu().numpy()
+    host = MM.Mesh3D(V, F, h, host_only=True).evaluate_host(P, H, k)
+    u = _units(H)
+    dh = np.abs(got - host) / u / (1 + np.abs(host) / u)
+    assert dh.max() <= 1e-11, (k, dh.max(axis=0))
+    w = OM.snapped_winding(V, F, P)
+    ref = OM.closed_surface_requests(V, F, P, k, h, list(REQUESTS22), chunk=32, winding=w)
+    tol = 1e-11 if k == "w6" else 1e-12
+    err = np.abs(got - ref) / u / (1 + np.abs(ref) / u)
+    assert err.max() <= tol, (k, err.max())
+    mix = m.evaluate(Q, HH, k, "mixed").cpu().numpy()
+    sig = np.abs(got / u).max(axis=0) + 1e-300
+    assert (np.abs(mix - got) / u <= 1e-6 * sig).all(), (np.abs(mix - got) / u / sig).max()
+
+
+def test_points_on_the_surface_take_the_exact_sum():
+    V, F = MM.box_surface((0, 0, 0), (1, 1, 1))
+    # on a face, on an edge, at a vertex, and just off the faces
+    P = np.array([[0.0, 0.3, 0.6], [0.0, 0.0, 0.4], [0.0, 0.0, 0.0], [1e-17, 0.3, 0.6], [0.5, 0.5, 1.0], [0.5, 0.5, 0.5]])
+    h = 0.1
+    m = MM.Mesh3D(V, F, h, device=DEV)
+    adj = m.adjacency(torch.tensor(P, device=DEV), torch.full((len(P),), h, dtype=torch.float64, device=DEV))
+    chi = adj.chi.cpu().numpy()
+    host = MM.Mesh3D(V, F, h, host_only=True).adjacency_host(P, h)[2]
+    np.testing.assert_allclose(chi, host, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(chi, [0.5, 0.25, 0.125, 1.0, 0.5, 1.0], atol=1e-14)
+
+
+def test_plane_limit_and_whole_support():
+    # a large box: the query deep inside sees the whole ball; near one face it sees the half-space
+    from warpsphboundaries_amd import sphere as S
+    from warpsphboundaries_amd.plan3d import whole_vector3
+    V, F = MM.box_surface((-10, -10, -10), (10, 10, 10))
+    m = MM.Mesh3D(V, F, 1.0, device=DEV)
+    d = np.array([0.9, 0.5, 0.1, 1e-6, 0.0])
+    P = np.zeros((len(d) + 1, 3))
+    P[: len(d), 2] = 10.0 - d
+    H = torch.ones(len(P), dtype=torch.float64, device=DEV)
+    for k in KERNELS:
+        out = m.evaluate(torch.tensor(P, device=DEV), H, k, "fp64").cpu().numpy()
+        np.testing.assert_allclose(out[-1], whole_vector3(k), atol=1e-13)
+        lam = np.array([float(S.half_space_exact(k, -x)) for x in d])
+        np.testing.assert_allclose(out[:-1, 0], lam, rtol=0, atol=1e-13)
+
+
+def test_empty_and_far_particles():
+    V, F = _cavity(1)
+    m = MM.Mesh3D(V, F, 0.05, device=DEV)
+    P = torch.tensor([[5.0, 5.0, 5.0], [0.5, 0.5, 0.5], [0.1, 0.1, 0.1]], dtype=torch.float64, device=DEV)
+    H = torch.full((3,), 0.05, dtype=torch.float64, device=DEV)
+    out = m.evaluate(P, H, "w2", "fp64").cpu().numpy()
+    from warpsphboundaries_amd.plan3d import whole_vector3
+    u = _units(np.full(3, 0.05))
+    np.testing.assert_allclose(out[0], 0.0, atol=0)
+    np.testing.assert_allclose(out[1], 0.0, atol=1e-15)              # inside the cavity (outside the solid)
+    np.testing.assert_allclose(out[2] / u[2], whole_vector3("w2"), atol=1e-13)
+    e = m.evaluate(P[:0], H[:0], "w2", "fp64")
+    assert e.shape == (0, 22)
+
+
+def test_evaluator_and_element_api():
+    c = _fixture_cases()[4]
+    V = np.array([[float(x) for x in v] for v in c["vertices"]])
+    ev = MM.Evaluator(MM.Geometry([V]), MM.baseline(3)[:22], kernel=c["kernel"])
+    got = ev.evaluate([[float(x) for x in c["query"]]], float(c["h"]))[0]
+    ref = np.array([float(v) for v in c["values"]])[:22]
+    assert (np.abs(got - ref) <= 1e-12 * (1 + np.abs(ref))).all()
+    tet = MM.Tetrahedron(V, [float(x) for x in c["query"]])
+    assert abs(tet.moment(c["kernel"], (1, 0, 0), float(c["h"])) - ref[1]) <= 1e-12
+    g = tet.kernel_gradient(c["kernel"], (0, 1, 0), float(c["h"]))
+    assert np.allclose(g, ref[16:19], atol=1e-12)
+    with pytest.raises(NotImplementedError):
+        tet.moment(c["kernel"], (3, 0, 0))

@@ -38,11 +38,25 @@ SIGNATURES = {
                                     c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_signed_distance2d": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wsb_sphere_closed_form": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "wsb_mesh3d_create": (c_int, [c_void_p, c_int, c_void_p, c_int, c_double, c_int, POINTER(c_void_p), POINTER(c_int64)]),
+    "wsb_mesh3d_destroy": (c_int, [c_void_p]),
+    "wsb_mesh3d_info": (c_int, [c_void_p, POINTER(c_int64)]),
+    "wsb_mesh3d_grid": (c_int, [c_void_p, POINTER(c_double)]),
+    "wsb_adjacency3d_scratch_bytes": (c_size_t, [c_int]),
+    "wsb_adjacency3d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_size_t,
+                                c_void_p]),
+    "wsb_moments3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                              c_void_p, c_void_p]),
+    "wsb_adjacency3d_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+    "wsb_winding3d_exact_host": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "wsb_face_terms3d_host": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "wsb_finalize22_host": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
 }
 
 # C enums
 FORMS = {"plain": 0, "fp64": 1, "mixed": 2}
 OUT_ACCUMULATE, OUT_PHYSICAL, OUT_WORLD, OUT_NO_INDICATOR = 1, 2, 4, 8
+MESH_HOST_ONLY, MESH_SOUP = 1, 2
 
 _lib = None
 
@@ -91,6 +105,11 @@ def require_device(*tensors):
 
 def ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def nptr(a):
+    """host pointer of a C-contiguous numpy array (host evaluation entry points)."""
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
 
 
 def stream_ptr(device=None):

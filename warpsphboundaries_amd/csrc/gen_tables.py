@@ -74,7 +74,55 @@ def render():
             out.append(f"  static constexpr double M{k}L[NB][{NMAX}] = {{{', '.join(_arr(x) for x in los)}}};")
         out.append("};")
         out.append("")
+    out += render3d()
     return "\n".join(out) + "\n"
+
+
+NM3 = 18           # array extent of the 3-D tables (>= largest power of Phi2 + 3)
+
+
+def render3d():
+    """3-D face closed form: per block the radial functions P, Phi, Phi2, psi_P, psi_Phi (dense by power), their
+    K-chain weights g_m/(m+2) (index m), G(R) = sum g_m R^(m+2)/(m+2), and the whole-ball constants."""
+    from warpsphboundaries_amd import plan3d as P3
+    out = ["template <int KID> struct K3Tab;"]
+    for kid, name in enumerate(NAMES):
+        bl = P3.blocks3(name)
+        nb = len(bl)
+
+        def dense(d):
+            assert min(d) >= 0 and max(d) < NM3, d
+            return [d.get(m, 0) for m in range(NM3)]
+
+        def q(d):
+            return {m: c / (m + 2) for m, c in d.items()}
+
+        def GR(d, R):
+            return sum(c * R ** (m + 2) / (m + 2) for m, c in d.items())
+        deg = max(max(max(b.Phi2), max(b.psiF) + 2, max(b.Phi) + 2) for b in bl)
+        out.append(f"template <> struct K3Tab<{kid}> {{   // {name}")
+        out.append(f"  static constexpr int NB = {nb};")
+        out.append(f"  static constexpr int DEG = {deg};")
+        out.append(f"  static constexpr double R[NB] = {{{', '.join(repr(float(b.R)) for b in bl)}}};")
+        for key, attr in (("P", "P"), ("F", "Phi"), ("F2", "Phi2"), ("pP", "psiP"), ("pF", "psiF")):
+            rows = [dense(getattr(b, attr)) for b in bl]
+            out.append(f"  static constexpr double {key}[NB][{NM3}] = {{{', '.join(_arr3(r) for r in rows)}}};")
+            if key != "F2":
+                rows = [dense(q(getattr(b, attr))) for b in bl]
+                out.append(f"  static constexpr double Q{key}[NB][{NM3}] = {{{', '.join(_arr3(r) for r in rows)}}};")
+                out.append(f"  static constexpr double GR{key}[NB] = "
+                           f"{{{', '.join(repr(float(GR(getattr(b, attr), b.R))) for b in bl)}}};")
+        out.append(f"  static constexpr double MR[NB] = {{{', '.join(repr(float(b.mR)) for b in bl)}}};")
+        out.append(f"  static constexpr double MRF[NB] = {{{', '.join(repr(float(b.mRF)) for b in bl)}}};")
+        out.append(f"  static constexpr double MRSUM = {float(sum(b.mR for b in bl))!r};")
+        out.append(f"  static constexpr double MRFSUM = {float(sum(b.mRF for b in bl))!r};")
+        out.append("};")
+        out.append("")
+    return out
+
+
+def _arr3(vals):
+    return "{" + ", ".join(repr(float(v)) for v in vals) + "}"
 
 
 def main(path=None):

@@ -1,0 +1,690 @@
+// Closed oriented triangle surfaces in 3-D: static grid build (host), face adjacency + winding number (device),
+// the 22 stored moments (device tiles), and host evaluations of the same arithmetic.
+//
+// Adjacency contract: a face is a neighbour of a particle iff the reference's whole-element test keeps it,
+//     sum_a (max(0, lo_a - x_a, x_a - hi_a) / h)^2 <= 1 + 1.4210854715202004e-14      (axis order, no FMA)
+// over the face's vertex box (fluxbridge/device_search.py:80-85 with each triangle an element,
+// fluxbridge/geometry.py:62), listed per particle in ascending face order (the sorted (query, element) order of
+// device_search.py:186-189).  The static grid only narrows the candidates: a cell lists every face whose box,
+// inflated by support_max (1 + 1e-9), overlaps it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <queue>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "mesh3d_device.h"
+#include "wsb_internal.h"
+
+struct wsb_mesh3d {
+    int device = 0;
+    bool host_only = false, soup = false;
+    int nv = 0, nf = 0;
+    double support_max = 0;
+    m3::Grid3 hg{};                      // host view (pointers into the vectors below)
+    m3::Grid3 dg{};                      // device view
+    int K = 0, ncomp = 0;
+    int64_t nitems = 0, ndirty = 0, bytes = 0;
+    std::vector<double> frec, fbox, cchi;
+    std::vector<int32_t> cstart, citems, cwalk;
+    double* d_frec = nullptr;
+    double* d_fbox = nullptr;
+    double* d_cchi = nullptr;
+    int32_t* d_cstart = nullptr;
+    int32_t* d_citems = nullptr;
+    int32_t* d_cwalk = nullptr;
+};
+
+namespace {
+
+// whole-element test of the reference, operation order kept (no contraction)
+__host__ __device__ inline bool keep_face(const double* __restrict__ box, double px, double py, double pz, double h) {
+#pragma clang fp contract(off)
+    const double p[3] = {px, py, pz};
+    double sq = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double delta = fmax(0.0, fmax(box[a] - p[a], p[a] - box[3 + a])) / h;
+        sq = sq + delta * delta;
+    }
+    return sq <= 1.0 + 1.4210854715202004e-14;
+}
+
+// a box farther than h (1 + 1e-9) on some axis fails keep_face (its rounded normalised distance exceeds 1 + 1e-9)
+__host__ __device__ inline bool box_far(const double* __restrict__ box, double px, double py, double pz, double reach) {
+    return box[0] - px > reach || px - box[3] > reach || box[1] - py > reach || py - box[4] > reach || box[2] - pz > reach ||
+           pz - box[5] > reach;
+}
+
+__host__ __device__ inline double exact_winding(const double* frec, int nf, double px, double py, double pz, double ih) {
+    double s = 0.0;
+    for (int f = 0; f < nf; ++f) s += m3::face_solid_angle<m3::FP64>(frec + m3::FREC * f, px, py, pz, ih);
+    return s / (4.0 * m3::PI);
+}
+
+// ------------------------------------------------------------------------------------------------ host build
+int build_host(wsb_mesh3d* m, const double* V, int nv, const int32_t* Fc, int nf, double smax) {
+    const int64_t nf64 = nf;
+    m->frec.assign((size_t)nf64 * m3::FREC, 0.0);
+    m->fbox.assign((size_t)nf64 * 6, 0.0);
+    double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int f = 0; f < nf; ++f) {
+        const int32_t* ix = Fc + 3 * (int64_t)f;
+        double* fr = m->frec.data() + (size_t)m3::FREC * f;
+        double* bx = m->fbox.data() + (size_t)6 * f;
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 3; ++a) fr[m3::FR_V + 3 * k + a] = V[3 * (int64_t)ix[k] + a];
+        const double* v0 = fr + m3::FR_V;
+        const double* v1 = fr + m3::FR_V + 3;
+        const double* v2 = fr + m3::FR_V + 6;
+        const double d1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+        const double d2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+        const double N[3] = {d1[1] * d2[2] - d1[2] * d2[1], d1[2] * d2[0] - d1[0] * d2[2], d1[0] * d2[1] - d1[1] * d2[0]};
+        const double nl = std::sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+        WSB_REQUIRE(nl > 0.0 && std::isfinite(nl), WSB_EINVAL, "degenerate triangle %d", f);
+        const double n[3] = {N[0] / nl, N[1] / nl, N[2] / nl};
+        for (int a = 0; a < 3; ++a) fr[m3::FR_N + a] = n[a];
+        for (int e = 0; e < 3; ++e) {
+            const double* p = fr + m3::FR_V + 3 * e;
+            const double* q = fr + m3::FR_V + 3 * ((e + 1) % 3);
+            const double D[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+            const double len = std::sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+            const double t[3] = {D[0] / len, D[1] / len, D[2] / len};
+            const double mu[3] = {t[1] * n[2] - t[2] * n[1], t[2] * n[0] - t[0] * n[2], t[0] * n[1] - t[1] * n[0]};
+            for (int a = 0; a < 3; ++a) {
+                fr[m3::FR_T + 3 * e + a] = t[a];
+                fr[m3::FR_MU + 3 * e + a] = mu[a];
+            }
+            fr[m3::FR_LEN + e] = len;
+        }
+        for (int a = 0; a < 3; ++a) {
+            bx[a] = std::min({v0[a], v1[a], v2[a]});
+            bx[3 + a] = std::max({v0[a], v1[a], v2[a]});
+            glo[a] = std::min(glo[a], bx[a]);
+            ghi[a] = std::max(ghi[a], bx[3 + a]);
+        }
+    }
+    // grid: cells of support_max over the inflated mesh box, coarsened if it would exceed 2^24 cells
+    const double pad = smax * (1.0 + 1e-9) + 1e-12 * (1.0 + std::max({std::fabs(glo[0]), std::fabs(glo[1]), std::fabs(glo[2]),
+                                                                            std::fabs(ghi[0]), std::fabs(ghi[1]), std::fabs(ghi[2])}));
+    double cell = smax;
+    double ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = (ghi[a] - glo[a]) + 2.0 * pad;
+    auto ncell_of = [&](double c) {
+        double tot = 1.0;
+        for (int a = 0; a < 3; ++a) tot *= std::max(1.0, std::ceil(ext[a] / c));
+        return tot;
+    };
+    while (ncell_of(cell) > 16777216.0) cell *= 1.25;
+    m3::Grid3& g = m->hg;
+    g.cell = cell;
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = glo[a] - pad;
+        g.n[a] = (int)std::max(1.0, std::ceil(ext[a] / cell));
+    }
+    const int64_t ncells = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    auto clampi = [](double v, int hi) { return (int)std::min<double>(std::max<double>(v, 0.0), (double)hi); };
+    // dilated candidate lists (counting sort by cell, faces ascending inside a cell) and the dirty flags
+    std::vector<int32_t> cnt((size_t)ncells + 1, 0);
+    std::vector<uint8_t> dirty((size_t)ncells, 0);
+    const double reach = smax * (1.0 + 1e-9);
+    auto range = [&](const double* bx, double r, int lo3[3], int hi3[3]) {
+        for (int a = 0; a < 3; ++a) {
+            const double eps = 1e-12 * (1.0 + std::fabs(bx[a]) + std::fabs(bx[3 + a]));
+            lo3[a] = clampi(std::floor((bx[a] - r - eps - g.lo[a]) / cell), g.n[a] - 1);
+            hi3[a] = clampi(std::floor((bx[3 + a] + r + eps - g.lo[a]) / cell), g.n[a] - 1);
+        }
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<int32_t> fill;
+        if (pass == 1) {
+            for (int64_t k = 0; k < ncells; ++k) cnt[k + 1] += cnt[k];
+            m->citems.assign((size_t)cnt[ncells], 0);
+            fill.assign(cnt.begin(), cnt.end() - 1);
+        }
+        for (int f = 0; f < nf; ++f) {
+            const double* bx = m->fbox.data() + (size_t)6 * f;
+            int lo3[3], hi3[3];
+            range(bx, reach, lo3, hi3);
+            for (int i = lo3[0]; i <= hi3[0]; ++i)
+                for (int j = lo3[1]; j <= hi3[1]; ++j)
+                    for (int k = lo3[2]; k <= hi3[2]; ++k) {
+                        const int64_t c = m3::cell_id(g, i, j, k);
+                        if (pass == 0) cnt[c + 1]++;
+                        else m->citems[fill[c]++] = f;
+                    }
+            if (pass == 0) {
+                range(bx, 0.0, lo3, hi3);
+                for (int i = lo3[0]; i <= hi3[0]; ++i)
+                    for (int j = lo3[1]; j <= hi3[1]; ++j)
+                        for (int k = lo3[2]; k <= hi3[2]; ++k) dirty[m3::cell_id(g, i, j, k)] = 1;
+            }
+        }
+    }
+    m->cstart.assign(cnt.begin(), cnt.end());
+    m->nitems = (int64_t)m->citems.size();
+    int K = 0;
+    for (int64_t c = 0; c < ncells; ++c) K = std::max(K, m->cstart[c + 1] - m->cstart[c]);
+    m->K = K;
+    m->cchi.assign((size_t)ncells, 0.0);
+    m->cwalk.assign((size_t)ncells, -1);
+    if (!m->soup) {
+        // winding number of each connected component of face-free cells, from one exact sum per component
+        std::vector<int32_t> comp((size_t)ncells, -1);
+        int ncomp = 0;
+        std::vector<int64_t> queue;
+        for (int64_t c0 = 0; c0 < ncells; ++c0) {
+            if (dirty[c0] || comp[c0] >= 0) continue;
+            const int id = ncomp++;
+            comp[c0] = id;
+            queue.assign(1, c0);
+            for (size_t qi = 0; qi < queue.size(); ++qi) {
+                const int64_t c = queue[qi];
+                const int k = (int)(c % g.n[2]), j = (int)((c / g.n[2]) % g.n[1]), i = (int)(c / ((int64_t)g.n[1] * g.n[2]));
+                const int nb[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+                for (auto& q : nb) {
+                    if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= g.n[0] || q[1] >= g.n[1] || q[2] >= g.n[2]) continue;
+                    const int64_t cq = m3::cell_id(g, q[0], q[1], q[2]);
+                    if (dirty[cq] || comp[cq] >= 0) continue;
+                    comp[cq] = id;
+                    queue.push_back(cq);
+                }
+            }
+            const int k = (int)(c0 % g.n[2]), j = (int)((c0 / g.n[2]) % g.n[1]), i = (int)(c0 / ((int64_t)g.n[1] * g.n[2]));
+            const double px = g.lo[0] + (i + 0.5) * cell, py = g.lo[1] + (j + 0.5) * cell, pz = g.lo[2] + (k + 0.5) * cell;
+            const double w = exact_winding(m->frec.data(), nf, px, py, pz, 1.0);
+            const double wr = std::nearbyint(w);
+            WSB_REQUIRE(std::fabs(w - wr) < 1e-6, WSB_EINVAL, "winding number %.17g of a face-free cell is not an integer", w);
+            for (int64_t c : queue) m->cchi[c] = wr;
+        }
+        m->ncomp = ncomp;
+        // dirty cells: shortest axis walk to a face-free cell or out of the grid (6 directional sweeps)
+        std::vector<int32_t> best((size_t)ncells, 0x7fffffff), bdir((size_t)ncells, 0);
+        for (int dir = 0; dir < 6; ++dir) {
+            const int ax = dir >> 1, s = (dir & 1) ? -1 : 1;
+            const int au = (ax + 1) % 3, av = (ax + 2) % 3;
+            for (int u = 0; u < g.n[au]; ++u)
+                for (int v = 0; v < g.n[av]; ++v) {
+                    int run = 0;
+                    for (int t = 0; t < g.n[ax]; ++t) {
+                        const int pos = s > 0 ? g.n[ax] - 1 - t : t;
+                        int ijk[3];
+                        ijk[ax] = pos;
+                        ijk[au] = u;
+                        ijk[av] = v;
+                        const int64_t c = m3::cell_id(g, ijk[0], ijk[1], ijk[2]);
+                        if (!dirty[c]) {
+                            run = 0;
+                            continue;
+                        }
+                        ++run;
+                        if (run < best[c]) {
+                            best[c] = run;
+                            bdir[c] = dir;
+                        }
+                    }
+                }
+        }
+        int64_t ndirty = 0;
+        for (int64_t c = 0; c < ncells; ++c)
+            if (dirty[c]) {
+                m->cwalk[c] = bdir[c] | (best[c] << 3);
+                ++ndirty;
+            }
+        m->ndirty = ndirty;
+    }
+    g.cstart = m->cstart.data();
+    g.citems = m->citems.data();
+    g.cchi = m->cchi.data();
+    g.cwalk = m->cwalk.data();
+    g.frec = m->frec.data();
+    g.fbox = m->fbox.data();
+    g.nf = nf;
+    return WSB_OK;
+}
+
+int validate_closed(const double* V, int nv, const int32_t* Fc, int nf) {
+    // every undirected edge twice, in opposite directions (mesh3d/reference.py:238-245)
+    std::map<std::pair<int, int>, std::pair<int, int>> edges;     // (min, max) -> (count, direction sum)
+    for (int f = 0; f < nf; ++f) {
+        const int32_t* ix = Fc + 3 * (int64_t)f;
+        for (int e = 0; e < 3; ++e) {
+            const int a = ix[e], b = ix[(e + 1) % 3];
+            auto& v = edges[{std::min(a, b), std::max(a, b)}];
+            v.first++;
+            v.second += a < b ? 1 : -1;
+        }
+    }
+    for (auto& kv : edges)
+        WSB_REQUIRE(kv.second.first == 2 && kv.second.second == 0, WSB_EINVAL,
+                    "surface must be watertight and consistently oriented (edge %d-%d)", kv.first.first, kv.first.second);
+    // positive signed volume, relative to the first vertex (mesh3d/reference.py:246-250)
+    double s6 = 0.0;
+    const double* o = V;
+    for (int f = 0; f < nf; ++f) {
+        const int32_t* ix = Fc + 3 * (int64_t)f;
+        double r[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 3; ++a) r[k][a] = V[3 * (int64_t)ix[k] + a] - o[a];
+        s6 += r[0][0] * (r[1][1] * r[2][2] - r[1][2] * r[2][1]) + r[0][1] * (r[1][2] * r[2][0] - r[1][0] * r[2][2]) +
+              r[0][2] * (r[1][0] * r[2][1] - r[1][1] * r[2][0]);
+    }
+    WSB_REQUIRE(s6 > 0.0, WSB_EINVAL, "surface must have positive signed volume");
+    (void)nv;
+    return WSB_OK;
+}
+
+template <class T>
+int upload(T** dst, const std::vector<T>& src, int64_t* bytes) {
+    const size_t nb = std::max<size_t>(src.size(), 1) * sizeof(T);
+    WSB_HIP(hipMalloc(dst, nb));
+    if (!src.empty()) WSB_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    *bytes += (int64_t)nb;
+    return WSB_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ device kernels
+struct Adj3Args {
+    m3::Grid3 g;
+    const double* pos;
+    const double* sup;
+    int n;
+    int64_t* counts;            // count pass: per particle
+    const int64_t* offsets;     // fill pass: CSR offsets
+    int32_t* faces;
+    int64_t capacity;
+    double* chi;
+};
+
+// one thread per particle: candidates of its cell through the element test; the count pass also evaluates the
+// winding number (face-free cell, axis walk, or -- when a predicate of the walk is within rounding -- the exact
+// per-face sum, evaluated by the whole wavefront for each such lane in turn)
+template <bool FILL>
+__global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < a.n;
+    double px = 0.0, py = 0.0, pz = 0.0, h = 1.0;
+    if (live) {
+        px = a.pos[3 * (int64_t)i];
+        py = a.pos[3 * (int64_t)i + 1];
+        pz = a.pos[3 * (int64_t)i + 2];
+        h = a.sup[i];
+    }
+    int c[3] = {0, 0, 0};
+    const bool in = live && m3::cell_of(a.g, px, py, pz, c);
+    int64_t cnt = 0;
+    if (in) {
+        const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
+        const double reach = h * (1.0 + 1e-9);
+        const int64_t base = FILL ? a.offsets[i] : 0;
+        for (int it = a.g.cstart[cid]; it < a.g.cstart[cid + 1]; ++it) {
+            const int f = a.g.citems[it];
+            const double* bx = a.g.fbox + 6 * (int64_t)f;
+            if (box_far(bx, px, py, pz, reach)) continue;
+            if (!keep_face(bx, px, py, pz, h)) continue;
+            if (FILL && base + cnt < a.capacity) a.faces[base + cnt] = f;
+            ++cnt;
+        }
+    }
+    if (FILL) return;
+    double chi = 0.0;
+    bool need = false;
+    if (in && a.g.cwalk) need = !m3::ray_winding(a.g, px, py, pz, c, &chi);
+    uint64_t mask = __ballot(need);
+    const int lane = threadIdx.x & 63;
+    while (mask) {
+        const int src = __ffsll((unsigned long long)mask) - 1;
+        const double qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src), qi = __shfl(1.0 / h, src);
+        double s = 0.0;
+        for (int f = lane; f < a.g.nf; f += 64) s += m3::face_solid_angle<m3::FP64>(a.g.frec + m3::FREC * (int64_t)f, qx, qy, qz, qi);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == src) chi = s / (4.0 * m3::PI);
+        mask &= mask - 1;
+    }
+    if (live) {
+        a.counts[i] = cnt;
+        if (a.chi) a.chi[i] = chi;
+    }
+}
+
+__global__ void offsets_head_kernel(int64_t* offsets) { offsets[0] = 0; }
+
+// ------------------------------------------------------------------------------------------------ moment tiles
+constexpr int TILE3 = 256;
+
+struct Mom3Args {
+    const int64_t* offsets;
+    const int32_t* faces;
+    int64_t capacity;
+    const double* chi;
+    const double* pos;
+    const double* sup;
+    int n;
+    const double* frec;
+    int nf;
+    int flags;
+    double* out;
+};
+
+// A workgroup owns TILE3 consecutive particles.  Their (particle, face) pairs are packed densely over the lanes
+// (pair j -> its particle by a binary search of the tile's offsets in LDS); every lane evaluates one pair, the 23
+// face accumulators go through LDS and each particle's lane adds its own pairs in face order (deterministic sums,
+// independent of the packing).
+template <int KID, int FORM>
+__global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
+    using BlockScan = hipcub::BlockScan<int, TILE3>;
+    __shared__ typename BlockScan::TempStorage scan_tmp;
+    __shared__ int s_off[TILE3 + 1];
+    __shared__ double s_p[4][TILE3];
+    __shared__ double s_res[m3::NACC][TILE3];
+    const int tid = threadIdx.x;
+    const int row = blockIdx.x * TILE3 + tid;
+    const bool live = row < a.n;
+    int64_t first = 0;
+    int cnt = 0;
+    double h = 1.0;
+    if (live) {
+        first = a.offsets[row];
+        cnt = (int)(a.offsets[row + 1] - first);
+        s_p[0][tid] = a.pos[3 * (int64_t)row];
+        s_p[1][tid] = a.pos[3 * (int64_t)row + 1];
+        s_p[2][tid] = a.pos[3 * (int64_t)row + 2];
+        h = a.sup[row];
+        s_p[3][tid] = 1.0 / h;
+    }
+    int off, total;
+    BlockScan(scan_tmp).ExclusiveSum(cnt, off, total);
+    s_off[tid] = off;
+    if (tid == 0) s_off[TILE3] = total;
+    __syncthreads();
+    const int64_t tile_first = blockIdx.x * (int64_t)TILE3 < a.n ? a.offsets[blockIdx.x * (int64_t)TILE3] : 0;
+    double acc[m3::NACC];
+#pragma unroll
+    for (int k = 0; k < m3::NACC; ++k) acc[k] = 0.0;
+    for (int base = 0; base < total; base += TILE3) {
+        const int j = base + tid;
+        double c[m3::NACC];
+#pragma unroll
+        for (int k = 0; k < m3::NACC; ++k) c[k] = 0.0;
+        if (j < total) {
+            int lo = 0, hi = TILE3;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            const int64_t slot = tile_first + j;
+            const int f = slot < a.capacity ? a.faces[slot] : -1;
+            if (f >= 0 && f < a.nf)
+                m3::face_terms<KID, FORM>(a.frec + m3::FREC * (int64_t)f, s_p[0][lo], s_p[1][lo], s_p[2][lo], s_p[3][lo], c);
+        }
+#pragma unroll
+        for (int k = 0; k < m3::NACC; ++k) s_res[k][tid] = c[k];
+        __syncthreads();
+        const int b0 = max(off, base), b1 = min(off + cnt, base + TILE3);
+        for (int jj = b0; jj < b1; ++jj) {
+#pragma unroll
+            for (int k = 0; k < m3::NACC; ++k) acc[k] += s_res[k][jj - base];
+        }
+        __syncthreads();
+    }
+    if (live) {
+        double o[m3::NOUT];
+        const double chi = (a.flags & WSB_OUT_NO_INDICATOR) ? 0.0 : a.chi[row];
+        m3::finalize22<KID>(acc, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, o);
+        double* dst = a.out + (int64_t)row * m3::NOUT;
+        if (a.flags & WSB_OUT_ACCUMULATE) {
+#pragma unroll
+            for (int k = 0; k < m3::NOUT; ++k) dst[k] += o[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < m3::NOUT; ++k) dst[k] = o[k];
+        }
+    }
+}
+
+template <int KID, int FORM>
+int launch22(const Mom3Args& a, hipStream_t s) {
+    hipLaunchKernelGGL((tile22_kernel<KID, FORM>), dim3((a.n + TILE3 - 1) / TILE3), dim3(TILE3), 0, s, a);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+template <int FORM>
+int dispatch22(int kid, const Mom3Args& a, hipStream_t s) {
+    switch (kid) {
+        case WSB_CUBIC: return launch22<WSB_CUBIC, FORM>(a, s);
+        case WSB_W2: return launch22<WSB_W2, FORM>(a, s);
+        case WSB_W4: return launch22<WSB_W4, FORM>(a, s);
+        case WSB_W6: return launch22<WSB_W6, FORM>(a, s);
+    }
+    return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kid);
+}
+
+template <int KID, int FORM>
+void host_pairs(const double* frec, const int32_t* pf, const double* pq, const double* ph, int P, double* out) {
+    for (int j = 0; j < P; ++j) {
+        double c[m3::NACC];
+        for (int k = 0; k < m3::NACC; ++k) c[k] = 0.0;
+        m3::face_terms<KID, FORM>(frec + m3::FREC * (int64_t)pf[j], pq[3 * j], pq[3 * j + 1], pq[3 * j + 2], 1.0 / ph[j], c);
+        for (int k = 0; k < m3::NACC; ++k) out[(int64_t)j * m3::NACC + k] = c[k];
+    }
+}
+
+template <int KID>
+void host_finalize(const double* acc, const double* chi, const double* h, int n, int flags, double* out) {
+    for (int i = 0; i < n; ++i) m3::finalize22<KID>(acc + (int64_t)i * m3::NACC, chi[i], h[i], (flags & WSB_OUT_PHYSICAL) != 0, out + (int64_t)i * m3::NOUT);
+}
+
+}  // namespace
+
+// ================================================================================================= C ABI
+extern "C" int wsb_mesh3d_create(const double* verts, int nv, const int32_t* faces, int nf, double support_max, int flags,
+                                 wsb_mesh3d** out, int64_t* info) {
+    WSB_REQUIRE(verts && faces && out, WSB_EINVAL, "null argument");
+    WSB_REQUIRE(nv >= 3 && nf >= 1, WSB_EINVAL, "closed surface needs faces (nv=%d nf=%d)", nv, nf);
+    WSB_REQUIRE(support_max > 0 && std::isfinite(support_max), WSB_EINVAL, "support_max must be positive and finite");
+    for (int64_t i = 0; i < 3 * (int64_t)nv; ++i) WSB_REQUIRE(std::isfinite(verts[i]), WSB_EINVAL, "non-finite vertex");
+    for (int f = 0; f < nf; ++f) {
+        const int32_t* ix = faces + 3 * (int64_t)f;
+        for (int k = 0; k < 3; ++k) WSB_REQUIRE(ix[k] >= 0 && ix[k] < nv, WSB_EINVAL, "invalid triangle indices (face %d)", f);
+        WSB_REQUIRE(ix[0] != ix[1] && ix[1] != ix[2] && ix[0] != ix[2], WSB_EINVAL, "invalid triangle indices (face %d)", f);
+    }
+    const bool soup = (flags & WSB_MESH_SOUP) != 0;
+    if (!soup) {
+        const int st = validate_closed(verts, nv, faces, nf);
+        if (st) return st;
+    }
+    auto* m = new wsb_mesh3d();
+    m->host_only = (flags & WSB_MESH_HOST_ONLY) != 0;
+    m->soup = soup;
+    m->nv = nv;
+    m->nf = nf;
+    m->support_max = support_max;
+    int st = build_host(m, verts, nv, faces, nf, support_max);
+    if (st) {
+        delete m;
+        return st;
+    }
+    if (soup) m->hg.cwalk = nullptr;
+    m->dg = m->hg;
+    if (!m->host_only) {
+        if (hipGetDevice(&m->device) != hipSuccess) {
+            delete m;
+            return wsb::set_error(WSB_EDEVICE, "no HIP device");
+        }
+        if (!st) st = upload(&m->d_frec, m->frec, &m->bytes);
+        if (!st) st = upload(&m->d_fbox, m->fbox, &m->bytes);
+        if (!st) st = upload(&m->d_cstart, m->cstart, &m->bytes);
+        if (!st) st = upload(&m->d_citems, m->citems, &m->bytes);
+        if (!st) st = upload(&m->d_cchi, m->cchi, &m->bytes);
+        if (!st) st = upload(&m->d_cwalk, m->cwalk, &m->bytes);
+        if (st) {
+            wsb_mesh3d_destroy(m);
+            return st;
+        }
+        m->dg.frec = m->d_frec;
+        m->dg.fbox = m->d_fbox;
+        m->dg.cstart = m->d_cstart;
+        m->dg.citems = m->d_citems;
+        m->dg.cchi = m->d_cchi;
+        m->dg.cwalk = soup ? nullptr : m->d_cwalk;
+    }
+    *out = m;
+    if (info) wsb_mesh3d_info(m, info);
+    return WSB_OK;
+}
+
+extern "C" int wsb_mesh3d_destroy(wsb_mesh3d* m) {
+    if (!m) return WSB_OK;
+    for (void* p : {(void*)m->d_frec, (void*)m->d_fbox, (void*)m->d_cstart, (void*)m->d_citems, (void*)m->d_cchi, (void*)m->d_cwalk})
+        if (p) (void)hipFree(p);
+    delete m;
+    return WSB_OK;
+}
+
+extern "C" int wsb_mesh3d_info(const wsb_mesh3d* m, int64_t* info) {
+    WSB_REQUIRE(m && info, WSB_EINVAL, "null argument");
+    info[0] = m->K;
+    info[1] = m->hg.n[0];
+    info[2] = m->hg.n[1];
+    info[3] = m->hg.n[2];
+    info[4] = m->nf;
+    info[5] = m->nitems;
+    info[6] = m->bytes;
+    info[7] = m->ncomp;
+    info[8] = m->ndirty;
+    return WSB_OK;
+}
+
+extern "C" int wsb_mesh3d_grid(const wsb_mesh3d* m, double* lo_cell) {
+    WSB_REQUIRE(m && lo_cell, WSB_EINVAL, "null argument");
+    lo_cell[0] = m->hg.lo[0];
+    lo_cell[1] = m->hg.lo[1];
+    lo_cell[2] = m->hg.lo[2];
+    lo_cell[3] = m->hg.cell;
+    return WSB_OK;
+}
+
+extern "C" size_t wsb_adjacency3d_scratch_bytes(int n) {
+    if (n <= 0) return 0;
+    size_t tmp = 0;
+    (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp, (const int64_t*)nullptr, (int64_t*)nullptr, n);
+    return ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255) + tmp + 256;
+}
+
+extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, int n, int64_t* offsets,
+                               int32_t* faces, int64_t capacity, double* chi, void* scratch, size_t scratch_bytes, void* stream) {
+    WSB_REQUIRE(m, WSB_EINVAL, "null mesh");
+    WSB_REQUIRE(!m->host_only, WSB_EINVAL, "host-only mesh has no device tables");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    WSB_REQUIRE(offsets, WSB_EINVAL, "null offsets");
+    hipStream_t s = wsb::as_stream(stream);
+    hipLaunchKernelGGL(offsets_head_kernel, dim3(1), dim3(1), 0, s, offsets);
+    WSB_HIP(hipGetLastError());
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(pos && sup, WSB_EINVAL, "null buffer");
+    const size_t need = wsb_adjacency3d_scratch_bytes(n);
+    WSB_REQUIRE(scratch && scratch_bytes >= need, WSB_ENOMEM, "adjacency needs %zu scratch bytes (got %zu)", need, scratch_bytes);
+    int64_t* counts = static_cast<int64_t*>(scratch);
+    char* tmp = static_cast<char*>(scratch) + ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255);
+    size_t tmp_bytes = scratch_bytes - ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255);
+    Adj3Args a{m->dg, pos, sup, n, counts, offsets, faces, capacity, chi};
+    const dim3 grid((n + 255) / 256), block(256);
+    hipLaunchKernelGGL(adj3_kernel<false>, grid, block, 0, s, a);
+    WSB_HIP(hipGetLastError());
+    WSB_HIP(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, counts, offsets + 1, n, s));
+    if (faces && capacity > 0) {
+        hipLaunchKernelGGL(adj3_kernel<true>, grid, block, 0, s, a);
+        WSB_HIP(hipGetLastError());
+    }
+    return WSB_OK;
+}
+
+extern "C" int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets, const int32_t* faces,
+                             int64_t capacity, const double* chi, const double* pos, const double* sup, int n, int flags, double* out,
+                             void* stream) {
+    WSB_REQUIRE(m, WSB_EINVAL, "null mesh");
+    WSB_REQUIRE(!m->host_only, WSB_EINVAL, "host-only mesh has no device tables");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(offsets && pos && sup && out, WSB_EINVAL, "null buffer");
+    WSB_REQUIRE(chi || (flags & WSB_OUT_NO_INDICATOR), WSB_EINVAL, "winding numbers required");
+    Mom3Args a{offsets, faces, faces ? capacity : 0, chi, pos, sup, n, m->d_frec, m->nf, flags, out};
+    hipStream_t s = wsb::as_stream(stream);
+    switch (form) {
+        case WSB_FORM_FP64: return dispatch22<m3::FP64>(kernel_id, a, s);
+        case WSB_FORM_MIXED: return dispatch22<m3::MIXED>(kernel_id, a, s);
+    }
+    return wsb::set_error(WSB_ENOTSUP, "3-D moments support the fp64 and mixed forms (form %d)", form);
+}
+
+// ------------------------------------------------------------------------------------------------ host evaluation
+extern "C" int wsb_adjacency3d_host(const wsb_mesh3d* m, const double* pos, const double* sup, int n, int64_t* offsets,
+                                    int32_t* faces, int64_t capacity, double* chi) {
+    WSB_REQUIRE(m && offsets, WSB_EINVAL, "null argument");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    offsets[0] = 0;
+    const m3::Grid3& g = m->hg;
+    for (int i = 0; i < n; ++i) {
+        const double px = pos[3 * (int64_t)i], py = pos[3 * (int64_t)i + 1], pz = pos[3 * (int64_t)i + 2], h = sup[i];
+        int c[3];
+        int64_t cnt = 0;
+        double w = 0.0;
+        if (m3::cell_of(g, px, py, pz, c)) {
+            const int64_t cid = m3::cell_id(g, c[0], c[1], c[2]);
+            for (int it = g.cstart[cid]; it < g.cstart[cid + 1]; ++it) {
+                const int f = g.citems[it];
+                const double* bx = g.fbox + 6 * (int64_t)f;
+                if (box_far(bx, px, py, pz, h * (1.0 + 1e-9)) || !keep_face(bx, px, py, pz, h)) continue;
+                if (faces && offsets[i] + cnt < capacity) faces[offsets[i] + cnt] = f;
+                ++cnt;
+            }
+            if (!m->soup && !m3::ray_winding(g, px, py, pz, c, &w)) w = exact_winding(g.frec, g.nf, px, py, pz, 1.0 / h);
+        }
+        offsets[i + 1] = offsets[i] + cnt;
+        if (chi) chi[i] = w;
+    }
+    return WSB_OK;
+}
+
+extern "C" int wsb_winding3d_exact_host(const wsb_mesh3d* m, const double* pos, int n, double* chi) {
+    WSB_REQUIRE(m && pos && chi, WSB_EINVAL, "null argument");
+    for (int i = 0; i < n; ++i)
+        chi[i] = exact_winding(m->hg.frec, m->nf, pos[3 * (int64_t)i], pos[3 * (int64_t)i + 1], pos[3 * (int64_t)i + 2], 1.0);
+    return WSB_OK;
+}
+
+template <int FORM>
+int host_pairs_form(int kernel_id, const double* frec, const int32_t* pf, const double* pq, const double* ph, int P, double* out) {
+    switch (kernel_id) {
+        case WSB_CUBIC: host_pairs<WSB_CUBIC, FORM>(frec, pf, pq, ph, P, out); return WSB_OK;
+        case WSB_W2: host_pairs<WSB_W2, FORM>(frec, pf, pq, ph, P, out); return WSB_OK;
+        case WSB_W4: host_pairs<WSB_W4, FORM>(frec, pf, pq, ph, P, out); return WSB_OK;
+        case WSB_W6: host_pairs<WSB_W6, FORM>(frec, pf, pq, ph, P, out); return WSB_OK;
+    }
+    return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
+}
+
+extern "C" int wsb_face_terms3d_host(const wsb_mesh3d* m, int kernel_id, int form, const int32_t* pair_face,
+                                     const double* pair_pos, const double* pair_sup, int P, double* out) {
+    WSB_REQUIRE(m && pair_face && pair_pos && pair_sup && out, WSB_EINVAL, "null argument");
+    for (int j = 0; j < P; ++j) WSB_REQUIRE(pair_face[j] >= 0 && pair_face[j] < m->nf, WSB_EINVAL, "face index out of range");
+    if (form == WSB_FORM_FP64) return host_pairs_form<m3::FP64>(kernel_id, m->hg.frec, pair_face, pair_pos, pair_sup, P, out);
+    if (form == WSB_FORM_MIXED) return host_pairs_form<m3::MIXED>(kernel_id, m->hg.frec, pair_face, pair_pos, pair_sup, P, out);
+    return wsb::set_error(WSB_ENOTSUP, "3-D face terms support the fp64 and mixed forms (form %d)", form);
+}
+
+extern "C" int wsb_finalize22_host(int kernel_id, const double* acc, const double* chi, const double* sup, int n, int flags,
+                                   double* out) {
+    WSB_REQUIRE(acc && chi && sup && out, WSB_EINVAL, "null argument");
+    switch (kernel_id) {
+        case WSB_CUBIC: host_finalize<WSB_CUBIC>(acc, chi, sup, n, flags, out); return WSB_OK;
+        case WSB_W2: host_finalize<WSB_W2>(acc, chi, sup, n, flags, out); return WSB_OK;
+        case WSB_W4: host_finalize<WSB_W4>(acc, chi, sup, n, flags, out); return WSB_OK;
+        case WSB_W6: host_finalize<WSB_W6>(acc, chi, sup, n, flags, out); return WSB_OK;
+    }
+    return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
+}
