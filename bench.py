@@ -153,6 +153,75 @@ def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, see
     return st
 
 
+def mesh3d_measure(torch, dev, steps, warmup):
+    """the 3-D mesh path on the 81,920-face sphere cavity workload: adjacency rebuild + the 22 stored moments (w4)."""
+    from warpsphboundaries_amd import workloads as WL
+    from warpsphboundaries_amd.mesh3d import Mesh3D
+    log("generating the 3-D workload ...")
+    w = WL.config3()
+    N = len(w.positions)
+    t0 = time.perf_counter()
+    mesh = Mesh3D(w.vertices, w.faces, w.h, device=dev)
+    t_build = time.perf_counter() - t0
+    pos = torch.tensor(w.positions, dtype=torch.float64, device=dev)
+    sup = torch.tensor(w.supports, dtype=torch.float64, device=dev)
+    adj = mesh.adjacency(pos, sup)
+    cap = adj.faces.numel()
+    out = torch.empty((N, 22), dtype=torch.float64, device=dev)
+    kern = w.kernels[0]
+
+    def step(form):
+        a = mesh.adjacency(pos, sup, capacity=cap)
+        mesh.moments(a, pos, sup, kern, form, out=out)
+
+    res = {}
+    for form in ("fp64", "mixed"):
+        for _ in range(warmup):
+            step(form)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(form)
+        torch.cuda.synchronize()
+        res[form] = (time.perf_counter() - t0) / steps
+        res[form + "_out"] = out.clone()
+        log(f"3-D {form}: {res[form] * 1e3:.3f} ms/step")
+    stream = torch.cuda.current_stream()
+    per = {}
+    for name, fn in (("adjacency", lambda: mesh.adjacency(pos, sup, capacity=cap)),
+                     ("moments_fp64", lambda: mesh.moments(adj, pos, sup, kern, "fp64", out=out)),
+                     ("moments_mixed", lambda: mesh.moments(adj, pos, sup, kern, "mixed", out=out))):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        per[name] = e0.elapsed_time(e1) / 1e3 / steps
+    ref = res["fp64_out"]
+    scale = ref.abs().amax(0).clamp(min=1e-300)
+    err = float(((res["mixed_out"] - ref).abs().amax(0) / scale).max())
+    chi = adj.chi
+    evals = 22 * N
+    # algorithmic bytes of the moment launch: positions, support, offsets, winding number in; 22 float64 out;
+    # the face list and the face records of every pair
+    bytes_mom = N * (24 + 8 + 8 + 8 + 22 * 8) + adj.count * (4 + 36 * 8)
+    return {
+        "workload": f"{w.name} (box [0,1]^3 minus a level-6 icosphere of radius 0.25), {N:,} particles, w4, h=0.01; "
+                    "step = face adjacency + winding rebuild + 22 stored moments",
+        "N": N, "faces": int(len(w.faces)), "pairs": int(adj.count), "pairs_per_particle": adj.count / N,
+        "winding_values": sorted({float(v) for v in torch.unique(chi).cpu().tolist()}),
+        "static_build_s": t_build, "grid": list(mesh.grid_shape), "K": mesh.K,
+        "ms_per_step": {"fp64": res["fp64"] * 1e3, "mixed": res["mixed"] * 1e3},
+        "evals_per_s": {"fp64": evals / res["fp64"], "mixed": evals / res["mixed"]},
+        "launch_ms": {k: v * 1e3 for k, v in per.items()},
+        "moments_hbm_GBps_algorithmic": bytes_mom / per["moments_fp64"] / 1e9,
+        "moments_algorithmic_bytes": bytes_mom,
+        "max_signal_relative_error_mixed_vs_fp64": err,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +232,8 @@ def main():
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--only-3d", action="store_true", help="run the 3-D section alone and print it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the wall-integral path shards by particle ranges with replicated geometry "
@@ -175,6 +246,9 @@ def main():
     from warpsphboundaries_amd import _lib as L
 
     dev = torch.device("cuda:0")
+    if args.only_3d:
+        print(json.dumps(mesh3d_measure(torch, dev, args.steps, args.warmup)), flush=True)
+        return
     log("generating the star workload ...")
     work = WL.config2(n=args.n)
     loop, solid = work.loops[0]
@@ -311,6 +385,9 @@ def main():
         "served_latency": None,
         "roofline": roof,
     }
+    if not args.no_3d:
+        line["mesh3d"] = mesh3d_measure(torch, dev, args.steps, args.warmup)
+        log("3-D:", line["mesh3d"])
     if not args.no_serve:
         log("served workload ...")
         served = serve_measure("mixed", args.qps, graphs=False)

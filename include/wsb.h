@@ -145,12 +145,21 @@ int wsb_mesh3d_grid(const wsb_mesh3d* m, double* lo_cell);
 
 /* ---- face adjacency + winding number (reference fluxbridge/device_search.py:68-195 count / fill / sort with
  * each triangle an element; winding number = the solid angle sum of mesh3d/reference.py:255-258 / 4 pi).
- * pos[n*3], sup[n] (<= support_max).  offsets[n+1] (CSR, int64) is always written; faces[capacity] receives the
+ * pos[n*3], sup[n] (<= support_max); order[n] nullable (wsb_sort3d).  offsets[n+1] (CSR, int64) is always written; faces[capacity] receives the
  * kept faces of each particle in ascending order (entries past `capacity` are dropped: the caller compares
  * offsets[n] with it); chi[n] (nullable) the winding number.  scratch: wsb_adjacency3d_scratch_bytes(n) bytes. */
 size_t wsb_adjacency3d_scratch_bytes(int n);
-int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, int n, int64_t* offsets, int32_t* faces,
-                    int64_t capacity, double* chi, void* scratch, size_t scratch_bytes, void* stream);
+int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, const int32_t* order, int n,
+                    int64_t* offsets, int32_t* faces, int64_t capacity, double* chi, void* scratch, size_t scratch_bytes,
+                    void* stream);
+
+/* ---- particles grouped by grid cell (device counting sort of the cell keys; no counterpart in the reference):
+ * order[n] is a permutation of 0..n-1: first the particles whose cell has candidate faces or faces of its own,
+ * grouped by cell, then every other particle in input order.  Passed to wsb_adjacency3d it changes which thread
+ * serves a particle, never a result. */
+size_t wsb_sort3d_scratch_bytes(const wsb_mesh3d* m, int n);
+int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t* order, void* scratch, size_t scratch_bytes,
+               void* stream);
 
 /* ---- the 22 stored moments per particle from the adjacency (reference fluxbridge/evaluate.py:158-175 batch +
  * reduce, fluxbridge/wp_func.py:260-351 element in compact mode): out[n*22] = sum over the particle's faces of

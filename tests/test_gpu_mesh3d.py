@@ -187,3 +187,22 @@ def test_evaluator_and_element_api():
     assert np.allclose(g, ref[16:19], atol=1e-12)
     with pytest.raises(NotImplementedError):
         tet.moment(c["kernel"], (3, 0, 0))
+
+
+def test_cell_sorted_schedule_changes_nothing():
+    V, F = _cavity(3)
+    h = 0.05
+    P = _points(3000, seed=11)
+    m = MM.Mesh3D(V, F, h, device=DEV)
+    Q = torch.tensor(P, device=DEV)
+    H = torch.full((len(P),), h, dtype=torch.float64, device=DEV)
+    order = m.sort(Q)
+    assert torch.equal(torch.sort(order.long()).values, torch.arange(len(P), device=DEV))
+    a = m.adjacency(Q, H, sort=True)
+    b = m.adjacency(Q, H, sort=False)
+    assert a.count == b.count
+    assert torch.equal(a.offsets, b.offsets) and torch.equal(a.faces[: a.count], b.faces[: b.count])
+    assert torch.equal(a.chi, b.chi)
+    ma = m.moments(a, Q, H, "w4", "fp64")
+    mb = m.moments(b, Q, H, "w4", "fp64")
+    assert torch.equal(ma, mb)

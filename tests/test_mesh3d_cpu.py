@@ -1,0 +1,272 @@
+"""3-D closed-surface path on the CPU: the oracle pinned by the reference's 70-digit fixtures and by closed forms, and
+the device arithmetic (compiled for the host by the same library) checked against both.
+
+Fixtures (copied data): tests/golden/fluxbridge-v1.json (tetrahedra and an open triangle, baseline(3) requests plus
+degree-9 moments and their derivatives) and tests/golden/fluxbridge-nearplane-v1.json (queries 1e-16 from faces,
+edges and vertices).  Closed forms: whole-ball moments (Gamma-function angular factors), the half-space mass
+(sphere.half_space_exact), the cavity identity outer - void.  The adjacency is compared with the brute-force element
+test, the winding number with the solid-angle sum.
+"""
+import json
+import os
+from math import gamma
+
+import numpy as np
+import pytest
+
+from oracle import mesh3d as OM
+from warpsphboundaries_amd import mesh3d as MM
+from warpsphboundaries_amd.plan3d import H_POWER3, REQUESTS22, ball_mass, ball_second_moment, whole_vector3
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+KERNELS = ["cubic", "w2", "w4", "w6"]
+
+
+def _load(fn):
+    return json.load(open(os.path.join(GOLD, fn)))["cases"]
+
+
+def _vq(c):
+    V = np.array([[float(x) for x in v] for v in c["vertices"]])
+    q = np.array([float(x) for x in c["query"]])
+    return V, q
+
+
+def _req(c):
+    return [(tuple(r["alpha"]), tuple(r["derivative"])) for r in c["requests"]]
+
+
+def _ref(c):
+    return np.array([float(v) for v in c["values"]])
+
+
+# ------------------------------------------------------------------------------------------------ oracle pins
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_oracle_tetrahedra_fixtures(kernel):
+    for c in _load("fluxbridge-v1.json"):
+        if c["dimension"] != 3 or c["measure"] != "volume" or c["kernel"] != kernel:
+            continue
+        V, q = _vq(c)
+        req, ref = _req(c), _ref(c)
+        if len(req) > 100:                      # the degree-9 coverage cases: every 7th request
+            idx = np.arange(0, len(req), 7)
+            req, ref = [req[i] for i in idx], ref[idx]
+        got = OM.closed_surface_requests(V, OM.tetrahedron_faces(V), q[None], kernel, float(c["h"]), req)[0]
+        err = np.abs(got - ref) / (1 + np.abs(ref))
+        assert err.max() <= 5e-12, (c["query"], err.max(), req[int(err.argmax())])
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_oracle_open_triangle_fixtures(kernel):
+    for c in _load("fluxbridge-v1.json"):
+        if c["measure"] != "surface" or c["kernel"] != kernel:
+            continue
+        V, q = _vq(c)
+        req = [r for r in _req(c) if len(r[1]) < 2]
+        ref = _ref(c)[[i for i, r in enumerate(_req(c)) if len(r[1]) < 2]]
+        got = OM.triangle_requests(V, q[None], kernel, float(c["h"]), req)[0]
+        assert (np.abs(got - ref) <= 1e-12 * (1 + np.abs(ref))).all()
+
+
+def test_oracle_near_surface_fixtures_both_forms():
+    base = list(REQUESTS22)
+    for c in _load("fluxbridge-nearplane-v1.json"):
+        V, q = _vq(c)
+        F = OM.tetrahedron_faces(V)
+        ref = _ref(c)[:22]
+        got = OM.closed_surface_requests(V, F, q[None], c["kernel"], float(c["h"]), base)[0]
+        assert (np.abs(got - ref) <= 1e-11 * (1 + np.abs(ref))).all(), c["label"]
+        # the compact form with the winding number of the point (integer off the surface)
+        w = OM.snapped_winding(V, F, q[None])
+        if "near_edge" in c["label"]:                # the corner-vector formula breaks down 1e-16 from an edge
+            w = np.array([1.0 if "inside" in c["label"] else 0.0])
+        got = OM.closed_surface_requests(V, F, q[None], c["kernel"], float(c["h"]), base, winding=w)[0]
+        assert (np.abs(got - ref) <= 5e-12 * (1 + np.abs(ref))).all(), c["label"]
+
+
+def _cube(r):
+    V, F = MM.box_surface((-r, -r, -r), (r, r, r))
+    return V, F
+
+
+def _ball_moment(kernel, alpha):
+    """int_ball y^alpha W dV at unit support (Gamma-function angular factor times the radial integral)."""
+    if any(a % 2 for a in alpha):
+        return 0.0
+    ang = 2 * np.prod([gamma((a + 1) / 2) for a in alpha]) / gamma((sum(alpha) + 3) / 2)
+    rad = sum(c * R ** (n + sum(alpha) + 3) / (n + sum(alpha) + 3) for n, R, c in OM.profile(kernel, 1.0))
+    return ang * rad
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_oracle_whole_support_closed_forms(kernel):
+    V, F = _cube(5.0)
+    x = np.zeros((1, 3))
+    alphas = OM.monomials(3, 4)
+    req = [(a, ()) for a in alphas] + [(a, (j,)) for a in OM.monomials(3, 2) for j in range(3)]
+    got = OM.closed_surface_requests(V, F, x, kernel, 1.0, req)[0]
+    ref = []
+    for a, d in req:
+        if not d:
+            ref.append(_ball_moment(kernel, a))
+        else:
+            j = d[0]
+            ref.append(a[j] * _ball_moment(kernel, tuple(t - int(k == j) for k, t in enumerate(a))) if a[j] else 0.0)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+    assert abs(_ball_moment(kernel, (0, 0, 0)) - float(ball_mass(kernel))) < 1e-13
+    assert abs(_ball_moment(kernel, (2, 0, 0)) - float(ball_second_moment(kernel))) < 1e-13
+
+
+def test_oracle_cavity_identity_and_half_space():
+    from warpsphboundaries_amd import sphere as S
+    Vo, Fo = _cube(3.0)
+    Vi, Fi = _cube(0.4)
+    V = np.vstack([Vo, Vi])
+    F = np.vstack([Fo, (Fi + len(Vo))[:, ::-1]])
+    q = np.array([[0, 0, 0], [0.4, 0, 0], [0.4, 0.4, 0], [0.45, 0.1, -0.2]])
+    req = list(REQUESTS22)
+    both = OM.closed_surface_requests(V, F, q, "w2", 1.0, req)
+    outer = OM.closed_surface_requests(Vo, Fo, q, "w2", 1.0, req)
+    void = OM.closed_surface_requests(Vi, Fi, q, "w2", 1.0, req)
+    np.testing.assert_allclose(both, outer - void, rtol=1e-12, atol=1e-12)
+    Vb, Fb = _cube(10.0)
+    for k in KERNELS:
+        for d in (0.9, 0.3, 1e-3):
+            got = OM.closed_surface_requests(Vb, Fb, np.array([[0.0, 0.0, 10.0 - d]]), k, 1.0, [((0, 0, 0), ())])[0, 0]
+            assert abs(got - float(S.half_space_exact(k, -d))) < 1e-12
+
+
+def test_oracle_predicates_restate_the_reference():
+    V, F = _cube(1.0)
+    T = V[F]
+    B = OM.element_bounds(T)
+    x = np.array([1.5, 0.0, 0.0])
+    # exactly at distance h on an axis: kept by both forms (<= 1 and <= h^2)
+    keep = OM.device_keep(B, x, 0.5)
+    assert keep.any() and not keep.all()
+    np.testing.assert_array_equal(keep, OM.host_keep(B, x, 0.5))
+
+
+# ------------------------------------------------------------------------------------------------ device arithmetic on the host
+def _fixture_cases():
+    out = []
+    for fn, lim in (("fluxbridge-v1.json", 44), ("fluxbridge-nearplane-v1.json", 10 ** 6)):
+        for ci, c in enumerate(_load(fn)):
+            if ci < lim and c.get("dimension", 3) == 3 and c.get("measure", "volume") == "volume":
+                out.append(c)
+    return out
+
+
+@pytest.mark.parametrize("form,tol", [("fp64", 1e-12), ("mixed", 1e-10)])
+def test_host_closed_form_matches_fixtures(form, tol):
+    worst = 0.0
+    for c in _fixture_cases():
+        V, q = _vq(c)
+        m = MM.Mesh3D(V, OM.tetrahedron_faces(V), float(c["h"]), host_only=True)
+        got = m.evaluate_host(q[None], float(c["h"]), c["kernel"], form=form)[0]
+        ref = _ref(c)[:22]
+        err = np.abs(got - ref) / (1 + np.abs(ref))
+        worst = max(worst, err.max())
+        assert err.max() <= tol, (c.get("label", c["query"]), c["kernel"], err.max())
+    assert worst > 0
+
+
+def test_numpy_model_of_the_regrouping_matches_fixtures():
+    import cpu_models3d as CM
+    for c in _fixture_cases():
+        V, q = _vq(c)
+        F = OM.tetrahedron_faces(V)
+        chi = OM.winding(V, F, q[None])
+        if "near_edge" in c.get("label", ""):
+            chi = np.array([1.0 if "inside" in c["label"] else 0.0])
+        got = CM.closed_surface22(V, F, q[None], float(c["h"]), c["kernel"], chi)[0]
+        ref = _ref(c)[:22]
+        assert (np.abs(got - ref) <= 1e-12 * (1 + np.abs(ref))).all(), (c.get("label"), c["kernel"])
+
+
+def _cavity(level=2):
+    Vb, Fb = MM.box_surface((0, 0, 0), (1, 1, 1))
+    Vs, Fs = MM.icosphere(level, 0.25, (0.5, 0.5, 0.5))
+    return np.vstack([Vb, Vs]), np.vstack([Fb, (Fs + len(Vb))[:, ::-1]])
+
+
+def _points(n, seed=1):
+    rng = np.random.default_rng(seed)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    p1 = 0.5 + dirs * (0.25 + rng.uniform(-0.06, 0.06, n))[:, None]
+    p2 = rng.uniform(0, 1, (n, 3))
+    p2[:, 0] = rng.uniform(-0.03, 0.05, n)
+    p3 = rng.uniform(-0.05, 1.05, (n, 3))
+    return np.vstack([p1, p2, p3])
+
+
+def test_host_adjacency_is_the_brute_force_element_test():
+    V, F = _cavity(2)
+    h = 0.08
+    P = _points(300)
+    # points exactly h from a face box along an axis, and just beyond
+    B = OM.element_bounds(V[F])
+    extra = []
+    for f in range(0, len(F), 37):
+        lo, hi = B[f]
+        extra.append([hi[0] + h, 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])])
+        extra.append([np.nextafter(hi[0] + h, 9.0), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])])
+    P = np.vstack([P, np.array(extra)])
+    H = np.full(len(P), h)
+    H[::5] *= 0.7
+    m = MM.Mesh3D(V, F, h, host_only=True)
+    off, faces, chi = m.adjacency_host(P, H)
+    cols, offs = OM.face_adjacency(V[F], P, H)
+    np.testing.assert_array_equal(off, offs)
+    np.testing.assert_array_equal(faces, cols)
+
+
+def test_host_winding_numbers():
+    V, F = _cavity(2)
+    m = MM.Mesh3D(V, F, 0.08, host_only=True)
+    P = _points(400, seed=3)
+    chi = m.adjacency_host(P, 0.08)[2]
+    np.testing.assert_allclose(chi, OM.winding(V, F, P), rtol=0, atol=1e-12)
+    assert set(np.unique(chi)) <= {0.0, 1.0}
+    Vb, Fb = MM.box_surface((0, 0, 0), (1, 1, 1))
+    mb = MM.Mesh3D(Vb, Fb, 0.1, host_only=True)
+    on = np.array([[0.0, 0.3, 0.6], [0.0, 0.0, 0.4], [0.0, 0.0, 0.0], [0.5, 0.5, 1.0], [1e-17, 0.3, 0.6]])
+    np.testing.assert_allclose(mb.adjacency_host(on, 0.1)[2], [0.5, 0.25, 0.125, 0.5, 1.0], atol=1e-14)
+
+
+@pytest.mark.parametrize("k", KERNELS)
+def test_host_closed_form_matches_compact_oracle(k):
+    V, F = _cavity(2)
+    h = 0.08
+    P = _points(40, seed=7)
+    got = MM.Mesh3D(V, F, h, host_only=True).evaluate_host(P, h, k)
+    w = OM.snapped_winding(V, F, P)
+    ref = OM.closed_surface_requests(V, F, P, k, h, list(REQUESTS22), chunk=40, winding=w)
+    u = h ** np.array(H_POWER3, dtype=np.float64)
+    err = np.abs(got - ref) / u / (1 + np.abs(ref) / u)
+    assert err.max() <= (1e-11 if k == "w6" else 1e-12), err.max()
+
+
+def test_host_whole_support_and_far_field():
+    V, F = _cube(10.0)
+    m = MM.Mesh3D(V, F, 1.0, host_only=True)
+    for k in KERNELS:
+        out = m.evaluate_host(np.array([[0.0, 0.0, 0.0], [30.0, 0.0, 0.0]]), 1.0, k)
+        np.testing.assert_allclose(out[0], whole_vector3(k), atol=1e-14)
+        np.testing.assert_array_equal(out[1], np.zeros(22))
+
+
+def test_surface_validation():
+    V, F = MM.box_surface((0, 0, 0), (1, 1, 1))
+    with pytest.raises(ValueError, match="watertight"):
+        MM.Mesh3D(V, F[:-1], 0.1, host_only=True)
+    with pytest.raises(ValueError, match="positive signed volume"):
+        MM.Mesh3D(V, F[:, ::-1], 0.1, host_only=True)
+    with pytest.raises(ValueError, match="invalid triangle"):
+        MM.Mesh3D(V, np.vstack([F[:-1], [[0, 0, 1]]]), 0.1, host_only=True)
+    with pytest.raises(ValueError, match="support_max"):
+        MM.Mesh3D(V, F, 0.0, host_only=True)
+    soup = MM.Mesh3D(V, F[:3], 0.1, host_only=True, soup=True)
+    off, faces, chi = soup.adjacency_host(np.array([[0.05, 0.5, 0.5]]), 0.1)
+    assert (chi == 0).all() and len(faces) >= 1

@@ -43,8 +43,10 @@ SIGNATURES = {
     "wsb_mesh3d_info": (c_int, [c_void_p, POINTER(c_int64)]),
     "wsb_mesh3d_grid": (c_int, [c_void_p, POINTER(c_double)]),
     "wsb_adjacency3d_scratch_bytes": (c_size_t, [c_int]),
-    "wsb_adjacency3d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_size_t,
-                                c_void_p]),
+    "wsb_adjacency3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                c_size_t, c_void_p]),
+    "wsb_sort3d_scratch_bytes": (c_size_t, [c_void_p, c_int]),
+    "wsb_sort3d": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_moments3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int,
                               c_void_p, c_void_p]),
     "wsb_adjacency3d_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p]),

@@ -29,13 +29,19 @@ struct wsb_mesh3d {
     int K = 0, ncomp = 0;
     int64_t nitems = 0, ndirty = 0, bytes = 0;
     std::vector<double> frec, fbox, cchi;
-    std::vector<int32_t> cstart, citems, cwalk;
+    std::vector<int32_t> cstart, citems, cwalk, wdir, wstart, witems;
+    std::vector<float4> crec, wrec;
     double* d_frec = nullptr;
     double* d_fbox = nullptr;
     double* d_cchi = nullptr;
     int32_t* d_cstart = nullptr;
     int32_t* d_citems = nullptr;
+    float4* d_crec = nullptr;
+    float4* d_wrec = nullptr;
     int32_t* d_cwalk = nullptr;
+    int32_t* d_wdir = nullptr;
+    int32_t* d_wstart = nullptr;
+    int32_t* d_witems = nullptr;
 };
 
 namespace {
@@ -67,6 +73,9 @@ __host__ __device__ inline double exact_winding(const double* frec, int nf, doub
 // ------------------------------------------------------------------------------------------------ host build
 int build_host(wsb_mesh3d* m, const double* V, int nv, const int32_t* Fc, int nf, double smax) {
     const int64_t nf64 = nf;
+    // FP32 rounding toward -inf / +inf (conservative boxes of the pre-filters)
+    auto down = [](double v) { float f = (float)v; if ((double)f > v) f = std::nextafter(f, -INFINITY); return f; };
+    auto up = [](double v) { float f = (float)v; if ((double)f < v) f = std::nextafter(f, INFINITY); return f; };
     m->frec.assign((size_t)nf64 * m3::FREC, 0.0);
     m->fbox.assign((size_t)nf64 * 6, 0.0);
     double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -165,6 +174,16 @@ int build_host(wsb_mesh3d* m, const double* V, int nv, const int32_t* Fc, int nf
     }
     m->cstart.assign(cnt.begin(), cnt.end());
     m->nitems = (int64_t)m->citems.size();
+    // per-item FP32 boxes rounded outward: a conservative pre-filter read with one load per candidate
+    m->crec.resize(2 * m->citems.size());
+    for (size_t it = 0; it < m->citems.size(); ++it) {
+        const int f = m->citems[it];
+        const double* bx = m->fbox.data() + (size_t)6 * f;
+        float id;
+        std::memcpy(&id, &f, sizeof(float));
+        m->crec[2 * it] = float4{down(bx[0]), down(bx[1]), down(bx[2]), id};
+        m->crec[2 * it + 1] = float4{up(bx[3]), up(bx[4]), up(bx[5]), 0.0f};
+    }
     int K = 0;
     for (int64_t c = 0; c < ncells; ++c) K = std::max(K, m->cstart[c + 1] - m->cstart[c]);
     m->K = K;
@@ -227,18 +246,68 @@ int build_host(wsb_mesh3d* m, const double* V, int nv, const int32_t* Fc, int nf
                     }
                 }
         }
+        // per cell with faces: its walk and the faces that overlap the walk's column (cross-section of the cell,
+        // walked range along the axis), deduplicated and ascending
         int64_t ndirty = 0;
-        for (int64_t c = 0; c < ncells; ++c)
-            if (dirty[c]) {
-                m->cwalk[c] = bdir[c] | (best[c] << 3);
-                ++ndirty;
+        m->wstart.assign(1, 0);
+        std::vector<int32_t> buf;
+        for (int64_t c = 0; c < ncells; ++c) {
+            if (!dirty[c]) continue;
+            const int d = (int)ndirty++;
+            m->cwalk[c] = d;
+            const int dir = bdir[c], steps = best[c];
+            m->wdir.push_back(dir | (steps << 3));
+            const int ax = dir >> 1, sg = (dir & 1) ? -1 : 1;
+            const int k = (int)(c % g.n[2]), j = (int)((c / g.n[2]) % g.n[1]), i = (int)(c / ((int64_t)g.n[1] * g.n[2]));
+            const int ijk[3] = {i, j, k};
+            double lo3[3], hi3[3];
+            for (int a = 0; a < 3; ++a) {
+                lo3[a] = g.lo[a] + ijk[a] * cell;
+                hi3[a] = g.lo[a] + (ijk[a] + 1) * cell;
             }
+            const int jend = ijk[ax] + sg * (steps - 1);
+            lo3[ax] = g.lo[ax] + std::min(ijk[ax], jend) * cell;
+            hi3[ax] = g.lo[ax] + (std::max(ijk[ax], jend) + 1) * cell;
+            buf.clear();
+            for (int st = 0; st < steps; ++st) {
+                int q[3] = {i, j, k};
+                q[ax] += sg * st;
+                const int64_t cq = m3::cell_id(g, q[0], q[1], q[2]);
+                for (int it = m->cstart[cq]; it < m->cstart[cq + 1]; ++it) {
+                    const int f = m->citems[it];
+                    const double* bx = m->fbox.data() + (size_t)6 * f;
+                    bool hit = true;
+                    for (int a = 0; a < 3; ++a) {
+                        const double eps = 1e-9 * cell + 1e-12 * (std::fabs(lo3[a]) + std::fabs(hi3[a]));
+                        if (bx[3 + a] < lo3[a] - eps || bx[a] > hi3[a] + eps) hit = false;
+                    }
+                    if (hit) buf.push_back(f);
+                }
+            }
+            std::sort(buf.begin(), buf.end());
+            buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+            m->witems.insert(m->witems.end(), buf.begin(), buf.end());
+            m->wstart.push_back((int32_t)m->witems.size());
+            const int au = (ax + 1) % 3, av = (ax + 2) % 3;
+            for (int f : buf) {
+                const double* bx = m->fbox.data() + (size_t)6 * f;
+                float id;
+                std::memcpy(&id, &f, sizeof(float));
+                m->wrec.push_back(float4{down(bx[au]), up(bx[3 + au]), down(bx[av]), up(bx[3 + av])});
+                m->wrec.push_back(float4{down(bx[ax]), up(bx[3 + ax]), id, 0.0f});
+            }
+        }
         m->ndirty = ndirty;
     }
     g.cstart = m->cstart.data();
     g.citems = m->citems.data();
+    g.crec = m->crec.data();
     g.cchi = m->cchi.data();
     g.cwalk = m->cwalk.data();
+    g.wdir = m->wdir.data();
+    g.wstart = m->wstart.data();
+    g.witems = m->witems.data();
+    g.wrec = m->wrec.data();
     g.frec = m->frec.data();
     g.fbox = m->fbox.data();
     g.nf = nf;
@@ -290,6 +359,7 @@ struct Adj3Args {
     m3::Grid3 g;
     const double* pos;
     const double* sup;
+    const int32_t* order;       // nullable: thread i serves particle order[i] (particles grouped by cell)
     int n;
     int64_t* counts;            // count pass: per particle
     const int64_t* offsets;     // fill pass: CSR offsets
@@ -303,8 +373,9 @@ struct Adj3Args {
 // per-face sum, evaluated by the whole wavefront for each such lane in turn)
 template <bool FILL>
 __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < a.n;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < a.n;
+    const int i = live ? (a.order ? a.order[t] : t) : 0;
     double px = 0.0, py = 0.0, pz = 0.0, h = 1.0;
     if (live) {
         px = a.pos[3 * (int64_t)i];
@@ -319,11 +390,15 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
         const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
         const double reach = h * (1.0 + 1e-9);
         const int64_t base = FILL ? a.offsets[i] : 0;
-        for (int it = a.g.cstart[cid]; it < a.g.cstart[cid + 1]; ++it) {
-            const int f = a.g.citems[it];
-            const double* bx = a.g.fbox + 6 * (int64_t)f;
-            if (box_far(bx, px, py, pz, reach)) continue;
-            if (!keep_face(bx, px, py, pz, h)) continue;
+        const int it1 = a.g.cstart[cid + 1];
+        for (int it = a.g.cstart[cid]; it < it1; ++it) {
+            // one independent 32-byte load per candidate; the exact test only for boxes within h on every axis
+            const float4 l = a.g.crec[2 * (int64_t)it], u = a.g.crec[2 * (int64_t)it + 1];
+            if ((double)l.x - px > reach || px - (double)u.x > reach || (double)l.y - py > reach || py - (double)u.y > reach ||
+                (double)l.z - pz > reach || pz - (double)u.z > reach)
+                continue;
+            const int f = m3::bits_int(l.w);
+            if (!keep_face(a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
             if (FILL && base + cnt < a.capacity) a.faces[base + cnt] = f;
             ++cnt;
         }
@@ -350,6 +425,68 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
 }
 
 __global__ void offsets_head_kernel(int64_t* offsets) { offsets[0] = 0; }
+
+// counting sort of the particles that have work by grid cell: histogram, exclusive scan, scatter (the order inside
+// a cell depends on the atomics' timing; it only decides which thread serves a particle, never a result).  A
+// particle has work when its cell lists candidate faces or has faces of its own (the axis walk); every other
+// particle (outside the grid, or in a face-free cell with no face within support_max) keeps its input order after
+// the sorted ones, placed by a scan of their flags, so their loads and stores stay coalesced.
+__global__ __launch_bounds__(256) void cell_count3_kernel(m3::Grid3 g, const double* pos, int n, int32_t* key, int32_t* outflag,
+                                                          int32_t* hist) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int c[3];
+    const bool in = m3::cell_of(g, pos[3 * (int64_t)r], pos[3 * (int64_t)r + 1], pos[3 * (int64_t)r + 2], c);
+    int k = -1;
+    if (in) {
+        const int64_t cid = m3::cell_id(g, c[0], c[1], c[2]);
+        if (g.cstart[cid + 1] > g.cstart[cid] || (g.cwalk && g.cwalk[cid] >= 0)) k = (int)cid;
+    }
+    key[r] = k;
+    outflag[r] = k >= 0 ? 0 : 1;
+    if (k >= 0) atomicAdd(hist + k, 1);
+}
+
+__global__ __launch_bounds__(256) void cell_scatter3_kernel(const int32_t* key, const int32_t* outflag, const int32_t* outpos, int n,
+                                                            int32_t* cursor, int32_t* order) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int k = key[r];
+    if (k >= 0) {
+        order[atomicAdd(cursor + k, 1)] = r;
+    } else {
+        const int n_out = outpos[n - 1] + outflag[n - 1];
+        order[(n - n_out) + outpos[r]] = r;
+    }
+}
+
+__global__ void zero3_kernel(int32_t* p, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+struct Sort3Layout {
+    size_t key, flag, opos, hist, cur, tmp, tmp_bytes, total;
+};
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+Sort3Layout sort3_layout(int n, int64_t nbins) {
+    Sort3Layout L{};
+    size_t tmp = 0, tmp2 = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, (int)nbins);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, (const int32_t*)nullptr, (int32_t*)nullptr, n);
+    tmp = std::max(tmp, tmp2);
+    L.key = 0;
+    L.flag = al256(L.key + sizeof(int32_t) * (size_t)n);
+    L.opos = al256(L.flag + sizeof(int32_t) * (size_t)n);
+    L.hist = al256(L.opos + sizeof(int32_t) * (size_t)n);
+    L.cur = al256(L.hist + sizeof(int32_t) * (size_t)nbins);
+    L.tmp = al256(L.cur + sizeof(int32_t) * (size_t)nbins);
+    L.tmp_bytes = tmp;
+    L.total = al256(L.tmp + tmp);
+    return L;
+}
 
 // ------------------------------------------------------------------------------------------------ moment tiles
 constexpr int TILE3 = 256;
@@ -430,18 +567,24 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
         }
         __syncthreads();
     }
+    // finalise into LDS, then store the tile's rows as one contiguous block (coalesced across the lanes)
     if (live) {
         double o[m3::NOUT];
         const double chi = (a.flags & WSB_OUT_NO_INDICATOR) ? 0.0 : a.chi[row];
         m3::finalize22<KID>(acc, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, o);
-        double* dst = a.out + (int64_t)row * m3::NOUT;
-        if (a.flags & WSB_OUT_ACCUMULATE) {
 #pragma unroll
-            for (int k = 0; k < m3::NOUT; ++k) dst[k] += o[k];
-        } else {
-#pragma unroll
-            for (int k = 0; k < m3::NOUT; ++k) dst[k] = o[k];
-        }
+        for (int k = 0; k < m3::NOUT; ++k) s_res[k][tid] = o[k];
+    }
+    __syncthreads();
+    const int64_t r0 = blockIdx.x * (int64_t)TILE3;
+    const int nrows = (int)min((int64_t)TILE3, (int64_t)a.n - r0);
+    double* dst = a.out + r0 * m3::NOUT;
+    const bool accumulate = (a.flags & WSB_OUT_ACCUMULATE) != 0;
+    for (int idx = tid; idx < nrows * m3::NOUT; idx += TILE3) {
+        const int rr = idx / m3::NOUT, k = idx - rr * m3::NOUT;
+        const double v = s_res[k][rr];
+        if (accumulate) dst[idx] += v;
+        else dst[idx] = v;
     }
 }
 
@@ -519,8 +662,13 @@ extern "C" int wsb_mesh3d_create(const double* verts, int nv, const int32_t* fac
         if (!st) st = upload(&m->d_fbox, m->fbox, &m->bytes);
         if (!st) st = upload(&m->d_cstart, m->cstart, &m->bytes);
         if (!st) st = upload(&m->d_citems, m->citems, &m->bytes);
+        if (!st) st = upload(&m->d_crec, m->crec, &m->bytes);
         if (!st) st = upload(&m->d_cchi, m->cchi, &m->bytes);
         if (!st) st = upload(&m->d_cwalk, m->cwalk, &m->bytes);
+        if (!st) st = upload(&m->d_wdir, m->wdir, &m->bytes);
+        if (!st) st = upload(&m->d_wstart, m->wstart, &m->bytes);
+        if (!st) st = upload(&m->d_witems, m->witems, &m->bytes);
+        if (!st) st = upload(&m->d_wrec, m->wrec, &m->bytes);
         if (st) {
             wsb_mesh3d_destroy(m);
             return st;
@@ -529,8 +677,13 @@ extern "C" int wsb_mesh3d_create(const double* verts, int nv, const int32_t* fac
         m->dg.fbox = m->d_fbox;
         m->dg.cstart = m->d_cstart;
         m->dg.citems = m->d_citems;
+        m->dg.crec = m->d_crec;
         m->dg.cchi = m->d_cchi;
         m->dg.cwalk = soup ? nullptr : m->d_cwalk;
+        m->dg.wdir = m->d_wdir;
+        m->dg.wstart = m->d_wstart;
+        m->dg.witems = m->d_witems;
+        m->dg.wrec = m->d_wrec;
     }
     *out = m;
     if (info) wsb_mesh3d_info(m, info);
@@ -539,7 +692,9 @@ extern "C" int wsb_mesh3d_create(const double* verts, int nv, const int32_t* fac
 
 extern "C" int wsb_mesh3d_destroy(wsb_mesh3d* m) {
     if (!m) return WSB_OK;
-    for (void* p : {(void*)m->d_frec, (void*)m->d_fbox, (void*)m->d_cstart, (void*)m->d_citems, (void*)m->d_cchi, (void*)m->d_cwalk})
+    for (void* p : {(void*)m->d_frec, (void*)m->d_fbox, (void*)m->d_cstart, (void*)m->d_citems, (void*)m->d_crec, (void*)m->d_cchi,
+                    (void*)m->d_cwalk,
+                    (void*)m->d_wdir, (void*)m->d_wstart, (void*)m->d_witems, (void*)m->d_wrec})
         if (p) (void)hipFree(p);
     delete m;
     return WSB_OK;
@@ -568,6 +723,41 @@ extern "C" int wsb_mesh3d_grid(const wsb_mesh3d* m, double* lo_cell) {
     return WSB_OK;
 }
 
+extern "C" size_t wsb_sort3d_scratch_bytes(const wsb_mesh3d* m, int n) {
+    if (!m || n <= 0) return 0;
+    return sort3_layout(n, (int64_t)m->hg.n[0] * m->hg.n[1] * m->hg.n[2]).total;
+}
+
+extern "C" int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t* order, void* scratch, size_t scratch_bytes,
+                          void* stream) {
+    WSB_REQUIRE(m, WSB_EINVAL, "null mesh");
+    WSB_REQUIRE(!m->host_only, WSB_EINVAL, "host-only mesh has no device tables");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(pos && order, WSB_EINVAL, "null buffer");
+    const int64_t nbins = (int64_t)m->hg.n[0] * m->hg.n[1] * m->hg.n[2];
+    WSB_REQUIRE(nbins < (int64_t)1 << 31, WSB_ENOTSUP, "grid too large for the cell sort");
+    const Sort3Layout L = sort3_layout(n, nbins);
+    WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", L.total, scratch_bytes);
+    char* base = static_cast<char*>(scratch);
+    int32_t* key = reinterpret_cast<int32_t*>(base + L.key);
+    int32_t* flag = reinterpret_cast<int32_t*>(base + L.flag);
+    int32_t* opos = reinterpret_cast<int32_t*>(base + L.opos);
+    int32_t* hist = reinterpret_cast<int32_t*>(base + L.hist);
+    int32_t* cur = reinterpret_cast<int32_t*>(base + L.cur);
+    hipStream_t s = wsb::as_stream(stream);
+    hipLaunchKernelGGL(zero3_kernel, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, s, hist, nbins);
+    hipLaunchKernelGGL(cell_count3_kernel, dim3((n + 255) / 256), dim3(256), 0, s, m->dg, pos, n, key, flag, hist);
+    WSB_HIP(hipGetLastError());
+    size_t tmp = L.tmp_bytes;
+    WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, hist, cur, (int)nbins, s));
+    tmp = L.tmp_bytes;
+    WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, flag, opos, n, s));
+    hipLaunchKernelGGL(cell_scatter3_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key, flag, opos, n, cur, order);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
 extern "C" size_t wsb_adjacency3d_scratch_bytes(int n) {
     if (n <= 0) return 0;
     size_t tmp = 0;
@@ -575,8 +765,9 @@ extern "C" size_t wsb_adjacency3d_scratch_bytes(int n) {
     return ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255) + tmp + 256;
 }
 
-extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, int n, int64_t* offsets,
-                               int32_t* faces, int64_t capacity, double* chi, void* scratch, size_t scratch_bytes, void* stream) {
+extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, const int32_t* order, int n,
+                               int64_t* offsets, int32_t* faces, int64_t capacity, double* chi, void* scratch,
+                               size_t scratch_bytes, void* stream) {
     WSB_REQUIRE(m, WSB_EINVAL, "null mesh");
     WSB_REQUIRE(!m->host_only, WSB_EINVAL, "host-only mesh has no device tables");
     WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
@@ -591,7 +782,7 @@ extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const dou
     int64_t* counts = static_cast<int64_t*>(scratch);
     char* tmp = static_cast<char*>(scratch) + ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255);
     size_t tmp_bytes = scratch_bytes - ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255);
-    Adj3Args a{m->dg, pos, sup, n, counts, offsets, faces, capacity, chi};
+    Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi};
     const dim3 grid((n + 255) / 256), block(256);
     hipLaunchKernelGGL(adj3_kernel<false>, grid, block, 0, s, a);
     WSB_HIP(hipGetLastError());

@@ -66,6 +66,13 @@ __host__ __device__ inline double xsqrt(double v) {
 
 __host__ __device__ inline double sgn(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
 
+// the face id stored in the bits of a float record field
+__host__ __device__ inline int bits_int(float v) {
+    int i;
+    __builtin_memcpy(&i, &v, sizeof(int));
+    return i;
+}
+
 // the pair geometry in units of h: relative vertices d, plane offset z, per edge (w, lo, hi); shared by the moment
 // terms and the solid-angle sum so that both see the same sign of z and w
 struct PairGeo {
@@ -399,8 +406,14 @@ struct Grid3 {
     int n[3];
     const int32_t* cstart;      // [ncells+1] dilated (support_max) candidate lists, ascending face ids
     const int32_t* citems;
+    const float4* crec;         // [2*items] per cell item: face box rounded outward to FP32 (lo.xyz, id), (hi.xyz, 0)
     const double* cchi;         // [ncells] winding number of face-free cells
-    const int32_t* cwalk;       // [ncells] dirty cells: dir | steps << 3 (dir 0..5 = +x -x +y -y +z -z), -1 clean
+    const int32_t* cwalk;       // [ncells] -1 for a face-free cell, else the index d of the cell's axis walk
+    const int32_t* wdir;        // [ndirty] dir | steps << 3 (dir 0..5 = +x -x +y -y +z -z)
+    const int32_t* wstart;      // [ndirty+1] faces a ray of the walk can cross (CSR), ascending face ids
+    const int32_t* witems;
+    const float4* wrec;         // [2*witems] per walk item, rounded outward to FP32: (u lo, u hi, v lo, v hi),
+                                // (axis lo, axis hi, id, 0) with (u, v) the two axes across the walk
     const double* frec;         // [F*FREC]
     const double* fbox;         // [F*6] lo xyz, hi xyz
     int nf;
@@ -421,80 +434,71 @@ __host__ __device__ inline int64_t cell_id(const Grid3& g, int i, int j, int k) 
     return ((int64_t)i * g.n[1] + j) * g.n[2] + k;
 }
 
-// signed crossings of the axis-aligned ray from p (axis ax, direction s) with the faces of the walked cells, each
-// intersection attributed to the walked cell whose slab holds it (clamped to the walked range); returns false
-// when a predicate is within its rounding bound (the caller then takes the exact per-face sum).
+// winding number at p from the axis walk of its cell: the winding number where the walk ends (a face-free cell, or
+// 0 outside the grid) plus the signed crossings of the ray from p with the faces that overlap the walk's column.
+// Every such face lies inside the walked range (a face reaching the end cell would make it a cell with faces), so
+// each crossing ahead of p is counted once.  Returns false when a predicate is within its rounding bound (the
+// caller then takes the exact per-face sum).
 __host__ __device__ inline bool ray_winding(const Grid3& g, double px, double py, double pz, const int c0[3], double* chi) {
     const int64_t cid0 = cell_id(g, c0[0], c0[1], c0[2]);
-    const int walk = g.cwalk[cid0];
-    if (walk < 0) {
+    const int d = g.cwalk[cid0];
+    if (d < 0) {
         *chi = g.cchi[cid0];
         return true;
     }
+    const int walk = g.wdir[d];
     const int dir = walk & 7, steps = walk >> 3;
     const int ax = dir >> 1;
     const int s = (dir & 1) ? -1 : 1;
     const int au = (ax + 1) % 3, av = (ax + 2) % 3;
     const double p[3] = {px, py, pz};
     const double Pu = p[au], Pv = p[av];
-    // end of the walk: a face-free cell or the outside of the grid (winding 0)
     int ce[3] = {c0[0], c0[1], c0[2]};
     ce[ax] += s * steps;
     double acc = 0.0;
     if (ce[ax] >= 0 && ce[ax] < g.n[ax]) acc = g.cchi[cell_id(g, ce[0], ce[1], ce[2])];
-    const int jfirst = c0[ax], jlast = c0[ax] + s * (steps - 1);
-    for (int st = 0; st < steps; ++st) {
-        int cc[3] = {c0[0], c0[1], c0[2]};
-        cc[ax] += s * st;
-        const int64_t cid = cell_id(g, cc[0], cc[1], cc[2]);
-        for (int it = g.cstart[cid]; it < g.cstart[cid + 1]; ++it) {
-            const int f = g.citems[it];
-            const double* bx = g.fbox + 6 * f;
-            if (Pu < bx[au] || Pu > bx[3 + au] || Pv < bx[av] || Pv > bx[3 + av]) continue;
-            if (s > 0 ? bx[3 + ax] < p[ax] : bx[ax] > p[ax]) continue;
-            const double* fr = g.frec + FREC * f;
-            const double Au = fr[FR_V + au], Av = fr[FR_V + av];
-            const double Bu = fr[FR_V + 3 + au], Bv = fr[FR_V + 3 + av];
-            const double Cu = fr[FR_V + 6 + au], Cv = fr[FR_V + 6 + av];
-            // projected point-in-triangle: three orientations with a static rounding bound
-            int pos = 0, neg = 0, unk = 0;
-            const double eu[3] = {Bu - Au, Cu - Bu, Au - Cu}, ev[3] = {Bv - Av, Cv - Bv, Av - Cv};
-            const double ou[3] = {Pu - Au, Pu - Bu, Pu - Cu}, ov[3] = {Pv - Av, Pv - Bv, Pv - Cv};
+    const int it1 = g.wstart[d + 1];
+    for (int it = g.wstart[d]; it < it1; ++it) {
+        // conservative FP32 box filter (one load), then the exact tests on the face's vertices
+        const float4 q0 = g.wrec[2 * (int64_t)it], q1 = g.wrec[2 * (int64_t)it + 1];
+        if (Pu < (double)q0.x || Pu > (double)q0.y || Pv < (double)q0.z || Pv > (double)q0.w) continue;
+        if (s > 0 ? (double)q1.y < p[ax] : (double)q1.x > p[ax]) continue;
+        const int f = bits_int(q1.z);
+        const double* fr = g.frec + FREC * (int64_t)f;
+        const double Au = fr[FR_V + au], Av = fr[FR_V + av];
+        const double Bu = fr[FR_V + 3 + au], Bv = fr[FR_V + 3 + av];
+        const double Cu = fr[FR_V + 6 + au], Cv = fr[FR_V + 6 + av];
+        // projected point-in-triangle: three orientations with a static rounding bound
+        int pos = 0, neg = 0, unk = 0;
+        const double eu[3] = {Bu - Au, Cu - Bu, Au - Cu}, ev[3] = {Bv - Av, Cv - Bv, Av - Cv};
+        const double ou[3] = {Pu - Au, Pu - Bu, Pu - Cu}, ov[3] = {Pv - Av, Pv - Bv, Pv - Cv};
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const double t1 = eu[q] * ov[q], t2 = ev[q] * ou[q];
-                const double o = t1 - t2;
-                const double bound = 1e-14 * (fabs(t1) + fabs(t2)) + 1e-300;
-                if (o > bound) ++pos;
-                else if (o < -bound) ++neg;
-                else ++unk;
-            }
-            if (pos > 0 && neg > 0) continue;            // certainly outside the projected triangle
-            if (unk > 0) return false;                   // ray through (or near) an edge or a vertex
-            // side of p against the face plane: N . (p - A), N = (B - A) x (C - A)
-            const double Ax = fr[FR_V], Ay = fr[FR_V + 1], Az = fr[FR_V + 2];
-            const double e1x = fr[FR_V + 3] - Ax, e1y = fr[FR_V + 4] - Ay, e1z = fr[FR_V + 5] - Az;
-            const double e2x = fr[FR_V + 6] - Ax, e2y = fr[FR_V + 7] - Ay, e2z = fr[FR_V + 8] - Az;
-            const double dx = px - Ax, dy = py - Ay, dz = pz - Az;
-            const double Nx = e1y * e2z - e1z * e2y, Ny = e1z * e2x - e1x * e2z, Nz = e1x * e2y - e1y * e2x;
-            const double side = Nx * dx + Ny * dy + Nz * dz;
-            const double mag = (fabs(e1y * e2z) + fabs(e1z * e2y)) * fabs(dx) + (fabs(e1z * e2x) + fabs(e1x * e2z)) * fabs(dy) +
-                               (fabs(e1x * e2y) + fabs(e1y * e2x)) * fabs(dz);
-            if (!(fabs(side) > 1e-13 * mag + 1e-300)) return false;     // p on (or within rounding of) the plane
-            // N_ax has the sign of the projected orientation (counter-clockwise in (au, av) <=> N_ax > 0)
-            const double Nk = ax == 0 ? Nx : (ax == 1 ? Ny : Nz);
-            const double sgnN = pos > 0 ? 1.0 : -1.0;
-            if (!(fabs(Nk) > 0.0)) return false;
-            // p + t s e_ax meets the plane at t = -side / (s N_ax): ahead of p <=> side and s N_ax differ in sign
-            if ((side > 0.0) == (s * sgnN > 0.0)) continue;
-            // slab of the intersection X_ax = p_ax - side / N_ax, clamped to the walked range
-            const double Xk = p[ax] - side / (sgnN * fabs(Nk));
-            const double fj = floor((Xk - g.lo[ax]) / g.cell);
-            const double jmin = (double)(jfirst < jlast ? jfirst : jlast), jmax = (double)(jfirst < jlast ? jlast : jfirst);
-            if ((int)fmin(fmax(fj, jmin), jmax) != cc[ax]) continue;
-            // the ray leaves the solid where s N_ax > 0 (outward normal along the ray)
-            acc += (s * sgnN > 0.0) ? 1.0 : -1.0;
+        for (int q = 0; q < 3; ++q) {
+            const double t1 = eu[q] * ov[q], t2 = ev[q] * ou[q];
+            const double o = t1 - t2;
+            const double bound = 1e-14 * (fabs(t1) + fabs(t2)) + 1e-300;
+            if (o > bound) ++pos;
+            else if (o < -bound) ++neg;
+            else ++unk;
         }
+        if (pos > 0 && neg > 0) continue;            // certainly outside the projected triangle
+        if (unk > 0) return false;                   // ray through (or near) an edge or a vertex
+        // side of p against the face plane: N . (p - A), N = (B - A) x (C - A)
+        const double Ax = fr[FR_V], Ay = fr[FR_V + 1], Az = fr[FR_V + 2];
+        const double e1x = fr[FR_V + 3] - Ax, e1y = fr[FR_V + 4] - Ay, e1z = fr[FR_V + 5] - Az;
+        const double e2x = fr[FR_V + 6] - Ax, e2y = fr[FR_V + 7] - Ay, e2z = fr[FR_V + 8] - Az;
+        const double dx = px - Ax, dy = py - Ay, dz = pz - Az;
+        const double Nx = e1y * e2z - e1z * e2y, Ny = e1z * e2x - e1x * e2z, Nz = e1x * e2y - e1y * e2x;
+        const double side = Nx * dx + Ny * dy + Nz * dz;
+        const double mag = (fabs(e1y * e2z) + fabs(e1z * e2y)) * fabs(dx) + (fabs(e1z * e2x) + fabs(e1x * e2z)) * fabs(dy) +
+                           (fabs(e1x * e2y) + fabs(e1y * e2x)) * fabs(dz);
+        if (!(fabs(side) > 1e-13 * mag + 1e-300)) return false;     // p on (or within rounding of) the plane
+        // N_ax has the sign of the projected orientation (counter-clockwise in (au, av) <=> N_ax > 0); the ray
+        // p + t s e_ax meets the plane at t = -side / (s N_ax): ahead of p <=> side and s N_ax differ in sign
+        const double sgnN = pos > 0 ? 1.0 : -1.0;
+        if ((side > 0.0) == (s * sgnN > 0.0)) continue;
+        // the ray leaves the solid where s N_ax > 0 (outward normal along the ray)
+        acc += (s * sgnN > 0.0) ? 1.0 : -1.0;
     }
     *chi = acc;
     return true;

@@ -84,6 +84,7 @@ class Mesh3D:
         L.check(lib.wsb_mesh3d_grid(h, g))
         self.grid_lo, self.cell = tuple(g[:3]), g[3]
         self._scratch = None
+        self._sort_scratch = None
         self._cap = 0
 
     def close(self):
@@ -116,33 +117,52 @@ class Mesh3D:
     def scratch_bytes(self, n: int) -> int:
         return int(L.load().wsb_adjacency3d_scratch_bytes(n))
 
-    def adjacency(self, pos: torch.Tensor, sup: torch.Tensor, capacity: int | None = None, scratch=None) -> Adjacency3D:
-        """face adjacency + winding numbers.  ``capacity`` (pairs) defaults to a remembered bound; the call retries
-        once with the exact count when it was too small (one host read of the pair count)."""
+    def _buf(self, name, nbytes, dev):
+        b = getattr(self, name)
+        if b is None or b.numel() < nbytes or b.device != dev:
+            b = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+            setattr(self, name, b)
+        return b
+
+    def sort(self, pos: torch.Tensor, order: torch.Tensor | None = None) -> torch.Tensor:
+        """particles grouped by grid cell (device counting sort): a permutation for ``adjacency(order=...)``."""
+        self._check_inputs(pos, pos[:, 0])
+        n = pos.shape[0]
+        if order is None:
+            order = torch.empty(n, dtype=torch.int32, device=pos.device)
+        lib = L.load()
+        scratch = self._buf("_sort_scratch", lib.wsb_sort3d_scratch_bytes(self._h, n), pos.device)
+        L.check(lib.wsb_sort3d(self._h, L.ptr(pos), n, L.ptr(order), L.ptr(scratch), scratch.numel(), L.stream_ptr(pos.device)))
+        return order
+
+    def adjacency(self, pos: torch.Tensor, sup: torch.Tensor, capacity: int | None = None, scratch=None,
+                  order: torch.Tensor | None = None, sort: bool = True) -> Adjacency3D:
+        """face adjacency + winding numbers.  Particles are grouped by cell first (``sort``; or pass ``order``), which
+        changes only the thread schedule.  ``capacity`` (pairs) defaults to a remembered bound; the call retries once
+        with the exact count when it was too small (one host read of the pair count)."""
         self._check_inputs(pos, sup)
         if float(sup.max()) > self.support_max * (1 + 1e-12) if len(sup) else False:
             raise ValueError("support exceeds the mesh's support_max")
         n = pos.shape[0]
         dev = pos.device
         lib = L.load()
+        if order is None and sort and n:
+            order = self.sort(pos)
         if scratch is None:
-            need = self.scratch_bytes(n)
-            if self._scratch is None or self._scratch.numel() < need or self._scratch.device != dev:
-                self._scratch = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
-            scratch = self._scratch
+            scratch = self._buf("_scratch", self.scratch_bytes(n), dev)
         offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
         chi = torch.empty(n, dtype=torch.float64, device=dev)
         cap = int(capacity if capacity is not None else max(self._cap, 4 * n, 1024))
         faces = torch.empty(cap, dtype=torch.int32, device=dev)
         st = L.stream_ptr(dev)
-        L.check(lib.wsb_adjacency3d(self._h, L.ptr(pos), L.ptr(sup), n, L.ptr(offsets), L.ptr(faces), cap, L.ptr(chi),
-                                    L.ptr(scratch), scratch.numel(), st))
+        L.check(lib.wsb_adjacency3d(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap,
+                                    L.ptr(chi), L.ptr(scratch), scratch.numel(), st))
         total = int(offsets[n]) if n else 0
         if total > cap:
             cap = total
             faces = torch.empty(cap, dtype=torch.int32, device=dev)
-            L.check(lib.wsb_adjacency3d(self._h, L.ptr(pos), L.ptr(sup), n, L.ptr(offsets), L.ptr(faces), cap, L.ptr(chi),
-                                        L.ptr(scratch), scratch.numel(), st))
+            L.check(lib.wsb_adjacency3d(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap,
+                                        L.ptr(chi), L.ptr(scratch), scratch.numel(), st))
         self._cap = max(self._cap, total)
         return Adjacency3D(offsets, faces, chi, total)
 

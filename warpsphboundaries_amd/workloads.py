@@ -6,6 +6,9 @@
   (0.5, 0.5), solid inside); 1,048,576 particles in [0,1]^2, 30 % within one support of the wall (an edge picked in
   proportion to its length, a point uniform on it, offset d = h U(0,1) along the fluid normal) and the rest uniform
   outside the star; h = 0.004; kernels cubic, w2, w4, w6.
+* ``config3``: the fluid in the unit box [0,1]^3 around an icosphere of radius 0.25 at (0.5, 0.5, 0.5) subdivided to
+  81,920 triangles: one closed surface of the 12 outward box triangles and the sphere's triangles turned into the
+  cavity; 2,097,152 particles uniform in the box outside the sphere; w4, h = 0.01.
 """
 from __future__ import annotations
 
@@ -128,6 +131,36 @@ def subsample(w: Workload, n: int, seed=7) -> Workload:
     idx = np.concatenate([g.choice(near, k, replace=False) if k else near[:0], g.choice(far, n - k, replace=False)])
     idx.sort()
     return Workload(w.name + f"-sub{n}", w.loops, w.positions[idx], w.supports[idx], w.kernels, w.h, w.near[idx])
+
+
+@dataclass
+class Workload3:
+    name: str
+    vertices: np.ndarray     # [nv, 3]
+    faces: np.ndarray        # [nf, 3] outward (cavity faces point into the void)
+    positions: np.ndarray    # [N, 3]
+    supports: np.ndarray     # [N]
+    kernels: tuple
+    h: float
+
+
+def box_with_sphere_cavity(level=6, radius=0.25, center=(0.5, 0.5, 0.5)):
+    from .mesh3d import box_surface, icosphere
+    Vb, Fb = box_surface((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    Vs, Fs = icosphere(level, radius, center)
+    return np.vstack([Vb, Vs]), np.vstack([Fb, (Fs + len(Vb))[:, ::-1]]).astype(np.int32)
+
+
+def config3(n=2097152, seed=3, level=6, h=0.01):
+    g = rng(seed)
+    V, F = box_with_sphere_cavity(level)
+    pts = np.empty((0, 3))
+    while len(pts) < n:
+        c = g.uniform(0.0, 1.0, (int(1.1 * (n - len(pts))) + 1024, 3))
+        c = c[np.sum((c - 0.5) ** 2, axis=1) > 0.25 ** 2]
+        pts = np.concatenate([pts, c])
+    pts = pts[:n]
+    return Workload3(f"box+icosphere{len(F) - 12}", V, F, pts, np.full(n, h), ("w4",), h)
 
 
 @dataclass
