@@ -133,7 +133,7 @@ def test_oracle_cavity_identity_and_half_space():
     for k in KERNELS:
         for d in (0.9, 0.3, 1e-3):
             got = OM.closed_surface_requests(Vb, Fb, np.array([[0.0, 0.0, 10.0 - d]]), k, 1.0, [((0, 0, 0), ())])[0, 0]
-            assert abs(got - float(S.half_space_exact(k, -d))) < 1e-12
+            assert abs(got - float(S.half_space_exact(k, -d))) < 5e-12      # the FP64 oracle sums O(1e3) terms for w6
 
 
 def test_oracle_predicates_restate_the_reference():
