@@ -391,16 +391,27 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
         const double reach = h * (1.0 + 1e-9);
         const int64_t base = FILL ? a.offsets[i] : 0;
         const int it1 = a.g.cstart[cid + 1];
-        for (int it = a.g.cstart[cid]; it < it1; ++it) {
-            // one independent 32-byte load per candidate; the exact test only for boxes within h on every axis
-            const float4 l = a.g.crec[2 * (int64_t)it], u = a.g.crec[2 * (int64_t)it + 1];
-            if ((double)l.x - px > reach || px - (double)u.x > reach || (double)l.y - py > reach || py - (double)u.y > reach ||
-                (double)l.z - pz > reach || pz - (double)u.z > reach)
-                continue;
-            const int f = m3::bits_int(l.w);
-            if (!keep_face(a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
-            if (FILL && base + cnt < a.capacity) a.faces[base + cnt] = f;
-            ++cnt;
+        // four candidates per step: their 32-byte records are loaded together (independent loads in flight), then
+        // filtered in list order; the exact test only for boxes within h on every axis
+        for (int it = a.g.cstart[cid]; it < it1; it += 4) {
+            float4 l[4], u[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = min(it + q, it1 - 1);
+                l[q] = a.g.crec[2 * (int64_t)j];
+                u[q] = a.g.crec[2 * (int64_t)j + 1];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (it + q >= it1) break;
+                if ((double)l[q].x - px > reach || px - (double)u[q].x > reach || (double)l[q].y - py > reach ||
+                    py - (double)u[q].y > reach || (double)l[q].z - pz > reach || pz - (double)u[q].z > reach)
+                    continue;
+                const int f = m3::bits_int(l[q].w);
+                if (!keep_face(a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
+                if (FILL && base + cnt < a.capacity) a.faces[base + cnt] = f;
+                ++cnt;
+            }
         }
     }
     if (FILL) return;

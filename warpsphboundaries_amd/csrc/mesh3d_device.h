@@ -138,10 +138,20 @@ __host__ __device__ inline double piece_phi(double a, double b, double c2, doubl
     return angle2<FORM>(a * b * D, b * b * r1 * r2 + a * a * x1 * x2);
 }
 
+// distances of the three vertices from the query (units of h): the endpoint radii of every unclipped edge end
+template <int FORM>
+__host__ __device__ inline void vertex_radii(const PairGeo& g, double rv[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double* v = g.d + 3 * k;
+        rv[k] = xsqrt<FORM>(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+}
+
 // solid angle of the whole face seen from the query with the reference's convention (0 for a face whose plane
 // contains the query): z S_f[r^-3] over the whole face = sgn(z) (Theta - sum_e sgn(w_e) dA_e), dA_e the angle of
 // a s/(|w| r) across the edge (mesh3d/reference.py:127-150 at R = None, n = -3).  FP64 angles.
-__host__ __device__ inline double omega_k(const PairGeo& g) {
+__host__ __device__ inline double omega_k(const PairGeo& g, const double rv[3]) {
     if (g.z == 0.0) return 0.0;
     const double a = fabs(g.z);
     double s = foot_angle<FP64>(g);
@@ -149,28 +159,27 @@ __host__ __device__ inline double omega_k(const PairGeo& g) {
     for (int e = 0; e < 3; ++e) {
         const double b = fabs(g.w[e]);
         if (b > 0.0) {
-            const double l = g.lo[e], u = g.hi[e];
             const double c2 = b * b + a * a;
-            s -= sgn(g.w[e]) * piece_phi<FP64>(a, b, c2, l, sqrt(l * l + c2), u, sqrt(u * u + c2));
+            s -= sgn(g.w[e]) * piece_phi<FP64>(a, b, c2, g.lo[e], rv[e], g.hi[e], rv[(e + 1) % 3]);
         }
     }
     return sgn(g.z) * s;
 }
 
 // the same solid angle for the mixed form: 2 atan2(det, den) of the corner vectors (van Oosterom and Strackee), one
-// angle instead of four; where det and den are both below 1e-6 of their scale (the query within ~1e-6 of an edge
-// line in the face's plane, where the formula loses digits) the form above is taken instead
+// angle instead of three; where det and den are both below 1e-6 of their scale (the query within ~1e-6 of an edge
+// line in the face's plane, where the formula loses digits) the form above is taken instead.  The FP64 form keeps
+// the form above everywhere: this angle enters the moments with weights of up to ~1e3 (w6), and the corner-vector
+// formula's rounding (~10 u of its scale) would then exceed 1e-12.
 template <int FORM>
-__host__ __device__ inline double omega_face(const PairGeo& g) {
+__host__ __device__ inline double omega_face(const PairGeo& g, const double rv[3]) {
     if (FORM == MIXED) {
         if (g.z == 0.0) return 0.0;
         const double* A = g.d;
         const double* B = g.d + 3;
         const double* C = g.d + 6;
         const double det = A[0] * (B[1] * C[2] - B[2] * C[1]) + A[1] * (B[2] * C[0] - B[0] * C[2]) + A[2] * (B[0] * C[1] - B[1] * C[0]);
-        const double la = xsqrt<FORM>(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
-        const double lb = xsqrt<FORM>(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
-        const double lc = xsqrt<FORM>(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
+        const double la = rv[0], lb = rv[1], lc = rv[2];
         const double ab = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
         const double bc = B[0] * C[0] + B[1] * C[1] + B[2] * C[2];
         const double ca = C[0] * A[0] + C[1] * A[1] + C[2] * A[2];
@@ -178,14 +187,16 @@ __host__ __device__ inline double omega_face(const PairGeo& g) {
         const double scale = la * lb * lc;
         if (fabs(det) > 1e-6 * scale || fabs(den) > 1e-6 * scale) return 2.0 * angle2<FORM>(det, den);
     }
-    return omega_k(g);
+    return omega_k(g, rv);
 }
 
 template <int FORM>
 __host__ __device__ inline double face_solid_angle(const double* __restrict__ fr, double qx, double qy, double qz, double ih) {
     PairGeo g;
     pair_geo(fr, qx, qy, qz, ih, g);
-    return omega_face<FORM>(g);
+    double rv[3];
+    vertex_radii<FP64>(g, rv);
+    return omega_face<FORM>(g, rv);
 }
 
 // One (query, face) pair: the 23 face accumulators in units of h (before 1/pi).  fr = face record, q = query,
@@ -211,12 +222,13 @@ __host__ __device__ inline void face_terms(const double* __restrict__ fr, double
     const double z = g.z, a = fabs(z), sz = sgn(z);
     const double nx = fr[FR_N], ny = fr[FR_N + 1], nz = fr[FR_N + 2];
     if (!(a < T::R[0])) return;                     // R[0] is the largest block radius
-    const double Om = omega_face<FORM>(g);
+    double rv[3];
+    vertex_radii<FORM>(g, rv);
+    const double Om = omega_face<FORM>(g, rv);
     double apow[DG + 3];
     apow[0] = 1.0;
 #pragma unroll
     for (int k = 1; k <= DG + 2; ++k) apow[k] = apow[k - 1] * a;
-    double rlo[3] = {-1.0, -1.0, -1.0}, rhi[3] = {-1.0, -1.0, -1.0};     // radii at the edge ends, computed once
     double sP = 0.0, sF = 0.0, zs0 = 0.0, zsV = 0.0;
     double l1x = 0.0, l1y = 0.0, l1z = 0.0, l2x = 0.0, l2y = 0.0, l2z = 0.0;
 #pragma unroll
@@ -239,27 +251,26 @@ __host__ __device__ inline void face_terms(const double* __restrict__ fr, double
                 u = chi ? L : hi;
                 chord = l < u;
             }
+            const double rlo = rv[e], rhi = rv[(e + 1) % 3];      // radii at the edge's vertices
             // angles of the pieces outside the ball
             if (bw > 0.0) {
-                auto rad_lo = [&]() { if (rlo[e] < 0.0) rlo[e] = xsqrt<FORM>(lo * lo + c2); return rlo[e]; };
-                auto rad_hi = [&]() { if (rhi[e] < 0.0) rhi[e] = xsqrt<FORM>(hi * hi + c2); return rhi[e]; };
                 if (!chord) {
                     thO += sw * piece_theta<FORM>(bw, lo, hi);
-                    if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, lo, rad_lo(), hi, rad_hi());
+                    if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, lo, rlo, hi, rhi);
                 } else {
                     if (clo) {
                         thO += sw * piece_theta<FORM>(bw, lo, -L);
-                        if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, lo, rad_lo(), -L, R);
+                        if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, lo, rlo, -L, R);
                     }
                     if (chi) {
                         thO += sw * piece_theta<FORM>(bw, L, hi);
-                        if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, L, R, hi, rad_hi());
+                        if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, L, R, hi, rhi);
                     }
                 }
             }
             if (!chord) continue;
-            const double rl = clo ? R : (rlo[e] >= 0.0 ? rlo[e] : (rlo[e] = xsqrt<FORM>(l * l + c2)));
-            const double ru = chi ? R : (rhi[e] >= 0.0 ? rhi[e] : (rhi[e] = xsqrt<FORM>(u * u + c2)));
+            const double rl = clo ? R : rlo;
+            const double ru = chi ? R : rhi;
             // I_-1 = asinh(s/c): one log of F(u)/F(l), F(s) = s + r (= c^2/(r - s) for s < 0)
             double dIm1 = 0.0;
             if (c2 > 0.0) {
@@ -458,9 +469,20 @@ __host__ __device__ inline bool ray_winding(const Grid3& g, double px, double py
     double acc = 0.0;
     if (ce[ax] >= 0 && ce[ax] < g.n[ax]) acc = g.cchi[cell_id(g, ce[0], ce[1], ce[2])];
     const int it1 = g.wstart[d + 1];
-    for (int it = g.wstart[d]; it < it1; ++it) {
-        // conservative FP32 box filter (one load), then the exact tests on the face's vertices
-        const float4 q0 = g.wrec[2 * (int64_t)it], q1 = g.wrec[2 * (int64_t)it + 1];
+    for (int it0 = g.wstart[d]; it0 < it1; it0 += 4) {
+      // four walk items per step: records loaded together, then the conservative FP32 box filter and the exact
+      // tests on the face's vertices
+      float4 r0[4], r1[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+          const int j = it0 + k < it1 ? it0 + k : it1 - 1;
+          r0[k] = g.wrec[2 * (int64_t)j];
+          r1[k] = g.wrec[2 * (int64_t)j + 1];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (it0 + k >= it1) break;
+        const float4 q0 = r0[k], q1 = r1[k];
         if (Pu < (double)q0.x || Pu > (double)q0.y || Pv < (double)q0.z || Pv > (double)q0.w) continue;
         if (s > 0 ? (double)q1.y < p[ax] : (double)q1.x > p[ax]) continue;
         const int f = bits_int(q1.z);
@@ -499,6 +521,7 @@ __host__ __device__ inline bool ray_winding(const Grid3& g, double px, double py
         if ((side > 0.0) == (s * sgnN > 0.0)) continue;
         // the ray leaves the solid where s N_ax > 0 (outward normal along the ray)
         acc += (s * sgnN > 0.0) ? 1.0 : -1.0;
+      }
     }
     *chi = acc;
     return true;
