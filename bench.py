@@ -109,20 +109,25 @@ def cpu_baseline(work, n_sample=8192, seed=7):
                       f"median of 3 runs of {t:.2f} s"}
 
 
-def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, seed=5):
+def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, seed=5, batched=None, device_requests=False,
+                  max_batch_rows=1 << 20):
     """the served mixed-size workload: every scene resident, one request per scene and timestep arriving as a Poisson
     process at `qps`; continuous batching over `streams` streams.  Returns latency percentiles and throughput."""
     import torch
     from warpsphboundaries_amd import workloads as WL
     from warpsphboundaries_amd.serve import BoundaryServer
     scenes = WL.config5(n_scenes=n_scenes, seed=seed)
-    srv = BoundaryServer(streams=streams, form=form, graphs=graphs, hbm_budget=64 << 30)
+    srv = BoundaryServer(streams=streams, form=form, graphs=graphs, hbm_budget=64 << 30, batched=batched,
+                         max_batch_rows=max_batch_rows)
     for sc in scenes:
         srv.add_scene(sc.name, sc.loops, sc.h)
     work = []
     for t in range(steps):
         for i, sc in enumerate(scenes):
-            work.append((sc.name, WL.jitter(sc.positions, sc.h, 1000 * t + i), sc.h, sc.kernel))
+            pos = WL.jitter(sc.positions, sc.h, 1000 * t + i)
+            if device_requests:             # the simulations' particle arrays live on this GPU
+                pos = torch.tensor(pos, dtype=torch.float64, device=srv.device)
+            work.append((sc.name, pos, sc.h, sc.kernel))
     order = WL.rng(seed + 1).permutation(len(work))
     work = [work[i] for i in order]
     # warm pass, not timed: every scene once (per-stream buffers, captured graphs, first kernel loads)
@@ -132,6 +137,7 @@ def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, see
     torch.cuda.synchronize()
     srv.completed.clear()
     srv.launched_batches = 0
+    srv.host_seconds = {k: 0.0 for k in srv.host_seconds}
     arrivals = WL.poisson_arrivals(len(work), qps, seed + 2)
     t0 = time.perf_counter()
     reqs = []
@@ -149,7 +155,9 @@ def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, see
     rows = sum(r.n for r in reqs)
     st = srv.latency_stats(reqs)
     st.update({"form": form, "qps_offered": qps, "requests": len(reqs), "rows": rows, "wall_s": wall,
-               "evals_per_s": 12 * rows / wall, "batches": srv.launched_batches, "streams": streams, "graphs": graphs})
+               "evals_per_s": 12 * rows / wall, "batches": srv.launched_batches, "streams": streams, "graphs": graphs,
+               "cross_scene_batches": srv.batched, "device_resident_requests": bool(device_requests),
+               "max_batch_rows": max_batch_rows, "host_seconds": dict(srv.host_seconds)})
     return st
 
 

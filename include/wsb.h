@@ -114,6 +114,39 @@ int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form, const int
 int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max,
                           double* d, double* nrm, int32_t* edge, void* stream);
 
+/* ---- batched evaluation across many resident geometries (continuous batching of a boundary server; no counterpart
+ * in the reference).  wsb_geom2d_desc writes wsb_geom2d_desc_bytes() bytes describing a geometry (grid parameters and
+ * device pointers); a caller gathers the descriptors of its resident geometries into one device table.  row_desc[n]
+ * names each row's geometry (index into that table; a negative index skips the row: no slots, indicator 0, its
+ * output untouched).  slots[n*Kstride] with Kstride >= the K of every geometry in the
+ * batch.  Each row's slots, indicator and moments are bitwise those of wsb_adjacency2d / wsb_moments2d (unsorted,
+ * local frame) on its own geometry. */
+size_t wsb_geom2d_desc_bytes(void);
+int wsb_geom2d_desc(const wsb_geom2d* g, void* out);
+int wsb_batch2d_adjacency(const void* descs, const int32_t* row_desc, const int32_t* order, const double* lpos,
+                          const double* lsup, int n, int Kstride, int32_t* slots, int32_t* counts, double* ind, void* stream);
+/* rows grouped by (geometry, cell): order[n] for wsb_batch2d_adjacency (device radix sort; the schedule only) */
+size_t wsb_batch2d_sort_scratch_bytes(int n);
+int wsb_batch2d_sort(const void* descs, const int32_t* row_desc, const double* lpos, int n, int ndesc, int32_t* order,
+                     void* scratch, size_t scratch_bytes, void* stream);
+int wsb_batch2d_moments(const void* descs, const int32_t* row_desc, int kernel_id, int form, const int32_t* slots,
+                        const int32_t* counts, int Kstride, const double* ind, const double* lpos, const double* lsup,
+                        int n, int flags, double* out, void* stream);
+
+/* the plain FP64 form of the batched moments: the plan interpreter of the per-scene plain path, one thread per row
+ * summing its pairs in slot order (bitwise the per-scene plain rows); the plan's kernel is the batch's kernel */
+int wsb_batch2d_plain_moments(const void* descs, const int32_t* row_desc, const wsb_plan* plan, const int32_t* slots,
+                              const int32_t* counts, int Kstride, const double* ind, const double* lpos, const double* lsup,
+                              int n, int flags, double* out, void* stream);
+
+/* rows of device-resident requests into a batch and back: segs is a device array of nseg records of
+ * wsb_row_segment_bytes() bytes, each {const double* src; double* dst; int64_t first; int64_t count; int64_t value}
+ * (src for a gather, dst for a scatter, value for a fill); rows are `width` doubles wide. */
+size_t wsb_row_segment_bytes(void);
+int wsb_gather_rows(const void* segs, int nseg, int width, double* batch, void* stream);
+int wsb_scatter_rows(const void* segs, int nseg, int width, const double* batch, void* stream);
+int wsb_fill_rows_i32(const void* segs, int nseg, int32_t* rows, void* stream);
+
 /* ---- solid-sphere closed form (3-D, FP64; reference curvbound/symbolic.py:166-178 outside the ball,
  * mesh3d/radial.py:102-140 on both sides): out[i] = int_ball W dV for a ball of radius R[i] and a query at signed
  * distance d[i] from its surface (d > 0 outside), units of h; NaN when R <= 0 or d < -R. */

@@ -76,3 +76,36 @@ def test_poisoned_request_is_isolated():
     _, (alone,) = _serve(scenes, [(s.name, good, s.h, s.kernel)], streams=1)
     assert np.array_equal(r_good.result, alone.result)
     assert srv.launched_batches == 1          # both were in one batch
+
+
+def test_cross_scene_batches_equal_per_scene_launches():
+    scenes = _scenes(5, seed=25)
+    box = WL.config1(n=3000, seed=26)                       # a two-representation scene (box + obstacle)
+    scenes.append(WL.ServedScene("box+gon", box.loops, 0.02, "w4", box.positions))
+    reqs = _requests(scenes, steps=2)
+    srv_b, batched = _serve(scenes, reqs, streams=2, batched=True)
+    srv_p, per_scene = _serve(scenes, reqs, streams=2, batched=False)
+    for b, p in zip(batched, per_scene):
+        assert b.error is None and p.error is None
+        assert np.array_equal(b.result, p.result), b.scene
+    assert srv_b.launched_batches < srv_p.launched_batches
+
+
+def test_device_resident_requests_equal_host_requests():
+    scenes = _scenes(4, seed=27)
+    reqs = _requests(scenes, steps=2)
+    _, host = _serve(scenes, reqs, streams=3)
+    dev_reqs = [(n, torch.tensor(p, device="cuda:0"), h, k) for n, p, h, k in reqs]
+    _, dev = _serve(scenes, dev_reqs, streams=3)
+    for a, b in zip(host, dev):
+        assert isinstance(b.result, torch.Tensor) and b.result.is_cuda
+        assert np.array_equal(a.result, b.result.cpu().numpy())
+
+
+def test_plain_form_batches_equal_per_scene_plain():
+    scenes = _scenes(4, seed=28)
+    reqs = _requests(scenes, steps=1)
+    _, b = _serve(scenes, reqs, streams=2, form="plain", batched=True)
+    _, p = _serve(scenes, reqs, streams=2, form="plain", batched=False)
+    for x, y in zip(b, p):
+        assert np.array_equal(x.result, y.result)

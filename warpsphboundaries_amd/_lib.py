@@ -38,6 +38,20 @@ SIGNATURES = {
                                     c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_signed_distance2d": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wsb_sphere_closed_form": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "wsb_geom2d_desc_bytes": (c_size_t, []),
+    "wsb_geom2d_desc": (c_int, [c_void_p, c_void_p]),
+    "wsb_batch2d_adjacency": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                      c_void_p, c_void_p]),
+    "wsb_batch2d_sort_scratch_bytes": (c_size_t, [c_int]),
+    "wsb_batch2d_sort": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "wsb_batch2d_moments": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                    c_int, c_int, c_void_p, c_void_p]),
+    "wsb_batch2d_plain_moments": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                          c_int, c_int, c_void_p, c_void_p]),
+    "wsb_row_segment_bytes": (c_size_t, []),
+    "wsb_gather_rows": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "wsb_scatter_rows": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "wsb_fill_rows_i32": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_mesh3d_create": (c_int, [c_void_p, c_int, c_void_p, c_int, c_double, c_int, POINTER(c_void_p), POINTER(c_int64)]),
     "wsb_mesh3d_destroy": (c_int, [c_void_p]),
     "wsb_mesh3d_info": (c_int, [c_void_p, POINTER(c_int64)]),

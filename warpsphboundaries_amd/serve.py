@@ -14,14 +14,19 @@ counterpart: the reference evaluates one position set per call).
 * Continuous batching: pending requests of one (scene, kernel) are merged into a batch (up to ``max_batch_rows``);
   each of ``streams`` HIP streams carries one batch at a time, so batches of different scenes overlap on the device;
   as soon as a stream's batch retires, the oldest pending work is admitted into it.
-* Captured launches: with ``graphs=True`` the adjacency + moment launches of a (scene, kernel, stream, row bucket)
-  are captured once in a HIP graph over fixed buffers and replayed (rows beyond the request count are parked outside
-  the geometry's grid, where they cost one cell lookup).
+* Batched launches across scenes (``batched=True``, the default for the fp64 and mixed forms): every resident
+  geometry has a device descriptor (grid parameters and array pointers) in one table; a batch gathers the pending
+  requests of one kernel from any number of scenes, each row names its geometry, and one adjacency launch plus one
+  moment launch (per surface representation index) serve the whole batch.  Host work per batch is a few copies.
+* Captured launches: with ``graphs=True`` (per-scene batches) the adjacency + moment launches of a (scene, kernel,
+  stream, row bucket) are captured once in a HIP graph over fixed buffers and replayed (rows beyond the request count
+  are parked outside the geometry's grid, where they cost one cell lookup).
 * Latency = arrival (``submit``) to retirement (the poll that sees the batch's event complete), host clock.
 Results are bitwise independent of batching: every row's channels are a fixed-order sum over its own slots.
 """
 from __future__ import annotations
 
+import ctypes
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -42,17 +47,23 @@ class Request:
     rid: int
     scene: str
     kernel: str
-    positions: np.ndarray
-    supports: np.ndarray
+    positions: object                         # [n, 2] float64: numpy (host request) or a device tensor
+    supports: object                          # [n] float64, same residency
     arrival: float
     done: Optional[float] = None
-    result: Optional[np.ndarray] = None       # [n, 12] float64, physical units, device channel order
+    result: Optional[object] = None           # [n, 12] float64, physical units, device channel order (numpy or tensor)
     error: Optional[BaseException] = None
     cancelled: bool = False
+    out: Optional[torch.Tensor] = None        # device output of a device-resident request
+    ready: Optional[torch.cuda.Event] = None  # device request: its inputs are written (submitter's stream)
 
     @property
     def n(self):
         return len(self.positions)
+
+    @property
+    def on_device(self):
+        return isinstance(self.positions, torch.Tensor)
 
     @property
     def latency(self):
@@ -67,6 +78,7 @@ class ResidentScene:
     geoms: list
     K: int
     bytes: int
+    desc: list = field(default_factory=list)     # descriptor-table index of each geometry
 
 
 @dataclass
@@ -90,6 +102,30 @@ class _Slot:
     filled: int = 0           # rows of lpos / lsup holding request data (the rest is parked)
 
 
+_SEG = np.dtype([("src", "<u8"), ("dst", "<u8"), ("first", "<i8"), ("count", "<i8"), ("value", "<i8")])
+
+
+@dataclass
+class _BSlot:
+    """per-stream fixed buffers of the cross-scene batches for one row capacity."""
+    cap: int
+    Kstride: int
+    lpos: torch.Tensor
+    lsup: torch.Tensor
+    rdesc: torch.Tensor
+    slots: torch.Tensor
+    counts: torch.Tensor
+    ind: torch.Tensor
+    out: torch.Tensor
+    host_pos: torch.Tensor
+    host_sup: torch.Tensor
+    host_desc: torch.Tensor
+    host_out: torch.Tensor
+    bytes: int
+    order: Optional[torch.Tensor] = None
+    sort_scratch: Optional[torch.Tensor] = None
+
+
 @dataclass
 class _Batch:
     stream_index: int
@@ -100,20 +136,31 @@ class _Batch:
     slot: _Slot
     event: torch.cuda.Event
     bytes: int
+    keep: Optional[list] = None       # host buffers an asynchronous copy of this batch still reads
 
 
 class BoundaryServer:
     def __init__(self, device="cuda:0", hbm_budget: int = 8 << 30, streams: int = 4, form: str = "mixed",
-                 max_batch_rows: int = 1 << 18, sort_threshold: int = 1 << 16, graphs: bool = False):
+                 max_batch_rows: int = 1 << 18, sort_threshold: int = 1 << 16, graphs: bool = False,
+                 batched: Optional[bool] = None):
         self.device = torch.device(device)
         self.budget = int(hbm_budget)
         self.form = form
         self.max_batch_rows = int(max_batch_rows)
         self.sort_threshold = int(sort_threshold)
         self.use_graphs = bool(graphs)
+        self.batched = (not graphs) if batched is None else bool(batched)
+        if self.batched and graphs:
+            raise NotImplementedError("cross-scene batches are launched eagerly (graph capture is per scene)")
+        self._desc_host = bytearray()
+        self._desc_dev = None
+        self._desc_bytes = int(L.load().wsb_geom2d_desc_bytes())
+        self._Kmax = 1
+        self.bslots: List[Dict[int, _BSlot]] = []
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(int(streams))]
         self.busy = [None] * len(self.streams)
         self.slots: List[Dict[int, _Slot]] = [dict() for _ in self.streams]
+        self.bslots = [dict() for _ in self.streams]
         self.scenes: Dict[str, ResidentScene] = {}
         self.pending: deque = deque()
         self.resident_bytes = 0
@@ -122,6 +169,7 @@ class BoundaryServer:
         self.completed: List[Request] = []
         self.launched_batches = 0
         self.launched_rows = 0
+        self.host_seconds = {"submit": 0.0, "launch": 0.0, "poll": 0.0}     # host time spent in the server
 
     # ------------------------------------------------------------------------------------------------ scenes
     def add_scene(self, name: str, loops, support_max: float):
@@ -135,24 +183,63 @@ class BoundaryServer:
             raise MemoryError(f"scene {name!r} needs {nbytes} resident bytes; {self.budget - self.resident_bytes} left")
         self.resident_bytes += nbytes
         sc = ResidentScene(name, reps, float(support_max), geoms, max(g.K for g in geoms), nbytes)
+        lib = L.load()
+        for g in geoms:
+            buf = ctypes.create_string_buffer(self._desc_bytes)
+            L.check(lib.wsb_geom2d_desc(g.handle, buf))
+            sc.desc.append(len(self._desc_host) // self._desc_bytes)
+            self._desc_host += buf.raw
+        self._desc_dev = torch.frombuffer(bytearray(self._desc_host), dtype=torch.uint8).to(self.device)
+        self._Kmax = max(self._Kmax, sc.K)
         self.scenes[name] = sc
         return sc
 
     def remove_scene(self, name: str):
+        """drop a scene (its descriptor-table entries stay, unreferenced, until the server ends)."""
+        if any(r.scene == name for r in self.pending) or any(b is not None and any(r.scene == name for r in b.requests)
+                                                              for b in self.busy):
+            raise ValueError(f"scene {name!r} has requests pending or in flight")
         sc = self.scenes.pop(name)
         self.resident_bytes -= sc.bytes
 
     # ------------------------------------------------------------------------------------------------ requests
     def submit(self, scene: str, positions, supports, kernel="w4") -> Request:
+        """queue one timestep of one simulation.  Host arrays give a numpy result; a device tensor of positions (a
+        simulation resident on this GPU) gives a device tensor result and never crosses PCIe."""
+        t0 = time.perf_counter()
+        try:
+            return self._submit(scene, positions, supports, kernel)
+        finally:
+            self.host_seconds["submit"] += time.perf_counter() - t0
+
+    def _submit(self, scene: str, positions, supports, kernel) -> Request:
         sc = self.scenes.get(scene)
         if sc is None:
             raise KeyError(f"unknown scene {scene!r}")
         k = kernelName(kernel)
-        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 2))
-        sup = np.ascontiguousarray(np.broadcast_to(np.asarray(supports, dtype=np.float64), (len(pos),)))
-        if len(pos) and not (np.all(sup > 0) and np.all(sup <= sc.support_max)):
-            raise ValueError("supports must lie in (0, support_max] of the scene (the resident cell list is built for it)")
+        if isinstance(positions, torch.Tensor):
+            if not self.batched:
+                raise NotImplementedError("device-resident requests use the cross-scene batches")
+            pos = positions.to(device=self.device, dtype=F64).reshape(-1, 2).contiguous()
+            if isinstance(supports, torch.Tensor):
+                sup = supports.to(device=self.device, dtype=F64).reshape(-1).contiguous()
+                bad = len(sup) != len(pos) or (len(sup) and bool(((sup <= 0) | (sup > sc.support_max)).any()))
+            else:
+                h = float(supports)
+                sup = torch.full((len(pos),), h, dtype=F64, device=self.device)
+                bad = not (0 < h <= sc.support_max)
+            if bad:
+                raise ValueError("supports must lie in (0, support_max] of the scene (the resident cell list is built for it)")
+        else:
+            pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 2))
+            sup = np.ascontiguousarray(np.broadcast_to(np.asarray(supports, dtype=np.float64), (len(pos),)))
+            if len(pos) and not (np.all(sup > 0) and np.all(sup <= sc.support_max)):
+                raise ValueError("supports must lie in (0, support_max] of the scene (the resident cell list is built for it)")
         req = Request(self._rid, scene, k, pos, sup, time.perf_counter())
+        if req.on_device:
+            req.out = torch.empty((len(pos), 12), dtype=F64, device=self.device)
+            req.ready = torch.cuda.Event()
+            req.ready.record(torch.cuda.current_stream(self.device))
         self._rid += 1
         if self._footprint(sc, max(len(pos), 1)) + self.resident_bytes > self.budget:
             req.error = MemoryError(f"request of {len(pos)} rows exceeds the HBM budget")
@@ -170,7 +257,8 @@ class BoundaryServer:
     # ------------------------------------------------------------------------------------------------ scheduling
     def _footprint(self, sc: ResidentScene, rows: int) -> int:
         cap = _bucket(rows)
-        per_row = 16 + 8 + 4 * sc.K + 4 + 8 + 4 + 96 + 16 + 16 + 96
+        K = self._Kmax if self.batched else sc.K
+        per_row = 16 + 8 + 4 * K + 4 + 8 + 4 + 96 + 16 + 16 + 96
         return cap * per_row + (1 << 20)
 
     def _in_flight_bytes(self):
@@ -178,15 +266,28 @@ class BoundaryServer:
 
     def poll(self) -> List[Request]:
         """retire finished batches, admit and launch pending work; returns the requests completed now."""
+        t0 = time.perf_counter()
+        l0 = self.host_seconds["launch"]
+        try:
+            return self._poll()
+        finally:
+            self.host_seconds["poll"] += (time.perf_counter() - t0) - (self.host_seconds["launch"] - l0)
+
+    def _poll(self) -> List[Request]:
         now_done = []
         for i, b in enumerate(self.busy):
             if b is not None and b.event.query():
                 now = time.perf_counter()
-                res = b.slot.host_out[:b.rows].numpy()
+                res = None
                 off = 0
                 for r in b.requests:
                     if not r.cancelled:
-                        r.result = res[off:off + r.n].copy()
+                        if r.on_device:
+                            r.result = r.out
+                        else:
+                            if res is None:
+                                res = b.slot.host_out[:b.rows].numpy()
+                            r.result = res[off:off + r.n].copy()
                     r.done = now
                     off += r.n
                     now_done.append(r)
@@ -199,7 +300,8 @@ class BoundaryServer:
             sc = self.scenes[head.scene]
             group, rows = [], 0
             for r in list(self.pending):
-                if r.scene == head.scene and r.kernel == head.kernel and (not group or rows + r.n <= self.max_batch_rows):
+                same = r.kernel == head.kernel and r.on_device == head.on_device and (self.batched or r.scene == head.scene)
+                if same and (not group or rows + r.n <= self.max_batch_rows):
                     group.append(r)
                     rows += r.n
                     if rows >= self.max_batch_rows:
@@ -222,7 +324,12 @@ class BoundaryServer:
                     if sl is not None and (sc.name, head.kernel) in sl.graphs:
                         si = i
                         break
-            self._launch(si, sc, head.kernel, group, rows, fp)
+            t1 = time.perf_counter()
+            if self.batched:
+                self._launch_batched(si, head.kernel, group, rows, fp)
+            else:
+                self._launch(si, sc, head.kernel, group, rows, fp)
+            self.host_seconds["launch"] += time.perf_counter() - t1
         self.completed.extend(now_done)
         return now_done
 
@@ -299,6 +406,108 @@ class BoundaryServer:
             ev = torch.cuda.Event()
             ev.record(st)
         self.busy[si] = _Batch(si, sc, kernel, group, rows, s, ev, fp)
+        self.launched_batches += 1
+        self.launched_rows += rows
+
+    def _bslot(self, si: int, rows: int) -> _BSlot:
+        cap = _bucket(rows)
+        pool = self.bslots[si]
+        s = pool.get(cap)
+        if s is None or s.Kstride < self._Kmax:
+            dev, K = self.device, self._Kmax
+            s = _BSlot(cap, K, torch.empty((cap, 2), dtype=F64, device=dev), torch.empty(cap, dtype=F64, device=dev),
+                       torch.empty(cap, dtype=torch.int32, device=dev), torch.empty(cap * K, dtype=torch.int32, device=dev),
+                       torch.empty(cap, dtype=torch.int32, device=dev), torch.empty(cap, dtype=F64, device=dev),
+                       torch.empty((cap, 12), dtype=F64, device=dev),
+                       torch.empty((cap, 2), dtype=F64).pin_memory(), torch.empty(cap, dtype=F64).pin_memory(),
+                       torch.empty(cap, dtype=torch.int32).pin_memory(), torch.empty((cap, 12), dtype=F64).pin_memory(),
+                       cap * (16 + 8 + 4 + 4 * K + 4 + 8 + 96))
+            pool[cap] = s
+        return s
+
+    def _launch_batched(self, si: int, kernel: str, group: List[Request], rows: int, fp: int):
+        """one cross-scene batch: rows in (pinned copies for host requests, a gather kernel for device requests), per
+        surface-representation index one fill of the row geometries, one adjacency and one moment launch, rows out."""
+        s = self._bslot(si, rows)
+        st = self.streams[si]
+        nrep = max(len(self.scenes[r.scene].geoms) for r in group)
+        on_dev = group[0].on_device
+        ng = len(group)
+        # segment tables: [pos gather | sup gather | out scatter | row geometry of pass 0 .. nrep-1]
+        seg = np.zeros((3 + nrep, ng), dtype=_SEG)
+        off = 0
+        hp = hs = None
+        if not on_dev:
+            hp, hs = s.host_pos.numpy(), s.host_sup.numpy()
+        for g, r in enumerate(group):
+            seg["first"][:, g] = off
+            seg["count"][:, g] = r.n
+            d = self.scenes[r.scene].desc
+            for j in range(nrep):
+                seg["value"][3 + j, g] = d[j] if j < len(d) else -1
+            if on_dev:
+                seg["src"][0, g] = r.positions.data_ptr()
+                seg["src"][1, g] = r.supports.data_ptr()
+                seg["dst"][2, g] = r.out.data_ptr()
+            else:
+                hp[off:off + r.n] = r.positions
+                hs[off:off + r.n] = r.supports
+            off += r.n
+        lib = L.load()
+        from .kernels import KERNEL_ID
+        from .device import surface_device_plan
+        kid, fid = KERNEL_ID[kernel], L.form_id(self.form)
+        if fid == 0:
+            plan = surface_device_plan(kernel, self.device.index or 0).handle
+        host_seg = torch.from_numpy(seg.view(np.uint8).reshape(-1)).pin_memory()
+        sb = seg.dtype.itemsize
+        with torch.cuda.stream(st):
+            if on_dev:
+                for r in group:
+                    st.wait_event(r.ready)
+            dev_seg = torch.empty(host_seg.numel(), dtype=torch.uint8, device=self.device)
+            dev_seg.copy_(host_seg, non_blocking=True)
+            sp = L.stream_ptr(self.device)
+            base = dev_seg.data_ptr()
+            tab = [ctypes.c_void_p(base + t * ng * sb) for t in range(3 + nrep)]
+            if on_dev:
+                L.check(lib.wsb_gather_rows(tab[0], ng, 2, L.ptr(s.lpos), sp))
+                L.check(lib.wsb_gather_rows(tab[1], ng, 1, L.ptr(s.lsup), sp))
+            else:
+                s.lpos[:rows].copy_(s.host_pos[:rows], non_blocking=True)
+                s.lsup[:rows].copy_(s.host_sup[:rows], non_blocking=True)
+            ndesc = len(self._desc_host) // self._desc_bytes
+            sort = rows >= self.sort_threshold
+            if sort and (s.order is None or s.order.numel() < s.cap):
+                s.order = torch.empty(s.cap, dtype=torch.int32, device=self.device)
+                s.sort_scratch = torch.empty(max(int(lib.wsb_batch2d_sort_scratch_bytes(s.cap)), 1), dtype=torch.uint8,
+                                             device=self.device)
+            for j in range(nrep):
+                L.check(lib.wsb_fill_rows_i32(tab[3 + j], ng, L.ptr(s.rdesc), sp))
+                order = None
+                if sort:
+                    L.check(lib.wsb_batch2d_sort(L.ptr(self._desc_dev), L.ptr(s.rdesc), L.ptr(s.lpos), rows, ndesc, L.ptr(s.order),
+                                                 L.ptr(s.sort_scratch), s.sort_scratch.numel(), sp))
+                    order = s.order
+                L.check(lib.wsb_batch2d_adjacency(L.ptr(self._desc_dev), L.ptr(s.rdesc), L.ptr(order), L.ptr(s.lpos), L.ptr(s.lsup),
+                                                  rows, s.Kstride, L.ptr(s.slots), L.ptr(s.counts), L.ptr(s.ind), sp))
+                flags = L.OUT_PHYSICAL | (L.OUT_ACCUMULATE if j else 0)
+                if fid == 0:
+                    L.check(lib.wsb_batch2d_plain_moments(L.ptr(self._desc_dev), L.ptr(s.rdesc), plan, L.ptr(s.slots),
+                                                          L.ptr(s.counts), s.Kstride, L.ptr(s.ind), L.ptr(s.lpos),
+                                                          L.ptr(s.lsup), rows, flags, L.ptr(s.out), sp))
+                else:
+                    L.check(lib.wsb_batch2d_moments(L.ptr(self._desc_dev), L.ptr(s.rdesc), kid, fid, L.ptr(s.slots),
+                                                    L.ptr(s.counts), s.Kstride, L.ptr(s.ind), L.ptr(s.lpos), L.ptr(s.lsup), rows,
+                                                    flags, L.ptr(s.out), sp))
+            if on_dev:
+                L.check(lib.wsb_scatter_rows(tab[2], ng, 12, L.ptr(s.out), sp))
+            else:
+                s.host_out[:rows].copy_(s.out[:rows], non_blocking=True)
+            dev_seg.record_stream(st)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        self.busy[si] = _Batch(si, None, kernel, group, rows, s, ev, fp, keep=[host_seg])
         self.launched_batches += 1
         self.launched_rows += rows
 
