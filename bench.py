@@ -55,7 +55,7 @@ def active_block_counts(torch, verts, edges, slots, counts, lpos, lsup, K, kname
     return len(rows), out
 
 
-def cpu_baseline(work, n_sample=8192, seed=7):
+def cpu_baseline(work, n_sample=131072, seed=7):
     """the CPU oracle (numpy FP64 restatement of the reference's edge reductions) on a stratified subsample:
     pair list from the oracle's cell-list slot fill, twelve channels per pair, row sums; single thread."""
     from oracle import adjacency as oadj
@@ -97,7 +97,7 @@ def cpu_baseline(work, n_sample=8192, seed=7):
 
     run()
     times = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         npairs = run()
         times.append(time.perf_counter() - t0)
@@ -106,7 +106,7 @@ def cpu_baseline(work, n_sample=8192, seed=7):
     return {"value": evals / t, "unit": "particle-moment evals/s", "cores": 1, "kind": "port",
             "sample": f"{len(pts)}-particle stratified subsample of the star workload ({int(sub.near.sum())} near-wall, "
                       f"{npairs} pairs), slot fill + twelve channels + row sums for 4 kernels, numpy FP64 oracle, "
-                      f"median of 3 runs of {t:.2f} s"}
+                      f"median of 5 runs of {t:.2f} s"}
 
 
 def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, seed=5, batched=None, device_requests=False,
@@ -130,9 +130,10 @@ def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, see
             work.append((sc.name, pos, sc.h, sc.kernel))
     order = WL.rng(seed + 1).permutation(len(work))
     work = [work[i] for i in order]
-    # warm pass, not timed: every scene once (per-stream buffers, captured graphs, first kernel loads)
-    for sc in scenes:
-        srv.submit(sc.name, sc.positions, sc.h, sc.kernel)
+    # warm pass, not timed: the same requests all at once (per-stream buffers of every row bucket, captured graphs,
+    # first kernel loads), then the timed pass with Poisson arrivals
+    for name, pos, h, k in work:
+        srv.submit(name, pos, h, k)
     srv.run()
     torch.cuda.synchronize()
     srv.completed.clear()
@@ -230,6 +231,17 @@ def mesh3d_measure(torch, dev, steps, warmup):
     }
 
 
+def _traffic(kernel):
+    """HBM bytes per launch of the mixed twelve-moment tile kernel on the star workload, from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs scaled on a copy kernel of known size (None when the summary is absent)."""
+    fn = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_tile12_star4096_1M.json")
+    try:
+        with open(fn) as f:
+            return json.load(f)["tile12_mixed"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +254,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--only-3d", action="store_true", help="run the 3-D section alone and print it")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter probe: one mixed step of the star workload and a 512 MiB device copy, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the wall-integral path shards by particle ranges with replicated geometry "
@@ -289,6 +303,18 @@ def main():
             step(form, scratch)
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
+
+    if args.probe:
+        step("mixed")
+        step("mixed")
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)                      # a plain copy of known size: 512 MiB read, 512 MiB written
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "star", "N": N, "tile_algorithmic_bytes": N * opmodel.BYTES_PER_PARTICLE_2D,
+                          "copy_bytes": src.numel() * 8}), flush=True)
+        return
 
     # ---- headline: mixed form, adjacency rebuild + evaluate
     t_mixed = timed("mixed", args.steps, args.warmup)
@@ -369,7 +395,7 @@ def main():
             "achieved": (flops / t_dom / 1e12) if compute_bound else (bytes_alg / t_dom / 1e9),
             "peak": PEAK_FP64 / 1e12 if compute_bound else PEAK_HBM / 1e9,
             "unit": "TFLOP/s" if compute_bound else "GB/s",
-            "frac": t_roof / t_dom, "traffic": None,
+            "frac": t_roof / t_dom, "traffic": _traffic(kdom),
             "algorithmic_flops": flops, "algorithmic_bytes": bytes_alg, "pairs": npairs, "active_chords": active,
             "launch_seconds": t_dom, "hbm_GBps_algorithmic": bytes_alg / t_dom / 1e9}
 
@@ -398,22 +424,35 @@ def main():
         log("3-D:", line["mesh3d"])
     if not args.no_serve:
         log("served workload ...")
-        served = serve_measure("mixed", args.qps, graphs=False)
-        log("served mixed:", served)
-        served_graph = serve_measure("mixed", args.qps, graphs=True, steps=3)
-        log("served mixed (captured graphs, 3 timesteps):", served_graph)
-        served_plain = serve_measure("plain", args.qps)
-        log("served plain:", served_plain)
-        hi_qps = 10 * args.qps
-        served_hi = serve_measure("mixed", hi_qps)
-        log("served mixed at", hi_qps, served_hi)
-        served_plain_hi = serve_measure("plain", hi_qps)
-        log("served plain at", hi_qps, served_plain_hi)
-        line["served_latency"] = {"p50_ms": served.get("p50_ms"), "p99_ms": served.get("p99_ms"), "qps": args.qps,
-                                  "mixed": served, "mixed_captured_graphs_3_steps": served_graph, "plain_fp64": served_plain,
-                                  "mixed_at_10x_qps": served_hi, "plain_fp64_at_10x_qps": served_plain_hi,
-                                  "workload": "256 2-D star scenes (16-8192 edges), 1e3-2e5 particles each, ~40 neighbours, "
-                                              "w2/w4/w6, one jittered timestep per scene, Poisson arrivals"}
+        # headline: the simulations' particle arrays are resident on this GPU (device-tensor requests and results)
+        sat = 1e9                          # every request arrives at t = 0: saturation throughput
+        mixed_sat = serve_measure("mixed", sat, device_requests=True)
+        log("served mixed, saturation:", mixed_sat)
+        plain_sat = serve_measure("plain", sat, device_requests=True)
+        log("served plain FP64, saturation:", plain_sat)
+        fp64_sat = serve_measure("fp64", sat, device_requests=True)
+        log("served FP64 closed form, saturation:", fp64_sat)
+        mixed_q = serve_measure("mixed", args.qps, device_requests=True)
+        log("served mixed at", args.qps, "QPS:", mixed_q)
+        plain_q = serve_measure("plain", args.qps, device_requests=True)
+        log("served plain FP64 at", args.qps, "QPS:", plain_q)
+        # host-array requests: every request and result crosses PCIe (pinned staging)
+        mixed_host = serve_measure("mixed", args.qps)
+        log("served mixed, host requests:", mixed_host)
+        mixed_host_sat = serve_measure("mixed", sat)
+        log("served mixed, host requests, saturation:", mixed_host_sat)
+        graph_q = serve_measure("mixed", args.qps, graphs=True, steps=2)
+        log("served mixed, per-scene captured graphs, host requests:", graph_q)
+        line["served_latency"] = {
+            "p50_ms": mixed_q.get("p50_ms"), "p99_ms": mixed_q.get("p99_ms"), "qps": args.qps,
+            "saturation_evals_per_s": {"mixed": mixed_sat["evals_per_s"], "plain_fp64": plain_sat["evals_per_s"],
+                                       "fp64": fp64_sat["evals_per_s"]},
+            "mixed_over_plain_fp64_at_saturation": mixed_sat["evals_per_s"] / plain_sat["evals_per_s"],
+            "mixed": mixed_q, "plain_fp64": plain_q, "mixed_saturation": mixed_sat, "plain_fp64_saturation": plain_sat,
+            "fp64_saturation": fp64_sat, "mixed_host_requests_pcie": mixed_host,
+            "mixed_host_requests_pcie_saturation": mixed_host_sat, "mixed_per_scene_graphs_host_requests": graph_q,
+            "workload": "256 2-D star scenes (16-8192 edges), 1e3-2e5 particles each, ~40 neighbours, w2/w4/w6, one "
+                        "jittered timestep per scene, Poisson arrivals; cross-scene batched launches, 4 streams"}
     if not args.no_cpu:
         log("timing the CPU oracle ...")
         line["cpu_baseline"] = cpu_baseline(work)

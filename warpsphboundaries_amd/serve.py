@@ -124,6 +124,8 @@ class _BSlot:
     bytes: int
     order: Optional[torch.Tensor] = None
     sort_scratch: Optional[torch.Tensor] = None
+    host_seg: Optional[torch.Tensor] = None      # pinned segment tables (rewritten only after the slot's batch retired)
+    dev_seg: Optional[torch.Tensor] = None
 
 
 @dataclass
@@ -459,14 +461,19 @@ class BoundaryServer:
         kid, fid = KERNEL_ID[kernel], L.form_id(self.form)
         if fid == 0:
             plan = surface_device_plan(kernel, self.device.index or 0).handle
-        host_seg = torch.from_numpy(seg.view(np.uint8).reshape(-1)).pin_memory()
+        raw = seg.view(np.uint8).reshape(-1)
+        if s.host_seg is None or s.host_seg.numel() < raw.size:
+            nb = max(raw.size, 1 << 16)
+            s.host_seg = torch.empty(nb, dtype=torch.uint8).pin_memory()
+            s.dev_seg = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        s.host_seg.numpy()[:raw.size] = raw
         sb = seg.dtype.itemsize
         with torch.cuda.stream(st):
             if on_dev:
                 for r in group:
                     st.wait_event(r.ready)
-            dev_seg = torch.empty(host_seg.numel(), dtype=torch.uint8, device=self.device)
-            dev_seg.copy_(host_seg, non_blocking=True)
+            dev_seg = s.dev_seg
+            dev_seg[:raw.size].copy_(s.host_seg[:raw.size], non_blocking=True)
             sp = L.stream_ptr(self.device)
             base = dev_seg.data_ptr()
             tab = [ctypes.c_void_p(base + t * ng * sb) for t in range(3 + nrep)]
@@ -504,10 +511,9 @@ class BoundaryServer:
                 L.check(lib.wsb_scatter_rows(tab[2], ng, 12, L.ptr(s.out), sp))
             else:
                 s.host_out[:rows].copy_(s.out[:rows], non_blocking=True)
-            dev_seg.record_stream(st)
             ev = torch.cuda.Event()
             ev.record(st)
-        self.busy[si] = _Batch(si, None, kernel, group, rows, s, ev, fp, keep=[host_seg])
+        self.busy[si] = _Batch(si, None, kernel, group, rows, s, ev, fp)
         self.launched_batches += 1
         self.launched_rows += rows
 
