@@ -1,4 +1,5 @@
-"""FP64 numpy restatement of the 2-D edge reductions per (query, edge) pair.  TEST INFRASTRUCTURE (see oracle/__init__).
+"""Numpy restatement of the 2-D edge reductions per (query, edge) pair, in FP64 or the extended precisions of
+``oracle.arith``.  TEST INFRASTRUCTURE (see oracle/__init__).
 
 This follows the stage-1 structure of the reference -- monomial-by-monomial block recursion, not the flattened
 plan the product uses -- so that the product's plan compiler and device kernels are checked against an
@@ -21,19 +22,18 @@ from math import comb
 
 import numpy as np
 
-from .kernels import blocks as kernel_blocks, plane_vector as _plane_vector
-
-INV_PI = 1.0 / np.pi
+from . import arith as A
+from .kernels import blocks as kernel_blocks, plane_vector_exact
 
 
 class EdgeGeo:
     """per-pair edge geometry, arrays of shape (P,)."""
 
     def __init__(self, p, q):
-        p = np.asarray(p, dtype=np.float64)
-        q = np.asarray(q, dtype=np.float64)
+        p = A.asarray(p)
+        q = A.asarray(q)
         d = q - p
-        ell = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1])
+        ell = A.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1])
         ok = ell > 0
         safe = np.where(ok, ell, 1.0)
         self.t0, self.t1 = d[:, 0] / safe, d[:, 1] / safe
@@ -51,7 +51,7 @@ class EdgeGeo:
         if v is None:
             Rf = float(R)
             az = np.abs(self.z)
-            L = np.sqrt(np.maximum((Rf - az) * (Rf + az), 0.0))
+            L = A.sqrt(np.maximum((Rf - az) * (Rf + az), 0.0))
             lo = np.maximum(self.s0, -L)
             hi = np.minimum(self.s1, L)
             m = (az < Rf) & (lo < hi) & self.ok
@@ -64,11 +64,11 @@ def I_all(M, s, z):
     """[I_0 .. I_M](s): odd antiderivatives of r^m (edge/primitives.py:19-32)."""
     zz = z * z
     r2 = s * s + zz
-    r = np.sqrt(r2)
+    r = A.sqrt(r2)
     out = [s]
     if M >= 1:
         safe = np.where(z == 0, 1.0, np.abs(z))
-        tail = np.where(z == 0, 0.0, zz * np.arcsinh(s / safe))
+        tail = np.where(z == 0, 0.0, zz * A.arcsinh(s / safe))
         out.append((s * r + tail) / 2)
     for m in range(2, M + 1):
         rpow = r2 ** (m // 2) if m % 2 == 0 else r * r2 ** ((m - 1) // 2)
@@ -88,7 +88,7 @@ def S(j, m, s, z):
     tot = 0.0
     for i in range(h + 1):
         mm = m + 2 * i
-        rp = r2 ** ((mm + 2) // 2) if mm % 2 == 0 else np.sqrt(r2) * r2 ** ((mm + 1) // 2)
+        rp = r2 ** ((mm + 2) // 2) if mm % 2 == 0 else A.sqrt(r2) * r2 ** ((mm + 1) // 2)
         tot = tot + comb(h, i) * (-zz) ** (h - i) * rp / (mm + 2)
     return tot
 
@@ -96,7 +96,7 @@ def S(j, m, s, z):
 def dangle(z, lo, hi):
     """atan2(z (hi - lo), z^2 + lo hi), 0 at z == 0 (edge/primitives.py:86-94)."""
     safe = np.where(z == 0, 1.0, z)
-    return np.where(z == 0, 0.0, np.arctan2(safe * (hi - lo), z * z + lo * hi))
+    return np.where(z == 0, 0.0, A.arctan2(safe * (hi - lo), z * z + lo * hi))
 
 
 def ypoly(g: EdgeGeo, beta):
@@ -193,7 +193,7 @@ def _sum_blocks(kname, fn):
             if c == 0:
                 continue
             v = fn(n, R)
-            w = float(Fraction(c)) * INV_PI
+            w = A.num(c) / A.pi()
             v = tuple(w * x for x in v) if isinstance(v, tuple) else w * v
             if tot is None:
                 tot = v
@@ -238,4 +238,6 @@ def moment_grad_edges(p, q, kname, alpha):
 
 
 def plane_vector(kname):
-    return np.array(_plane_vector(kname), dtype=np.float64)
+    """the twelve-channel whole-support vector in the working precision (exact rationals rounded once)."""
+    return A.asarray([A.num(v) for v in plane_vector_exact(kname)]) if A.mode() != "mp" else \
+        np.array([A.num(v) for v in plane_vector_exact(kname)], dtype=object)

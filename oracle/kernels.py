@@ -74,9 +74,17 @@ def radial_integral(name, power):
     return sum(sp.integrate(c2pi * e * _q ** power, (_q, lo, hi)) for (lo, hi), e in pieces)
 
 
-def plane_vector(name):
-    """twelve-channel whole-support vector (device order), from exact integrals: mass and int y_x^2 W."""
+@lru_cache(maxsize=None)
+def plane_vector_exact(name):
+    """twelve-channel whole-support vector (device order) as exact Fractions: mass and int y_x^2 W."""
     mass = 2 * radial_integral(name, 1)                 # 2 pi int r W dr / pi * pi ... (pi W normalised: 2 int r (pi W) dr)
     second = radial_integral(name, 3)                   # int y_x^2 W = (1/pi) pi int r^3 (pi W) dr
-    m, s = float(mass), float(second)
-    return [m, 0.0, 0.0, 0.0, 0.0, m, 0.0, 0.0, m, s, 0.0, s]
+    m = Fraction(int(sp.fraction(mass)[0]), int(sp.fraction(mass)[1]))
+    s = Fraction(int(sp.fraction(second)[0]), int(sp.fraction(second)[1]))
+    z = Fraction(0)
+    return (m, z, z, z, z, m, z, z, m, s, z, s)
+
+
+def plane_vector(name):
+    """twelve-channel whole-support vector (device order) in float64."""
+    return [float(v) for v in plane_vector_exact(name)]

@@ -1,4 +1,6 @@
-"""FP64 numpy restatement of the 3-D face->edge reductions (TEST INFRASTRUCTURE, see oracle/__init__).
+"""Numpy restatement of the 3-D face->edge reductions in FP64 or extended precision (``oracle.arith``; the "mp"
+mode is not supported here: the formulas evaluate guarded divisions on both branches).  TEST INFRASTRUCTURE (see
+oracle/__init__).
 
 This follows ``mesh3d/reference.py`` of the reference operation by operation, vectorised over (query, face) pairs:
 
@@ -18,7 +20,8 @@ This follows ``mesh3d/reference.py`` of the reference operation by operation, ve
                                                                          ``fluxbridge/device_search.py:80-85``
 * compact-mode face skip and face solid angle                           : ``fluxbridge/wp_func.py:290-318``
 
-The reference evaluates these formulas in mpmath; here they run in FP64 with the same operation structure.  The
+The reference evaluates these formulas in mpmath; here they run in FP64 (or x86 extended precision) with the same
+operation structure.  The
 oracle is pinned by the reference's 70-digit fixtures (``tests/fixtures/fluxbridge-v1.json``,
 ``fluxbridge-nearplane-v1.json``), exact whole-support moments, the cavity identity and the planar limit
 (``tests/test_oracle3d.py``).
@@ -31,6 +34,7 @@ from math import comb
 
 import numpy as np
 
+from . import arith as A
 from .kernels import blocks as kernel_blocks, c3_over_c2
 
 ZERO = (0, 0, 0)
@@ -45,9 +49,9 @@ def profile_exact(name):
 
 
 def profile(name, h=1.0):
-    """physical profile (n, R h, c / (pi h^(n+3))) as floats."""
-    h = float(h)
-    return tuple((n, float(R) * h, float(c) / np.pi / h ** (n + 3)) for n, R, c in profile_exact(name))
+    """physical profile (n, R h, c / (pi h^(n+3))) in the working precision."""
+    h = A.asarray(float(h)) if A.mode() != "f64" else float(h)
+    return tuple((n, float(R) * h, A.num(c) / A.pi() / h ** (n + 3)) for n, R, c in profile_exact(name))
 
 
 def _gradient_profile(prof):
@@ -58,11 +62,11 @@ def _gradient_profile(prof):
 def _I_all(M, s, z):
     zz = z * z
     r2 = s * s + zz
-    r = np.sqrt(r2)
+    r = A.sqrt(r2)
     out = [s]
     if M >= 1:
         safe = np.where(z == 0, 1.0, np.abs(z))
-        tail = np.where(z == 0, 0.0, zz * np.arcsinh(s / safe))
+        tail = np.where(z == 0, 0.0, zz * A.arcsinh(s / safe))
         out.append((s * r + tail) / 2)
     for m in range(2, M + 1):
         rpow = r2 ** (m // 2) if m % 2 == 0 else r * r2 ** ((m - 1) // 2)
@@ -77,9 +81,9 @@ def Ig(m, s, z):
     zz = z * z
     safe = np.where(z == 0, 1.0, z)
     if m == -1:
-        return np.arcsinh(s / np.abs(safe))
+        return A.arcsinh(s / np.abs(safe))
     if m == -2:
-        return np.arctan(s / safe) / safe
+        return A.arctan(s / safe) / safe
     r2 = s * s + zz
     return ((m + 3) * Ig(m + 2, s, z) - s * r2 ** ((m + 2) / 2)) / ((m + 2) * zz)
 
@@ -88,10 +92,10 @@ def Jg(m, s, z):
     """int s r^m ds (edge/primitives.py:75-78)."""
     r2 = s * s + z * z
     if m == -2:
-        return np.log(r2) / 2
+        return A.log(r2) / 2
     if m % 2 == 0:
         return r2 ** ((m + 2) // 2) / (m + 2)
-    return np.sqrt(r2) * r2 ** ((m + 1) // 2) / (m + 2)
+    return A.sqrt(r2) * r2 ** ((m + 1) // 2) / (m + 2)
 
 
 def Sg(j, m, s, z):
@@ -107,15 +111,15 @@ def Sg(j, m, s, z):
 def K(m, s, w, a):
     """int r^m/(s^2+w^2) ds with r^2 = s^2 + w^2 + a^2, w != 0 (mesh3d/reference.py:74-88)."""
     b = np.abs(w)
-    c = np.sqrt(w * w + a * a)
+    c = A.sqrt(w * w + a * a)
     if m == 0:
-        return np.arctan(s / b) / b
+        return A.arctan(s / b) / b
     if m == -1:
-        r = np.sqrt(s * s + c * c)
+        r = A.sqrt(s * s + c * c)
         sa = np.where(a == 0, 1.0, a)
-        return np.where(a != 0, np.arctan(sa * s / (b * r)) / (sa * b), s / (b * b * r))
+        return np.where(a != 0, A.arctan(sa * s / (b * r)) / (sa * b), s / (b * b * r))
     if m == 1:
-        return np.arcsinh(s / c) + a * a * K(-1, s, w, a)
+        return A.arcsinh(s / c) + a * a * K(-1, s, w, a)
     if m >= 2:
         return Ig(m - 2, s, c) + a * a * K(m - 2, s, w, a)
     raise ValueError("K supports m >= -1")
@@ -151,15 +155,15 @@ class Faces:
     """P triangles, each relative to its own query: vertices V[P,3,3], queries x[P,3] (any consistent units)."""
 
     def __init__(self, V, x):
-        V = np.asarray(V, dtype=np.float64)
-        x = np.asarray(x, dtype=np.float64)
+        V = A.asarray(V)
+        x = A.asarray(x)
         p = V - x[:, None, :]
         d = p[:, 1] - p[:, 0]
         N = _cross(d, p[:, 2] - p[:, 0])
-        nlen = np.sqrt(_dot(N, N))
+        nlen = A.sqrt(_dot(N, N))
         if np.any(nlen == 0):
             raise ValueError("degenerate triangle")
-        ell = np.sqrt(_dot(d, d))
+        ell = A.sqrt(_dot(d, d))
         self.P = len(V)
         self.normal = N / nlen[:, None]
         self.z = _dot(N, p[:, 0]) / nlen
@@ -170,7 +174,7 @@ class Faces:
         for j in range(3):
             v, q = p[:, j], p[:, (j + 1) % 3]
             D = q - v
-            length = np.sqrt(_dot(D, D))
+            length = A.sqrt(_dot(D, D))
             t = D / length[:, None]
             mu = _cross(t, self.normal)
             w = _dot(_cross(D, N), v) / (length * nlen)
@@ -208,7 +212,7 @@ class Faces:
                 else:
                     c2 = a * a + ws * ws
                     cut = R * R > c2
-                    L = np.sqrt(np.where(cut, R * R - c2, 0.0))
+                    L = A.sqrt(np.where(cut, R * R - c2, 0.0))
                     x1 = np.where(cut & (lo < -L) & (-L < hi), -L, lo)
                     x2 = np.where(cut & (lo < L) & (L < hi), L, hi)
                     pts = [lo, x1, x2, hi]
@@ -235,7 +239,7 @@ class Faces:
         w, lo, hi, mu1, mu2, t1, t2 = self.edges[e]
         c2 = w * w + self.a * self.a
         ok = c2 < R * R
-        L = np.sqrt(np.where(ok, R * R - c2, 0.0))
+        L = A.sqrt(np.where(ok, R * R - c2, 0.0))
         l, u = np.maximum(lo, -L), np.minimum(hi, L)
         ok = ok & (l < u)
         poly = [np.ones(self.P)]
@@ -246,7 +250,7 @@ class Faces:
                 for j, v2 in enumerate(q):
                     new[i + j] = new[i + j] + v * v2
             poly = new
-        c = np.sqrt(c2)
+        c = A.sqrt(c2)
         tot = np.zeros(self.P)
         with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
             for j, v in enumerate(poly):
@@ -308,11 +312,11 @@ class ClosedSurfacePairs:
         """winding=None: the reference's regular radial flux with every face's complete solid angle
         (mesh3d/reference.py:255-258).  winding=[Q]: the compact form of fluxbridge/wp_func.py:290-306 with the total
         solid angle 4 pi winding supplied (the solid angle sum of the faces, or its integer value off the surface)."""
-        V = np.asarray(vertices, dtype=np.float64)
+        V = A.asarray(vertices)
         Fi = np.asarray(faces, dtype=np.int64)
-        X = np.asarray(queries, dtype=np.float64).reshape(-1, 3)
+        X = A.asarray(queries).reshape(-1, 3)
         self.Q, self.F = len(X), len(Fi)
-        self.winding = None if winding is None else np.asarray(winding, dtype=np.float64).reshape(-1)
+        self.winding = None if winding is None else A.asarray(winding).reshape(-1)
         tri = V[Fi]                                            # [F,3,3]
         self.faces = Faces(np.broadcast_to(tri, (self.Q,) + tri.shape).reshape(-1, 3, 3),
                            np.repeat(X, self.F, axis=0))
@@ -334,7 +338,7 @@ class ClosedSurfacePairs:
         if not any(alpha):
             if self.winding is not None:
                 term = f.z * (f.scalar(n, R) - R ** (n + 3) * f.scalar(-3, R)) / (n + 3)
-                return self._sum(np.where(f.z != 0, term, 0.0)) + 4 * np.pi * self.winding * R ** (n + 3) / (n + 3)
+                return self._sum(np.where(f.z != 0, term, 0.0)) + 4 * A.pi() * self.winding * R ** (n + 3) / (n + 3)
             # regular radial flux: z (S(n,R) + R^(n+3) (S(-3,inf) - S(-3,R))) / (n+3), faces with z != 0
             term = f.z * (f.scalar(n, R) + R ** (n + 3) * (f.scalar(-3, None) - f.scalar(-3, R))) / (n + 3)
             return self._sum(np.where(f.z != 0, term, 0.0))
@@ -436,7 +440,7 @@ def closed_surface_requests(vertices, faces, queries, kernel, h, requests, chunk
     (the evaluate() dispatch of fluxbridge/reference.py:9-30 for a ClosedSurface); winding[Q] selects the compact
     form (see ClosedSurfacePairs)."""
     X = np.asarray(queries, dtype=np.float64).reshape(-1, 3)
-    out = np.zeros((len(X), len(requests)))
+    out = A.zeros((len(X), len(requests)))
     W = None if winding is None else np.asarray(winding, dtype=np.float64).reshape(-1)
     for s in range(0, len(X), chunk):
         cs = ClosedSurfacePairs(vertices, faces, X[s:s + chunk], None if W is None else W[s:s + chunk])
@@ -455,7 +459,7 @@ def triangle_requests(vertices, queries, kernel, h, requests):
     X = np.asarray(queries, dtype=np.float64).reshape(-1, 3)
     V = np.broadcast_to(np.asarray(vertices, dtype=np.float64), (len(X), 3, 3))
     f = Faces(V, X)
-    out = np.zeros((len(X), len(requests)))
+    out = A.zeros((len(X), len(requests)))
     for k, (a, ds) in enumerate(requests):
         if not ds:
             out[:, k] = f.moment(kernel, a, h)
@@ -469,23 +473,23 @@ def triangle_requests(vertices, queries, kernel, h, requests):
 # ------------------------------------------------------------------ solid angle, predicates
 def face_solid_angle(V, x):
     """2 atan2(det, den) of the normalised face corners seen from x (fluxbridge/wp_func.py:296-302), V[P,3,3]."""
-    a, b, c = (np.asarray(V[:, k], dtype=np.float64) - x for k in range(3))
+    a, b, c = (A.asarray(V[:, k]) - A.asarray(x) for k in range(3))
     det = _dot(a, _cross(b - a, c - a))
-    la, lb, lc = (np.sqrt(_dot(u, u)) for u in (a, b, c))
+    la, lb, lc = (A.sqrt(_dot(u, u)) for u in (a, b, c))
     den = la * lb * lc + _dot(a, b) * lc + _dot(b, c) * la + _dot(c, a) * lb
-    return np.where(det != 0, 2.0 * np.arctan2(det, den), 0.0)
+    return np.where(det != 0, 2.0 * A.arctan2(det, den), 0.0)
 
 
 def winding(vertices, faces, queries, chunk=256):
     """sum of face solid angles / 4 pi over every face."""
     V = np.asarray(vertices, dtype=np.float64)[np.asarray(faces, dtype=np.int64)]
     X = np.asarray(queries, dtype=np.float64).reshape(-1, 3)
-    out = np.zeros(len(X))
+    out = A.zeros(len(X))
     for s in range(0, len(X), chunk):
         xs = X[s:s + chunk]
         VV = np.broadcast_to(V, (len(xs),) + V.shape).reshape(-1, 3, 3)
         xx = np.repeat(xs, len(V), axis=0)
-        out[s:s + chunk] = face_solid_angle(VV, xx).reshape(len(xs), len(V)).sum(axis=1) / (4 * np.pi)
+        out[s:s + chunk] = face_solid_angle(VV, xx).reshape(len(xs), len(V)).sum(axis=1) / (4 * A.pi())
     return out
 
 
