@@ -6,16 +6,18 @@ here, and only as the checker (or the timed CPU baseline), never as the product 
 
 Contents (each function cites the reference file:line it restates):
 
-* ``edge2d``    FP64 numpy restatement of the stage-1 edge reductions (``edge/core.py``, ``edge/primitives.py``,
+* ``edge2d``    numpy restatement of the stage-1 edge reductions (``edge/core.py``, ``edge/primitives.py``,
                 ``edge/geometry.py``) per (query, edge) pair: the twelve edge-local channels.
 * ``adjacency`` FP64 numpy restatement of the static cell list, the fixed-capacity slot fill and the winding
                 indicator (``scene/scene.py:100-138,273-371``, ``scene/fixedadj.py:67-147``).
-* ``scene2d``   the per-particle twelve moments of a surface representation (slots + channels + indicator x plane
-                vector + physical scaling; ``scene/scene.py:384-416``) and the signed distance (``:1312-1372``).
-* ``mp_edge``   small-input mpmath restatement used to pin single open-edge values at high precision.
+* ``mesh3d``    numpy restatement of the 3-D face->edge reductions of ``mesh3d/reference.py`` (closed surfaces,
+                tetrahedra, open triangles), the element adjacency predicates and the face solid angle.
+* ``kernels``   the kernel blocks re-derived with sympy from ``curvbound/kernels.py``.
+* ``arith``     the arithmetic the restatements run in: float64 (default), x86 long double, or mpmath.
 
-How each part is pinned (tests/test_oracle_*.py): the reference's 40-digit golden fixtures
-(``tests/fixtures/edge2d_golden.json`` of the reference, copied as data under ``tests/golden/``), exact rational
-whole-support moments, the half-plane closed form, edge-subdivision invariance, an independent fan-triangle area
-quadrature for the open-edge value channel, and brute force for the adjacency.
+How each part is pinned (tests/test_oracle_*.py, tests/test_mesh3d_cpu.py): the reference's 40-digit 2-D golden
+fixtures (``tests/fixtures/edge2d_golden.json`` of the reference, copied as data under ``tests/golden/``) and its
+70-digit 3-D fixtures (``fluxbridge-v1.json``, ``fluxbridge-nearplane-v1.json``), exact rational whole-support
+moments, the half-plane closed form, the cavity identity, and brute force for the adjacency; the extended-precision
+modes against the same fixtures at 1e-15 (long double) and 1e-33 (mpmath).
 """
