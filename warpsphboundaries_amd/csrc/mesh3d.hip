@@ -95,6 +95,14 @@ int build_host(wsb_mesh3d* m, const double* V, int nv, const int32_t* Fc, int nf
         WSB_REQUIRE(nl > 0.0 && std::isfinite(nl), WSB_EINVAL, "degenerate triangle %d", f);
         const double n[3] = {N[0] / nl, N[1] / nl, N[2] / nl};
         for (int a = 0; a < 3; ++a) fr[m3::FR_N + a] = n[a];
+        fr[m3::FR_NLEN] = nl;
+        double lmax2 = 0.0;
+        for (int e = 0; e < 3; ++e) {
+            const double* p = fr + m3::FR_V + 3 * e;
+            const double* q = fr + m3::FR_V + 3 * ((e + 1) % 3);
+            lmax2 = std::max(lmax2, (q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) + (q[2] - p[2]) * (q[2] - p[2]));
+        }
+        fr[m3::FR_SLIVER] = nl <= 1e-2 * lmax2 ? 1.0 : 0.0;      // 2 area <= 1e-2 longest^2
         for (int e = 0; e < 3; ++e) {
             const double* p = fr + m3::FR_V + 3 * e;
             const double* q = fr + m3::FR_V + 3 * ((e + 1) % 3);

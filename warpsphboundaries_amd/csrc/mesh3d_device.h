@@ -34,8 +34,9 @@ constexpr int FREC = 36;     // doubles per face record
 enum { FP64 = 1, MIXED = 2 };
 
 // face record: v0 v1 v2 (9), unit normal (3), unit edge tangents t_e (9), in-plane outward edge normals mu_e = t_e x n
-// (9), edge lengths (3), padding (3).  Edge e runs from vertex e to vertex (e+1) % 3.
-enum { FR_V = 0, FR_N = 9, FR_T = 12, FR_MU = 21, FR_LEN = 30 };
+// (9), edge lengths (3), |(v1 - v0) x (v2 - v0)| (1), sliver flag (1: twice the area at most 1e-2 of the longest edge
+// squared), padding (1).  Edge e runs from vertex e to vertex (e+1) % 3.
+enum { FR_V = 0, FR_N = 9, FR_T = 12, FR_MU = 21, FR_LEN = 30, FR_NLEN = 33, FR_SLIVER = 34 };
 
 // Angles stay FP64 in both forms.  In the face closed form every angle is weighted by a partial sum of the kernel
 // polynomial (its odd part through 1/(r + a), its even part through the clipped pieces), and those partial sums
@@ -73,13 +74,73 @@ __host__ __device__ inline int bits_int(float v) {
     return i;
 }
 
+// double-double arithmetic (an unevaluated sum hi + lo): exact differences and near-exact triple products for the
+// pairs whose plane offset and edge distances are both tiny (the query near an edge of the face).  Contraction is
+// off inside these helpers: a product fused into the following sum would count the product's rounding error twice.
+struct D2 {
+    double hi, lo;
+};
+__host__ __device__ inline D2 two_sum(double a, double b) {
+#pragma clang fp contract(off)
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__host__ __device__ inline D2 quick_sum(double a, double b) {
+#pragma clang fp contract(off)
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+__host__ __device__ inline D2 dd_add(D2 x, D2 y) {
+#pragma clang fp contract(off)
+    const D2 s = two_sum(x.hi, y.hi);
+    return quick_sum(s.hi, s.lo + (x.lo + y.lo));
+}
+__host__ __device__ inline D2 dd_sub(D2 x, D2 y) { return dd_add(x, {-y.hi, -y.lo}); }
+__host__ __device__ inline D2 dd_mul(D2 x, D2 y) {
+#pragma clang fp contract(off)
+    const double p = x.hi * y.hi;
+    const double e = fma(x.hi, y.hi, -p);
+    return quick_sum(p, e + (x.hi * y.lo + x.lo * y.hi));
+}
+__host__ __device__ inline void dd_cross(const D2 a[3], const D2 b[3], D2 c[3]) {
+    c[0] = dd_sub(dd_mul(a[1], b[2]), dd_mul(a[2], b[1]));
+    c[1] = dd_sub(dd_mul(a[2], b[0]), dd_mul(a[0], b[2]));
+    c[2] = dd_sub(dd_mul(a[0], b[1]), dd_mul(a[1], b[0]));
+}
+__host__ __device__ inline double dd_dot(const D2 a[3], const D2 b[3]) {
+    return dd_add(dd_add(dd_mul(a[0], b[0]), dd_mul(a[1], b[1])), dd_mul(a[2], b[2])).hi;
+}
+
 // the pair geometry in units of h: relative vertices d, plane offset z, per edge (w, lo, hi); shared by the moment
 // terms and the solid-angle sum so that both see the same sign of z and w
 struct PairGeo {
     double d[9];
+    double r2[3];              // squared vertex distances
     double z;
     double w[3], lo[3], hi[3];
+    double det;                // d0 . (d1 x d2), exact to rounding where `refined`
+    bool refined;
 };
+
+// exact (double-double) vertex-minus-query and edge vectors, formed where they are used to keep few registers live
+__host__ __device__ inline void dd_vq(const double* __restrict__ fr, int k, const double q[3], D2 out[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = two_sum(fr[FR_V + 3 * k + c], -q[c]);
+}
+__host__ __device__ inline void dd_edge(const double* __restrict__ fr, int e, D2 out[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = two_sum(fr[FR_V + 3 * ((e + 1) % 3) + c], -fr[FR_V + 3 * e + c]);
+}
+
+// z, det and per edge (w, lo, hi) of a pair, recomputed from exact vertex differences
+struct RefinedGeo {
+    double z, det;
+    double w[3], lo[3], hi[3];
+};
+__host__ __device__ __attribute__((noinline)) RefinedGeo refine_geo(const double* __restrict__ fr, double qx, double qy,
+                                                                    double qz, double ih, int kz, bool near_start0,
+                                                                    bool near_start1, bool near_start2);
 
 __host__ __device__ inline void pair_geo(const double* __restrict__ fr, double qx, double qy, double qz, double ih, PairGeo& g) {
 #pragma unroll
@@ -88,19 +149,95 @@ __host__ __device__ inline void pair_geo(const double* __restrict__ fr, double q
         g.d[3 * k + 1] = (fr[FR_V + 3 * k + 1] - qy) * ih;
         g.d[3 * k + 2] = (fr[FR_V + 3 * k + 2] - qz) * ih;
     }
-    g.z = fr[FR_N] * g.d[0] + fr[FR_N + 1] * g.d[1] + fr[FR_N + 2] * g.d[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g.r2[k] = g.d[3 * k] * g.d[3 * k] + g.d[3 * k + 1] * g.d[3 * k + 1] + g.d[3 * k + 2] * g.d[3 * k + 2];
+    // The plane offset from the vertex nearest the query and each edge's line distance from its nearer end: a
+    // product taken from a far vertex carries an absolute rounding of u |d_far|, which near another vertex is a
+    // large relative error of z and w; the faces meeting at that vertex would then see it at inconsistent
+    // positions, and their solid angles (O(1) there, weighted O(1) in the moments) would no longer cancel.
+    const int kz = (g.r2[0] <= g.r2[1]) ? (g.r2[0] <= g.r2[2] ? 0 : 2) : (g.r2[1] <= g.r2[2] ? 1 : 2);
+    const double* dz = g.d + 3 * kz;
+    g.z = fr[FR_N] * dz[0] + fr[FR_N + 1] * dz[1] + fr[FR_N + 2] * dz[2];
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         const double* t = fr + FR_T + 3 * e;
         const double* mu = fr + FR_MU + 3 * e;
         const double* v = g.d + 3 * e;
         const double* q = g.d + 3 * ((e + 1) % 3);
+        const double* wv = g.r2[e] <= g.r2[(e + 1) % 3] ? v : q;
         // both ends projected from their own rounded vertices: near a corner the two edges meeting there must see
         // the same vertex, or the foot-point angles of the two edges disagree at the rounding scale of w
-        g.w[e] = mu[0] * v[0] + mu[1] * v[1] + mu[2] * v[2];
+        g.w[e] = mu[0] * wv[0] + mu[1] * wv[1] + mu[2] * wv[2];
         g.lo[e] = t[0] * v[0] + t[1] * v[1] + t[2] * v[2];
         g.hi[e] = t[0] * q[0] + t[1] * q[1] + t[2] * q[2];
     }
+    // Where the foot point of the query is near an edge line (|w_e| below 1e-3 of the distance of the edge's nearer
+    // vertex) the edge angles atan(a s / (|w| r)) are steep in w and in the chord ends lo, hi, and the rounded unit
+    // vectors of the record leave w_e, lo, hi (and z) with an absolute error of u |d|: the faces meeting at the edge
+    // see the query at inconsistent positions (z small too) or the face's own angles lose digits at a / |w| (z large).
+    // On a sliver face the rounded normal is off by ~u x aspect ratio, so every w_e carries that error against a
+    // small width of the face.  In these cases z, w_e, lo_e, hi_e are recomputed from exact vertex differences
+    // (double-double): z = N.(v_k - q) / |N|, w_e = N.((v - q) x E_e) / (|N| |E_e|), lo_e = E_e.(v_e - q) / |E_e|,
+    // hi_e = E_e.(v_(e+1) - q) / |E_e|, with N = (v1 - v0) x (v2 - v0) and E_e the edge vector.
+    constexpr double TAU2 = 1e-6;
+    bool refine = fr[FR_SLIVER] != 0.0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) refine |= g.w[e] * g.w[e] < TAU2 * fmin(g.r2[e], g.r2[(e + 1) % 3]);
+    g.refined = refine;
+    if (refine) {
+        const RefinedGeo r = refine_geo(fr, qx, qy, qz, ih, kz, g.r2[0] <= g.r2[1], g.r2[1] <= g.r2[2], g.r2[2] <= g.r2[0]);
+        g.z = r.z;
+        g.det = r.det;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            g.w[e] = r.w[e];
+            g.lo[e] = r.lo[e];
+            g.hi[e] = r.hi[e];
+        }
+    }
+}
+
+// z, w_e, lo_e, hi_e from exact vertex differences (see pair_geo); out of line: a rare path that would otherwise
+// add its registers to every caller's.  near_start_e: the edge's start vertex is the nearer end.
+__host__ __device__ __attribute__((noinline)) RefinedGeo refine_geo(const double* __restrict__ fr, double qx, double qy,
+                                                                    double qz, double ih, int kz, bool near_start0,
+                                                                    bool near_start1, bool near_start2) {
+    RefinedGeo r;
+    const double qv[3] = {qx, qy, qz};
+    const bool near_start[3] = {near_start0, near_start1, near_start2};
+    D2 N[3];
+    {
+        D2 E0[3], E2[3];
+        dd_edge(fr, 0, E0);
+        dd_edge(fr, 2, E2);
+        dd_cross(E2, E0, N);                          // (v0 - v2) x (v1 - v0) = (v1 - v0) x (v2 - v0)
+    }
+    const double inl = ih / fr[FR_NLEN];
+    {
+        D2 D[3];
+        dd_vq(fr, kz, qv, D);
+        const double det = dd_dot(N, D);              // N.(v_k - q), the same for every k
+        r.z = det * inl;
+        r.det = det * (ih * ih * ih);
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        D2 E[3], D[3], C[3];
+        dd_edge(fr, e, E);
+        const double il = ih / fr[FR_LEN + e];
+        dd_vq(fr, e, qv, D);
+        r.lo[e] = dd_dot(E, D) * il;
+        if (near_start[e]) {
+            dd_cross(D, E, C);
+            dd_vq(fr, (e + 1) % 3, qv, D);
+        } else {
+            dd_vq(fr, (e + 1) % 3, qv, D);
+            dd_cross(D, E, C);
+        }
+        r.w[e] = dd_dot(N, C) * (inl / fr[FR_LEN + e]);
+        r.hi[e] = dd_dot(E, D) * il;
+    }
+    return r;
 }
 
 // angle of the triangle at the foot point of the query: 2 pi when the foot is strictly inside (every w_e > 0),
@@ -142,10 +279,7 @@ __host__ __device__ inline double piece_phi(double a, double b, double c2, doubl
 template <int FORM>
 __host__ __device__ inline void vertex_radii(const PairGeo& g, double rv[3]) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double* v = g.d + 3 * k;
-        rv[k] = xsqrt<FORM>(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    }
+    for (int k = 0; k < 3; ++k) rv[k] = xsqrt<FORM>(g.r2[k]);
 }
 
 // solid angle of the whole face seen from the query with the reference's convention (0 for a face whose plane
@@ -167,12 +301,30 @@ __host__ __device__ inline double omega_k(const PairGeo& g, const double rv[3]) 
 }
 
 // the same solid angle for the mixed form: 2 atan2(det, den) of the corner vectors (van Oosterom and Strackee), one
-// angle instead of three; where det and den are both below 1e-6 of their scale (the query within ~1e-6 of an edge
-// line in the face's plane, where the formula loses digits) the form above is taken instead.  The FP64 form keeps
+// angle instead of three; where det and den are both below 1e-2 of their scale (the query near an edge of the face,
+// where the formula's angle carries u / max(|det|, |den|) of rounding and the faces of the edge see the query at
+// inconsistent positions) the form above is taken instead.  The FP64 form keeps
 // the form above everywhere: this angle enters the moments with weights of up to ~1e3 (w6), and the corner-vector
 // formula's rounding (~10 u of its scale) would then exceed 1e-12.
 template <int FORM>
 __host__ __device__ inline double omega_face(const PairGeo& g, const double rv[3]) {
+    if (g.refined) {
+        // exact triple product: 2 atan2(det, den) keeps full relative accuracy where den has no cancellation (den
+        // at least la lb lc), in particular for the small solid angle of a sliver seen from afar, which the three-angle
+        // form obtains as 2 pi minus angles of ~pi
+        if (g.z == 0.0) return 0.0;
+        const double* A = g.d;
+        const double* B = g.d + 3;
+        const double* C = g.d + 6;
+        const double la = rv[0], lb = rv[1], lc = rv[2];
+        const double ab = A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
+        const double bc = B[0] * C[0] + B[1] * C[1] + B[2] * C[2];
+        const double ca = C[0] * A[0] + C[1] * A[1] + C[2] * A[2];
+        const double lll = la * lb * lc;
+        const double den = lll + ab * lc + bc * la + ca * lb;
+        if (den >= lll) return 2.0 * angle2<FP64>(g.det, den);
+        return omega_k(g, rv);
+    }
     if (FORM == MIXED) {
         if (g.z == 0.0) return 0.0;
         const double* A = g.d;
@@ -185,7 +337,7 @@ __host__ __device__ inline double omega_face(const PairGeo& g, const double rv[3
         const double ca = C[0] * A[0] + C[1] * A[1] + C[2] * A[2];
         const double den = la * lb * lc + ab * lc + bc * la + ca * lb;
         const double scale = la * lb * lc;
-        if (fabs(det) > 1e-6 * scale || fabs(den) > 1e-6 * scale) return 2.0 * angle2<FORM>(det, den);
+        if (fabs(det) > 1e-2 * scale || fabs(den) > 1e-2 * scale) return 2.0 * angle2<FORM>(det, den);
     }
     return omega_k(g, rv);
 }

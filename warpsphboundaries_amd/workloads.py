@@ -9,6 +9,11 @@
 * ``config3``: the fluid in the unit box [0,1]^3 around an icosphere of radius 0.25 at (0.5, 0.5, 0.5) subdivided to
   81,920 triangles: one closed surface of the 12 outward box triangles and the sphere's triangles turned into the
   cavity; 2,097,152 particles uniform in the box outside the sphere; w4, h = 0.01.
+* ``config4`` (precision stress): 2-D wedge / notch polygons with interior angle U(5, 355) degrees at the tip and
+  particles at d/h = 10^U(-7, 0) from an edge interior (normal offset, both sides) or from a vertex (random
+  direction); 3-D sliver tetrahedra whose base triangle has aspect ratio 10^U(0, 4), particles at the same distances
+  from the base face interior (both sides) or from a base vertex; kernels cubic, w2, w4, w6.
+* ``config5`` (served mix): see ``config5``.
 """
 from __future__ import annotations
 
@@ -161,6 +166,136 @@ def config3(n=2097152, seed=3, level=6, h=0.01):
         pts = np.concatenate([pts, c])
     pts = pts[:n]
     return Workload3(f"box+icosphere{len(F) - 12}", V, F, pts, np.full(n, h), ("w4",), h)
+
+
+@dataclass
+class StressPolygons:
+    """precision stress, 2-D: ``polygons[i]`` counter-clockwise around the solid (the wedge); particle j belongs to
+    polygon ``poly[j]``, sits d_over_h[j] * h from its anchor (``anchor[j]`` 0: an edge interior, 1: a vertex)."""
+    name: str
+    h: float
+    kernels: tuple
+    polygons: list
+    angles_deg: np.ndarray
+    poly: np.ndarray
+    positions: np.ndarray
+    d_over_h: np.ndarray
+    anchor: np.ndarray
+
+
+@dataclass
+class StressTetrahedra:
+    """precision stress, 3-D: tetrahedron i (vertices ``vertices[i]`` [4,3], outward faces ``faces`` [4,3]) has a
+    sliver base triangle (vertices 0, 1, 2) of aspect ratio ``aspect[i]``; particle j belongs to tetrahedron
+    ``tet[j]``, sits d_over_h[j] * h from the base face interior (anchor 0) or from a base vertex (anchor 1)."""
+    name: str
+    h: float
+    kernels: tuple
+    vertices: np.ndarray
+    faces: np.ndarray
+    aspect: np.ndarray
+    tet: np.ndarray
+    positions: np.ndarray
+    d_over_h: np.ndarray
+    anchor: np.ndarray
+
+
+def wedge_polygon(theta, arm, center, rotation, arc_step=np.pi / 8):
+    """tip at ``center`` with interior angle ``theta`` (radians, in (0, 2 pi)): the solid is the sector of radius
+    ``arm`` around the direction ``rotation + pi``, closed by a polygonal arc; counter-clockwise vertices."""
+    m = max(2, int(np.ceil(theta / arc_step)))
+    phi = rotation + np.pi + np.linspace(-theta / 2, theta / 2, m + 1)
+    arc = arm * np.stack([np.cos(phi), np.sin(phi)], 1)
+    return np.vstack([[0.0, 0.0], arc]) + np.asarray(center)
+
+
+def config4_2d(n=262144, n_poly=256, h=0.01, seed=4):
+    """precision stress in 2-D (see the module docstring): ``n_poly`` wedges of arm 4 h, ``n // n_poly`` particles
+    each, half anchored on edge interiors (an edge picked uniformly, t ~ U(0, 1), side +-1), half on vertices (the
+    tip with probability 1/2, else a vertex picked uniformly; a uniform direction)."""
+    g = rng(seed)
+    per = n // n_poly
+    polys, angles, P, D, A, I = [], [], [], [], [], []
+    for i in range(n_poly):
+        theta = np.deg2rad(g.uniform(5.0, 355.0))
+        V = wedge_polygon(theta, 4 * h, g.uniform(0.2, 0.8, 2), g.uniform(0, 2 * np.pi))
+        polys.append(V)
+        angles.append(np.rad2deg(theta))
+        ne = len(V)
+        d = h * 10.0 ** g.uniform(-7.0, 0.0, per)
+        anchor = (np.arange(per) >= per // 2).astype(np.int32)
+        pts = np.empty((per, 2))
+        ke = per // 2
+        e = g.integers(0, ne, ke)
+        a, b = V[e], V[(e + 1) % ne]
+        t = g.uniform(0.0, 1.0, ke)[:, None]
+        tan = (b - a) / np.linalg.norm(b - a, axis=1)[:, None]
+        nrm = np.stack([tan[:, 1], -tan[:, 0]], 1)
+        side = np.where(g.uniform(size=ke) < 0.5, -1.0, 1.0)[:, None]
+        pts[:ke] = a + t * (b - a) + side * d[:ke, None] * nrm
+        kv = per - ke
+        v = np.where(g.uniform(size=kv) < 0.5, 0, g.integers(0, ne, kv))
+        ang = g.uniform(0, 2 * np.pi, kv)
+        pts[ke:] = V[v] + d[ke:, None] * np.stack([np.cos(ang), np.sin(ang)], 1)
+        P.append(pts)
+        D.append(d / h)
+        A.append(anchor)
+        I.append(np.full(per, i, dtype=np.int32))
+    return StressPolygons("stress-2d", h, ("cubic", "w2", "w4", "w6"), polys, np.array(angles), np.concatenate(I),
+                          np.concatenate(P), np.concatenate(D), np.concatenate(A))
+
+
+def _rotation(g):
+    q, r = np.linalg.qr(g.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def config4_3d(n=262144, n_tet=256, h=0.01, seed=44):
+    """precision stress in 3-D (see the module docstring): base triangle A = 0, B = (2h, 0, 0),
+    C = (2h u, 2h / aspect, 0) with u ~ U(0.1, 0.9) and aspect = 10^U(0, 4); apex 2h below the base; a random rotation
+    and a centre in [0.3, 0.7]^3.  Half the particles are anchored on the base face interior (uniform barycentric
+    point, offset d along the normal, either side), half on a base vertex (uniform direction)."""
+    g = rng(seed)
+    per = n // n_tet
+    ell = 2 * h
+    Vs, asp, P, D, A, I = [], [], [], [], [], []
+    faces = np.array([[0, 1, 2], [0, 3, 1], [1, 3, 2], [2, 3, 0]], dtype=np.int32)
+    for i in range(n_tet):
+        aspect = 10.0 ** g.uniform(0.0, 4.0)
+        u = g.uniform(0.1, 0.9)
+        base = np.array([[0.0, 0.0, 0.0], [ell, 0.0, 0.0], [u * ell, ell / aspect, 0.0]])
+        apex = np.array([[base[:, 0].mean(), base[:, 1].mean(), -ell]])
+        R = _rotation(g)
+        c = g.uniform(0.3, 0.7, 3)
+        V = np.vstack([base, apex]) @ R.T + c
+        Vs.append(V)
+        asp.append(aspect)
+        nrm = R[:, 2]                          # the base plane's normal (either orientation)
+        d = h * 10.0 ** g.uniform(-7.0, 0.0, per)
+        ke = per // 2
+        w = g.dirichlet(np.ones(3), ke)
+        side = np.where(g.uniform(size=ke) < 0.5, -1.0, 1.0)[:, None]
+        pts = np.empty((per, 3))
+        pts[:ke] = w @ V[:3] + side * d[:ke, None] * nrm
+        kv = per - ke
+        dirs = g.normal(size=(kv, 3))
+        dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+        pts[ke:] = V[g.integers(0, 3, kv)] + d[ke:, None] * dirs
+        P.append(pts)
+        D.append(d / h)
+        A.append((np.arange(per) >= ke).astype(np.int32))
+        I.append(np.full(per, i, dtype=np.int32))
+    V = np.array(Vs)
+    # outward orientation: the apex lies below the base, so (B - A) x (C - A) must point away from it
+    for i in range(n_tet):
+        a, b, c2, dd = V[i]
+        if np.dot(np.cross(b - a, c2 - a), dd - a) >= 0:
+            raise AssertionError("tetrahedron orientation")
+    return StressTetrahedra("stress-3d", h, ("cubic", "w2", "w4", "w6"), V, faces, np.array(asp), np.concatenate(I),
+                            np.concatenate(P), np.concatenate(D), np.concatenate(A))
 
 
 @dataclass
