@@ -1,5 +1,6 @@
 """The continuous-batching server: results equal alone and in the mix (bitwise), captured-graph replays equal
-eager launches on every replay, HBM admission, a poisoned request isolated from its batch."""
+eager launches on every replay, HBM admission, a poisoned request isolated from its batch; 3-D mesh scenes equal
+the direct Mesh3D evaluation bitwise, including a batch relaunched because its face list was too short."""
 import numpy as np
 import pytest
 import torch
@@ -17,7 +18,7 @@ def _scenes(k=6, seed=21):
 def _serve(scenes, requests, **kw):
     srv = BoundaryServer(**kw)
     for s in scenes:
-        srv.add_scene(s.name, s.loops, s.h)
+        srv.add_scene(s.name, s.loops, s.h, mesh=s.mesh)
     reqs = [srv.submit(name, pos, h, kern) for name, pos, h, kern in requests]
     srv.run()
     return srv, reqs
@@ -108,4 +109,54 @@ def test_plain_form_batches_equal_per_scene_plain():
     _, b = _serve(scenes, reqs, streams=2, form="plain", batched=True)
     _, p = _serve(scenes, reqs, streams=2, form="plain", batched=False)
     for x, y in zip(b, p):
-        assert np.array_equal(x.result, y.result)
+        assert x.error is None and y.error is None, (x.scene, x.error, y.error)
+        assert np.array_equal(x.result, y.result), (x.scene, x.result.shape, float(np.nanmax(np.abs(x.result - y.result))))
+
+
+def _scenes3d(k=3, seed=29):
+    return [s for s in WL.config5(n_scenes=4 * k, seed=seed, n_min=500, n_max=20000, e_min=16, e_max=64) if s.dim == 3]
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_mesh_scenes_equal_direct_evaluation(form):
+    from warpsphboundaries_amd.mesh3d import Mesh3D
+    scenes = _scenes3d()
+    assert scenes and all(s.dim == 3 for s in scenes)
+    reqs = _requests(scenes, steps=2)
+    srv, served = _serve(scenes, reqs, streams=3, form=form)
+    for (name, pos, h, kern), r in zip(reqs, served):
+        sc = next(s for s in scenes if s.name == name)
+        m = Mesh3D(sc.mesh[0], sc.mesh[1], h, device="cuda:0")
+        P = torch.tensor(pos, device="cuda:0")
+        H = torch.full((len(pos),), h, dtype=torch.float64, device="cuda:0")
+        ref = m.evaluate(P, H, kern, form).cpu().numpy()
+        assert r.error is None and r.result.shape == (len(pos), 22)
+        assert np.array_equal(r.result, ref), name
+
+
+def test_mesh_batch_with_short_face_list_is_relaunched():
+    scenes = _scenes3d(2, seed=30)
+    reqs = _requests(scenes, steps=1)
+    _, ref = _serve(scenes, reqs, streams=2)
+    srv = BoundaryServer(streams=2)
+    srv.min_pair_capacity = 1
+    for s in scenes:
+        srv.add_scene(s.name, s.loops, s.h, mesh=s.mesh)
+        srv.scenes[s.name].pairs_per_row = 1e-3            # the first batch's face list is far too short
+    out = [srv.submit(n, p, h, k) for n, p, h, k in reqs]
+    srv.run()
+    assert srv.relaunched_batches >= 1
+    for a, b in zip(ref, out):
+        assert np.array_equal(a.result, b.result)
+
+
+def test_mixed_dimension_device_requests():
+    scenes = _scenes(8, seed=31)
+    assert {s.dim for s in scenes} == {2, 3}
+    reqs = _requests(scenes, steps=1)
+    _, host = _serve(scenes, reqs, streams=4)
+    dev_reqs = [(n, torch.tensor(p, device="cuda:0"), h, k) for n, p, h, k in reqs]
+    _, dev = _serve(scenes, dev_reqs, streams=4)
+    for (n, p, h, k), a, b in zip(reqs, host, dev):
+        assert b.result.shape == (len(p), 12 if p.shape[1] == 2 else 22)
+        assert np.array_equal(a.result, b.result.cpu().numpy()), n

@@ -18,6 +18,12 @@ counterpart: the reference evaluates one position set per call).
   geometry has a device descriptor (grid parameters and array pointers) in one table; a batch gathers the pending
   requests of one kernel from any number of scenes, each row names its geometry, and one adjacency launch plus one
   moment launch (per surface representation index) serve the whole batch.  Host work per batch is a few copies.
+* 3-D scenes (``add_scene(name, None, support_max, mesh=(vertices, faces))``): a resident ``Mesh3D`` (face records,
+  cell grid, walk lists); their requests are batched per (scene, kernel) into one cell sort, one face adjacency +
+  winding launch and one launch of the 22 stored moments.  The face list of a batch is sized from the scene's
+  previous pair counts; the count is read back with the batch's results, and a batch whose count exceeded the list is
+  launched again with the exact size before its requests complete (no host synchronisation on the launch path).  The
+  3-D path has the fp64 and mixed forms; a server of the plain form runs its 3-D scenes in fp64.
 * Captured launches: with ``graphs=True`` (per-scene batches) the adjacency + moment launches of a (scene, kernel,
   stream, row bucket) are captured once in a HIP graph over fixed buffers and replayed (rows beyond the request count
   are parked outside the geometry's grid, where they cost one cell lookup).
@@ -79,6 +85,12 @@ class ResidentScene:
     K: int
     bytes: int
     desc: list = field(default_factory=list)     # descriptor-table index of each geometry
+    mesh: Optional[object] = None                # 3-D scene: its Mesh3D
+    pairs_per_row: float = 8.0                   # 3-D: face-list sizing from the previous batches
+
+    @property
+    def dim(self):
+        return 2 if self.mesh is None else 3
 
 
 @dataclass
@@ -129,16 +141,39 @@ class _BSlot:
 
 
 @dataclass
+class _MSlot:
+    """per-stream fixed buffers of the 3-D batches for one row capacity (the face list grows with the pair count)."""
+    cap: int
+    pos: torch.Tensor
+    sup: torch.Tensor
+    order: torch.Tensor
+    offsets: torch.Tensor
+    chi: torch.Tensor
+    out: torch.Tensor
+    adj_scratch: torch.Tensor
+    host_pos: torch.Tensor
+    host_sup: torch.Tensor
+    host_out: torch.Tensor
+    host_count: torch.Tensor
+    bytes: int
+    faces: Optional[torch.Tensor] = None
+    sort_scratch: Optional[torch.Tensor] = None
+    host_seg: Optional[torch.Tensor] = None
+    dev_seg: Optional[torch.Tensor] = None
+
+
+@dataclass
 class _Batch:
     stream_index: int
     scene: ResidentScene
     kernel: str
     requests: List[Request]
     rows: int
-    slot: _Slot
+    slot: object
     event: torch.cuda.Event
     bytes: int
     keep: Optional[list] = None       # host buffers an asynchronous copy of this batch still reads
+    pair_capacity: int = 0            # 3-D: the face list size this batch was launched with
 
 
 class BoundaryServer:
@@ -163,6 +198,9 @@ class BoundaryServer:
         self.busy = [None] * len(self.streams)
         self.slots: List[Dict[int, _Slot]] = [dict() for _ in self.streams]
         self.bslots = [dict() for _ in self.streams]
+        self.mslots: List[Dict[int, _MSlot]] = [dict() for _ in self.streams]
+        self.relaunched_batches = 0
+        self.min_pair_capacity = 4096       # face-list slack of a 3-D batch beyond the scene's running estimate
         self.scenes: Dict[str, ResidentScene] = {}
         self.pending: deque = deque()
         self.resident_bytes = 0
@@ -174,10 +212,13 @@ class BoundaryServer:
         self.host_seconds = {"submit": 0.0, "launch": 0.0, "poll": 0.0}     # host time spent in the server
 
     # ------------------------------------------------------------------------------------------------ scenes
-    def add_scene(self, name: str, loops, support_max: float):
-        """make a scene resident: ``loops`` = [(vertices [n,2], 'inside' | 'outside')] or SurfaceReps."""
+    def add_scene(self, name: str, loops, support_max: float, mesh=None):
+        """make a scene resident: ``loops`` = [(vertices [n,2], 'inside' | 'outside')] or SurfaceReps (2-D), or
+        ``mesh`` = (vertices [V,3], outward faces [F,3]) or a ``Mesh3D`` built for at least ``support_max`` (3-D)."""
         if name in self.scenes:
             raise ValueError(f"scene {name!r} exists")
+        if mesh is not None:
+            return self._add_mesh_scene(name, mesh, float(support_max))
         reps = [l if isinstance(l, SurfaceRep) else SurfaceRep.polygon(l[0], l[1]) for l in loops]
         geoms = [r.geom(float(support_max), self.device) for r in reps]
         nbytes = sum(int(g.info["device_bytes"]) for g in geoms)
@@ -193,6 +234,23 @@ class BoundaryServer:
             self._desc_host += buf.raw
         self._desc_dev = torch.frombuffer(bytearray(self._desc_host), dtype=torch.uint8).to(self.device)
         self._Kmax = max(self._Kmax, sc.K)
+        self.scenes[name] = sc
+        return sc
+
+    def _add_mesh_scene(self, name, mesh, support_max):
+        from .mesh3d import Mesh3D
+        if isinstance(mesh, Mesh3D):
+            m = mesh
+            if m.host_only or m.device != self.device or m.support_max < support_max:
+                raise ValueError("the Mesh3D must be resident on the server's device with support_max >= the scene's")
+        else:
+            V, F = mesh
+            m = Mesh3D(V, F, support_max, device=self.device)
+        nbytes = int(m.bytes)
+        if self.resident_bytes + nbytes > self.budget:
+            raise MemoryError(f"scene {name!r} needs {nbytes} resident bytes; {self.budget - self.resident_bytes} left")
+        self.resident_bytes += nbytes
+        sc = ResidentScene(name, [], support_max, [], 1, nbytes, mesh=m)
         self.scenes[name] = sc
         return sc
 
@@ -219,10 +277,11 @@ class BoundaryServer:
         if sc is None:
             raise KeyError(f"unknown scene {scene!r}")
         k = kernelName(kernel)
+        dim = sc.dim
         if isinstance(positions, torch.Tensor):
-            if not self.batched:
+            if not self.batched and dim == 2:
                 raise NotImplementedError("device-resident requests use the cross-scene batches")
-            pos = positions.to(device=self.device, dtype=F64).reshape(-1, 2).contiguous()
+            pos = positions.to(device=self.device, dtype=F64).reshape(-1, dim).contiguous()
             if isinstance(supports, torch.Tensor):
                 sup = supports.to(device=self.device, dtype=F64).reshape(-1).contiguous()
                 bad = len(sup) != len(pos) or (len(sup) and bool(((sup <= 0) | (sup > sc.support_max)).any()))
@@ -233,13 +292,13 @@ class BoundaryServer:
             if bad:
                 raise ValueError("supports must lie in (0, support_max] of the scene (the resident cell list is built for it)")
         else:
-            pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 2))
+            pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, dim))
             sup = np.ascontiguousarray(np.broadcast_to(np.asarray(supports, dtype=np.float64), (len(pos),)))
             if len(pos) and not (np.all(sup > 0) and np.all(sup <= sc.support_max)):
                 raise ValueError("supports must lie in (0, support_max] of the scene (the resident cell list is built for it)")
         req = Request(self._rid, scene, k, pos, sup, time.perf_counter())
         if req.on_device:
-            req.out = torch.empty((len(pos), 12), dtype=F64, device=self.device)
+            req.out = torch.empty((len(pos), 12 if dim == 2 else 22), dtype=F64, device=self.device)
             req.ready = torch.cuda.Event()
             req.ready.record(torch.cuda.current_stream(self.device))
         self._rid += 1
@@ -259,6 +318,9 @@ class BoundaryServer:
     # ------------------------------------------------------------------------------------------------ scheduling
     def _footprint(self, sc: ResidentScene, rows: int) -> int:
         cap = _bucket(rows)
+        if sc.dim == 3:
+            pairs = int(sc.pairs_per_row * cap * 1.25) + 4096
+            return cap * (24 + 8 + 4 + 8 + 8 + 176 + 16) + 4 * pairs + int(L.load().wsb_adjacency3d_scratch_bytes(cap)) + (1 << 20)
         K = self._Kmax if self.batched else sc.K
         per_row = 16 + 8 + 4 * K + 4 + 8 + 4 + 96 + 16 + 16 + 96
         return cap * per_row + (1 << 20)
@@ -279,6 +341,13 @@ class BoundaryServer:
         now_done = []
         for i, b in enumerate(self.busy):
             if b is not None and b.event.query():
+                if b.scene is not None and b.scene.dim == 3:
+                    total = int(b.slot.host_count[0])
+                    b.scene.pairs_per_row = max(b.scene.pairs_per_row, total / max(b.rows, 1))
+                    if total > b.pair_capacity:           # face list too short: launch again, exactly sized
+                        self.relaunched_batches += 1
+                        self._launch_mesh(i, b.scene, b.kernel, b.requests, b.rows, b.bytes, pairs=total)
+                        continue
                 now = time.perf_counter()
                 res = None
                 off = 0
@@ -301,8 +370,10 @@ class BoundaryServer:
             head = self.pending[0]
             sc = self.scenes[head.scene]
             group, rows = [], 0
+            dim = sc.dim
             for r in list(self.pending):
-                same = r.kernel == head.kernel and r.on_device == head.on_device and (self.batched or r.scene == head.scene)
+                same = (r.kernel == head.kernel and r.on_device == head.on_device and self.scenes[r.scene].dim == dim
+                        and ((self.batched and dim == 2) or r.scene == head.scene))
                 if same and (not group or rows + r.n <= self.max_batch_rows):
                     group.append(r)
                     rows += r.n
@@ -327,7 +398,9 @@ class BoundaryServer:
                         si = i
                         break
             t1 = time.perf_counter()
-            if self.batched:
+            if dim == 3:
+                self._launch_mesh(si, sc, head.kernel, group, rows, fp)
+            elif self.batched:
                 self._launch_batched(si, head.kernel, group, rows, fp)
             else:
                 self._launch(si, sc, head.kernel, group, rows, fp)
@@ -516,6 +589,98 @@ class BoundaryServer:
         self.busy[si] = _Batch(si, None, kernel, group, rows, s, ev, fp)
         self.launched_batches += 1
         self.launched_rows += rows
+
+    def _mslot(self, si: int, rows: int) -> _MSlot:
+        cap = _bucket(rows)
+        pool = self.mslots[si]
+        s = pool.get(cap)
+        if s is None:
+            dev = self.device
+            ab = max(int(L.load().wsb_adjacency3d_scratch_bytes(cap)), 1)
+            s = _MSlot(cap, torch.empty((cap, 3), dtype=F64, device=dev), torch.empty(cap, dtype=F64, device=dev),
+                       torch.empty(cap, dtype=torch.int32, device=dev), torch.empty(cap + 1, dtype=torch.int64, device=dev),
+                       torch.empty(cap, dtype=F64, device=dev), torch.empty((cap, 22), dtype=F64, device=dev),
+                       torch.empty(ab, dtype=torch.uint8, device=dev),
+                       torch.empty((cap, 3), dtype=F64).pin_memory(), torch.empty(cap, dtype=F64).pin_memory(),
+                       torch.empty((cap, 22), dtype=F64).pin_memory(), torch.empty(1, dtype=torch.int64).pin_memory(),
+                       cap * (24 + 8 + 4 + 8 + 8 + 176) + ab)
+            pool[cap] = s
+        return s
+
+    def _launch_mesh(self, si: int, sc: ResidentScene, kernel: str, group: List[Request], rows: int, fp: int,
+                     pairs: Optional[int] = None):
+        """one 3-D batch of one scene: rows in, cell sort, face adjacency + winding, 22 moments, rows out, the pair
+        count back to the host (pinned) for the face-list check at retirement."""
+        from .kernels import KERNEL_ID
+        s = self._mslot(si, rows)
+        st = self.streams[si]
+        m = sc.mesh
+        lib = L.load()
+        fid = L.form_id(self.form)
+        if fid == 0:
+            fid = L.form_id("fp64")          # the 3-D path's all-FP64 form
+        cap_pairs = int(pairs if pairs is not None else sc.pairs_per_row * rows * 1.25) + self.min_pair_capacity
+        if s.faces is None or s.faces.numel() < cap_pairs:
+            s.faces = torch.empty(cap_pairs, dtype=torch.int32, device=self.device)
+        cap_pairs = s.faces.numel()
+        sb = int(lib.wsb_sort3d_scratch_bytes(m.handle, s.cap))
+        if s.sort_scratch is None or s.sort_scratch.numel() < sb:
+            s.sort_scratch = torch.empty(max(sb, 1), dtype=torch.uint8, device=self.device)
+        on_dev = group[0].on_device
+        ng = len(group)
+        seg = np.zeros((3, ng), dtype=_SEG)
+        off = 0
+        if not on_dev:
+            hp, hs = s.host_pos.numpy(), s.host_sup.numpy()
+        for g, r in enumerate(group):
+            seg["first"][:, g] = off
+            seg["count"][:, g] = r.n
+            if on_dev:
+                seg["src"][0, g] = r.positions.data_ptr()
+                seg["src"][1, g] = r.supports.data_ptr()
+                seg["dst"][2, g] = r.out.data_ptr()
+            else:
+                hp[off:off + r.n] = r.positions
+                hs[off:off + r.n] = r.supports
+            off += r.n
+        raw = seg.view(np.uint8).reshape(-1)
+        if on_dev:
+            if s.host_seg is None or s.host_seg.numel() < raw.size:
+                nb = max(raw.size, 1 << 12)
+                s.host_seg = torch.empty(nb, dtype=torch.uint8).pin_memory()
+                s.dev_seg = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            s.host_seg.numpy()[:raw.size] = raw
+        with torch.cuda.stream(st):
+            sp = L.stream_ptr(self.device)
+            if on_dev:
+                for r in group:
+                    st.wait_event(r.ready)
+                s.dev_seg[:raw.size].copy_(s.host_seg[:raw.size], non_blocking=True)
+                base = s.dev_seg.data_ptr()
+                tab = [ctypes.c_void_p(base + t * ng * seg.dtype.itemsize) for t in range(3)]
+                L.check(lib.wsb_gather_rows(tab[0], ng, 3, L.ptr(s.pos), sp))
+                L.check(lib.wsb_gather_rows(tab[1], ng, 1, L.ptr(s.sup), sp))
+            else:
+                s.pos[:rows].copy_(s.host_pos[:rows], non_blocking=True)
+                s.sup[:rows].copy_(s.host_sup[:rows], non_blocking=True)
+            L.check(lib.wsb_sort3d(m.handle, L.ptr(s.pos), rows, L.ptr(s.order), L.ptr(s.sort_scratch),
+                                   s.sort_scratch.numel(), sp))
+            L.check(lib.wsb_adjacency3d(m.handle, L.ptr(s.pos), L.ptr(s.sup), L.ptr(s.order), rows, L.ptr(s.offsets),
+                                        L.ptr(s.faces), cap_pairs, L.ptr(s.chi), L.ptr(s.adj_scratch),
+                                        s.adj_scratch.numel(), sp))
+            L.check(lib.wsb_moments3d(m.handle, KERNEL_ID[kernel], fid, L.ptr(s.offsets), L.ptr(s.faces), cap_pairs,
+                                      L.ptr(s.chi), L.ptr(s.pos), L.ptr(s.sup), rows, L.OUT_PHYSICAL, L.ptr(s.out), sp))
+            s.host_count.copy_(s.offsets[rows:rows + 1], non_blocking=True)
+            if on_dev:
+                L.check(lib.wsb_scatter_rows(tab[2], ng, 22, L.ptr(s.out), sp))
+            else:
+                s.host_out[:rows].copy_(s.out[:rows], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        self.busy[si] = _Batch(si, sc, kernel, group, rows, s, ev, fp, pair_capacity=cap_pairs)
+        if pairs is None:
+            self.launched_batches += 1
+            self.launched_rows += rows
 
     # ------------------------------------------------------------------------------------------------ stats
     def latency_stats(self, requests=None):

@@ -13,7 +13,7 @@
   particles at d/h = 10^U(-7, 0) from an edge interior (normal offset, both sides) or from a vertex (random
   direction); 3-D sliver tetrahedra whose base triangle has aspect ratio 10^U(0, 4), particles at the same distances
   from the base face interior (both sides) or from a base vertex; kernels cubic, w2, w4, w6.
-* ``config5`` (served mix): see ``config5``.
+* ``config5`` (served mix): 256 scenes, a quarter of them 3-D meshes; see ``config5`` and ``served_scene_3d``.
 """
 from __future__ import annotations
 
@@ -301,19 +301,83 @@ def config4_3d(n=262144, n_tet=256, h=0.01, seed=44):
 @dataclass
 class ServedScene:
     name: str
-    loops: list
+    loops: list                 # 2-D: [(vertices, 'inside' | 'outside')]; None for a 3-D scene
     h: float
     kernel: str
-    positions: np.ndarray
+    positions: np.ndarray       # [n, 2] or [n, 3]
+    mesh: tuple = None          # 3-D: (vertices [V, 3], outward faces [F, 3])
+
+    @property
+    def dim(self):
+        return 2 if self.mesh is None else 3
 
 
-def config5(n_scenes=256, seed=5, n_min=1000, n_max=200000, e_min=16, e_max=8192):
-    """the served mixed-size workload (2-D scenes): star polygons with E = 10^U(log 16, log 8192) edges, particle
-    counts 10^U(3, log 2e5) uniform in [0,1]^2 outside the star, kernel in {w2, w4, w6}, h = dx sqrt(40/pi)
-    (about 40 neighbours, dx = 1/sqrt(n))."""
+def box_surface_subdivided(k, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    """(vertices, faces) of an axis-aligned box whose six sides are each split into k x k squares of two triangles
+    (12 k^2 outward faces), vertices shared along the edges."""
+    lo, hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+    index = {}
+    V, F = [], []
+
+    def vid(ijk):
+        key = tuple(int(v) for v in ijk)
+        j = index.get(key)
+        if j is None:
+            j = index[key] = len(V)
+            V.append(lo + (hi - lo) * np.array(key, dtype=np.float64) / k)
+        return j
+    for axis in range(3):
+        u, v = (axis + 1) % 3, (axis + 2) % 3
+        for side in (0, k):
+            for a in range(k):
+                for b in range(k):
+                    q = []
+                    for da, db in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = [0, 0, 0]
+                        p[axis], p[u], p[v] = side, a + da, b + db
+                        q.append(vid(p))
+                    # (u, v) is counter-clockwise seen from +axis: outward on the high side, reversed on the low side
+                    tris = [(q[0], q[1], q[2]), (q[0], q[2], q[3])]
+                    if side == 0:
+                        tris = [(t[0], t[2], t[1]) for t in tris]
+                    F.extend(tris)
+    return np.array(V), np.array(F, dtype=np.int32)
+
+
+def served_scene_3d(i, g, n_min, n_max):
+    """a served 3-D scene: the unit box around a spherical cavity of radius 0.25 (icosphere of level 4 or 5: 5,120 or
+    20,480 faces) or a box subdivided into 12 k^2 faces with k = 10^U(1, log 81) (1,200 to 78,732 faces, fluid
+    inside); n = 10^U(log n_min, log n_max) particles uniform in the fluid; h = dx (30/pi)^(1/3) (about 40
+    neighbours, dx = n^(-1/3))."""
+    n = int(10 ** g.uniform(np.log10(n_min), np.log10(n_max)))
+    kind = int(g.integers(0, 3))
+    if kind < 2:
+        V, F = box_with_sphere_cavity(level=4 + kind)
+        pts = np.empty((0, 3))
+        while len(pts) < n:
+            c = g.uniform(0.0, 1.0, (int(1.2 * (n - len(pts))) + 64, 3))
+            c = c[np.linalg.norm(c - 0.5, axis=1) > 0.25]
+            pts = np.concatenate([pts, c])
+    else:
+        k = int(round(10 ** g.uniform(1.0, np.log10(81.0))))
+        V, F = box_surface_subdivided(k)
+        pts = g.uniform(0.0, 1.0, (n, 3))
+    h = float((30.0 / np.pi) ** (1.0 / 3.0) / n ** (1.0 / 3.0))
+    return ServedScene(f"scene{i:03d}", None, h, ["w2", "w4", "w6"][int(g.integers(0, 3))], pts[:n], (V, F))
+
+
+def config5(n_scenes=256, seed=5, n_min=1000, n_max=200000, e_min=16, e_max=8192, frac3d=0.25):
+    """the served mixed-size workload: every 1/frac3d-th scene 3-D (``served_scene_3d``, its own random stream), the
+    others 2-D star polygons with E = 10^U(log 16, log 8192) edges, particle counts 10^U(3, log 2e5) uniform in
+    [0,1]^2 outside the star, kernel in {w2, w4, w6}, h = dx sqrt(40/pi) (about 40 neighbours, dx = 1/sqrt(n))."""
     g = rng(seed)
+    g3 = rng(seed + 1000)
+    every = int(round(1.0 / frac3d)) if frac3d > 0 else 0
     scenes = []
     for i in range(n_scenes):
+        if every and i % every == every - 1:
+            scenes.append(served_scene_3d(i, g3, n_min, n_max))
+            continue
         E = int(10 ** g.uniform(np.log10(e_min), np.log10(e_max)))
         n = int(10 ** g.uniform(np.log10(n_min), np.log10(n_max)))
         star = star_polygon(max(E, 3), g=g)
