@@ -198,10 +198,12 @@ int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t* order, vo
  * reduce, fluxbridge/wp_func.py:260-351 element in compact mode): out[n*22] = sum over the particle's faces of
  * the compact face terms + chi * whole-ball vector; flags WSB_OUT_PHYSICAL (h^(|alpha| - |derivative|)),
  * WSB_OUT_NO_INDICATOR, WSB_OUT_ACCUMULATE.  form WSB_FORM_FP64 or WSB_FORM_MIXED.  Pairs at or past
- * `capacity` are ignored. */
+ * `capacity` are ignored.  `scratch` (device, wsb_moments3d_scratch_bytes(n) bytes) holds the list of the
+ * particles with more than 64 faces, each of which one wavefront serves; no allocation, no host synchronisation. */
+size_t wsb_moments3d_scratch_bytes(int n);
 int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets, const int32_t* faces,
                   int64_t capacity, const double* chi, const double* pos, const double* sup, int n, int flags,
-                  double* out, void* stream);
+                  double* out, void* scratch, size_t scratch_bytes, void* stream);
 
 /* ---- host evaluations of the same arithmetic (validation; every pointer is host memory) */
 int wsb_adjacency3d_host(const wsb_mesh3d* m, const double* pos, const double* sup, int n, int64_t* offsets,

@@ -206,3 +206,40 @@ def test_cell_sorted_schedule_changes_nothing():
     ma = m.moments(a, Q, H, "w4", "fp64")
     mb = m.moments(b, Q, H, "w4", "fp64")
     assert torch.equal(ma, mb)
+
+
+def test_heavy_rows_match_host_and_are_batch_independent():
+    # a fine sphere inside a box with a support of ~10 face sizes: particles near the sphere have hundreds of faces
+    V, F = MM.box_surface((0, 0, 0), (1, 1, 1))
+    Vs, Fs = MM.icosphere(4, 0.25, (0.5, 0.5, 0.5))
+    V, F = np.vstack([V, Vs]), np.vstack([F, (Fs + 8)[:, ::-1]])
+    h = 0.12
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(700, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    P = np.vstack([0.5 + d * (0.25 + rng.uniform(0.0, 0.1, 700))[:, None], rng.uniform(0, 1, (300, 3))])
+    H = np.full(len(P), h)
+    m = MM.Mesh3D(V, F, h, device=DEV)
+    Q, HH = torch.tensor(P, device=DEV), torch.tensor(H, device=DEV)
+    adj = m.adjacency(Q, HH)
+    counts = np.diff(adj.offsets.cpu().numpy())
+    assert (counts > 64).sum() > 100 and (counts <= 64).sum() > 50, (counts.max(), (counts > 64).sum())
+    u = _units(H)
+    host = MM.Mesh3D(V, F, h, host_only=True)
+    for k in ("w2", "w6"):
+        fp = None
+        for form in ("fp64", "mixed"):
+            got = m.moments(adj, Q, HH, k, form).cpu().numpy()
+            if form == "fp64":
+                # the device arithmetic against the same arithmetic on the host, summed in order
+                ref = host.evaluate_host(P, H, k, form=form)
+                err = np.abs(got - ref) / u / (1 + np.abs(ref) / u)
+                assert err.max() <= 1e-11, (k, form, err.max())
+                fp = got
+            else:
+                sig = np.abs(fp / u).max(axis=0) + 1e-300
+                assert (np.abs(got - fp) / u <= 1e-6 * sig).all(), (k, (np.abs(got - fp) / u / sig).max())
+            # the same rows in another launch (every other particle dropped): bitwise the same values
+            sub = np.arange(0, len(P), 2)
+            got2 = m.evaluate(Q[sub], HH[sub], k, form).cpu().numpy()
+            assert np.array_equal(got2, got[sub])

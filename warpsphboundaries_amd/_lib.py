@@ -61,8 +61,9 @@ SIGNATURES = {
                                 c_size_t, c_void_p]),
     "wsb_sort3d_scratch_bytes": (c_size_t, [c_void_p, c_int]),
     "wsb_sort3d": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "wsb_moments3d_scratch_bytes": (c_size_t, [c_int]),
     "wsb_moments3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int,
-                              c_void_p, c_void_p]),
+                              c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_adjacency3d_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
     "wsb_winding3d_exact_host": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "wsb_face_terms3d_host": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),

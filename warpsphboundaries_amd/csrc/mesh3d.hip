@@ -522,12 +522,19 @@ struct Mom3Args {
     int nf;
     int flags;
     double* out;
+    int32_t* heavy_count;       // scratch: the number of heavy particles, then their row indices
+    int32_t* heavy_list;
 };
 
-// A workgroup owns TILE3 consecutive particles.  Their (particle, face) pairs are packed densely over the lanes
-// (pair j -> its particle by a binary search of the tile's offsets in LDS); every lane evaluates one pair, the 23
-// face accumulators go through LDS and each particle's lane adds its own pairs in face order (deterministic sums,
-// independent of the packing).
+// A particle with more than HEAVY3 pairs is served by one wavefront (heavy22_kernel): pair j of it goes to lane j % 64,
+// each lane sums its pairs in order and a butterfly over the 64 lanes adds the lane sums (every lane ends with the
+// same value: each level adds the same two operands).  The other particles are served in tiles (tile22_kernel): a
+// workgroup owns TILE3 consecutive particles, their pairs are packed densely over the lanes (pair j -> its particle
+// by a binary search of the tile's offsets in LDS); every lane evaluates one pair, the 23 face accumulators go through
+// LDS and each particle's lane adds its own pairs in face order.  Either way a particle's sums depend only on its own
+// pairs, never on the packing or on the other particles of its batch.
+constexpr int HEAVY3 = 64;
+
 template <int KID, int FORM>
 __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
     using BlockScan = hipcub::BlockScan<int, TILE3>;
@@ -538,18 +545,20 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
     const int tid = threadIdx.x;
     const int row = blockIdx.x * TILE3 + tid;
     const bool live = row < a.n;
-    int64_t first = 0;
     int cnt = 0;
     double h = 1.0;
     if (live) {
-        first = a.offsets[row];
-        cnt = (int)(a.offsets[row + 1] - first);
+        cnt = (int)(a.offsets[row + 1] - a.offsets[row]);
         s_p[0][tid] = a.pos[3 * (int64_t)row];
         s_p[1][tid] = a.pos[3 * (int64_t)row + 1];
         s_p[2][tid] = a.pos[3 * (int64_t)row + 2];
         h = a.sup[row];
-        s_p[3][tid] = 1.0 / h;
+        // a heavy particle (served by heavy22_kernel) is marked by a negative inverse support in LDS: its pairs stay
+        // in the packing (the slot of pair j is then the tile's first slot + j) but are not evaluated here
+        s_p[3][tid] = cnt > HEAVY3 ? -1.0 : 1.0 / h;
     }
+    const bool heavy = cnt > HEAVY3;
+    if (heavy) a.heavy_list[atomicAdd(a.heavy_count, 1)] = row;
     int off, total;
     BlockScan(scan_tmp).ExclusiveSum(cnt, off, total);
     s_off[tid] = off;
@@ -573,13 +582,14 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
             }
             const int64_t slot = tile_first + j;
             const int f = slot < a.capacity ? a.faces[slot] : -1;
-            if (f >= 0 && f < a.nf)
-                m3::face_terms<KID, FORM>(a.frec + m3::FREC * (int64_t)f, s_p[0][lo], s_p[1][lo], s_p[2][lo], s_p[3][lo], c);
+            const double ihl = s_p[3][lo];
+            if (f >= 0 && f < a.nf && ihl > 0.0)
+                m3::face_terms<KID, FORM>(a.frec + m3::FREC * (int64_t)f, s_p[0][lo], s_p[1][lo], s_p[2][lo], ihl, c);
         }
 #pragma unroll
         for (int k = 0; k < m3::NACC; ++k) s_res[k][tid] = c[k];
         __syncthreads();
-        const int b0 = max(off, base), b1 = min(off + cnt, base + TILE3);
+        const int b0 = max(off, base), b1 = heavy ? b0 : min(off + cnt, base + TILE3);
         for (int jj = b0; jj < b1; ++jj) {
 #pragma unroll
             for (int k = 0; k < m3::NACC; ++k) acc[k] += s_res[k][jj - base];
@@ -601,15 +611,66 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
     const bool accumulate = (a.flags & WSB_OUT_ACCUMULATE) != 0;
     for (int idx = tid; idx < nrows * m3::NOUT; idx += TILE3) {
         const int rr = idx / m3::NOUT, k = idx - rr * m3::NOUT;
+        if (s_p[3][rr] < 0.0) continue;                // stored by heavy22_kernel
         const double v = s_res[k][rr];
         if (accumulate) dst[idx] += v;
         else dst[idx] = v;
     }
 }
 
+// the particles with more than HEAVY3 pairs (listed by tile22_kernel), one wavefront each, a fixed grid of wavefronts
+// taking them in turn
+template <int KID, int FORM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void heavy22_kernel(Mom3Args a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave_id = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const bool accumulate = (a.flags & WSB_OUT_ACCUMULATE) != 0;
+    const int nheavy = *a.heavy_count;
+    for (int64_t q = wave_id; q < nheavy; q += nwaves) {
+        const int64_t r = a.heavy_list[q];
+        const int64_t first = a.offsets[r];
+        const int rc = (int)(a.offsets[r + 1] - first);
+        const double px = a.pos[3 * r], py = a.pos[3 * r + 1], pz = a.pos[3 * r + 2];
+        const double h = a.sup[r];
+        const double ih = 1.0 / h;
+        double acc[m3::NACC];
+#pragma unroll
+        for (int k = 0; k < m3::NACC; ++k) acc[k] = 0.0;
+        for (int j = lane; j < rc; j += 64) {
+            const int64_t slot = first + j;
+            const int f = slot < a.capacity ? a.faces[slot] : -1;
+            if (f >= 0 && f < a.nf) m3::face_terms<KID, FORM>(a.frec + m3::FREC * (int64_t)f, px, py, pz, ih, acc);
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+            for (int k = 0; k < m3::NACC; ++k) acc[k] += __shfl_xor(acc[k], o);
+        }
+        double out[m3::NOUT];
+        const double chi = (a.flags & WSB_OUT_NO_INDICATOR) ? 0.0 : a.chi[r];
+        m3::finalize22<KID>(acc, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, out);
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < m3::NOUT; ++k) v = (k == lane) ? out[k] : v;
+        if (lane < m3::NOUT) {
+            double* dst = a.out + r * m3::NOUT + lane;
+            if (accumulate) *dst += v;
+            else *dst = v;
+        }
+    }
+}
+
+__global__ void zero_count3_kernel(int32_t* c) { *c = 0; }
+
 template <int KID, int FORM>
 int launch22(const Mom3Args& a, hipStream_t s) {
+    hipLaunchKernelGGL(zero_count3_kernel, dim3(1), dim3(1), 0, s, a.heavy_count);
     hipLaunchKernelGGL((tile22_kernel<KID, FORM>), dim3((a.n + TILE3 - 1) / TILE3), dim3(TILE3), 0, s, a);
+    WSB_HIP(hipGetLastError());
+    // up to 4,096 wavefronts (16 per CU), one per listed particle at a time
+    const int blocks = (int)std::min<int64_t>(((int64_t)a.n + 3) / 4, 1024);
+    hipLaunchKernelGGL((heavy22_kernel<KID, FORM>), dim3(blocks), dim3(256), 0, s, a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
@@ -813,16 +874,25 @@ extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const dou
     return WSB_OK;
 }
 
+extern "C" size_t wsb_moments3d_scratch_bytes(int n) {
+    if (n <= 0) return 0;
+    return 256 + (((size_t)n * sizeof(int32_t) + 255) & ~(size_t)255);
+}
+
 extern "C" int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets, const int32_t* faces,
                              int64_t capacity, const double* chi, const double* pos, const double* sup, int n, int flags, double* out,
-                             void* stream) {
+                             void* scratch, size_t scratch_bytes, void* stream) {
     WSB_REQUIRE(m, WSB_EINVAL, "null mesh");
     WSB_REQUIRE(!m->host_only, WSB_EINVAL, "host-only mesh has no device tables");
     WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(offsets && pos && sup && out, WSB_EINVAL, "null buffer");
     WSB_REQUIRE(chi || (flags & WSB_OUT_NO_INDICATOR), WSB_EINVAL, "winding numbers required");
-    Mom3Args a{offsets, faces, faces ? capacity : 0, chi, pos, sup, n, m->d_frec, m->nf, flags, out};
+    WSB_REQUIRE(scratch && scratch_bytes >= wsb_moments3d_scratch_bytes(n), WSB_ENOMEM,
+                "3-D moments need %zu scratch bytes (got %zu)", wsb_moments3d_scratch_bytes(n), scratch_bytes);
+    int32_t* hc = static_cast<int32_t*>(scratch);
+    int32_t* hl = reinterpret_cast<int32_t*>(static_cast<char*>(scratch) + 256);
+    Mom3Args a{offsets, faces, faces ? capacity : 0, chi, pos, sup, n, m->d_frec, m->nf, flags, out, hc, hl};
     hipStream_t s = wsb::as_stream(stream);
     switch (form) {
         case WSB_FORM_FP64: return dispatch22<m3::FP64>(kernel_id, a, s);

@@ -85,6 +85,7 @@ class Mesh3D:
         self.grid_lo, self.cell = tuple(g[:3]), g[3]
         self._scratch = None
         self._sort_scratch = None
+        self._mom_scratch = None
         self._cap = 0
 
     def close(self):
@@ -177,9 +178,11 @@ class Mesh3D:
         fid = L.form_id(form)
         if fid == 0:
             raise NotImplementedError("the 3-D path provides the fp64 and mixed forms")
-        L.check(L.load().wsb_moments3d(self._h, KERNEL_ID[_kernel_name(kernel)], fid, L.ptr(adj.offsets), L.ptr(adj.faces),
-                                       adj.faces.numel(), L.ptr(adj.chi), L.ptr(pos), L.ptr(sup), n, flags, L.ptr(out),
-                                       L.stream_ptr(pos.device)))
+        lib = L.load()
+        scratch = self._buf("_mom_scratch", lib.wsb_moments3d_scratch_bytes(n), pos.device)
+        L.check(lib.wsb_moments3d(self._h, KERNEL_ID[_kernel_name(kernel)], fid, L.ptr(adj.offsets), L.ptr(adj.faces),
+                                  adj.faces.numel(), L.ptr(adj.chi), L.ptr(pos), L.ptr(sup), n, flags, L.ptr(out),
+                                  L.ptr(scratch), scratch.numel(), L.stream_ptr(pos.device)))
         return out
 
     def evaluate(self, pos, sup, kernel="w2", form="fp64", physical=True):

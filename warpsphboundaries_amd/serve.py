@@ -112,6 +112,7 @@ class _Slot:
     bytes: int
     graphs: dict = field(default_factory=dict)
     filled: int = 0           # rows of lpos / lsup holding request data (the rest is parked)
+    ready: Optional[torch.cuda.Event] = None   # the initialising fills (current stream) are done
 
 
 _SEG = np.dtype([("src", "<u8"), ("dst", "<u8"), ("first", "<i8"), ("count", "<i8"), ("value", "<i8")])
@@ -138,6 +139,7 @@ class _BSlot:
     sort_scratch: Optional[torch.Tensor] = None
     host_seg: Optional[torch.Tensor] = None      # pinned segment tables (rewritten only after the slot's batch retired)
     dev_seg: Optional[torch.Tensor] = None
+    ready: Optional[torch.cuda.Event] = None
 
 
 @dataclass
@@ -160,6 +162,22 @@ class _MSlot:
     sort_scratch: Optional[torch.Tensor] = None
     host_seg: Optional[torch.Tensor] = None
     dev_seg: Optional[torch.Tensor] = None
+    ready: Optional[torch.cuda.Event] = None
+    mom_scratch: Optional[torch.Tensor] = None
+
+
+def _mark_ready(slot, device):
+    """slot buffers are allocated (and filled) on the current stream; a batch stream waits for that once."""
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream(device))
+    slot.ready = ev
+    return slot
+
+
+def _wait_ready(slot, stream):
+    if slot.ready is not None:
+        stream.wait_event(slot.ready)
+        slot.ready = None
 
 
 @dataclass
@@ -432,7 +450,7 @@ class BoundaryServer:
                       torch.empty((cap, 2), dtype=F64).pin_memory(), torch.empty(cap, dtype=F64).pin_memory(),
                       torch.empty((cap, 12), dtype=F64).pin_memory(),
                       torch.empty(max(sort_bytes, 1), dtype=torch.uint8, device=dev), cap * (16 + 8 + 4 * Kcap + 4 + 8 + 4 + 96))
-            pool[key] = s
+            pool[key] = _mark_ready(s, dev)
         return s
 
     def _evaluate(self, sc: ResidentScene, kernel: str, s: _Slot, n: int, sort: bool):
@@ -458,6 +476,7 @@ class BoundaryServer:
             off += r.n
         use_graph = self.use_graphs
         with torch.cuda.stream(st):
+            _wait_ready(s, st)
             s.lpos[:rows].copy_(s.host_pos[:rows], non_blocking=True)
             s.lsup[:rows].copy_(s.host_sup[:rows], non_blocking=True)
             if use_graph:
@@ -497,7 +516,7 @@ class BoundaryServer:
                        torch.empty((cap, 2), dtype=F64).pin_memory(), torch.empty(cap, dtype=F64).pin_memory(),
                        torch.empty(cap, dtype=torch.int32).pin_memory(), torch.empty((cap, 12), dtype=F64).pin_memory(),
                        cap * (16 + 8 + 4 + 4 * K + 4 + 8 + 96))
-            pool[cap] = s
+            pool[cap] = _mark_ready(s, dev)
         return s
 
     def _launch_batched(self, si: int, kernel: str, group: List[Request], rows: int, fp: int):
@@ -542,6 +561,7 @@ class BoundaryServer:
         s.host_seg.numpy()[:raw.size] = raw
         sb = seg.dtype.itemsize
         with torch.cuda.stream(st):
+            _wait_ready(s, st)
             if on_dev:
                 for r in group:
                     st.wait_event(r.ready)
@@ -597,14 +617,16 @@ class BoundaryServer:
         if s is None:
             dev = self.device
             ab = max(int(L.load().wsb_adjacency3d_scratch_bytes(cap)), 1)
+            mb = max(int(L.load().wsb_moments3d_scratch_bytes(cap)), 1)
             s = _MSlot(cap, torch.empty((cap, 3), dtype=F64, device=dev), torch.empty(cap, dtype=F64, device=dev),
                        torch.empty(cap, dtype=torch.int32, device=dev), torch.empty(cap + 1, dtype=torch.int64, device=dev),
                        torch.empty(cap, dtype=F64, device=dev), torch.empty((cap, 22), dtype=F64, device=dev),
                        torch.empty(ab, dtype=torch.uint8, device=dev),
                        torch.empty((cap, 3), dtype=F64).pin_memory(), torch.empty(cap, dtype=F64).pin_memory(),
                        torch.empty((cap, 22), dtype=F64).pin_memory(), torch.empty(1, dtype=torch.int64).pin_memory(),
-                       cap * (24 + 8 + 4 + 8 + 8 + 176) + ab)
-            pool[cap] = s
+                       cap * (24 + 8 + 4 + 8 + 8 + 176) + ab + mb)
+            s.mom_scratch = torch.empty(mb, dtype=torch.uint8, device=dev)
+            pool[cap] = _mark_ready(s, dev)
         return s
 
     def _launch_mesh(self, si: int, sc: ResidentScene, kernel: str, group: List[Request], rows: int, fp: int,
@@ -652,6 +674,7 @@ class BoundaryServer:
             s.host_seg.numpy()[:raw.size] = raw
         with torch.cuda.stream(st):
             sp = L.stream_ptr(self.device)
+            _wait_ready(s, st)
             if on_dev:
                 for r in group:
                     st.wait_event(r.ready)
@@ -669,7 +692,8 @@ class BoundaryServer:
                                         L.ptr(s.faces), cap_pairs, L.ptr(s.chi), L.ptr(s.adj_scratch),
                                         s.adj_scratch.numel(), sp))
             L.check(lib.wsb_moments3d(m.handle, KERNEL_ID[kernel], fid, L.ptr(s.offsets), L.ptr(s.faces), cap_pairs,
-                                      L.ptr(s.chi), L.ptr(s.pos), L.ptr(s.sup), rows, L.OUT_PHYSICAL, L.ptr(s.out), sp))
+                                      L.ptr(s.chi), L.ptr(s.pos), L.ptr(s.sup), rows, L.OUT_PHYSICAL, L.ptr(s.out),
+                                      L.ptr(s.mom_scratch), s.mom_scratch.numel(), sp))
             s.host_count.copy_(s.offsets[rows:rows + 1], non_blocking=True)
             if on_dev:
                 L.check(lib.wsb_scatter_rows(tab[2], ng, 22, L.ptr(s.out), sp))
