@@ -47,6 +47,29 @@ struct wsb_mesh3d {
 namespace {
 
 // whole-element test of the reference, operation order kept (no contraction)
+// keep_face decided without its divisions wherever the outcome is certain: the squared distance to the face's FP32
+// box rounded outward (a lower bound of the true one, evaluated with relative rounding below 1e-15) beyond
+// h^2 (1 + 1e-9) rejects, the squared distance to the exact box below h^2 (1 - 1e-9) keeps (the reference's
+// rounded sum differs from the exact quotient by less than 1e-15 relative, its threshold is 1 + 1.42e-14); only the
+// band between them takes the reference's arithmetic
+__host__ __device__ inline bool keep_face(const double* __restrict__ box, double px, double py, double pz, double h);
+
+__host__ __device__ inline bool keep_face_fast(const float4& l, const float4& u, const double* __restrict__ box, double px,
+                                               double py, double pz, double h) {
+    const double h2 = h * h;
+    {
+        const double dx = fmax(0.0, fmax((double)l.x - px, px - (double)u.x));
+        const double dy = fmax(0.0, fmax((double)l.y - py, py - (double)u.y));
+        const double dz = fmax(0.0, fmax((double)l.z - pz, pz - (double)u.z));
+        if (dx * dx + dy * dy + dz * dz > h2 * (1.0 + 1e-9)) return false;
+    }
+    const double dx = fmax(0.0, fmax(box[0] - px, px - box[3]));
+    const double dy = fmax(0.0, fmax(box[1] - py, py - box[4]));
+    const double dz = fmax(0.0, fmax(box[2] - pz, pz - box[5]));
+    if (dx * dx + dy * dy + dz * dz < h2 * (1.0 - 1e-9)) return true;
+    return keep_face(box, px, py, pz, h);
+}
+
 __host__ __device__ inline bool keep_face(const double* __restrict__ box, double px, double py, double pz, double h) {
 #pragma clang fp contract(off)
     const double p[3] = {px, py, pz};
@@ -376,56 +399,82 @@ struct Adj3Args {
     double* chi;
 };
 
+constexpr int ADJ_UNROLL = 8;
+
 // one thread per particle: candidates of its cell through the element test; the count pass also evaluates the
 // winding number (face-free cell, axis walk, or -- when a predicate of the walk is within rounding -- the exact
 // per-face sum, evaluated by the whole wavefront for each such lane in turn)
-template <bool FILL>
+// MODE 0: pair counts, 1: face lists, 2: winding numbers (apart from the candidate loop: the walk and the exact sum
+// need three times the registers of the loop, which then runs at a quarter of the occupancy)
+template <int MODE>
 __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
+    constexpr bool FILL = MODE == 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = t < a.n;
     const int i = live ? (a.order ? a.order[t] : t) : 0;
+    // MODE 2 serves only the particles the count pass left undecided (NaN winding number)
+    double chi = 0.0;
+    bool todo = live;
+    if (MODE == 2 && live) {
+        chi = a.chi[i];
+        todo = chi != chi;
+    }
     double px = 0.0, py = 0.0, pz = 0.0, h = 1.0;
-    if (live) {
+    if (todo) {
         px = a.pos[3 * (int64_t)i];
         py = a.pos[3 * (int64_t)i + 1];
         pz = a.pos[3 * (int64_t)i + 2];
         h = a.sup[i];
     }
     int c[3] = {0, 0, 0};
-    const bool in = live && m3::cell_of(a.g, px, py, pz, c);
+    const bool in = todo && m3::cell_of(a.g, px, py, pz, c);
     int64_t cnt = 0;
-    if (in) {
+    if (MODE != 2 && in) {
         const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
         const double reach = h * (1.0 + 1e-9);
         const int64_t base = FILL ? a.offsets[i] : 0;
         const int it1 = a.g.cstart[cid + 1];
-        // four candidates per step: their 32-byte records are loaded together (independent loads in flight), then
-        // filtered in list order; the exact test only for boxes within h on every axis
-        for (int it = a.g.cstart[cid]; it < it1; it += 4) {
-            float4 l[4], u[4];
+        // ADJ_UNROLL candidates per step: their 32-byte records are loaded together (independent loads in flight),
+        // then filtered in list order; the exact test only for boxes within h on every axis
+        for (int it = a.g.cstart[cid]; it < it1; it += ADJ_UNROLL) {
+            float4 l[ADJ_UNROLL], u[ADJ_UNROLL];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < ADJ_UNROLL; ++q) {
                 const int j = min(it + q, it1 - 1);
                 l[q] = a.g.crec[2 * (int64_t)j];
                 u[q] = a.g.crec[2 * (int64_t)j + 1];
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < ADJ_UNROLL; ++q) {
                 if (it + q >= it1) break;
                 if ((double)l[q].x - px > reach || px - (double)u[q].x > reach || (double)l[q].y - py > reach ||
                     py - (double)u[q].y > reach || (double)l[q].z - pz > reach || pz - (double)u[q].z > reach)
                     continue;
                 const int f = m3::bits_int(l[q].w);
-                if (!keep_face(a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
+                if (!keep_face_fast(l[q], u[q], a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
                 if (FILL && base + cnt < a.capacity) a.faces[base + cnt] = f;
                 ++cnt;
             }
         }
     }
-    if (FILL) return;
-    double chi = 0.0;
+    if (MODE == 1) return;
+    if (MODE == 0) {
+        if (live) {
+            a.counts[i] = cnt;
+            // the winding number where it is a lookup (outside the grid, a face-free cell); NaN: the walk decides
+            if (a.chi) {
+                double w = 0.0;
+                if (in) {
+                    const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
+                    w = (a.g.cwalk && a.g.cwalk[cid] >= 0) ? __builtin_nan("") : a.g.cchi[cid];
+                }
+                a.chi[i] = w;
+            }
+        }
+        return;
+    }
     bool need = false;
-    if (in && a.g.cwalk) need = !m3::ray_winding(a.g, px, py, pz, c, &chi);
+    if (todo) need = !m3::ray_winding(a.g, px, py, pz, c, &chi);
     uint64_t mask = __ballot(need);
     const int lane = threadIdx.x & 63;
     while (mask) {
@@ -437,10 +486,7 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
         if (lane == src) chi = s / (4.0 * m3::PI);
         mask &= mask - 1;
     }
-    if (live) {
-        a.counts[i] = cnt;
-        if (a.chi) a.chi[i] = chi;
-    }
+    if (todo) a.chi[i] = chi;
 }
 
 __global__ void offsets_head_kernel(int64_t* offsets) { offsets[0] = 0; }
@@ -864,11 +910,15 @@ extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const dou
     size_t tmp_bytes = scratch_bytes - ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255);
     Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi};
     const dim3 grid((n + 255) / 256), block(256);
-    hipLaunchKernelGGL(adj3_kernel<false>, grid, block, 0, s, a);
+    hipLaunchKernelGGL(adj3_kernel<0>, grid, block, 0, s, a);
     WSB_HIP(hipGetLastError());
     WSB_HIP(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, counts, offsets + 1, n, s));
     if (faces && capacity > 0) {
-        hipLaunchKernelGGL(adj3_kernel<true>, grid, block, 0, s, a);
+        hipLaunchKernelGGL(adj3_kernel<1>, grid, block, 0, s, a);
+        WSB_HIP(hipGetLastError());
+    }
+    if (chi) {
+        hipLaunchKernelGGL(adj3_kernel<2>, grid, block, 0, s, a);
         WSB_HIP(hipGetLastError());
     }
     return WSB_OK;
