@@ -47,22 +47,14 @@ struct wsb_mesh3d {
 namespace {
 
 // whole-element test of the reference, operation order kept (no contraction)
-// keep_face decided without its divisions wherever the outcome is certain: the squared distance to the face's FP32
-// box rounded outward (a lower bound of the true one, evaluated with relative rounding below 1e-15) beyond
-// h^2 (1 + 1e-9) rejects, the squared distance to the exact box below h^2 (1 - 1e-9) keeps (the reference's
-// rounded sum differs from the exact quotient by less than 1e-15 relative, its threshold is 1 + 1.42e-14); only the
-// band between them takes the reference's arithmetic
+// keep_face decided without its divisions wherever the outcome is certain: the squared distance to the exact box
+// below h^2 (1 - 1e-9) keeps (the reference's rounded sum differs from the exact quotient by less than 1e-15
+// relative, its threshold is 1 + 1.42e-14); only the band above takes the reference's arithmetic.  (Candidates far
+// from the particle are rejected before, in FP32, by prefilter32.)
 __host__ __device__ inline bool keep_face(const double* __restrict__ box, double px, double py, double pz, double h);
 
-__host__ __device__ inline bool keep_face_fast(const float4& l, const float4& u, const double* __restrict__ box, double px,
-                                               double py, double pz, double h) {
+__host__ __device__ inline bool keep_face_fast(const double* __restrict__ box, double px, double py, double pz, double h) {
     const double h2 = h * h;
-    {
-        const double dx = fmax(0.0, fmax((double)l.x - px, px - (double)u.x));
-        const double dy = fmax(0.0, fmax((double)l.y - py, py - (double)u.y));
-        const double dz = fmax(0.0, fmax((double)l.z - pz, pz - (double)u.z));
-        if (dx * dx + dy * dy + dz * dz > h2 * (1.0 + 1e-9)) return false;
-    }
     const double dx = fmax(0.0, fmax(box[0] - px, px - box[3]));
     const double dy = fmax(0.0, fmax(box[1] - py, py - box[4]));
     const double dz = fmax(0.0, fmax(box[2] - pz, pz - box[5]));
@@ -401,6 +393,32 @@ struct Adj3Args {
 
 constexpr int ADJ_UNROLL = 8;
 
+// Conservative FP32 rejection of a candidate from its FP32 box rounded outward (l, u): farther than h (1 + 1e-6) on one
+// axis, or squared distance beyond h^2 (1 + 1e-5).  `tol` bounds the FP32 rounding of the particle's coordinates and of the
+// differences (2^-23 (|p| + |box|) at most; tol = 1e-6 (1 + the largest coordinate) covers it with a wide margin),
+// and every distance is shortened by tol before it is compared, so a rejected candidate is farther than h on that axis
+// (or than h (1 + 4e-6) in all) in exact arithmetic: the element test would reject it too.
+struct Near32 {
+    float px, py, pz, reach, h2, tol;
+};
+__device__ inline Near32 near32(double px, double py, double pz, double h) {
+    Near32 q;
+    q.px = (float)px;
+    q.py = (float)py;
+    q.pz = (float)pz;
+    q.tol = 1e-6f * (1.0f + fmaxf(fabsf(q.px), fmaxf(fabsf(q.py), fabsf(q.pz))));
+    q.reach = (float)(h * (1.0 + 1e-6));     // above h after the FP32 rounding (relative 6e-8)
+    q.h2 = (float)(h * h);
+    return q;
+}
+__device__ inline bool prefilter32(const Near32& q, const float4& l, const float4& u) {
+    const float dx = fmaxf(0.0f, fmaxf(l.x - q.px, q.px - u.x) - q.tol);
+    const float dy = fmaxf(0.0f, fmaxf(l.y - q.py, q.py - u.y) - q.tol);
+    const float dz = fmaxf(0.0f, fmaxf(l.z - q.pz, q.pz - u.z) - q.tol);
+    if (dx > q.reach || dy > q.reach || dz > q.reach) return false;
+    return dx * dx + dy * dy + dz * dz <= q.h2 * (1.0f + 1e-5f);
+}
+
 // one thread per particle: candidates of its cell through the element test; the count pass also evaluates the
 // winding number (face-free cell, axis walk, or -- when a predicate of the walk is within rounding -- the exact
 // per-face sum, evaluated by the whole wavefront for each such lane in turn)
@@ -431,7 +449,7 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     int64_t cnt = 0;
     if (MODE != 2 && in) {
         const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
-        const double reach = h * (1.0 + 1e-9);
+        const Near32 nq = near32(px, py, pz, h);
         const int64_t base = FILL ? a.offsets[i] : 0;
         const int it1 = a.g.cstart[cid + 1];
         // ADJ_UNROLL candidates per step: their 32-byte records are loaded together (independent loads in flight),
@@ -447,11 +465,9 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
 #pragma unroll
             for (int q = 0; q < ADJ_UNROLL; ++q) {
                 if (it + q >= it1) break;
-                if ((double)l[q].x - px > reach || px - (double)u[q].x > reach || (double)l[q].y - py > reach ||
-                    py - (double)u[q].y > reach || (double)l[q].z - pz > reach || pz - (double)u[q].z > reach)
-                    continue;
+                if (!prefilter32(nq, l[q], u[q])) continue;
                 const int f = m3::bits_int(l[q].w);
-                if (!keep_face_fast(l[q], u[q], a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
+                if (!keep_face_fast(a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
                 if (FILL && base + cnt < a.capacity) a.faces[base + cnt] = f;
                 ++cnt;
             }
