@@ -411,12 +411,16 @@ __device__ inline Near32 near32(double px, double py, double pz, double h) {
     q.h2 = (float)(h * h);
     return q;
 }
-__device__ inline bool prefilter32(const Near32& q, const float4& l, const float4& u) {
-    const float dx = fmaxf(0.0f, fmaxf(l.x - q.px, q.px - u.x) - q.tol);
-    const float dy = fmaxf(0.0f, fmaxf(l.y - q.py, q.py - u.y) - q.tol);
-    const float dz = fmaxf(0.0f, fmaxf(l.z - q.pz, q.pz - u.z) - q.tol);
-    if (dx > q.reach || dy > q.reach || dz > q.reach) return false;
-    return dx * dx + dy * dy + dz * dz <= q.h2 * (1.0f + 1e-5f);
+// 0: rejected, 1: kept, 2: undecided (the FP64 test decides).  Kept: the distances to the box shrunk by tol (inside the
+// exact box, whose bounds lie within an FP32 ulp of l and u) lengthened by tol again bound the exact distances from
+// above; below h^2 (1 - 1e-5) in all, the element test keeps the candidate.
+__device__ inline int classify32(const Near32& q, const float4& l, const float4& u) {
+    const float ax = fmaxf(l.x - q.px, q.px - u.x), ay = fmaxf(l.y - q.py, q.py - u.y), az = fmaxf(l.z - q.pz, q.pz - u.z);
+    const float dx = fmaxf(0.0f, ax - q.tol), dy = fmaxf(0.0f, ay - q.tol), dz = fmaxf(0.0f, az - q.tol);
+    if (dx > q.reach || dy > q.reach || dz > q.reach) return 0;
+    if (dx * dx + dy * dy + dz * dz > q.h2 * (1.0f + 1e-5f)) return 0;
+    const float ux = fmaxf(0.0f, ax + 3.0f * q.tol), uy = fmaxf(0.0f, ay + 3.0f * q.tol), uz = fmaxf(0.0f, az + 3.0f * q.tol);
+    return ux * ux + uy * uy + uz * uz < q.h2 * (1.0f - 1e-5f) ? 1 : 2;
 }
 
 // one thread per particle: candidates of its cell through the element test; the count pass also evaluates the
@@ -465,9 +469,10 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
 #pragma unroll
             for (int q = 0; q < ADJ_UNROLL; ++q) {
                 if (it + q >= it1) break;
-                if (!prefilter32(nq, l[q], u[q])) continue;
+                const int k32 = classify32(nq, l[q], u[q]);
+                if (k32 == 0) continue;
                 const int f = m3::bits_int(l[q].w);
-                if (!keep_face_fast(a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
+                if (k32 == 2 && !keep_face_fast(a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
                 if (FILL && base + cnt < a.capacity) a.faces[base + cnt] = f;
                 ++cnt;
             }
