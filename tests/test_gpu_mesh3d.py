@@ -243,3 +243,39 @@ def test_heavy_rows_match_host_and_are_batch_independent():
             sub = np.arange(0, len(P), 2)
             got2 = m.evaluate(Q[sub], HH[sub], k, form).cpu().numpy()
             assert np.array_equal(got2, got[sub])
+
+
+def test_wide_candidate_cells_bitexact():
+    # a fine sphere (5,120 faces) and a support of ~7 face sizes: cells near the sphere list hundreds of candidate
+    # faces (scanned one wavefront per particle), cells near the box walls a few (one lane per particle); both kinds
+    # share wavefronts when the particles are not sorted by cell
+    V, F = MM.box_surface((0, 0, 0), (1, 1, 1))
+    Vs, Fs = MM.icosphere(4, 0.25, (0.5, 0.5, 0.5))
+    V, F = np.vstack([V, Vs]), np.vstack([F, (Fs + 8)[:, ::-1]])
+    h = 0.12
+    rng = np.random.default_rng(8)
+    d = rng.normal(size=(900, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    P = np.vstack([0.5 + d * (0.25 + rng.uniform(0.0, 0.15, 900))[:, None], rng.uniform(0, 1, (600, 3))])
+    P = P[rng.permutation(len(P))]
+    H = np.full(len(P), h)
+    H[::4] *= 0.5
+    m = MM.Mesh3D(V, F, h, device=DEV)
+    # candidates of each particle's cell: faces whose box, grown by support_max, meets the cell
+    lo, cell = np.array(m.grid_lo), m.cell
+    c0 = lo + np.floor((P - lo) / cell) * cell
+    T = V[F]
+    bl, bh = T.min(1) - h, T.max(1) + h
+    cand = ((bl[None] <= c0[:, None] + cell) & (bh[None] >= c0[:, None])).all(-1).sum(1)
+    assert (cand > 400).sum() > 200 and (cand < 64).sum() > 50, (cand.min(), cand.max())
+    Q, HH = torch.tensor(P, device=DEV), torch.tensor(H, device=DEV)
+    hoff, hfaces, hchi = MM.Mesh3D(V, F, h, host_only=True).adjacency_host(P, H)
+    cols, offs = OM.face_adjacency(T, P, H)
+    assert np.array_equal(hoff, offs) and np.array_equal(hfaces, cols)
+    for sort in (True, False):
+        adj = m.adjacency(Q, HH, sort=sort)
+        q, el, off = adj.numpy()
+        assert np.array_equal(off, hoff) and np.array_equal(el, hfaces), sort
+        np.testing.assert_allclose(adj.chi.cpu().numpy(), hchi, rtol=0, atol=1e-14)
+    short = m.adjacency(Q, HH, capacity=100)              # a face list too short at first: counted, then refilled
+    assert short.count == len(hfaces) and np.array_equal(short.faces[: short.count].cpu().numpy(), hfaces)

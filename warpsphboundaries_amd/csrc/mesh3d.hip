@@ -389,9 +389,18 @@ struct Adj3Args {
     int32_t* faces;
     int64_t capacity;
     double* chi;
+    int32_t* wide_count;        // scratch: the number of wide particles, then their indices
+    int32_t* wide_list;
 };
 
 constexpr int ADJ_UNROLL = 8;
+
+// A particle whose cell lists more than WIDE3 candidates is scanned by one wavefront (adj3_wide_kernel) instead of one
+// lane: lane k takes candidates k, k + 64, ... in chunks of 64, and the kept candidates of a chunk are placed in list
+// order by a ballot and the number of kept lanes below.  Scenes whose mesh is fine against the particle spacing have
+// thousands of candidates per particle and too few particles to fill the chip at one lane each.  Either way a
+// particle's face list is the same ascending list.
+constexpr int WIDE3 = 128;
 
 // Conservative FP32 rejection of a candidate from its FP32 box rounded outward (l, u): farther than h (1 + 1e-6) on one
 // axis, or squared distance beyond h^2 (1 + 1e-5).  `tol` bounds the FP32 rounding of the particle's coordinates and of the
@@ -451,14 +460,31 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     int c[3] = {0, 0, 0};
     const bool in = todo && m3::cell_of(a.g, px, py, pz, c);
     int64_t cnt = 0;
+    int it0 = 0, it1 = 0;
     if (MODE != 2 && in) {
         const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
+        it0 = a.g.cstart[cid];
+        it1 = a.g.cstart[cid + 1];
+    }
+    const bool wide = it1 - it0 > WIDE3;
+    if (MODE == 0) {
+        // wide particles go to the list of adj3_wide_kernel (one atomic per wavefront)
+        const uint64_t wm = __ballot(wide);
+        if (wm) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((unsigned long long)wm) - 1;
+            int first = 0;
+            if (lane == leader) first = atomicAdd(a.wide_count, __popcll(wm));
+            first = __shfl(first, leader);
+            if (wide) a.wide_list[first + __popcll(wm & ((1ull << lane) - 1))] = i;
+        }
+    }
+    if (MODE != 2 && in && !wide) {
         const Near32 nq = near32(px, py, pz, h);
         const int64_t base = FILL ? a.offsets[i] : 0;
-        const int it1 = a.g.cstart[cid + 1];
         // ADJ_UNROLL candidates per step: their 32-byte records are loaded together (independent loads in flight),
         // then filtered in list order; the exact test only for boxes within h on every axis
-        for (int it = a.g.cstart[cid]; it < it1; it += ADJ_UNROLL) {
+        for (int it = it0; it < it1; it += ADJ_UNROLL) {
             float4 l[ADJ_UNROLL], u[ADJ_UNROLL];
 #pragma unroll
             for (int q = 0; q < ADJ_UNROLL; ++q) {
@@ -481,7 +507,7 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     if (MODE == 1) return;
     if (MODE == 0) {
         if (live) {
-            a.counts[i] = cnt;
+            if (!wide) a.counts[i] = cnt;
             // the winding number where it is a lookup (outside the grid, a face-free cell); NaN: the walk decides
             if (a.chi) {
                 double w = 0.0;
@@ -508,6 +534,47 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
         mask &= mask - 1;
     }
     if (todo) a.chi[i] = chi;
+}
+
+// the particles listed by adj3_kernel<0> as wide, one wavefront each, a fixed grid of wavefronts taking them in turn:
+// FILL false writes the pair counts, true the face lists
+template <bool FILL>
+__global__ __launch_bounds__(256) void adj3_wide_kernel(Adj3Args a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave_id = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int nwide = *a.wide_count;
+    const uint64_t below = (1ull << lane) - 1;
+    for (int64_t q = wave_id; q < nwide; q += nwaves) {
+        const int i = a.wide_list[q];
+        const double px = a.pos[3 * (int64_t)i], py = a.pos[3 * (int64_t)i + 1], pz = a.pos[3 * (int64_t)i + 2];
+        const double h = a.sup[i];
+        int c[3] = {0, 0, 0};
+        (void)m3::cell_of(a.g, px, py, pz, c);        // inside the grid: it was listed from its cell
+        const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
+        const int it0 = a.g.cstart[cid], it1 = a.g.cstart[cid + 1];
+        const Near32 nq = near32(px, py, pz, h);
+        const int64_t base = FILL ? a.offsets[i] : 0;
+        int64_t cnt = 0;
+        for (int it = it0; it < it1; it += 64) {
+            const int j = it + lane;
+            bool keep = false;
+            int f = 0;
+            if (j < it1) {
+                const float4 l = a.g.crec[2 * (int64_t)j], u = a.g.crec[2 * (int64_t)j + 1];
+                const int k32 = classify32(nq, l, u);
+                f = m3::bits_int(l.w);
+                keep = k32 == 1 || (k32 == 2 && keep_face_fast(a.g.fbox + 6 * (int64_t)f, px, py, pz, h));
+            }
+            const uint64_t m = __ballot(keep);
+            if (FILL && keep) {
+                const int64_t p = base + cnt + __popcll(m & below);
+                if (p < a.capacity) a.faces[p] = f;
+            }
+            cnt += __popcll(m);
+        }
+        if (!FILL && lane == 0) a.counts[i] = cnt;
+    }
 }
 
 __global__ void offsets_head_kernel(int64_t* offsets) { offsets[0] = 0; }
@@ -905,11 +972,25 @@ extern "C" int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t
     return WSB_OK;
 }
 
-extern "C" size_t wsb_adjacency3d_scratch_bytes(int n) {
-    if (n <= 0) return 0;
+// adjacency scratch: pair counts [n] i64 | wide count (256 bytes) | wide list [n] i32 | scan temporary
+struct Adj3Layout {
+    size_t counts, wide_count, wide_list, tmp, total;
+};
+Adj3Layout adj3_layout(int n) {
+    Adj3Layout L{};
     size_t tmp = 0;
     (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp, (const int64_t*)nullptr, (int64_t*)nullptr, n);
-    return ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255) + tmp + 256;
+    L.counts = 0;
+    L.wide_count = al256(sizeof(int64_t) * (size_t)n);
+    L.wide_list = L.wide_count + 256;
+    L.tmp = al256(L.wide_list + sizeof(int32_t) * (size_t)n);
+    L.total = al256(L.tmp + tmp);
+    return L;
+}
+
+extern "C" size_t wsb_adjacency3d_scratch_bytes(int n) {
+    if (n <= 0) return 0;
+    return adj3_layout(n).total;
 }
 
 extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, const int32_t* order, int n,
@@ -926,16 +1007,27 @@ extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const dou
     WSB_REQUIRE(pos && sup, WSB_EINVAL, "null buffer");
     const size_t need = wsb_adjacency3d_scratch_bytes(n);
     WSB_REQUIRE(scratch && scratch_bytes >= need, WSB_ENOMEM, "adjacency needs %zu scratch bytes (got %zu)", need, scratch_bytes);
-    int64_t* counts = static_cast<int64_t*>(scratch);
-    char* tmp = static_cast<char*>(scratch) + ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255);
-    size_t tmp_bytes = scratch_bytes - ((sizeof(int64_t) * (size_t)n + 255) & ~(size_t)255);
-    Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi};
+    const Adj3Layout lay = adj3_layout(n);
+    char* sb = static_cast<char*>(scratch);
+    int64_t* counts = reinterpret_cast<int64_t*>(sb + lay.counts);
+    int32_t* wide_count = reinterpret_cast<int32_t*>(sb + lay.wide_count);
+    int32_t* wide_list = reinterpret_cast<int32_t*>(sb + lay.wide_list);
+    char* tmp = sb + lay.tmp;
+    size_t tmp_bytes = scratch_bytes - lay.tmp;
+    Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, wide_count, wide_list};
     const dim3 grid((n + 255) / 256), block(256);
+    // up to 8,192 wavefronts (32 per CU) for the wide particles, one per particle at a time
+    const dim3 wgrid((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 2048));
+    hipLaunchKernelGGL(zero_count3_kernel, dim3(1), dim3(1), 0, s, wide_count);
     hipLaunchKernelGGL(adj3_kernel<0>, grid, block, 0, s, a);
+    WSB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(adj3_wide_kernel<false>, wgrid, block, 0, s, a);
     WSB_HIP(hipGetLastError());
     WSB_HIP(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, counts, offsets + 1, n, s));
     if (faces && capacity > 0) {
         hipLaunchKernelGGL(adj3_kernel<1>, grid, block, 0, s, a);
+        WSB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(adj3_wide_kernel<true>, wgrid, block, 0, s, a);
         WSB_HIP(hipGetLastError());
     }
     if (chi) {
