@@ -167,8 +167,8 @@ enum wsb_mesh_flags {
  * fluxbridge/geometry.py:74-80 Geometry.closed_surface).  verts[nv*3], faces[nf*3] (outward orientation: a cavity's
  * faces point into the cavity).  Builds the face records, a grid of cells of support_max (coarser above 2^24 cells)
  * with each cell's candidate faces (vertex box inflated by support_max), the winding number of every face-free cell
- * and the axis walk of every other cell.  info[9] (nullable): K (largest candidate list), nx, ny, nz, nf, cell items,
- * device bytes, face-free components, cells with faces. */
+ * and the axis walk of every other cell.  info[10] (nullable): K (largest candidate list), nx, ny, nz, nf, cell items,
+ * device bytes, face-free components, cells with faces, longest axis walk (items). */
 int wsb_mesh3d_create(const double* verts, int nv, const int32_t* faces, int nf, double support_max, int flags,
                       wsb_mesh3d** out, int64_t* info);
 int wsb_mesh3d_destroy(wsb_mesh3d* m);

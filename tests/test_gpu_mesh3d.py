@@ -248,7 +248,7 @@ def test_heavy_rows_match_host_and_are_batch_independent():
 def test_wide_candidate_cells_bitexact():
     # a fine sphere (5,120 faces) and a support of ~7 face sizes: cells near the sphere list hundreds of candidate
     # faces (scanned one wavefront per particle), cells near the box walls a few (one lane per particle); both kinds
-    # share wavefronts when the particles are not sorted by cell
+    # share wavefronts when the particles are not sorted by cell.  Likewise for the axis walks of the winding number.
     V, F = MM.box_surface((0, 0, 0), (1, 1, 1))
     Vs, Fs = MM.icosphere(4, 0.25, (0.5, 0.5, 0.5))
     V, F = np.vstack([V, Vs]), np.vstack([F, (Fs + 8)[:, ::-1]])
@@ -268,6 +268,7 @@ def test_wide_candidate_cells_bitexact():
     bl, bh = T.min(1) - h, T.max(1) + h
     cand = ((bl[None] <= c0[:, None] + cell) & (bh[None] >= c0[:, None])).all(-1).sum(1)
     assert (cand > 400).sum() > 200 and (cand < 64).sum() > 50, (cand.min(), cand.max())
+    assert m.longest_walk > 200 and m.dirty > 100      # axis walks over more than 128 faces: one wavefront each
     Q, HH = torch.tensor(P, device=DEV), torch.tensor(H, device=DEV)
     hoff, hfaces, hchi = MM.Mesh3D(V, F, h, host_only=True).adjacency_host(P, H)
     cols, offs = OM.face_adjacency(T, P, H)

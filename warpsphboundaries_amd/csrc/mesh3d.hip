@@ -8,6 +8,7 @@
 // device_search.py:186-189).  The static grid only narrows the candidates: a cell lists every face whose box,
 // inflated by support_max (1 + 1e-9), overlaps it.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -391,6 +392,11 @@ struct Adj3Args {
     double* chi;
     int32_t* wide_count;        // scratch: the number of wide particles, then their indices
     int32_t* wide_list;
+    int32_t* walk_count;        // scratch: the number of particles with a long axis walk, then their indices
+    int32_t* walk_list;
+    int wide_min;               // candidate lists / axis walks longer than this go to the wavefront kernels
+    int32_t* exact_count;       // scratch: the number of particles whose walk hit a rounding bound, then their indices
+    int32_t* exact_list;
 };
 
 constexpr int ADJ_UNROLL = 8;
@@ -399,8 +405,12 @@ constexpr int ADJ_UNROLL = 8;
 // lane: lane k takes candidates k, k + 64, ... in chunks of 64, and the kept candidates of a chunk are placed in list
 // order by a ballot and the number of kept lanes below.  Scenes whose mesh is fine against the particle spacing have
 // thousands of candidates per particle and too few particles to fill the chip at one lane each.  Either way a
-// particle's face list is the same ascending list.
+// particle's face list is the same ascending list.  Likewise a particle whose cell's axis walk has more than WIDE3
+// items takes its winding number from adj3_walk_kernel, one wavefront per particle (the crossings are integers, so
+// their sum does not depend on the order).  Launches of WIDE3_MAX_N particles or more keep one lane per particle
+// throughout: they fill the chip as they are.
 constexpr int WIDE3 = 128;
+constexpr int WIDE3_MAX_N = 1 << 18;
 
 // Conservative FP32 rejection of a candidate from its FP32 box rounded outward (l, u): farther than h (1 + 1e-6) on one
 // axis, or squared distance beyond h^2 (1 + 1e-5).  `tol` bounds the FP32 rounding of the particle's coordinates and of the
@@ -432,11 +442,21 @@ __device__ inline int classify32(const Near32& q, const float4& l, const float4&
     return ux * ux + uy * uy + uz * uz < q.h2 * (1.0f - 1e-5f) ? 1 : 2;
 }
 
+// the reference's per-face solid-angle sum at one point, evaluated by the whole wavefront (face f on lane f % 64, a
+// butterfly over the lane sums: every lane ends with the same value)
+__device__ inline double wave_exact_winding(const m3::Grid3& g, double qx, double qy, double qz, double qi) {
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int f = lane; f < g.nf; f += 64) s += m3::face_solid_angle<m3::FP64>(g.frec + m3::FREC * (int64_t)f, qx, qy, qz, qi);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    return s / (4.0 * m3::PI);
+}
+
 // one thread per particle: candidates of its cell through the element test; the count pass also evaluates the
 // winding number (face-free cell, axis walk, or -- when a predicate of the walk is within rounding -- the exact
-// per-face sum, evaluated by the whole wavefront for each such lane in turn)
-// MODE 0: pair counts, 1: face lists, 2: winding numbers (apart from the candidate loop: the walk and the exact sum
-// need three times the registers of the loop, which then runs at a quarter of the occupancy)
+// per-face sum, evaluated by adj3_exact_kernel for the particles listed here)
+// MODE 0: pair counts, 1: face lists, 2: winding numbers by the walk (apart from the candidate loop: the walk needs
+// twice the registers of the loop; the exact sum, in a kernel of its own, four times)
 template <int MODE>
 __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     constexpr bool FILL = MODE == 1;
@@ -466,17 +486,39 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
         it0 = a.g.cstart[cid];
         it1 = a.g.cstart[cid + 1];
     }
-    const bool wide = it1 - it0 > WIDE3;
+    const bool wide = it1 - it0 > a.wide_min;
+    // the winding number where it is a lookup (outside the grid, a face-free cell); NaN: the walk decides
+    double w0 = 0.0;
+    bool long_walk = false;
+    if (MODE == 0 && a.chi && in) {
+        const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
+        const int d = a.g.cwalk ? a.g.cwalk[cid] : -1;
+        if (d >= 0) {
+            w0 = __builtin_nan("");
+            long_walk = a.g.wstart[d + 1] - a.g.wstart[d] > a.wide_min;
+        } else {
+            w0 = a.g.cchi[cid];
+        }
+    }
     if (MODE == 0) {
-        // wide particles go to the list of adj3_wide_kernel (one atomic per wavefront)
+        // wide particles go to the list of adj3_wide_kernel, long walks to that of adj3_walk_kernel (one atomic per
+        // wavefront and list)
+        const int lane = threadIdx.x & 63;
         const uint64_t wm = __ballot(wide);
         if (wm) {
-            const int lane = threadIdx.x & 63;
             const int leader = __ffsll((unsigned long long)wm) - 1;
             int first = 0;
             if (lane == leader) first = atomicAdd(a.wide_count, __popcll(wm));
             first = __shfl(first, leader);
             if (wide) a.wide_list[first + __popcll(wm & ((1ull << lane) - 1))] = i;
+        }
+        const uint64_t lm = __ballot(long_walk);
+        if (lm) {
+            const int leader = __ffsll((unsigned long long)lm) - 1;
+            int first = 0;
+            if (lane == leader) first = atomicAdd(a.walk_count, __popcll(lm));
+            first = __shfl(first, leader);
+            if (long_walk) a.walk_list[first + __popcll(lm & ((1ull << lane) - 1))] = i;
         }
     }
     if (MODE != 2 && in && !wide) {
@@ -508,32 +550,73 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     if (MODE == 0) {
         if (live) {
             if (!wide) a.counts[i] = cnt;
-            // the winding number where it is a lookup (outside the grid, a face-free cell); NaN: the walk decides
-            if (a.chi) {
-                double w = 0.0;
-                if (in) {
-                    const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
-                    w = (a.g.cwalk && a.g.cwalk[cid] >= 0) ? __builtin_nan("") : a.g.cchi[cid];
-                }
-                a.chi[i] = w;
-            }
+            if (a.chi) a.chi[i] = w0;
         }
         return;
     }
     bool need = false;
     if (todo) need = !m3::ray_winding(a.g, px, py, pz, c, &chi);
-    uint64_t mask = __ballot(need);
-    const int lane = threadIdx.x & 63;
-    while (mask) {
-        const int src = __ffsll((unsigned long long)mask) - 1;
-        const double qx = __shfl(px, src), qy = __shfl(py, src), qz = __shfl(pz, src), qi = __shfl(1.0 / h, src);
-        double s = 0.0;
-        for (int f = lane; f < a.g.nf; f += 64) s += m3::face_solid_angle<m3::FP64>(a.g.frec + m3::FREC * (int64_t)f, qx, qy, qz, qi);
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if (lane == src) chi = s / (4.0 * m3::PI);
-        mask &= mask - 1;
+    const uint64_t mask = __ballot(need);
+    if (mask) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int first = 0;
+        if (lane == leader) first = atomicAdd(a.exact_count, __popcll(mask));
+        first = __shfl(first, leader);
+        if (need) a.exact_list[first + __popcll(mask & ((1ull << lane) - 1))] = i;
     }
-    if (todo) a.chi[i] = chi;
+    if (todo && !need) a.chi[i] = chi;          // the listed ones keep NaN until adj3_exact_kernel
+}
+
+// the particles listed by adj3_kernel<0> for their long axis walk, one wavefront each: lane k takes walk items k,
+// k + 64, ...; any item within its rounding bound sends the particle to the exact per-face sum (as the one-lane walk
+// does).  Writes the winding number, so adj3_kernel<2> passes these particles by.
+__global__ __launch_bounds__(256) void adj3_walk_kernel(Adj3Args a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave_id = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int nwalk = *a.walk_count;
+    for (int64_t q = wave_id; q < nwalk; q += nwaves) {
+        const int i = a.walk_list[q];
+        const double p[3] = {a.pos[3 * (int64_t)i], a.pos[3 * (int64_t)i + 1], a.pos[3 * (int64_t)i + 2]};
+        const double h = a.sup[i];
+        int c[3] = {0, 0, 0};
+        (void)m3::cell_of(a.g, p[0], p[1], p[2], c);       // inside the grid: it was listed from its cell
+        const m3::Walk w = m3::walk_of(a.g, c, a.g.cwalk[m3::cell_id(a.g, c[0], c[1], c[2])]);
+        const double Pa = p[w.ax], Pu = p[w.au], Pv = p[w.av];
+        int net = 0;
+        bool unsure = false;
+        for (int it = w.it0; it < w.it1; it += 64) {
+            const int j = it + lane;
+            int r = 0;
+            if (j < w.it1)
+                r = m3::walk_item(a.g.frec, p[0], p[1], p[2], Pa, Pu, Pv, w.au, w.av, w.s, a.g.wrec[2 * (int64_t)j],
+                                  a.g.wrec[2 * (int64_t)j + 1]);
+            unsure = unsure || r == 2;
+            net += (r == 2) ? 0 : r;
+        }
+        if (__ballot(unsure)) {
+            if (lane == 0) a.exact_list[atomicAdd(a.exact_count, 1)] = i;     // NaN until adj3_exact_kernel
+        } else {
+            for (int o = 32; o > 0; o >>= 1) net += __shfl_xor(net, o);
+            if (lane == 0) a.chi[i] = w.acc + (double)net;
+        }
+    }
+}
+
+// the reference's per-face sum for the particles whose walk hit a rounding bound (listed by adj3_kernel<2> and
+// adj3_walk_kernel), one wavefront each
+__global__ __launch_bounds__(256) void adj3_exact_kernel(Adj3Args a) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave_id = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int nexact = *a.exact_count;
+    for (int64_t q = wave_id; q < nexact; q += nwaves) {
+        const int i = a.exact_list[q];
+        const double s = wave_exact_winding(a.g, a.pos[3 * (int64_t)i], a.pos[3 * (int64_t)i + 1], a.pos[3 * (int64_t)i + 2],
+                                            1.0 / a.sup[i]);
+        if (lane == 0) a.chi[i] = s;
+    }
 }
 
 // the particles listed by adj3_kernel<0> as wide, one wavefront each, a fixed grid of wavefronts taking them in turn:
@@ -658,14 +741,17 @@ struct Mom3Args {
     double* out;
     int32_t* heavy_count;       // scratch: the number of heavy particles, then their row indices
     int32_t* heavy_list;
+    int rows_per_tile;          // particles per tile22_kernel workgroup, at most TILE3
 };
 
-// A particle with more than HEAVY3 pairs is served by one wavefront (heavy22_kernel): pair j of it goes to lane j % 64,
-// each lane sums its pairs in order and a butterfly over the 64 lanes adds the lane sums (every lane ends with the
-// same value: each level adds the same two operands).  The other particles are served in tiles (tile22_kernel): a
-// workgroup owns TILE3 consecutive particles, their pairs are packed densely over the lanes (pair j -> its particle
-// by a binary search of the tile's offsets in LDS); every lane evaluates one pair, the 23 face accumulators go through
-// LDS and each particle's lane adds its own pairs in face order.  Either way a particle's sums depend only on its own
+// A particle with more than HEAVY3 pairs is served by one workgroup (heavy22_kernel): pair j of it goes to thread
+// j % 256, each thread sums its pairs in order, a butterfly over each wavefront's 64 lanes adds the thread sums (every
+// lane ends with the same value: each level adds the same two operands) and the four wavefront sums are added in a
+// fixed order.  The other particles are served in tiles (tile22_kernel): a
+// workgroup of TILE3 threads owns rows_per_tile <= TILE3 consecutive particles (fewer when the launch is small, so
+// that it still has thousands of workgroups), their pairs are packed densely over the threads (pair j -> its particle
+// by a binary search of the tile's offsets in LDS); every thread evaluates one pair, the 23 face accumulators go
+// through LDS and each particle's thread adds its own pairs in face order.  Either way a particle's sums depend only on its own
 // pairs, never on the packing or on the other particles of its batch.
 constexpr int HEAVY3 = 64;
 
@@ -674,31 +760,35 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
     using BlockScan = hipcub::BlockScan<int, TILE3>;
     __shared__ typename BlockScan::TempStorage scan_tmp;
     __shared__ int s_off[TILE3 + 1];
+    __shared__ int s_first[TILE3];
     __shared__ double s_p[4][TILE3];
     __shared__ double s_res[m3::NACC][TILE3];
     const int tid = threadIdx.x;
-    const int row = blockIdx.x * TILE3 + tid;
-    const bool live = row < a.n;
+    const int64_t r0 = blockIdx.x * (int64_t)a.rows_per_tile;
+    const int64_t row = r0 + tid;
+    const bool live = tid < a.rows_per_tile && row < a.n;
     int cnt = 0;
     double h = 1.0;
     if (live) {
         cnt = (int)(a.offsets[row + 1] - a.offsets[row]);
-        s_p[0][tid] = a.pos[3 * (int64_t)row];
-        s_p[1][tid] = a.pos[3 * (int64_t)row + 1];
-        s_p[2][tid] = a.pos[3 * (int64_t)row + 2];
+        s_p[0][tid] = a.pos[3 * row];
+        s_p[1][tid] = a.pos[3 * row + 1];
+        s_p[2][tid] = a.pos[3 * row + 2];
         h = a.sup[row];
-        // a heavy particle (served by heavy22_kernel) is marked by a negative inverse support in LDS: its pairs stay
-        // in the packing (the slot of pair j is then the tile's first slot + j) but are not evaluated here
+        // a heavy particle (served by heavy22_kernel) is marked by a negative inverse support in LDS
         s_p[3][tid] = cnt > HEAVY3 ? -1.0 : 1.0 / h;
     }
     const bool heavy = cnt > HEAVY3;
-    if (heavy) a.heavy_list[atomicAdd(a.heavy_count, 1)] = row;
+    if (heavy) a.heavy_list[atomicAdd(a.heavy_count, 1)] = (int32_t)row;
+    // heavy particles' pairs are left out of the packing: then pair j of the tile sits at slot tile_first +
+    // s_first[r] + (j - s_off[r]) for its row r; a tile without heavy particles keeps slot tile_first + j
+    const int64_t tile_first = r0 < a.n ? a.offsets[r0] : 0;
+    if (live) s_first[tid] = (int)(a.offsets[row] - tile_first);
     int off, total;
-    BlockScan(scan_tmp).ExclusiveSum(cnt, off, total);
+    BlockScan(scan_tmp).ExclusiveSum(heavy ? 0 : cnt, off, total);
     s_off[tid] = off;
     if (tid == 0) s_off[TILE3] = total;
-    __syncthreads();
-    const int64_t tile_first = blockIdx.x * (int64_t)TILE3 < a.n ? a.offsets[blockIdx.x * (int64_t)TILE3] : 0;
+    const bool mapped = __syncthreads_or(heavy);
     double acc[m3::NACC];
 #pragma unroll
     for (int k = 0; k < m3::NACC; ++k) acc[k] = 0.0;
@@ -714,7 +804,7 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
                 if (s_off[mid] <= j) lo = mid;
                 else hi = mid;
             }
-            const int64_t slot = tile_first + j;
+            const int64_t slot = tile_first + (mapped ? s_first[lo] + (j - s_off[lo]) : j);
             const int f = slot < a.capacity ? a.faces[slot] : -1;
             const double ihl = s_p[3][lo];
             if (f >= 0 && f < a.nf && ihl > 0.0)
@@ -723,7 +813,7 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
 #pragma unroll
         for (int k = 0; k < m3::NACC; ++k) s_res[k][tid] = c[k];
         __syncthreads();
-        const int b0 = max(off, base), b1 = heavy ? b0 : min(off + cnt, base + TILE3);
+        const int b0 = max(off, base), b1 = heavy ? b0 : min(off + cnt, base + TILE3);   // off + 0 for heavy
         for (int jj = b0; jj < b1; ++jj) {
 #pragma unroll
             for (int k = 0; k < m3::NACC; ++k) acc[k] += s_res[k][jj - base];
@@ -739,8 +829,7 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
         for (int k = 0; k < m3::NOUT; ++k) s_res[k][tid] = o[k];
     }
     __syncthreads();
-    const int64_t r0 = blockIdx.x * (int64_t)TILE3;
-    const int nrows = (int)min((int64_t)TILE3, (int64_t)a.n - r0);
+    const int nrows = (int)min((int64_t)a.rows_per_tile, (int64_t)a.n - r0);
     double* dst = a.out + r0 * m3::NOUT;
     const bool accumulate = (a.flags & WSB_OUT_ACCUMULATE) != 0;
     for (int idx = tid; idx < nrows * m3::NOUT; idx += TILE3) {
@@ -752,16 +841,17 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
     }
 }
 
-// the particles with more than HEAVY3 pairs (listed by tile22_kernel), one wavefront each, a fixed grid of wavefronts
-// taking them in turn
+// the particles with more than HEAVY3 pairs (listed by tile22_kernel), one workgroup of four wavefronts each (a
+// fixed grid of workgroups taking them in turn): pair j of a particle goes to thread j % 256, each thread sums its
+// pairs in order, a butterfly adds the 64 sums of each wavefront, and the four wavefront sums are added in wavefront
+// order -- the same order whatever else is in the batch
 template <int KID, int FORM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void heavy22_kernel(Mom3Args a) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave_id = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    __shared__ double s_acc[4][m3::NACC];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const bool accumulate = (a.flags & WSB_OUT_ACCUMULATE) != 0;
     const int nheavy = *a.heavy_count;
-    for (int64_t q = wave_id; q < nheavy; q += nwaves) {
+    for (int64_t q = blockIdx.x; q < nheavy; q += gridDim.x) {
         const int64_t r = a.heavy_list[q];
         const int64_t first = a.offsets[r];
         const int rc = (int)(a.offsets[r + 1] - first);
@@ -771,7 +861,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void h
         double acc[m3::NACC];
 #pragma unroll
         for (int k = 0; k < m3::NACC; ++k) acc[k] = 0.0;
-        for (int j = lane; j < rc; j += 64) {
+        for (int j = tid; j < rc; j += 256) {
             const int64_t slot = first + j;
             const int f = slot < a.capacity ? a.faces[slot] : -1;
             if (f >= 0 && f < a.nf) m3::face_terms<KID, FORM>(a.frec + m3::FREC * (int64_t)f, px, py, pz, ih, acc);
@@ -781,29 +871,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void h
 #pragma unroll
             for (int k = 0; k < m3::NACC; ++k) acc[k] += __shfl_xor(acc[k], o);
         }
-        double out[m3::NOUT];
-        const double chi = (a.flags & WSB_OUT_NO_INDICATOR) ? 0.0 : a.chi[r];
-        m3::finalize22<KID>(acc, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, out);
-        double v = 0.0;
+        if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < m3::NOUT; ++k) v = (k == lane) ? out[k] : v;
-        if (lane < m3::NOUT) {
-            double* dst = a.out + r * m3::NOUT + lane;
-            if (accumulate) *dst += v;
-            else *dst = v;
+            for (int k = 0; k < m3::NACC; ++k) s_acc[wv][k] = acc[k];
         }
+        __syncthreads();
+        if (wv == 0) {
+            double tot[m3::NACC];
+#pragma unroll
+            for (int k = 0; k < m3::NACC; ++k) tot[k] = ((s_acc[0][k] + s_acc[1][k]) + s_acc[2][k]) + s_acc[3][k];
+            double out[m3::NOUT];
+            const double chi = (a.flags & WSB_OUT_NO_INDICATOR) ? 0.0 : a.chi[r];
+            m3::finalize22<KID>(tot, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, out);
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < m3::NOUT; ++k) v = (k == lane) ? out[k] : v;
+            if (lane < m3::NOUT) {
+                double* dst = a.out + r * m3::NOUT + lane;
+                if (accumulate) *dst += v;
+                else *dst = v;
+            }
+        }
+        __syncthreads();            // s_acc is reused by the next particle
     }
 }
 
 __global__ void zero_count3_kernel(int32_t* c) { *c = 0; }
+__global__ void zero3_counts_kernel(int32_t* c) {
+    c[0] = 0;
+    c[1] = 0;
+    c[2] = 0;
+}
 
 template <int KID, int FORM>
 int launch22(const Mom3Args& a, hipStream_t s) {
     hipLaunchKernelGGL(zero_count3_kernel, dim3(1), dim3(1), 0, s, a.heavy_count);
-    hipLaunchKernelGGL((tile22_kernel<KID, FORM>), dim3((a.n + TILE3 - 1) / TILE3), dim3(TILE3), 0, s, a);
+    // rows per tile: TILE3, halved (down to 16) while the launch has fewer than 2,048 workgroups
+    Mom3Args t = a;
+    t.rows_per_tile = TILE3;
+    while (t.rows_per_tile > 16 && ((int64_t)a.n + t.rows_per_tile - 1) / t.rows_per_tile < 2048) t.rows_per_tile >>= 1;
+    hipLaunchKernelGGL((tile22_kernel<KID, FORM>), dim3((unsigned)(((int64_t)a.n + t.rows_per_tile - 1) / t.rows_per_tile)),
+                       dim3(TILE3), 0, s, t);
     WSB_HIP(hipGetLastError());
-    // up to 4,096 wavefronts (16 per CU), one per listed particle at a time
-    const int blocks = (int)std::min<int64_t>(((int64_t)a.n + 3) / 4, 1024);
+    // up to 2,048 workgroups (8 per CU), one per listed particle at a time
+    const int blocks = (int)std::min<int64_t>((int64_t)a.n, 2048);
     hipLaunchKernelGGL((heavy22_kernel<KID, FORM>), dim3(blocks), dim3(256), 0, s, a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
@@ -925,6 +1036,9 @@ extern "C" int wsb_mesh3d_info(const wsb_mesh3d* m, int64_t* info) {
     info[6] = m->bytes;
     info[7] = m->ncomp;
     info[8] = m->ndirty;
+    int64_t wmax = 0;
+    for (size_t d = 0; d + 1 < m->wstart.size(); ++d) wmax = std::max<int64_t>(wmax, m->wstart[d + 1] - m->wstart[d]);
+    info[9] = wmax;
     return WSB_OK;
 }
 
@@ -972,18 +1086,21 @@ extern "C" int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t
     return WSB_OK;
 }
 
-// adjacency scratch: pair counts [n] i64 | wide count (256 bytes) | wide list [n] i32 | scan temporary
+// adjacency scratch: pair counts [n] i64 | the three list counts (256 bytes) | wide list [n] i32 | walk list [n] i32 |
+// exact list [n] i32 | scan temporary
 struct Adj3Layout {
-    size_t counts, wide_count, wide_list, tmp, total;
+    size_t counts, list_counts, wide_list, walk_list, exact_list, tmp, total;
 };
 Adj3Layout adj3_layout(int n) {
     Adj3Layout L{};
     size_t tmp = 0;
     (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp, (const int64_t*)nullptr, (int64_t*)nullptr, n);
     L.counts = 0;
-    L.wide_count = al256(sizeof(int64_t) * (size_t)n);
-    L.wide_list = L.wide_count + 256;
-    L.tmp = al256(L.wide_list + sizeof(int32_t) * (size_t)n);
+    L.list_counts = al256(sizeof(int64_t) * (size_t)n);
+    L.wide_list = L.list_counts + 256;
+    L.walk_list = al256(L.wide_list + sizeof(int32_t) * (size_t)n);
+    L.exact_list = al256(L.walk_list + sizeof(int32_t) * (size_t)n);
+    L.tmp = al256(L.exact_list + sizeof(int32_t) * (size_t)n);
     L.total = al256(L.tmp + tmp);
     return L;
 }
@@ -1010,28 +1127,43 @@ extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const dou
     const Adj3Layout lay = adj3_layout(n);
     char* sb = static_cast<char*>(scratch);
     int64_t* counts = reinterpret_cast<int64_t*>(sb + lay.counts);
-    int32_t* wide_count = reinterpret_cast<int32_t*>(sb + lay.wide_count);
+    int32_t* list_counts = reinterpret_cast<int32_t*>(sb + lay.list_counts);     // wide, walk, exact
     int32_t* wide_list = reinterpret_cast<int32_t*>(sb + lay.wide_list);
+    int32_t* walk_list = reinterpret_cast<int32_t*>(sb + lay.walk_list);
+    int32_t* exact_list = reinterpret_cast<int32_t*>(sb + lay.exact_list);
     char* tmp = sb + lay.tmp;
     size_t tmp_bytes = scratch_bytes - lay.tmp;
-    Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, wide_count, wide_list};
+    const bool split = n < WIDE3_MAX_N;
+    Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, list_counts, wide_list, list_counts + 1,
+               walk_list, split ? WIDE3 : INT_MAX, list_counts + 2, exact_list};
     const dim3 grid((n + 255) / 256), block(256);
     // up to 8,192 wavefronts (32 per CU) for the wide particles, one per particle at a time
     const dim3 wgrid((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 2048));
-    hipLaunchKernelGGL(zero_count3_kernel, dim3(1), dim3(1), 0, s, wide_count);
+    hipLaunchKernelGGL(zero3_counts_kernel, dim3(1), dim3(1), 0, s, list_counts);
     hipLaunchKernelGGL(adj3_kernel<0>, grid, block, 0, s, a);
     WSB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(adj3_wide_kernel<false>, wgrid, block, 0, s, a);
-    WSB_HIP(hipGetLastError());
+    if (split) {
+        hipLaunchKernelGGL(adj3_wide_kernel<false>, wgrid, block, 0, s, a);
+        WSB_HIP(hipGetLastError());
+    }
     WSB_HIP(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, counts, offsets + 1, n, s));
     if (faces && capacity > 0) {
         hipLaunchKernelGGL(adj3_kernel<1>, grid, block, 0, s, a);
         WSB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(adj3_wide_kernel<true>, wgrid, block, 0, s, a);
-        WSB_HIP(hipGetLastError());
+        if (split) {
+            hipLaunchKernelGGL(adj3_wide_kernel<true>, wgrid, block, 0, s, a);
+            WSB_HIP(hipGetLastError());
+        }
     }
     if (chi) {
+        if (split) {
+            hipLaunchKernelGGL(adj3_walk_kernel, wgrid, block, 0, s, a);
+            WSB_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL(adj3_kernel<2>, grid, block, 0, s, a);
+        WSB_HIP(hipGetLastError());
+        // rare (particles within rounding of a face plane, rays through edges): 1,024 wavefronts at most
+        hipLaunchKernelGGL(adj3_exact_kernel, dim3((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 256)), block, 0, s, a);
         WSB_HIP(hipGetLastError());
     }
     return WSB_OK;

@@ -597,11 +597,80 @@ __host__ __device__ inline int64_t cell_id(const Grid3& g, int i, int j, int k) 
     return ((int64_t)i * g.n[1] + j) * g.n[2] + k;
 }
 
-// winding number at p from the axis walk of its cell: the winding number where the walk ends (a face-free cell, or
-// 0 outside the grid) plus the signed crossings of the ray from p with the faces that overlap the walk's column.
-// Every such face lies inside the walked range (a face reaching the end cell would make it a cell with faces), so
-// each crossing ahead of p is counted once.  Returns false when a predicate is within its rounding bound (the
-// caller then takes the exact per-face sum).
+// The axis walk of a cell with faces: direction (axis ax, sign s), the winding number where the walk ends (a face-free
+// cell, or 0 outside the grid; an integer) and the range [it0, it1) of walk items (the faces overlapping the walk's
+// column, FP32 box records in wrec).
+struct Walk {
+    int ax, au, av, s, it0, it1;
+    double acc;
+};
+__host__ __device__ inline Walk walk_of(const Grid3& g, const int c0[3], int d) {
+    Walk w;
+    const int walk = g.wdir[d];
+    const int dir = walk & 7, steps = walk >> 3;
+    w.ax = dir >> 1;
+    w.au = (w.ax + 1) % 3;
+    w.av = (w.ax + 2) % 3;
+    w.s = (dir & 1) ? -1 : 1;
+    int ce[3] = {c0[0], c0[1], c0[2]};
+    ce[w.ax] += w.s * steps;
+    w.acc = (ce[w.ax] >= 0 && ce[w.ax] < g.n[w.ax]) ? g.cchi[cell_id(g, ce[0], ce[1], ce[2])] : 0.0;
+    w.it0 = g.wstart[d];
+    w.it1 = g.wstart[d + 1];
+    return w;
+}
+
+// one walk item (record q0, q1) against the ray p + t s e_ax, t > 0 (Pa, Pu, Pv: the coordinates of p along ax and
+// the two other axes au, av): 0 no crossing, +1 / -1 a crossing where the ray leaves / enters the solid, 2 a
+// predicate within its rounding bound (the ray through or near an edge or a vertex of the face, or p on or within
+// rounding of its plane)
+__host__ __device__ inline int walk_item(const double* __restrict__ frec, double px, double py, double pz, double Pa,
+                                         double Pu, double Pv, int au, int av, int s, const float4& q0, const float4& q1) {
+    if (Pu < (double)q0.x || Pu > (double)q0.y || Pv < (double)q0.z || Pv > (double)q0.w) return 0;
+    if (s > 0 ? (double)q1.y < Pa : (double)q1.x > Pa) return 0;
+    const int f = bits_int(q1.z);
+    const double* fr = frec + FREC * (int64_t)f;
+    const double Au = fr[FR_V + au], Av = fr[FR_V + av];
+    const double Bu = fr[FR_V + 3 + au], Bv = fr[FR_V + 3 + av];
+    const double Cu = fr[FR_V + 6 + au], Cv = fr[FR_V + 6 + av];
+    // projected point-in-triangle: three orientations with a static rounding bound
+    int pos = 0, neg = 0, unk = 0;
+    const double eu[3] = {Bu - Au, Cu - Bu, Au - Cu}, ev[3] = {Bv - Av, Cv - Bv, Av - Cv};
+    const double ou[3] = {Pu - Au, Pu - Bu, Pu - Cu}, ov[3] = {Pv - Av, Pv - Bv, Pv - Cv};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double t1 = eu[q] * ov[q], t2 = ev[q] * ou[q];
+        const double o = t1 - t2;
+        const double bound = 1e-14 * (fabs(t1) + fabs(t2)) + 1e-300;
+        if (o > bound) ++pos;
+        else if (o < -bound) ++neg;
+        else ++unk;
+    }
+    if (pos > 0 && neg > 0) return 0;            // certainly outside the projected triangle
+    if (unk > 0) return 2;
+    // side of p against the face plane: N . (p - A), N = (B - A) x (C - A)
+    const double Ax = fr[FR_V], Ay = fr[FR_V + 1], Az = fr[FR_V + 2];
+    const double e1x = fr[FR_V + 3] - Ax, e1y = fr[FR_V + 4] - Ay, e1z = fr[FR_V + 5] - Az;
+    const double e2x = fr[FR_V + 6] - Ax, e2y = fr[FR_V + 7] - Ay, e2z = fr[FR_V + 8] - Az;
+    const double dx = px - Ax, dy = py - Ay, dz = pz - Az;
+    const double Nx = e1y * e2z - e1z * e2y, Ny = e1z * e2x - e1x * e2z, Nz = e1x * e2y - e1y * e2x;
+    const double side = Nx * dx + Ny * dy + Nz * dz;
+    const double mag = (fabs(e1y * e2z) + fabs(e1z * e2y)) * fabs(dx) + (fabs(e1z * e2x) + fabs(e1x * e2z)) * fabs(dy) +
+                       (fabs(e1x * e2y) + fabs(e1y * e2x)) * fabs(dz);
+    if (!(fabs(side) > 1e-13 * mag + 1e-300)) return 2;
+    // N_ax has the sign of the projected orientation (counter-clockwise in (au, av) <=> N_ax > 0); the ray
+    // p + t s e_ax meets the plane at t = -side / (s N_ax): ahead of p <=> side and s N_ax differ in sign
+    const double sgnN = pos > 0 ? 1.0 : -1.0;
+    if ((side > 0.0) == (s * sgnN > 0.0)) return 0;
+    // the ray leaves the solid where s N_ax > 0 (outward normal along the ray)
+    return (s * sgnN > 0.0) ? 1 : -1;
+}
+
+// winding number at p from the axis walk of its cell: the winding number where the walk ends plus the signed
+// crossings of the ray from p with the faces that overlap the walk's column.  Every such face lies inside the walked
+// range (a face reaching the end cell would make it a cell with faces), so each crossing ahead of p is counted once.
+// Returns false when a predicate is within its rounding bound (the caller then takes the exact per-face sum).  All
+// terms are integers, so the sum is exact in any order.
 __host__ __device__ inline bool ray_winding(const Grid3& g, double px, double py, double pz, const int c0[3], double* chi) {
     const int64_t cid0 = cell_id(g, c0[0], c0[1], c0[2]);
     const int d = g.cwalk[cid0];
@@ -609,71 +678,27 @@ __host__ __device__ inline bool ray_winding(const Grid3& g, double px, double py
         *chi = g.cchi[cid0];
         return true;
     }
-    const int walk = g.wdir[d];
-    const int dir = walk & 7, steps = walk >> 3;
-    const int ax = dir >> 1;
-    const int s = (dir & 1) ? -1 : 1;
-    const int au = (ax + 1) % 3, av = (ax + 2) % 3;
+    const Walk w = walk_of(g, c0, d);
     const double p[3] = {px, py, pz};
-    const double Pu = p[au], Pv = p[av];
-    int ce[3] = {c0[0], c0[1], c0[2]};
-    ce[ax] += s * steps;
-    double acc = 0.0;
-    if (ce[ax] >= 0 && ce[ax] < g.n[ax]) acc = g.cchi[cell_id(g, ce[0], ce[1], ce[2])];
-    const int it1 = g.wstart[d + 1];
-    for (int it0 = g.wstart[d]; it0 < it1; it0 += 4) {
-      // four walk items per step: records loaded together, then the conservative FP32 box filter and the exact
-      // tests on the face's vertices
-      float4 r0[4], r1[4];
+    const double Pa = p[w.ax], Pu = p[w.au], Pv = p[w.av];
+    double acc = w.acc;
+    for (int it0 = w.it0; it0 < w.it1; it0 += 4) {
+        // four walk items per step: records loaded together, then the conservative FP32 box filter and the exact
+        // tests on the face's vertices
+        float4 r0[4], r1[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-          const int j = it0 + k < it1 ? it0 + k : it1 - 1;
-          r0[k] = g.wrec[2 * (int64_t)j];
-          r1[k] = g.wrec[2 * (int64_t)j + 1];
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (it0 + k >= it1) break;
-        const float4 q0 = r0[k], q1 = r1[k];
-        if (Pu < (double)q0.x || Pu > (double)q0.y || Pv < (double)q0.z || Pv > (double)q0.w) continue;
-        if (s > 0 ? (double)q1.y < p[ax] : (double)q1.x > p[ax]) continue;
-        const int f = bits_int(q1.z);
-        const double* fr = g.frec + FREC * (int64_t)f;
-        const double Au = fr[FR_V + au], Av = fr[FR_V + av];
-        const double Bu = fr[FR_V + 3 + au], Bv = fr[FR_V + 3 + av];
-        const double Cu = fr[FR_V + 6 + au], Cv = fr[FR_V + 6 + av];
-        // projected point-in-triangle: three orientations with a static rounding bound
-        int pos = 0, neg = 0, unk = 0;
-        const double eu[3] = {Bu - Au, Cu - Bu, Au - Cu}, ev[3] = {Bv - Av, Cv - Bv, Av - Cv};
-        const double ou[3] = {Pu - Au, Pu - Bu, Pu - Cu}, ov[3] = {Pv - Av, Pv - Bv, Pv - Cv};
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const double t1 = eu[q] * ov[q], t2 = ev[q] * ou[q];
-            const double o = t1 - t2;
-            const double bound = 1e-14 * (fabs(t1) + fabs(t2)) + 1e-300;
-            if (o > bound) ++pos;
-            else if (o < -bound) ++neg;
-            else ++unk;
+        for (int k = 0; k < 4; ++k) {
+            const int j = it0 + k < w.it1 ? it0 + k : w.it1 - 1;
+            r0[k] = g.wrec[2 * (int64_t)j];
+            r1[k] = g.wrec[2 * (int64_t)j + 1];
         }
-        if (pos > 0 && neg > 0) continue;            // certainly outside the projected triangle
-        if (unk > 0) return false;                   // ray through (or near) an edge or a vertex
-        // side of p against the face plane: N . (p - A), N = (B - A) x (C - A)
-        const double Ax = fr[FR_V], Ay = fr[FR_V + 1], Az = fr[FR_V + 2];
-        const double e1x = fr[FR_V + 3] - Ax, e1y = fr[FR_V + 4] - Ay, e1z = fr[FR_V + 5] - Az;
-        const double e2x = fr[FR_V + 6] - Ax, e2y = fr[FR_V + 7] - Ay, e2z = fr[FR_V + 8] - Az;
-        const double dx = px - Ax, dy = py - Ay, dz = pz - Az;
-        const double Nx = e1y * e2z - e1z * e2y, Ny = e1z * e2x - e1x * e2z, Nz = e1x * e2y - e1y * e2x;
-        const double side = Nx * dx + Ny * dy + Nz * dz;
-        const double mag = (fabs(e1y * e2z) + fabs(e1z * e2y)) * fabs(dx) + (fabs(e1z * e2x) + fabs(e1x * e2z)) * fabs(dy) +
-                           (fabs(e1x * e2y) + fabs(e1y * e2x)) * fabs(dz);
-        if (!(fabs(side) > 1e-13 * mag + 1e-300)) return false;     // p on (or within rounding of) the plane
-        // N_ax has the sign of the projected orientation (counter-clockwise in (au, av) <=> N_ax > 0); the ray
-        // p + t s e_ax meets the plane at t = -side / (s N_ax): ahead of p <=> side and s N_ax differ in sign
-        const double sgnN = pos > 0 ? 1.0 : -1.0;
-        if ((side > 0.0) == (s * sgnN > 0.0)) continue;
-        // the ray leaves the solid where s N_ax > 0 (outward normal along the ray)
-        acc += (s * sgnN > 0.0) ? 1.0 : -1.0;
-      }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (it0 + k >= w.it1) break;
+            const int r = walk_item(g.frec, px, py, pz, Pa, Pu, Pv, w.au, w.av, w.s, r0[k], r1[k]);
+            if (r == 2) return false;
+            acc += (double)r;
+        }
     }
     *chi = acc;
     return true;

@@ -71,14 +71,15 @@ class Mesh3D:
         self.device = torch.device(device) if not host_only else None
         lib = L.load()
         h = ctypes.c_void_p()
-        info = (ctypes.c_int64 * 9)()
+        info = (ctypes.c_int64 * 10)()
         flags = (L.MESH_HOST_ONLY if host_only else 0) | (L.MESH_SOUP if soup else 0)
         if not host_only:
             torch.cuda.set_device(self.device)
         L.check(lib.wsb_mesh3d_create(L.nptr(V), len(V), L.nptr(Fc), len(Fc), self.support_max, flags, ctypes.byref(h),
                                       info))
         self._h = h
-        self.K, nx, ny, nz, self.nf, self.items, self.bytes, self.components, self.dirty = (int(v) for v in info)
+        (self.K, nx, ny, nz, self.nf, self.items, self.bytes, self.components, self.dirty,
+         self.longest_walk) = (int(v) for v in info)
         self.grid_shape = (nx, ny, nz)
         g = (ctypes.c_double * 4)()
         L.check(lib.wsb_mesh3d_grid(h, g))
