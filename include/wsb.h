@@ -114,6 +114,25 @@ int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form, const int
 int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max,
                           double* d, double* nrm, int32_t* edge, void* stream);
 
+/* ---- conditioned small / sliver triangle (reference edge/warptriangle.py:136-181 `_triangle_kernel` and `channels`,
+ * applied by scene/scene.py:407-408 and scene/fused.py:556-564).  For a closed 3-vertex loop, a row whose support h
+ * makes the triangle small (longest edge <= 0.02 h) or a sliver (twice the area <= 0.01 longest edge squared) takes
+ * an adaptive area quadrature (tolerance relative to the triangle-scaled channels, at most max_panels panels) in
+ * place of the edge-local closed form.  tri[8] (host): v0 (x, y), v1 - v0, v2 - v0 of the loop ordered with its
+ * shortest edge first (reference `prepare`), |twice the signed area|, its sign (+1 counter-clockwise).
+ * values[n*12], estimate[n*12] (nullable; units of h, device channel order, complete: background x plane vector
+ * + sign x the area integral), status[n] (nullable): 0 the closed-form route (values and estimate 0), 1 converged,
+ * -1 budget exhausted (values NaN).  valid[n] nullable.  out[n*12] (nullable): the rows with status != 0 are
+ * overwritten with the values finalised by `flags` (WSB_OUT_PHYSICAL, WSB_OUT_WORLD with rot4): out must hold
+ * this representation's contribution alone (WSB_OUT_ACCUMULATE is rejected). */
+int wsb_triangle2d(const double* tri, int background, int kernel_id, const double* lpos, const double* lsup,
+                   const uint8_t* valid, int n, double tolerance, int max_panels, double* values, double* estimate,
+                   int32_t* status, const double* rot4, int flags, double* out, void* stream);
+/* the same rows on the host (host pointers; no `out`) */
+int wsb_triangle2d_host(const double* tri, int background, int kernel_id, const double* lpos, const double* lsup,
+                        const uint8_t* valid, int n, double tolerance, int max_panels, double* values, double* estimate,
+                        int32_t* status);
+
 /* ---- batched evaluation across many resident geometries (continuous batching of a boundary server; no counterpart
  * in the reference).  wsb_geom2d_desc writes wsb_geom2d_desc_bytes() bytes describing a geometry (grid parameters and
  * device pointers); a caller gathers the descriptors of its resident geometries into one device table.  row_desc[n]

@@ -10,6 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from warpsphboundaries_amd import plan as PL          # noqa: E402
+from fractions import Fraction as F                   # noqa: E402
+
 from warpsphboundaries_amd.kernels import NAMES, plane_channels   # noqa: E402
 
 NMAX = 16          # fixed array extent (>= max power + 1 of every profile)
@@ -75,7 +77,51 @@ def render():
         out.append("};")
         out.append("")
     out += render3d()
+    out += render_triangle()
     return "\n".join(out) + "\n"
+
+
+def render_triangle():
+    """conditioned triangle quadrature: pi W per support radius in powers of u = R - r (ascending radii), the first
+    piece of pi W about the origin (its regular -W'(r)/r near r = 0), and Gauss-Legendre rules of 8, 16, 32 nodes on
+    [0, 1]."""
+    import numpy as np
+    from math import comb
+    from warpsphboundaries_amd.kernels import kernel
+    out = ["template <int KID> struct TTab;"]
+    for kid, name in enumerate(NAMES):
+        k = kernel(name)
+        ub = {}
+        for b in k.blocks:
+            d = ub.setdefault(b.R, {})
+            for n, c in enumerate(b.coeffs):
+                for j in range(n + 1):
+                    d[j] = d.get(j, 0) + c * comb(n, j) * b.R ** (n - j) * (-1) ** j
+        radii = sorted(ub)
+        deg = max(max(d) for d in ub.values())
+        assert deg < NMAX and len(radii) <= 2
+        lo, hi, org = k.pieces[0]
+        assert lo == 0 and len(org) <= NMAX
+        out.append(f"template <> struct TTab<{kid}> {{   // {name}")
+        out.append(f"  static constexpr int NRAD = {len(radii)};")
+        out.append(f"  static constexpr int DEG = {deg};")
+        out.append(f"  static constexpr double RAD[NRAD] = {{{', '.join(repr(float(R)) for R in radii)}}};")
+        out.append(f"  static constexpr double UB[NRAD][{NMAX}] = "
+                   f"{{{', '.join(_arr([ub[R].get(j, 0) for j in range(deg + 1)]) for R in radii)}}};")
+        out.append(f"  static constexpr int NORIG = {len(org)};")
+        out.append(f"  static constexpr bool REGULAR = {'true' if len(org) > 1 and org[1] == 0 else 'false'};")
+        out.append(f"  static constexpr double ORIG_HI = {float(min(F(1, 8), hi))!r};")
+        out.append(f"  static constexpr double ORIG[{NMAX}] = {_arr(org)};")
+        out.append("};")
+        out.append("")
+    out.append("struct GLTab {")
+    for n in (8, 16, 32):
+        x, w = np.polynomial.legendre.leggauss(n)
+        out.append(f"  static constexpr double X{n}[{n}] = {{{', '.join(repr(float(v)) for v in (x + 1) / 2)}}};")
+        out.append(f"  static constexpr double W{n}[{n}] = {{{', '.join(repr(float(v)) for v in w / 2)}}};")
+    out.append("};")
+    out.append("")
+    return out
 
 
 NM3 = 18           # array extent of the 3-D tables (>= largest power of Phi2 + 3)

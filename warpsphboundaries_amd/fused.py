@@ -5,7 +5,9 @@
 
 Stage 1 evaluates all twelve channels of every row with the row sums, the indicator x plane vector, the physical
 scaling and the world rotation fused into the kernel (``WSB_OUT_PHYSICAL | WSB_OUT_WORLD``); several
-representations of one body accumulate into the same rows.  Stage 2 only slices:
+representations of one body accumulate into the same rows.  A conditioned triangle (a closed 3-vertex loop) replaces
+its closed form by the adaptive quadrature of ``triangle.py`` in the rows where it is small or a sliver at their
+support.  Stage 2 only slices:
 
     "lam"  [B, N]        int_solid W                  "g0"   [B, N, 2]     int_solid grad_x W
     "cov"  [B, N, 2, 2]  int_solid y (x) grad_x W      "m1"   [B, N, 2]     int_solid y W
@@ -20,6 +22,7 @@ from typing import Optional
 import torch
 
 from . import _lib as L
+from . import triangle
 from .scene import FixedTopology, SurfaceRep, kernelName
 
 F64 = torch.float64
@@ -69,9 +72,20 @@ class FusedWall(WallAggregate):
                 for rep, topo in zip(reps, ba.reps):
                     if not isinstance(rep, SurfaceRep) or not isinstance(topo, FixedTopology):
                         raise NotImplementedError("FusedWall: surface representations with the fixed-capacity adjacency only")
-                    flags = L.OUT_PHYSICAL | L.OUT_WORLD | L.OUT_ACCUMULATE
-                    topo.geom.moments(grp.kernel, topo.e, topo.counts, topo.ind, ba.lpos, ba.lsup, form=grp.form, rot4=R,
-                                      flags=flags, out=self.channels[gi, bi])
+                    if rep._triangle is None:
+                        flags = L.OUT_PHYSICAL | L.OUT_WORLD | L.OUT_ACCUMULATE
+                        topo.geom.moments(grp.kernel, topo.e, topo.counts, topo.ind, ba.lpos, ba.lsup, form=grp.form,
+                                          rot4=R, flags=flags, out=self.channels[gi, bi])
+                    else:
+                        # a conditioned triangle: its rows where the triangle is small or a sliver at their support
+                        # take the quadrature in place of this representation's closed form (reference
+                        # scene/fused.py:556-564), then the representation accumulates like the others
+                        flags = L.OUT_PHYSICAL | L.OUT_WORLD
+                        own = topo.geom.moments(grp.kernel, topo.e, topo.counts, topo.ind, ba.lpos, ba.lsup,
+                                                form=grp.form, rot4=R, flags=flags)
+                        triangle.override(rep.vertices, rep._triangle, grp.kernel, rep.background, ba.lpos, ba.lsup,
+                                          valid=ba.valid, rot4=R, flags=flags, out=own)
+                        self.channels[gi, bi] += own
             if grp.channels is not None:
                 keep = torch.zeros(12, dtype=F64, device=dev)
                 keep[list(grp.channels)] = 1.0

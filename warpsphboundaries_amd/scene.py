@@ -16,6 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import triangle
 from .device import Geom2D
 from .kernels import KERNEL_ID, plane_channels
 
@@ -102,6 +103,9 @@ class SurfaceRep:
         self.edges = torch.as_tensor(np.asarray(edges, dtype=np.int32), dtype=torch.int32)
         self.background = int(background)
         self.conditioned = bool(conditioned)
+        # a closed 3-vertex loop: rows whose support makes it small or a sliver take the conditioned quadrature
+        # (reference scene/scene.py:228 and edge/warptriangle.py); None: the closed form everywhere
+        self._triangle = triangle.prepare(self.vertices, self.edges) if conditioned else None
         self._geoms = {}
 
     @staticmethod

@@ -18,6 +18,8 @@ counterpart: the reference evaluates one position set per call).
   geometry has a device descriptor (grid parameters and array pointers) in one table; a batch gathers the pending
   requests of one kernel from any number of scenes, each row names its geometry, and one adjacency launch plus one
   moment launch (per surface representation index) serve the whole batch.  Host work per batch is a few copies.
+  A scene with a conditioned triangle (a closed 3-vertex loop whose rows may take the small / sliver quadrature of
+  ``triangle.py``) is launched per scene, where each representation's rows are finalised on their own.
 * 3-D scenes (``add_scene(name, None, support_max, mesh=(vertices, faces))``): a resident ``Mesh3D`` (face records,
   cell grid, walk lists); their requests are batched per (scene, kernel) into one cell sort, one face adjacency +
   winding launch and one launch of the 22 stored moments.  The face list of a batch is sized from the scene's
@@ -42,6 +44,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import triangle
 from .scene import SurfaceRep, kernelName
 
 F64 = torch.float64
@@ -87,6 +90,7 @@ class ResidentScene:
     desc: list = field(default_factory=list)     # descriptor-table index of each geometry
     mesh: Optional[object] = None                # 3-D scene: its Mesh3D
     pairs_per_row: float = 8.0                   # 3-D: face-list sizing from the previous batches
+    conditioned: bool = False                    # 2-D: a representation is a conditioned triangle (per-scene launches)
 
     @property
     def dim(self):
@@ -243,7 +247,8 @@ class BoundaryServer:
         if self.resident_bytes + nbytes > self.budget:
             raise MemoryError(f"scene {name!r} needs {nbytes} resident bytes; {self.budget - self.resident_bytes} left")
         self.resident_bytes += nbytes
-        sc = ResidentScene(name, reps, float(support_max), geoms, max(g.K for g in geoms), nbytes)
+        sc = ResidentScene(name, reps, float(support_max), geoms, max(g.K for g in geoms), nbytes,
+                           conditioned=any(r._triangle is not None for r in reps))
         lib = L.load()
         for g in geoms:
             buf = ctypes.create_string_buffer(self._desc_bytes)
@@ -297,8 +302,9 @@ class BoundaryServer:
         k = kernelName(kernel)
         dim = sc.dim
         if isinstance(positions, torch.Tensor):
-            if not self.batched and dim == 2:
-                raise NotImplementedError("device-resident requests use the cross-scene batches")
+            if dim == 2 and (not self.batched or sc.conditioned):
+                raise NotImplementedError("device-resident requests use the cross-scene batches" + (
+                    ", which a scene with a conditioned triangle does not take" if sc.conditioned else ""))
             pos = positions.to(device=self.device, dtype=F64).reshape(-1, dim).contiguous()
             if isinstance(supports, torch.Tensor):
                 sup = supports.to(device=self.device, dtype=F64).reshape(-1).contiguous()
@@ -390,8 +396,10 @@ class BoundaryServer:
             group, rows = [], 0
             dim = sc.dim
             for r in list(self.pending):
-                same = (r.kernel == head.kernel and r.on_device == head.on_device and self.scenes[r.scene].dim == dim
-                        and ((self.batched and dim == 2) or r.scene == head.scene))
+                rs = self.scenes[r.scene]
+                cross = self.batched and dim == 2 and not sc.conditioned and not rs.conditioned
+                same = (r.kernel == head.kernel and r.on_device == head.on_device and rs.dim == dim
+                        and (cross or r.scene == head.scene))
                 if same and (not group or rows + r.n <= self.max_batch_rows):
                     group.append(r)
                     rows += r.n
@@ -418,7 +426,7 @@ class BoundaryServer:
             t1 = time.perf_counter()
             if dim == 3:
                 self._launch_mesh(si, sc, head.kernel, group, rows, fp)
-            elif self.batched:
+            elif self.batched and not sc.conditioned:
                 self._launch_batched(si, head.kernel, group, rows, fp)
             else:
                 self._launch(si, sc, head.kernel, group, rows, fp)
@@ -461,9 +469,20 @@ class BoundaryServer:
                 order = g.sort(s.lpos[:n], order=s.order[:n], scratch=s.sort_scratch)
             g.adjacency(s.lpos[:n], s.lsup[:n], pad=False, slots=s.slots[:n * g.K], counts=s.counts[:n], ind=s.ind[:n],
                         order=order)
-            flags = L.OUT_PHYSICAL | (L.OUT_ACCUMULATE if j else 0)
-            g.moments(kernel, s.slots[:n * g.K], s.counts[:n], s.ind[:n], s.lpos[:n], s.lsup[:n], form=self.form,
-                      flags=flags, out=s.out[:n])
+            rep = sc.reps[j]
+            if rep._triangle is None:
+                flags = L.OUT_PHYSICAL | (L.OUT_ACCUMULATE if j else 0)
+                g.moments(kernel, s.slots[:n * g.K], s.counts[:n], s.ind[:n], s.lpos[:n], s.lsup[:n], form=self.form,
+                          flags=flags, out=s.out[:n])
+            else:
+                # a conditioned triangle: its own rows first, the quadrature rows overwritten, then accumulated
+                own = s.out[:n] if j == 0 else torch.empty((n, 12), dtype=F64, device=self.device)
+                g.moments(kernel, s.slots[:n * g.K], s.counts[:n], s.ind[:n], s.lpos[:n], s.lsup[:n], form=self.form,
+                          flags=L.OUT_PHYSICAL, out=own)
+                triangle.override(rep.vertices, rep._triangle, kernel, rep.background, s.lpos[:n], s.lsup[:n],
+                                  flags=L.OUT_PHYSICAL, out=own)
+                if j:
+                    s.out[:n] += own
 
     def _launch(self, si: int, sc: ResidentScene, kernel: str, group: List[Request], rows: int, fp: int):
         s = self._slot(si, sc, rows)
