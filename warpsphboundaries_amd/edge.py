@@ -15,6 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import precision
 from .device import edge_channels12, plan_channels
 from .kernels import disk_moment, plane_channels
 from .scene import kernelName
@@ -44,10 +45,11 @@ def edge_channels(pair_q, pair_e, positions, supports, vertices, edges, kernel, 
                   stable=None, channels=None, form=None):
     """edge-local channels c[P, 9] (lam, m10, m01, Gx, Gy, g1xx, g1xy, g1yx, g1yy) in units of h of every (query, edge)
     pair; edges counter-clockwise around the solid.  ``channels`` keeps a subset (the others are zero).  ``form``:
-    'fp64' (closed form, default), 'mixed', or 'plain' (the plan interpreter)."""
+    'fp64' (closed form), 'mixed', or 'plain' (the plan interpreter); None: the process precision
+    (``precision.DEFAULT_FORM``, from warpSPHCore_PRECISION)."""
     if isinstance(stable, tuple):
         raise NotImplementedError("the Chebyshev quadrature route is not built: the closed forms are FP64-accurate")
-    c = channels12(pair_q, pair_e, positions, supports, vertices, edges, kernel, device, form or "fp64")[:, :9]
+    c = channels12(pair_q, pair_e, positions, supports, vertices, edges, kernel, device, precision.default_form(form))[:, :9]
     if channels is not None:
         keep = torch.zeros(9, dtype=F64, device=c.device)
         keep[[int(k) for k in channels]] = 1.0
@@ -59,7 +61,7 @@ def second_channels(pair_q, pair_e, positions, supports, vertices, edges, kernel
     """edge-local m20, m11, m02 in units of h; add indicator * plane moment."""
     if isinstance(stable, tuple):
         raise NotImplementedError("the Chebyshev quadrature route is not built")
-    return channels12(pair_q, pair_e, positions, supports, vertices, edges, kernel, device, form or "fp64")[:, 9:]
+    return channels12(pair_q, pair_e, positions, supports, vertices, edges, kernel, device, precision.default_form(form))[:, 9:]
 
 
 def indicator_vector(kernel):

@@ -22,7 +22,7 @@ from typing import Optional
 import torch
 
 from . import _lib as L
-from . import triangle
+from . import precision, triangle
 from .scene import FixedTopology, SurfaceRep, kernelName
 
 F64 = torch.float64
@@ -42,11 +42,11 @@ class FusedGroup:
     ('fp64' closed form, 'mixed', or 'plain' = the plan interpreter)."""
     kernel: str
     channels: Optional[tuple] = None
-    form: str = "fp64"
+    form: Optional[str] = None          # None: the process precision (warpSPHCore_PRECISION)
 
     def resolved(self):
         ch = None if self.channels is None else tuple(sorted(set(int(c) for c in self.channels)))
-        return FusedGroup(kernelName(self.kernel), ch, self.form)
+        return FusedGroup(kernelName(self.kernel), ch, precision.default_form(self.form))
 
 
 class WallAggregate:
@@ -57,7 +57,7 @@ class FusedWall(WallAggregate):
     def __init__(self, scene, adjacency, groups):
         self.scene = scene
         self.adj = adjacency
-        self.groups = tuple(g.resolved() if isinstance(g, FusedGroup) else FusedGroup(kernelName(g)) for g in groups)
+        self.groups = tuple((g if isinstance(g, FusedGroup) else FusedGroup(kernelName(g))).resolved() for g in groups)
         self.N = adjacency.numQueries
         dev = torch.device(scene.device)
         self.dev = dev

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 from typing import Protocol, runtime_checkable
 
+from . import precision
 from .fused import FusedGroup, FusedWall, WallAggregate, WallOutput
 from .scene import Scene, fixed_adjacency, kernelName
 
@@ -28,9 +29,9 @@ class BoundaryProvider(Protocol):
 class AnalyticBoundary:
     """the analytic bodies of a ``Scene`` as a boundary provider (exact edge-reduction integrals on the MI355X)."""
 
-    def __init__(self, scene: Scene, form: str = "fp64"):
+    def __init__(self, scene: Scene, form: str = None):
         self.scene = scene
-        self.form = form
+        self.form = precision.default_form(form)      # None: the process precision (warpSPHCore_PRECISION)
         self._layouts = {}
 
     @property
