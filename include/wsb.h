@@ -33,7 +33,13 @@ enum wsb_status {
 };
 
 /* kernels: the exact table of the reference (curvbound/kernels.py:41-50) */
-enum wsb_kernel { WSB_CUBIC = 0, WSB_W2 = 1, WSB_W4 = 2, WSB_W6 = 3 };
+enum wsb_kernel {
+    WSB_CUBIC = 0, WSB_W2 = 1, WSB_W4 = 2, WSB_W6 = 3,
+    /* the wall-operation kernels of the reference's full aggregate layout (scene/provider.py:73-89), 2-D only:
+     * the cover vector's cone (1 - q), the wall Laplacian's L = W'/r of w2 / w4, and W^5 of w2 / w4 (tensile; the
+     * gradient channels 3, 4 only, the other channels are 0; no plain form, no per-edge channels) */
+    WSB_CONE = 4, WSB_LW2 = 5, WSB_LW4 = 6, WSB_W2P5 = 7, WSB_W4P5 = 8
+};
 
 /* arithmetic forms of the evaluation */
 enum wsb_form {

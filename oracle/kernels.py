@@ -20,6 +20,22 @@ _SPEC = {
 }
 
 
+def _normalised(shape):
+    """(c2 pi, [((0, 1), shape)]) with 2 int_0^1 q (c2 pi) shape dq = 1 (mass 1 in the plane)."""
+    return (1 / (2 * sp.integrate(_q * sp.expand(shape), (_q, 0, 1))), [((0, 1), shape)])
+
+
+# the wall-operation kernels (reference edge/kernels.py:168-170, scene/viscosity.py:1-40, scene/tensile.py:1-30):
+# the cone (1 - q) of the cover vector, L = W'/r of w2 / w4 divided by its constant (the wall Laplacian), and W^5
+_W2 = (1 - _q) ** 4 * (1 + 4 * _q)
+_W4 = (1 - _q) ** 6 * (1 + 6 * _q + sp.Rational(35, 3) * _q ** 2)
+_SPEC["cone"] = _normalised(1 - _q)
+_SPEC["lw2"] = _normalised(sp.expand(sp.diff(_W2, _q) / _q) / -20)
+_SPEC["lw4"] = _normalised(sp.expand(sp.diff(_W4, _q) / _q) / sp.Rational(-56, 3))
+_SPEC["w2p5"] = _normalised(_W2 ** 5)
+_SPEC["w4p5"] = _normalised(_W4 ** 5)
+
+
 def _coeffs(expr):
     p = sp.Poly(sp.expand(expr), _q)
     deg = p.degree()

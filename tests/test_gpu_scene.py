@@ -89,8 +89,11 @@ def test_provider_protocol_and_lean_aggregate():
     fw = ab.aggregate(ps, 0.04, kernel="w2", lean=True)
     assert set(fw.out) == {"lam", "G", "Cov"}
     assert fw.out["lam"].shape == (1, 1000) and fw.out["Cov"].shape == (1, 1000, 2, 2)
+    full = ab.aggregate(ps, 0.04, kernel="w2")                  # the full layout (cover and tensile groups)
+    assert set(full.out) == {"lam", "G", "Cov", "cover", "tens"}
+    torch.testing.assert_close(full.out["lam"], fw.out["lam"], rtol=0, atol=0)
     with pytest.raises(NotImplementedError):
-        ab.aggregate(ps, 0.04, kernel="w2")
+        ab.aggregate(ps, 0.04, kernel="w6")                    # the wall operations: Wendland C2 and C4 only
     # lap identity 2 lam - tr cov
     lap = fw.evaluate((WallOutput("lap", 0, "lap"),))["lap"]
     torch.testing.assert_close(lap, 2 * fw.out["lam"] - fw.out["Cov"][..., 0, 0] - fw.out["Cov"][..., 1, 1])

@@ -311,9 +311,92 @@ __device__ __forceinline__ void pair_channels(double ux, double uy, double vx, d
 
 }  // namespace closed
 
+// ================================================================================================= W^5 kernels
+// The tensile kernels (W^5 of w2 / w4, degree 25 / 40) are used for their gradient channels only (reference
+// scene/tensile.py: T = (1/5) grad int_solid W^5).  In monomials their coefficients reach 1e10 and the closed form
+// cancels to 1e-3 (which is why the reference gives them a Chebyshev plan); here the gradient's edge term
+// -n int_chord W ds is integrated directly: W >= 0 evaluated as a power of the positive Wendland shape, on 24-node
+// Gauss-Legendre panels that double in width away from the foot of the perpendicular (each panel stays at a fixed
+// ratio from the branch points s = +-i z of r = sqrt(z^2 + s^2); through the foot, z = 0, W is a polynomial of
+// degree <= 40 in s and one panel is exact).
+namespace chord {
+
+// pi W5(r) and -(pi W5)'(r) / r from the Wendland shape s (w2: u^4 (1 + 4r), w4: u^6 (1 + 6r + 35/3 r^2), u = 1 - r),
+// W5 = c5 s^5: every factor positive, so no cancellation (the 40 powers of u alone cancel by 1e9 near r = 0 for w4);
+// -W5'/r = 5 c5 s^4 (-s'/r) with -s'/r = 20 u^3 (w2), 56/3 (1 + 5r) u^5 (w4)
+template <int KID>
+__host__ __device__ inline void p5_wr(double r, double& W, double& radial) {
+    const double u = 1.0 - r;
+    if (!(u > 0.0)) {
+        W = radial = 0.0;
+        return;
+    }
+    double s, ds;
+    if constexpr (KID == WSB_W2P5) {
+        const double u3 = u * u * u;
+        s = u3 * u * (1.0 + 4.0 * r);
+        ds = 20.0 * u3;
+    } else {
+        const double u2 = u * u, u5 = u2 * u2 * u;
+        s = u5 * u * (1.0 + 6.0 * r + (35.0 / 3.0) * r * r);
+        ds = (56.0 / 3.0) * (1.0 + 5.0 * r) * u5;
+    }
+    const double s2 = s * s, s4 = s2 * s2;
+    W = P5Tab<KID>::C * s4 * s;
+    radial = P5Tab<KID>::C * 5.0 * s4 * ds;
+}
+
+template <int KID>
+__device__ __forceinline__ double wpi(double r) {
+    double W, radial;
+    p5_wr<KID>(r, W, radial);
+    return W;
+}
+
+// int_a^b pi W(sqrt(z^2 + s^2)) ds for 0 <= a < b (units of h)
+template <int KID>
+__device__ inline double half_chord(double az, double a, double b) {
+    if (az < 1e-17 * b) az = 0.0;            // then r = s to within 1e-34 relative on the whole interval
+    double sum = 0.0, p = a;
+    for (int it = 0; it < 80 && p < b; ++it) {
+        const double w = az > 0.0 ? fmax(p, az) : b - p;
+        const double q = fmin(b, p + w);
+        double part = 0.0;
+        for (int j = 0; j < 24; ++j) {
+            const double s = p + (q - p) * GLTab::X24[j];
+            part += GLTab::W24[j] * wpi<KID>(sqrt(az * az + s * s));
+        }
+        sum += (q - p) * part;
+        p = q;
+    }
+    return sum;
+}
+
+// the gradient channels (3, 4) of one (query, edge) pair; the arguments as closed::pair_channels
+template <int KID>
+__device__ __forceinline__ void pair_grad(double ux, double uy, double vx, double vy, double h, double ih, double tx,
+                                          double ty, double iell, double c[wsb::NCH]) {
+    if (iell == 0.0) return;
+    const double z = (ux * vy - uy * vx) * ih * iell;
+    const double az = fabs(z);
+    if (!(az < 1.0)) return;
+    const double s0 = (tx * ux + ty * uy) * ih, s1 = (tx * vx + ty * vy) * ih;
+    const double L = sqrt((1.0 - az) * (1.0 + az));
+    const double lo = fmax(s0, -L), hi = fmin(s1, L);
+    if (!(lo < hi)) return;
+    double A;
+    if (lo >= 0.0) A = half_chord<KID>(az, lo, hi);
+    else if (hi <= 0.0) A = half_chord<KID>(az, -hi, -lo);
+    else A = half_chord<KID>(az, 0.0, -lo) + half_chord<KID>(az, 0.0, hi);
+    c[3] -= ty * A * WSB_INV_PI;
+    c[4] += tx * A * WSB_INV_PI;
+}
+
+}  // namespace chord
+
 // ================================================================================================= row output
 template <int KID>
-__device__ __forceinline__ const double* plane_of() { return KTab<KID>::PLANE; }
+__device__ __forceinline__ const double* plane_of() { return TTab<KID>::PLANE; }
 
 // indicator x plane vector, physical scaling, world rotation, store / accumulate one row of twelve channels
 __device__ __forceinline__ void finalize_row(double c[wsb::NCH], double ind, double h, const double* plane, const double* rot4,

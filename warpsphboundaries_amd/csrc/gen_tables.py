@@ -12,7 +12,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from warpsphboundaries_amd import plan as PL          # noqa: E402
 from fractions import Fraction as F                   # noqa: E402
 
-from warpsphboundaries_amd.kernels import NAMES, plane_channels   # noqa: E402
+from warpsphboundaries_amd.kernels import ALL_NAMES, CLOSED_NAMES, NAMES, plane_channels   # noqa: E402
 
 NMAX = 16          # fixed array extent (>= max power + 1 of every profile)
 
@@ -25,7 +25,7 @@ def _arr(vals):
 def render():
     out = ["// generated by gen_tables.py from the exact kernel table -- do not edit",
            "#pragma once", "", "template <int KID> struct KTab;", ""]
-    for kid, name in enumerate(NAMES):
+    for kid, name in enumerate(CLOSED_NAMES):
         bl = PL.block_profiles(name)
         nb = len(bl)
         degP = [len(b.P) - 1 for b in bl]
@@ -81,15 +81,24 @@ def render():
     return "\n".join(out) + "\n"
 
 
+NT = 48           # array extent of the u-basis tables (>= degree + 1 of every kernel, W^5 of w4 included)
+
+
+def _arrn(vals, n):
+    vals = list(vals) + [0] * (n - len(vals))
+    return "{" + ", ".join(repr(float(v)) for v in vals) + "}"
+
+
 def render_triangle():
-    """conditioned triangle quadrature: pi W per support radius in powers of u = R - r (ascending radii), the first
-    piece of pi W about the origin (its regular -W'(r)/r near r = 0), and Gauss-Legendre rules of 8, 16, 32 nodes on
-    [0, 1]."""
+    """every kernel (the wall-operation kernels included): pi W per support radius in powers of u = R - r (ascending
+    radii; the conditioned triangle quadrature and the chord quadrature of the W^5 kernels), the first piece of pi W
+    about the origin (its regular -W'(r)/r near r = 0), the whole-support vector, and Gauss-Legendre rules of 8, 16,
+    24, 32 nodes on [0, 1]."""
     import numpy as np
     from math import comb
     from warpsphboundaries_amd.kernels import kernel
     out = ["template <int KID> struct TTab;"]
-    for kid, name in enumerate(NAMES):
+    for kid, name in enumerate(ALL_NAMES):
         k = kernel(name)
         ub = {}
         for b in k.blocks:
@@ -99,23 +108,32 @@ def render_triangle():
                     d[j] = d.get(j, 0) + c * comb(n, j) * b.R ** (n - j) * (-1) ** j
         radii = sorted(ub)
         deg = max(max(d) for d in ub.values())
-        assert deg < NMAX and len(radii) <= 2
+        assert deg < NT and len(radii) <= 2
         lo, hi, org = k.pieces[0]
-        assert lo == 0 and len(org) <= NMAX
+        assert lo == 0 and len(org) <= NT
         out.append(f"template <> struct TTab<{kid}> {{   // {name}")
         out.append(f"  static constexpr int NRAD = {len(radii)};")
         out.append(f"  static constexpr int DEG = {deg};")
         out.append(f"  static constexpr double RAD[NRAD] = {{{', '.join(repr(float(R)) for R in radii)}}};")
-        out.append(f"  static constexpr double UB[NRAD][{NMAX}] = "
-                   f"{{{', '.join(_arr([ub[R].get(j, 0) for j in range(deg + 1)]) for R in radii)}}};")
+        out.append(f"  static constexpr double UB[NRAD][{NT}] = "
+                   f"{{{', '.join(_arrn([ub[R].get(j, 0) for j in range(deg + 1)], NT) for R in radii)}}};")
         out.append(f"  static constexpr int NORIG = {len(org)};")
         out.append(f"  static constexpr bool REGULAR = {'true' if len(org) > 1 and org[1] == 0 else 'false'};")
         out.append(f"  static constexpr double ORIG_HI = {float(min(F(1, 8), hi))!r};")
-        out.append(f"  static constexpr double ORIG[{NMAX}] = {_arr(org)};")
+        out.append(f"  static constexpr double ORIG[{NT}] = {_arrn(org, NT)};")
+        out.append(f"  static constexpr double PLANE[12] = {{{', '.join(repr(v) for v in plane_channels(name))}}};")
         out.append("};")
         out.append("")
+    # the W^5 kernels: normalisation c5 (pi W5 = c5 shape^5 with the unnormalised Wendland shape)
+    from warpsphboundaries_amd.kernels import KERNEL_ID, P5_NAMES
+    out.append("template <int KID> struct P5Tab;")
+    for name in P5_NAMES:
+        out.append(f"template <> struct P5Tab<{KERNEL_ID[name]}> {{   // {name}")
+        out.append(f"  static constexpr double C = {float(kernel(name).c2_pi)!r};")
+        out.append("};")
+    out.append("")
     out.append("struct GLTab {")
-    for n in (8, 16, 32):
+    for n in (8, 16, 24, 32):
         x, w = np.polynomial.legendre.leggauss(n)
         out.append(f"  static constexpr double X{n}[{n}] = {{{', '.join(repr(float(v)) for v in (x + 1) / 2)}}};")
         out.append(f"  static constexpr double W{n}[{n}] = {{{', '.join(repr(float(v)) for v in w / 2)}}};")

@@ -173,7 +173,11 @@ __global__ __launch_bounds__(TILE) void tile12_kernel(TileArgs a) {
                 const double x = s_px[r], y = s_py[r];
                 const double4 ab = a.erec[e];
                 const double4 tn = a.etan[e];
-                closed::pair_channels<KID, FORM>(ab.x - x, ab.y - y, ab.z - x, ab.w - y, s_h[r], s_ih[r], tn.x, tn.y, tn.z, c);
+                if constexpr (KID >= WSB_W2P5)
+                    chord::pair_grad<KID>(ab.x - x, ab.y - y, ab.z - x, ab.w - y, s_h[r], s_ih[r], tn.x, tn.y, tn.z, c);
+                else
+                    closed::pair_channels<KID, FORM>(ab.x - x, ab.y - y, ab.z - x, ab.w - y, s_h[r], s_ih[r], tn.x, tn.y,
+                                                     tn.z, c);
             }
         }
 #pragma unroll
@@ -186,7 +190,9 @@ __global__ __launch_bounds__(TILE) void tile12_kernel(TileArgs a) {
         }
         __syncthreads();
     }
-    if (live) finalize_row(acc, a.ind[row], h, plane_of<KID>(), a.rot4, a.flags, a.out + (int64_t)row * wsb::NCH);
+    // the W^5 kernels carry their gradient channels only (the whole-support gradient is zero: no indicator term)
+    const int flags = KID >= WSB_W2P5 ? (a.flags | WSB_OUT_NO_INDICATOR) : a.flags;
+    if (live) finalize_row(acc, a.ind[row], h, plane_of<KID>(), a.rot4, flags, a.out + (int64_t)row * wsb::NCH);
 }
 
 template <int KID, int FORM>
@@ -203,6 +209,11 @@ int dispatch_tile(int kid, const TileArgs& a, hipStream_t s) {
         case WSB_W2: return launch_tile<WSB_W2, FORM>(a, s);
         case WSB_W4: return launch_tile<WSB_W4, FORM>(a, s);
         case WSB_W6: return launch_tile<WSB_W6, FORM>(a, s);
+        case WSB_CONE: return launch_tile<WSB_CONE, FORM>(a, s);
+        case WSB_LW2: return launch_tile<WSB_LW2, FORM>(a, s);
+        case WSB_LW4: return launch_tile<WSB_LW4, FORM>(a, s);
+        case WSB_W2P5: return launch_tile<WSB_W2P5, closed::FP64>(a, s);
+        case WSB_W4P5: return launch_tile<WSB_W4P5, closed::FP64>(a, s);
     }
     return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kid);
 }
@@ -247,6 +258,9 @@ int dispatch_pairs(int kid, const PairArgs& a, hipStream_t s) {
         case WSB_W2: hipLaunchKernelGGL((pairs12_kernel<WSB_W2, FORM>), g, b, 0, s, a); break;
         case WSB_W4: hipLaunchKernelGGL((pairs12_kernel<WSB_W4, FORM>), g, b, 0, s, a); break;
         case WSB_W6: hipLaunchKernelGGL((pairs12_kernel<WSB_W6, FORM>), g, b, 0, s, a); break;
+        case WSB_CONE: hipLaunchKernelGGL((pairs12_kernel<WSB_CONE, FORM>), g, b, 0, s, a); break;
+        case WSB_LW2: hipLaunchKernelGGL((pairs12_kernel<WSB_LW2, FORM>), g, b, 0, s, a); break;
+        case WSB_LW4: hipLaunchKernelGGL((pairs12_kernel<WSB_LW4, FORM>), g, b, 0, s, a); break;
         default: return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kid);
     }
     WSB_HIP(hipGetLastError());
@@ -254,10 +268,9 @@ int dispatch_pairs(int kid, const PairArgs& a, hipStream_t s) {
 }
 
 void plane_host(int kid, double* out) {
-    const double* p = kid == WSB_CUBIC ? KTab<WSB_CUBIC>::PLANE
-                    : kid == WSB_W2    ? KTab<WSB_W2>::PLANE
-                    : kid == WSB_W4    ? KTab<WSB_W4>::PLANE
-                                       : KTab<WSB_W6>::PLANE;
+    static const double* const planes[] = {TTab<0>::PLANE, TTab<1>::PLANE, TTab<2>::PLANE, TTab<3>::PLANE, TTab<4>::PLANE,
+                                           TTab<5>::PLANE, TTab<6>::PLANE, TTab<7>::PLANE, TTab<8>::PLANE};
+    const double* p = planes[kid];
     for (int k = 0; k < 12; ++k) out[k] = p[k];
 }
 
@@ -284,8 +297,10 @@ extern "C" int wsb_moments2d(const wsb_geom2d* g, const wsb_plan* plan, int kern
                              size_t scratch_bytes, void* stream) {
     WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
     WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes n=%d K=%d", n, K);
-    WSB_REQUIRE(kernel_id >= 0 && kernel_id <= 3, WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
+    WSB_REQUIRE(kernel_id >= 0 && kernel_id <= WSB_W4P5, WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
     WSB_REQUIRE(!(flags & WSB_OUT_WORLD) || rot4, WSB_EINVAL, "WSB_OUT_WORLD needs rot4");
+    WSB_REQUIRE(kernel_id < WSB_W2P5 || form != WSB_FORM_PLAIN, WSB_ENOTSUP,
+                "the W^5 kernels have no plan interpreter (their monomial plan cancels to 1e-3)");
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(slots && counts && ind && lpos && lsup && out, WSB_EINVAL, "null buffer");
     hipStream_t s = wsb::as_stream(stream);
@@ -323,7 +338,7 @@ extern "C" int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form
                                    int P, const double* pos, const double* sup, int nq, const double* verts,
                                    const int32_t* edges, int ne, double* out, void* stream) {
     WSB_REQUIRE(P >= 0, WSB_EINVAL, "negative pair count");
-    WSB_REQUIRE(kernel_id >= 0 && kernel_id <= 3, WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
+    WSB_REQUIRE(kernel_id >= 0 && kernel_id <= WSB_LW4, WSB_ENOTSUP, "kernel id %d has no per-edge channels", kernel_id);
     if (P == 0) return WSB_OK;
     WSB_REQUIRE(pair_q && pair_e && pos && sup && verts && edges && out, WSB_EINVAL, "null buffer");
     hipStream_t s = wsb::as_stream(stream);

@@ -67,6 +67,10 @@ __host__ __device__ inline void sort_cuts(double* cuts, int n) {
 // first piece's polynomial, whose -W'/r is a polynomial (W'(0) = 0)
 template <int KID>
 __host__ __device__ inline void kernel_wr(double r, double& W, double& radial) {
+    if constexpr (KID >= WSB_W2P5) {           // W^5: the factored form (its powers of u cancel near r = 0)
+        chord::p5_wr<KID>(r, W, radial);
+        return;
+    }
     using T = TTab<KID>;
     W = 0.0;
     double dw = 0.0;
@@ -220,7 +224,7 @@ __host__ __device__ inline int triangle_row(const Tri& T, double px, double py, 
     const double jac = T.area2 * ih * ih;
 #pragma unroll
     for (int k = 0; k < wsb::NCH; ++k) {
-        values[k] = status < 0 ? __builtin_nan("") : T.background * KTab<KID>::PLANE[k] + T.sign * total[k] * scale[k] * jac;
+        values[k] = status < 0 ? __builtin_nan("") : T.background * TTab<KID>::PLANE[k] + T.sign * total[k] * scale[k] * jac;
         estimate[k] = errors[k] * scale[k] * jac;
     }
     return status;
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(64) void triangle_kernel(TriArgs a) {
     }
     if (a.status) a.status[i] = st;
     if (st != 0 && a.out)
-        finalize_row(v, 0.0, h, KTab<KID>::PLANE, a.rot4, a.flags | WSB_OUT_NO_INDICATOR, a.out + (int64_t)i * wsb::NCH);
+        finalize_row(v, 0.0, h, TTab<KID>::PLANE, a.rot4, a.flags | WSB_OUT_NO_INDICATOR, a.out + (int64_t)i * wsb::NCH);
 }
 
 template <int KID>
@@ -332,6 +336,11 @@ extern "C" int wsb_triangle2d(const double* tri, int background, int kernel_id, 
         case WSB_W2: hipLaunchKernelGGL(tri::triangle_kernel<WSB_W2>, g, b, 0, s, a); break;
         case WSB_W4: hipLaunchKernelGGL(tri::triangle_kernel<WSB_W4>, g, b, 0, s, a); break;
         case WSB_W6: hipLaunchKernelGGL(tri::triangle_kernel<WSB_W6>, g, b, 0, s, a); break;
+        case WSB_CONE: hipLaunchKernelGGL(tri::triangle_kernel<WSB_CONE>, g, b, 0, s, a); break;
+        case WSB_LW2: hipLaunchKernelGGL(tri::triangle_kernel<WSB_LW2>, g, b, 0, s, a); break;
+        case WSB_LW4: hipLaunchKernelGGL(tri::triangle_kernel<WSB_LW4>, g, b, 0, s, a); break;
+        case WSB_W2P5: hipLaunchKernelGGL(tri::triangle_kernel<WSB_W2P5>, g, b, 0, s, a); break;
+        case WSB_W4P5: hipLaunchKernelGGL(tri::triangle_kernel<WSB_W4P5>, g, b, 0, s, a); break;
         default: return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
     }
     WSB_HIP(hipGetLastError());
@@ -350,6 +359,11 @@ extern "C" int wsb_triangle2d_host(const double* tri, int background, int kernel
         case WSB_W2: tri::host_rows<WSB_W2>(a); break;
         case WSB_W4: tri::host_rows<WSB_W4>(a); break;
         case WSB_W6: tri::host_rows<WSB_W6>(a); break;
+        case WSB_CONE: tri::host_rows<WSB_CONE>(a); break;
+        case WSB_LW2: tri::host_rows<WSB_LW2>(a); break;
+        case WSB_LW4: tri::host_rows<WSB_LW4>(a); break;
+        case WSB_W2P5: tri::host_rows<WSB_W2P5>(a); break;
+        case WSB_W4P5: tri::host_rows<WSB_W4P5>(a); break;
         default: return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
     }
     return WSB_OK;

@@ -23,6 +23,7 @@ import torch
 
 from . import _lib as L
 from . import precision, triangle
+from .kernels import P5_NAMES
 from .scene import FixedTopology, SurfaceRep, kernelName
 
 F64 = torch.float64
@@ -46,7 +47,10 @@ class FusedGroup:
 
     def resolved(self):
         ch = None if self.channels is None else tuple(sorted(set(int(c) for c in self.channels)))
-        return FusedGroup(kernelName(self.kernel), ch, precision.default_form(self.form))
+        name = kernelName(self.kernel)
+        if name in P5_NAMES and (ch is None or not set(ch) <= {3, 4}):
+            raise NotImplementedError(f"{name}: the W^5 kernels give the gradient channels (3, 4) only")
+        return FusedGroup(name, ch, precision.default_form(self.form))
 
 
 class WallAggregate:

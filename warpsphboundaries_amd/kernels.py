@@ -26,7 +26,7 @@ from typing import Dict, Tuple
 F = Fraction
 
 __all__ = ["Block", "Kernel", "KERNELS", "kernel", "disk_moment", "plane_mass", "plane_second_moment",
-           "plane_channels", "NAMES"]
+           "plane_channels", "NAMES", "AUX_NAMES", "P5_NAMES", "ALL_NAMES", "CLOSED_NAMES", "KERNEL_ID"]
 
 
 def _poly_mul(a, b):
@@ -123,7 +123,41 @@ KERNELS: Dict[str, Kernel] = {
     "w6": _build("w6", "wendland", F(78, 7), F(1365, 64), 8, ((0, 1), (1, 8), (2, 25), (3, 32))),
 }
 NAMES = ("cubic", "w2", "w4", "w6")
-KERNEL_ID = {n: i for i, n in enumerate(NAMES)}
+
+
+# ---- auxiliary kernels of the wall operations (reference edge/kernels.py:168-243, scene/viscosity.py:40-60,
+# scene/tensile.py:31-40): truncated powers of u = 1 - q with knot 1, normalised so that int_{R^2} W dA = 1
+def _from_u_terms(name, terms):
+    """terms [(coefficient, power)]: shape(q) = sum c (1 - q)^p on [0, 1]; C = 1 / (2 int_0^1 q shape dq)."""
+    shape = [F(0)]
+    for c, p in terms:
+        shape = _poly_add(shape, [F(c) * x for x in _poly_pow([F(1), F(-1)], p)])
+    qs = _poly_mul([F(0), F(1)], shape)
+    c2_pi = 1 / (2 * sum(x / (n + 1) for n, x in enumerate(qs)))
+    coeffs = tuple(_trim([c2_pi * x for x in shape]))
+    return Kernel(name, c2_pi, F(0), ((F(0), F(1), coeffs),), (Block(F(1), coeffs),))
+
+
+def _power_terms(k, base, p0):
+    """(shape)^k as u-power terms for a Wendland shape u^p0 * sum_m base[m] u^m (reference POWER_FAMILIES)."""
+    pol = [F(1)]
+    for _ in range(k):
+        pol = _poly_mul(pol, [F(b) for b in base])
+    return [(c, p0 * k + m) for m, c in enumerate(pol) if c != 0]
+
+
+_W2_U = ([F(5), F(-4)], 4)                              # (1-q)^4 (1 + 4q)        = u^4 (5 - 4u)
+_W4_U = ([F(56, 3), F(-88, 3), F(35, 3)], 6)           # (1-q)^6 (1+6q+35/3 q^2) = u^6 (56/3 - 88/3 u + 35/3 u^2)
+KERNELS["cone"] = _from_u_terms("cone", [(1, 1)])                       # the cover vector's kernel
+KERNELS["lw2"] = _from_u_terms("lw2", [(1, 3)])                         # L = W'/r of w2 (wall Laplacian)
+KERNELS["lw4"] = _from_u_terms("lw4", [(6, 5), (-5, 6)])               # L = W'/r of w4
+KERNELS["w2p5"] = _from_u_terms("w2p5", _power_terms(5, *_W2_U))       # W^5 of w2 (tensile), degree 25
+KERNELS["w4p5"] = _from_u_terms("w4p5", _power_terms(5, *_W4_U))       # W^5 of w4, degree 40
+AUX_NAMES = ("cone", "lw2", "lw4")          # evaluated by the closed form, like NAMES
+P5_NAMES = ("w2p5", "w4p5")                 # gradient channels only, by chord quadrature
+ALL_NAMES = NAMES + AUX_NAMES + P5_NAMES
+CLOSED_NAMES = NAMES + AUX_NAMES
+KERNEL_ID = {n: i for i, n in enumerate(ALL_NAMES)}
 
 
 def kernel(name: str) -> Kernel:

@@ -2,6 +2,8 @@
 
     aggregate(ps, support, ...)   the integrals of every body at the particle positions of ``ps``: a ``FusedWall``
                                   whose ``.out`` holds lam [B,N], G [B,N,2], Cov [B,N,2,2] (world frame, float64)
+                                  and, in the full layout, cover [B,N,2], lap [B,N], tens [B,N,2] (raw, see
+                                  ``wallops`` for the factors)
     signed_distance(x, ...)       (d, n, hit[, body]) of points to the nearest body: d > 0 in the fluid, n into the fluid
     bodies                        the rigid bodies of the scene
     supported(...)                whether the fused path serves the scene
@@ -52,22 +54,44 @@ class AnalyticBoundary:
         return {"kernel": kernelName(kernel), "operation": operation, "operationMode": "BoundaryToFluid"}
 
     def layout(self, kernel, laplacian, fixedAdjacency=True):
-        """(groups, roles) of the full layout: the scheme kernel plus the cover ('cone'), wall-Laplacian ('l<fam>') and
-        tensile ('<fam>p5') kernel groups of the reference; only the scheme kernel group is built here."""
-        self.family(kernel)
-        raise NotImplementedError("the cover / wall-Laplacian / tensile kernel groups (cone, l<fam>, <fam>p5) are not built: "
-                                  "use aggregate(..., lean=True)")
+        """(groups, roles): the fused kernel groups the scheme needs and the group index of each role (reference
+        ``scene/provider.py:73-89``): the scheme kernel 'w', the cover kernel 'cone' (gradient channels), the wall
+        Laplacian 'lap' (``l<fam>``) when ``laplacian``, and the tensile kernel 'tens' (``<fam>p5``, gradient
+        channels); built once per configuration."""
+        key = (kernelName(kernel), bool(laplacian), bool(fixedAdjacency))
+        if key not in self._layouts:
+            fam = self.family(kernel)
+            groups, idx = [FusedGroup(kernelName(kernel), None, self.form), FusedGroup("cone", (3, 4), self.form)], \
+                {"w": 0, "cone": 1}
+            if laplacian:
+                idx["lap"] = len(groups)
+                groups.append(FusedGroup("l" + fam, None, self.form))
+            idx["tens"] = len(groups)
+            groups.append(FusedGroup(fam + "p5", (3, 4), self.form))
+            self._layouts[key] = (tuple(groups), idx)
+        return self._layouts[key]
 
     def aggregate(self, ps, support, kernel="w2", laplacian=False, fixedAdjacency=True, lean=False):
-        """the fused wall evaluation at the positions of ``ps``; ``support``: host float >= every support of ``ps``."""
-        if not lean:
-            self.layout(kernel, laplacian, fixedAdjacency)
+        """the fused wall evaluation at the positions of ``ps``; ``support``: host float >= every support of ``ps``.
+        ``.out``: lam, G, Cov of the scheme kernel, and unless ``lean`` the raw cover (gradient of the cone kernel),
+        lap (2 lam - tr Cov of ``l<fam>``, with ``laplacian``) and tens (gradient of ``<fam>p5``): the scheme applies
+        the factors (``wallops.cover_factor``, ``lap_factor``, ``tensile_factor``)."""
         if not fixedAdjacency:
             raise NotImplementedError("only the fixed-capacity (graph-capturable) adjacency is built")
-        groups = (FusedGroup(kernelName(kernel), None, self.form),)
+        if lean:
+            groups, idx = (FusedGroup(kernelName(kernel), None, self.form),), {"w": 0}
+        else:
+            groups, idx = self.layout(kernel, laplacian, fixedAdjacency)
         adj = fixed_adjacency(self.scene, ps, None, support)
         fw = FusedWall(self.scene, adj, groups)
-        fw.out = fw.evaluate((WallOutput("lam", 0, "lam"), WallOutput("G", 0, "g0"), WallOutput("Cov", 0, "cov")))
+        outs = [WallOutput("lam", 0, "lam"), WallOutput("G", 0, "g0"), WallOutput("Cov", 0, "cov")]
+        if not lean:
+            outs.append(WallOutput("cover", idx["cone"], "g0"))
+            if "lap" in idx:
+                outs.append(WallOutput("lap", idx["lap"], "lap"))
+            outs.append(WallOutput("tens", idx["tens"], "g0"))
+        fw.out = fw.evaluate(outs)
+        fw.roles = idx
         return fw
 
     def signed_distance(self, x, body=None, supportMax=None, want_body=False):

@@ -23,7 +23,9 @@ from .kernels import KERNEL_ID, plane_channels
 F64 = torch.float64
 
 KERNEL_NAMES = {"CubicSpline": "cubic", "cubic": "cubic", "Wendland2": "w2", "Wendland4": "w4", "Wendland6": "w6",
-                "w2": "w2", "w4": "w4", "w6": "w6"}
+                "w2": "w2", "w4": "w4", "w6": "w6",
+                # the wall-operation kernels (cover, wall Laplacian, tensile) of the full aggregate layout
+                "cone": "cone", "lw2": "lw2", "lw4": "lw4", "w2p5": "w2p5", "w4p5": "w4p5"}
 
 
 def kernelName(kernel) -> str:
