@@ -120,6 +120,20 @@ int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form, const int
 int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max,
                           double* d, double* nrm, int32_t* edge, void* stream);
 
+/* ---- wedge areas and directional extremes of the solid within one support (reference scene/fused.py:157-309
+ * `_cone_area_kernel`, `_dir_extreme_kernel`, used by `FusedWall.cone_area` / `dir_extreme`, :741-835), FP64, over
+ * the slots of wsb_adjacency2d.  The body's pose enters as (cos, sin) of its rotation: world vectors are taken to
+ * the body frame by R^T.  wsb_cone_area2d adds area(solid cap disk(x, h) cap wedge(x, axis, half_angle)) to out[q]
+ * (half_angle >= pi: the full disk) and the full-disk area to out[n + q]; axes[n*2] world.
+ * wsb_dir_extreme2d max-accumulates into out[q] the largest |cos| between the world vector vecs[q] and a direction
+ * from the query to a solid point within h (1 where the vector's line meets the solid, 0 for no solid or w = 0). */
+int wsb_cone_area2d(const wsb_geom2d* g, const int32_t* slots, const int32_t* counts, int K, const double* ind,
+                    const double* lpos, const double* lsup, int n, const double* axes, double pose_cos, double pose_sin,
+                    double half_angle, double* out, void* stream);
+int wsb_dir_extreme2d(const wsb_geom2d* g, const int32_t* slots, const int32_t* counts, int K, const double* ind,
+                      const double* lpos, const double* lsup, int n, const double* vecs, double pose_cos, double pose_sin,
+                      double* out, void* stream);
+
 /* ---- conditioned small / sliver triangle (reference edge/warptriangle.py:136-181 `_triangle_kernel` and `channels`,
  * applied by scene/scene.py:407-408 and scene/fused.py:556-564).  For a closed 3-vertex loop, a row whose support h
  * makes the triangle small (longest edge <= 0.02 h) or a sliver (twice the area <= 0.01 longest edge squared) takes

@@ -38,6 +38,10 @@ SIGNATURES = {
                                     c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_signed_distance2d": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wsb_sphere_closed_form": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "wsb_cone_area2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double,
+                                c_double, c_double, c_void_p, c_void_p]),
+    "wsb_dir_extreme2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                  c_double, c_double, c_void_p, c_void_p]),
     "wsb_triangle2d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_triangle2d_host": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_double, c_int,

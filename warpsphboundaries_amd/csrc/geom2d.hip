@@ -508,6 +508,175 @@ extern "C" int wsb_adjacency2d(const wsb_geom2d* g, const double* lpos, const do
     return WSB_OK;
 }
 
+namespace {
+
+// ------------------------------------------------------------------------------------------------ cone area, extreme
+// One thread per row over its slots (the edges within one support), FP64 (angle differences near pi/2 amplify FP32
+// rounding).  The pose R = (c, -s, s, c) brings the world axis / vector into the body frame (R^T w).
+
+__device__ inline double wrap_pi(double a) {
+    const double two_pi = 6.283185307179586476925286766559;
+    double r = a + 3.1415926535897932384626433832795;
+    r = r - two_pi * floor(r / two_pi);
+    return r - 3.1415926535897932384626433832795;
+}
+
+// the part of the sector (1/2) H^2 (h - l) that the chord of an edge at distance z removes from [l, h]:
+// (1/2) H^2 (c_hi - c_lo) - (1/2) z^2 (tan(c_hi - pn) - tan(c_lo - pn)) over [c_lo, c_hi] = [l, h] cap the chord's
+// angular interval (pn - be, pn + be) (the 2 pi copy nearest the midpoint); 0 when empty
+// (reference scene/fused.py:165-176)
+__device__ inline double chord_deficit(double l, double h, double z, double H, double be, double pn) {
+    const double two_pi = 6.283185307179586476925286766559;
+    const double m = 0.5 * (l + h);
+    const double k = floor((m - pn) / two_pi + 0.5);
+    const double c_lo = fmax(l, pn + two_pi * k - be);
+    const double c_hi = fmin(h, pn + two_pi * k + be);
+    if (!(c_lo < c_hi)) return 0.0;
+    return 0.5 * H * H * (c_hi - c_lo) - 0.5 * z * z * (tan(c_hi - pn) - tan(c_lo - pn));
+}
+
+struct ConeArgs {
+    const int32_t* slots;
+    const int32_t* counts;
+    int K;
+    const double* ind;
+    const double* lpos;
+    const double* lsup;
+    int n;
+    const double* V;
+    const int32_t* E;
+    const double* vec;          // world axes / vectors [n*2]
+    double c, s;                // pose cos, sin
+    double half_angle;
+    double* out;                // cone area: out[q] (wedge), out[n + q] (full disk); extreme: out[q] (max)
+};
+
+// area(solid cap disk(p, H) cap wedge(p, axis, half_angle)) and the full-disk area: (1/2) H^2 W indicator minus the
+// chord deficits of the edges within H (reference scene/fused.py:179-240), added to out
+__global__ __launch_bounds__(256) void cone_area2d_kernel(ConeArgs a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.n) return;
+    const double pi = 3.1415926535897932384626433832795, two_pi = 6.283185307179586476925286766559;
+    const double H = a.lsup[q], px = a.lpos[2 * q], py = a.lpos[2 * q + 1];
+    const double wx = a.vec[2 * q], wy = a.vec[2 * q + 1];
+    const double th = atan2(-a.s * wx + a.c * wy, a.c * wx + a.s * wy);
+    const double al = a.half_angle;
+    const bool wedge_on = al < pi;
+    double a_wedge = 0.0, a_full = 0.0;
+    const int cnt = a.counts[q];
+    for (int i = 0; i < cnt; ++i) {
+        const int e = a.slots[(int64_t)q * a.K + i];
+        if (e < 0) continue;
+        const int v0 = a.E[2 * e], v1 = a.E[2 * e + 1];
+        const double Ax = a.V[2 * v0] - px, Ay = a.V[2 * v0 + 1] - py;
+        const double Bx = a.V[2 * v1] - px, By = a.V[2 * v1 + 1] - py;
+        const double cr = Ax * By - Ay * Bx;
+        const double L = sqrt((Bx - Ax) * (Bx - Ax) + (By - Ay) * (By - Ay));
+        if (!(L > 0.0 && fabs(cr) >= 1.0e-300)) continue;
+        const double sg = cr < 0.0 ? -1.0 : 1.0;
+        const double z = fabs(cr) / L;
+        if (!(z < H)) continue;
+        const double pa = atan2(Ay, Ax);
+        const double dphi = wrap_pi(atan2(By, Bx) - pa);
+        const double lo = dphi < 0.0 ? pa + dphi : pa, hi = dphi < 0.0 ? pa : pa + dphi;
+        const double dx = (Bx - Ax) / L, dy = (By - Ay) / L;
+        const double t = Ax * dx + Ay * dy;
+        const double pn = lo + wrap_pi(atan2(Ay - t * dy, Ax - t * dx) - lo);
+        const double be = acos(z / H);
+        a_full += sg * chord_deficit(lo, hi, z, H, be, pn);
+        if (wedge_on) {
+            const double cc = lo + wrap_pi(th - lo);
+            for (int kk = -1; kk <= 1; ++kk) {
+                const double l = fmax(lo, cc + two_pi * kk - al), h = fmin(hi, cc + two_pi * kk + al);
+                if (l < h) a_wedge += sg * chord_deficit(l, h, z, H, be, pn);
+            }
+        } else {
+            a_wedge += sg * chord_deficit(lo, hi, z, H, be, pn);
+        }
+    }
+    const double idv = a.ind[q];
+    const double w_wedge = wedge_on ? 2.0 * al : two_pi;
+    a.out[q] += 0.5 * H * H * w_wedge * idv - a_wedge;
+    a.out[a.n + q] += 0.5 * H * H * two_pi * idv - a_full;
+}
+
+// max over the solid points within H of |w_hat . x_hat|, x_hat the direction from the query (reference
+// scene/fused.py:243-309): 1 where the line of w meets an edge piece inside the disk (or the query is in the solid),
+// else the larger |cos| at the piece's ends; max-accumulated into out
+__global__ __launch_bounds__(256) void dir_extreme2d_kernel(ConeArgs a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.n) return;
+    const double H = a.lsup[q], px = a.lpos[2 * q], py = a.lpos[2 * q + 1];
+    const double wx = a.vec[2 * q], wy = a.vec[2 * q + 1];
+    double bx = a.c * wx + a.s * wy, by = -a.s * wx + a.c * wy;
+    const double nb = sqrt(bx * bx + by * by);
+    if (!(nb > 0.0)) return;
+    bx /= nb;
+    by /= nb;
+    double best = a.ind[q] > 0.5 ? 1.0 : 0.0;
+    const int cnt = a.counts[q];
+    for (int i = 0; i < cnt; ++i) {
+        const int e = a.slots[(int64_t)q * a.K + i];
+        if (e < 0) continue;
+        const int v0 = a.E[2 * e], v1 = a.E[2 * e + 1];
+        const double Ax = a.V[2 * v0] - px, Ay = a.V[2 * v0 + 1] - py;
+        const double Dx = a.V[2 * v1] - px - Ax, Dy = a.V[2 * v1 + 1] - py - Ay;
+        const double aa = Dx * Dx + Dy * Dy;
+        if (!(aa > 0.0)) continue;
+        const double bq = Ax * Dx + Ay * Dy;
+        const double disc = bq * bq - aa * (Ax * Ax + Ay * Ay - H * H);
+        if (!(disc > 0.0)) continue;
+        const double sq = sqrt(disc);
+        const double t0 = fmax(0.0, (-bq - sq) / aa), t1 = fmin(1.0, (-bq + sq) / aa);
+        if (!(t0 < t1)) continue;
+        double f = 0.0;
+        const double crd = Dx * by - Dy * bx;
+        if (fabs(crd) > 0.0) {
+            const double ts = -(Ax * by - Ay * bx) / crd;
+            if (ts >= t0 && ts <= t1) f = 1.0;
+        }
+        if (f < 1.0) {
+            for (int k = 0; k < 2; ++k) {
+                const double t = k ? t1 : t0;
+                const double rx = Ax + t * Dx, ry = Ay + t * Dy;
+                const double rr = sqrt(rx * rx + ry * ry);
+                if (rr > 0.0) f = fmax(f, fabs(rx * bx + ry * by) / rr);
+            }
+        }
+        best = fmax(best, f);
+    }
+    a.out[q] = fmax(a.out[q], best);
+}
+
+}  // namespace
+
+extern "C" int wsb_cone_area2d(const wsb_geom2d* g, const int32_t* slots, const int32_t* counts, int K, const double* ind,
+                               const double* lpos, const double* lsup, int n, const double* axes, double pose_cos,
+                               double pose_sin, double half_angle, double* out, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes n=%d K=%d", n, K);
+    WSB_REQUIRE(half_angle >= 0.0, WSB_EINVAL, "negative half angle");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(slots && counts && ind && lpos && lsup && axes && out, WSB_EINVAL, "null buffer");
+    ConeArgs a{slots, counts, K, ind, lpos, lsup, n, g->verts, g->edges, axes, pose_cos, pose_sin, half_angle, out};
+    hipLaunchKernelGGL(cone_area2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+extern "C" int wsb_dir_extreme2d(const wsb_geom2d* g, const int32_t* slots, const int32_t* counts, int K, const double* ind,
+                                 const double* lpos, const double* lsup, int n, const double* vecs, double pose_cos,
+                                 double pose_sin, double* out, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes n=%d K=%d", n, K);
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(slots && counts && ind && lpos && lsup && vecs && out, WSB_EINVAL, "null buffer");
+    ConeArgs a{slots, counts, K, ind, lpos, lsup, n, g->verts, g->edges, vecs, pose_cos, pose_sin, 0.0, out};
+    hipLaunchKernelGGL(dir_extreme2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
 extern "C" int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max, double* d,
                                      double* nrm, int32_t* edge, void* stream) {
     WSB_REQUIRE(g, WSB_EINVAL, "null geometry");

@@ -16,6 +16,7 @@ support.  Stage 2 only slices:
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 from typing import Optional
 
@@ -131,6 +132,52 @@ class FusedWall(WallAggregate):
     def reset_derived(self):
         """nothing is derived lazily: the channels are complete after stage 1."""
         return None
+
+    # ---- geometric queries of the solid within one support (reference scene/fused.py:741-835), FP64
+    def _reps(self):
+        for bi, ba in enumerate(self.adj.bodies):
+            c, s = ba.body.pose._cs()
+            reps = ba.repList if ba.repList is not None else ba.body.reps
+            for rep, topo in zip(reps, ba.reps):
+                yield bi, ba, rep, topo, c, s
+
+    def cone_area(self, axes, half_angle):
+        """[2, N] float64: area(solid cap disk(x, h) cap wedge(x, axis, half_angle)) (row 0, units length^2;
+        ``half_angle >= pi``: the full disk) and the full-disk area (row 1), for the world ``axes`` [N, 2] (only the
+        direction matters), summed over the bodies; h = the query's support."""
+        N, dev = self.N, self.dev
+        out = torch.zeros(2 * N, dtype=F64, device=dev)
+        ax = axes.to(dev, F64).contiguous()
+        lib = L.load()
+        for bi, ba, rep, topo, c, s in self._reps():
+            L.check(lib.wsb_cone_area2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K, L.ptr(topo.ind),
+                                        L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(ax), c, s, float(half_angle),
+                                        L.ptr(out), L.stream_ptr(dev)))
+            if rep.background and ba.valid is not None:
+                # a query the adjacency left out of a background representation is deep in its solid: the full sector
+                out_of = (~ba.valid).to(F64)
+                wedge = 2.0 * float(half_angle) if float(half_angle) < math.pi else 2.0 * math.pi
+                out[:N] += out_of * 0.5 * ba.lsup * ba.lsup * wedge
+                out[N:] += out_of * math.pi * ba.lsup * ba.lsup
+        return out.reshape(2, N)
+
+    def dir_extreme(self, vecs):
+        """[B, N] float64: for the world vectors ``vecs`` [B, N, 2] the largest |cos| between the vector and a
+        direction from the query to a solid point of that body within one support (1 where the vector's line meets
+        the solid, 0 where there is no solid in support or the vector vanishes)."""
+        N, dev = self.N, self.dev
+        B = len(self.adj.bodies)
+        out = torch.zeros(B * N, dtype=F64, device=dev)
+        v = vecs.to(dev, F64).contiguous()
+        lib = L.load()
+        for bi, ba, rep, topo, c, s in self._reps():
+            seg = out[bi * N:(bi + 1) * N]
+            L.check(lib.wsb_dir_extreme2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K, L.ptr(topo.ind),
+                                          L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(v[bi]), c, s, L.ptr(seg),
+                                          L.stream_ptr(dev)))
+            if rep.background and ba.valid is not None:
+                seg[(~ba.valid) & (v[bi].norm(dim=1) > 0)] = 1.0
+        return out.reshape(B, N)
 
     @staticmethod
     def supported(scene, lowering=True):
