@@ -55,6 +55,20 @@ def active_block_counts(torch, verts, edges, slots, counts, lpos, lsup, K, kname
     return len(rows), out
 
 
+def active_radius_counts(torch, verts, edges, slots, counts, lpos, lsup, K, knames):
+    """{block radius: active (pair, radius) chords} over the union of the kernels' block radii (the fused pass
+    computes each radius's chord primitives once for every kernel that has a block there)."""
+    from warpsphboundaries_amd.plan import block_profiles
+    radii = sorted({float(b.R) for k in knames for b in block_profiles(k)})
+    res = {}
+    for k in knames:
+        npairs, act = active_block_counts(torch, verts, edges, slots, counts, lpos, lsup, K, k)
+        for b, a in zip(block_profiles(k), act):
+            res[float(b.R)] = a
+    assert set(res) == set(radii)
+    return npairs, res
+
+
 def cpu_baseline(work, n_sample=131072, seed=7):
     """the CPU oracle (numpy FP64 restatement of the reference's edge reductions) on a stratified subsample:
     pair list from the oracle's cell-list slot fill, twelve channels per pair, row sums; single thread."""
@@ -263,12 +277,12 @@ def mesh3d_measure(torch, dev, steps, warmup):
 
 
 def _traffic(kernel):
-    """HBM bytes per launch of the mixed twelve-moment tile kernel on the star workload, from the committed
+    """HBM bytes per launch of the mixed-form fused twelve-moment pass on the star workload, from the committed
     FETCH_SIZE / WRITE_SIZE counter runs scaled on a copy kernel of known size (None when the summary is absent)."""
-    fn = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_tile12_star4096_1M.json")
+    fn = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_tile12_multi_star4096_1M.json")
     try:
         with open(fn) as f:
-            return json.load(f)["tile12_mixed"][kernel]["traffic_bytes"]
+            return json.load(f)["kernels"][kernel]["traffic_bytes"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -319,11 +333,17 @@ def main():
     sort_scratch = torch.empty(int(L.load().wsb_sort2d_scratch_bytes(geo.handle, N)), dtype=torch.uint8, device=dev)
     outs = {k: torch.empty((N, 12), dtype=torch.float64, device=dev) for k in work.kernels}
 
+    def evaluate(form, scratch=None):
+        if form == "plain":                 # the plan interpreter: one launch per kernel
+            for k in work.kernels:
+                geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k], scratch=scratch)
+        else:                               # the closed form: the four kernels in one pass sharing the chords
+            geo.moments_multi(work.kernels, slots, counts, ind, lpos, lsup, form=form, outs=outs)
+
     def step(form, scratch=None):
         geo.sort(lpos, order=order, scratch=sort_scratch)
         geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind, order=order)
-        for k in work.kernels:
-            geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k], scratch=scratch)
+        evaluate(form, scratch)
 
     def timed(form, steps, warmup, scratch=None):
         for _ in range(warmup):
@@ -343,7 +363,8 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)                      # a plain copy of known size: 512 MiB read, 512 MiB written
         torch.cuda.synchronize()
-        print(json.dumps({"probe": "star", "N": N, "tile_algorithmic_bytes": N * opmodel.BYTES_PER_PARTICLE_2D,
+        print(json.dumps({"probe": "star", "N": N,
+                          "tile_algorithmic_bytes": N * opmodel.bytes_per_particle_2d(len(work.kernels)),
                           "copy_bytes": src.numel() * 8}), flush=True)
         return
 
@@ -370,18 +391,29 @@ def main():
                   "plain": float(((res_plain[k] - ref).abs().amax(0) / scale).max())}
     log("signal-relative error vs fp64:", err)
 
-    # ---- evaluate-only timing of each kernel's launch (events on the launch stream), the dominant kernel
+    # ---- evaluate-only timing (events on the launch stream): the fused pass, and each kernel's own launch beside it
     geo.sort(lpos, order=order, scratch=sort_scratch)
     geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind, order=order)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
+    reps = max(5, args.steps)
+    t_multi = {}
+    for form in ("mixed", "fp64"):
+        for _ in range(2):
+            evaluate(form)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            evaluate(form)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t_multi[form] = e0.elapsed_time(e1) / 1e3 / reps
     per_kernel = {}
     for form in ("mixed", "fp64"):
         for k in work.kernels:
             for _ in range(2):
                 geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k])
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            reps = max(5, args.steps)
             e0.record(stream)
             for _ in range(reps):
                 geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k])
@@ -407,27 +439,28 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     t_adj_unsorted = e0.elapsed_time(e1) / 1e3 / args.steps
-    log("per-launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()}, "adjacency", t_adj, "sort", t_sort,
+    log("fused-pass seconds:", t_multi, "single-kernel launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()},
+        "adjacency", t_adj, "sort", t_sort,
         "adjacency in input order", t_adj_unsorted)
 
-    # ---- roofline of the dominant kernel (the mixed-form tile kernel, w6 is the heaviest profile)
-    kdom = max(work.kernels, key=lambda k: per_kernel[("mixed", k)])
-    npairs, active = active_block_counts(torch, rep.vertices.to(dev), rep.edges.to(dev).long(), slots, counts, lpos, lsup, K, kdom)
-    from warpsphboundaries_amd.plan import block_profiles
-    flops = npairs * (opmodel.pair_geometry_flops() + 12) + sum(a * opmodel.block_flops(b) for a, b in
-                                                                  zip(active, block_profiles(kdom))) + N * opmodel.row_flops()
-    bytes_alg = N * opmodel.BYTES_PER_PARTICLE_2D
-    t_dom = per_kernel[("mixed", kdom)]
+    # ---- roofline of the dominant kernel: the mixed-form fused pass over the four kernels
+    npairs, active = active_radius_counts(torch, rep.vertices.to(dev), rep.edges.to(dev).long(), slots, counts, lpos,
+                                          lsup, K, work.kernels)
+    flops = opmodel.multi_flops(work.kernels, npairs, active, N)
+    bytes_alg = N * opmodel.bytes_per_particle_2d(len(work.kernels))
+    t_dom = t_multi["mixed"]
     t_roof = max(bytes_alg / PEAK_HBM, flops / PEAK_FP64)
     compute_bound = flops / PEAK_FP64 >= bytes_alg / PEAK_HBM
+    kname = "tile12_multi_kernel<15, 2>"
     roof = {"bound": "mfma" if compute_bound else "hbm",
             "pipe": "FP64 VALU (no MFMA: transcendental gather-reduce; the MI355X FP64 vector peak equals its FP64 matrix peak)",
-            "kernel": f"tile12_kernel<{kdom}, mixed>",
+            "kernel": kname + " (cubic, w2, w4, w6 in one pass, mixed form)",
             "achieved": (flops / t_dom / 1e12) if compute_bound else (bytes_alg / t_dom / 1e9),
             "peak": PEAK_FP64 / 1e12 if compute_bound else PEAK_HBM / 1e9,
             "unit": "TFLOP/s" if compute_bound else "GB/s",
-            "frac": t_roof / t_dom, "traffic": _traffic(kdom),
-            "algorithmic_flops": flops, "algorithmic_bytes": bytes_alg, "pairs": npairs, "active_chords": active,
+            "frac": t_roof / t_dom, "traffic": _traffic(kname),
+            "algorithmic_flops": flops, "algorithmic_bytes": bytes_alg, "pairs": npairs,
+            "active_chords_by_radius": {str(r): a for r, a in sorted(active.items())},
             "launch_seconds": t_dom, "hbm_GBps_algorithmic": bytes_alg / t_dom / 1e9}
 
     evals = 12 * len(work.kernels) * N
@@ -443,8 +476,10 @@ def main():
                   "fp64": {"evals_per_s": evals / t_fp64, "ms_per_step": t_fp64 * 1e3},
                   "plain_fp64": {"evals_per_s": evals / t_plain, "ms_per_step": t_plain * 1e3}},
         "speedup_mixed_vs_plain": t_plain / t_mixed,
-        "evaluate_only_evals_per_s": {"mixed": evals / sum(per_kernel[("mixed", k)] for k in work.kernels),
-                                      "fp64": evals / sum(per_kernel[("fp64", k)] for k in work.kernels)},
+        "evaluate_only_evals_per_s": {"mixed": evals / t_multi["mixed"], "fp64": evals / t_multi["fp64"]},
+        "evaluate_only_single_kernel_launches_evals_per_s": {
+            "mixed": evals / sum(per_kernel[("mixed", k)] for k in work.kernels),
+            "fp64": evals / sum(per_kernel[("fp64", k)] for k in work.kernels)},
         "adjacency_ms": t_adj * 1e3, "cell_sort_ms": t_sort * 1e3, "adjacency_input_order_ms": t_adj_unsorted * 1e3,
         "max_signal_relative_error_vs_fp64": err,
         "served_latency": None,

@@ -120,6 +120,15 @@ int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form, const int
 int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max,
                           double* d, double* nrm, int32_t* edge, void* stream);
 
+/* ---- several scheme kernels in one pass (the reference's fused stage 1 evaluates the kernel groups of a layout in one
+ * launch family, scene/fused.py:521-564): outs[4] (a host array, indexed by kernel id WSB_CUBIC..WSB_W6; nullptr =
+ * not requested) receive what wsb_moments2d gives for each kernel, with the chord primitives of every pair (endpoint
+ * radii, logarithm, angle, I_m recurrence) computed once per support radius for all of them.  Forms WSB_FORM_FP64
+ * and WSB_FORM_MIXED; the other arguments as wsb_moments2d. */
+int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t* order, const int32_t* slots, const int32_t* counts,
+                        int K, const double* ind, const double* lpos, const double* lsup, int n, const double* rot4,
+                        int flags, double* const* outs, void* stream);
+
 /* ---- wedge areas and directional extremes of the solid within one support (reference scene/fused.py:157-309
  * `_cone_area_kernel`, `_dir_extreme_kernel`, used by `FusedWall.cone_area` / `dir_extreme`, :741-835), FP64, over
  * the slots of wsb_adjacency2d.  The body's pose enters as (cos, sin) of its rotation: world vectors are taken to

@@ -183,3 +183,67 @@ def test_golden_fixture_elements_on_device(form, kname):
     else:
         tol = np.array([1e-12] * 3 + [1e-11] * 6 + [1e-12] * 3) * (5 if kname == "w6" else 1)
         assert (err <= tol).all(), err.max(0)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+@pytest.mark.parametrize("kset", [("cubic", "w2", "w4", "w6"), ("w2",), ("cubic", "w6"), ("w4", "cubic", "w2")])
+def test_moments_multi_matches_single_launches_and_oracle(form, kset):
+    """wsb_moments2d_multi (several kernels in one pass sharing each pair's chord primitives) against the oracle
+    (the single-launch gates) and against one launch per kernel (the same closed form, summed in the same slot
+    order; the shared primitives round as the single kernels' own within 1e-13 of the signal); rows in cell order
+    and in input order bitwise equal; WSB_OUT_ACCUMULATE doubles; physical units and a world rotation as the single
+    launch."""
+    from warpsphboundaries_amd import _lib as L
+    loop, solid = _geometries()["star256"]
+    rep = _rep(loop, solid)
+    h = 0.01
+    pts = _particles(rep.vertices.numpy(), h, 2000, 13)
+    sup = np.where(WL.rng(14).uniform(0, 1, len(pts)) < 0.3, 0.7 * h, h)
+    st = oadj.Static(rep.vertices.numpy(), rep.edges.numpy(), rep.background, h)
+    e_ref, c_ref, i_ref = oadj.fixed_topology(st, pts, sup)
+    geo = rep.geom(h, DEV)
+    lp = torch.tensor(pts, dtype=torch.float64, device=DEV)
+    ls = torch.tensor(sup, dtype=torch.float64, device=DEV)
+    slots, counts, ind = geo.adjacency(lp, ls)
+    multi = geo.moments_multi(kset, slots, counts, ind, lp, ls, form=form)
+    order = geo.sort(lp)
+    grouped = geo.moments_multi(kset, slots, counts, ind, lp, ls, form=form, order=order)
+    acc = geo.moments_multi(kset, slots, counts, ind, lp, ls, form=form, flags=L.OUT_ACCUMULATE)
+    acc = geo.moments_multi(kset, slots, counts, ind, lp, ls, form=form, flags=L.OUT_ACCUMULATE, outs=acc)
+    a = 0.83
+    rot4 = torch.tensor([np.cos(a), -np.sin(a), np.sin(a), np.cos(a)], dtype=torch.float64, device=DEV)
+    fl = L.OUT_PHYSICAL | L.OUT_WORLD
+    world = geo.moments_multi(kset, slots, counts, ind, lp, ls, form=form, rot4=rot4, flags=fl)
+    assert set(multi) == set(kset)
+    for k in kset:
+        got = multi[k].cpu().numpy()
+        ref = _oracle_rows(st, k, pts, sup, e_ref, c_ref, i_ref, edge2d.plane_vector(k))
+        err = np.abs(got - ref)
+        sig = err.max(0) / np.maximum(np.abs(ref).max(0), 1e-300)
+        if form == "mixed":
+            assert sig.max() <= 1e-6, (k, sig.max())
+        else:
+            assert err.max() <= (5e-12 if k == "w6" else 1e-12), (k, err.max())
+        single = geo.moments(k, slots, counts, ind, lp, ls, form=form).cpu().numpy()
+        scale = np.maximum(np.abs(single).max(0), 1.0)
+        d = (np.abs(got - single) / scale).max()
+        print(form, kset, k, "oracle", err.max(), "multi vs single", d)
+        assert d <= 1e-13, (k, d)
+        assert torch.equal(grouped[k], multi[k])
+        assert torch.equal(acc[k], 2.0 * multi[k])
+        sw = geo.moments(k, slots, counts, ind, lp, ls, form=form, rot4=rot4, flags=fl)
+        dw = ((world[k] - sw).abs().amax(0) / sw.abs().amax(0).clamp(min=1.0)).max().item()
+        assert dw <= 1e-13, (k, dw)
+
+
+def test_moments_multi_rejects_what_it_does_not_fuse():
+    loop, solid = _geometries()["64gon"]
+    rep = _rep(loop, solid)
+    geo = rep.geom(0.02, DEV)
+    lp = torch.tensor([[0.5, 0.36]], dtype=torch.float64, device=DEV)
+    ls = torch.full((1,), 0.02, dtype=torch.float64, device=DEV)
+    slots, counts, ind = geo.adjacency(lp, ls)
+    with pytest.raises(NotImplementedError):
+        geo.moments_multi(("w2", "lw2"), slots, counts, ind, lp, ls)
+    with pytest.raises(NotImplementedError):
+        geo.moments_multi(("w2",), slots, counts, ind, lp, ls, form="plain")

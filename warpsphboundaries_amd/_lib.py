@@ -38,6 +38,8 @@ SIGNATURES = {
                                     c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_signed_distance2d": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wsb_sphere_closed_form": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "wsb_moments2d_multi": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                                    c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_cone_area2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double,
                                 c_double, c_double, c_void_p, c_void_p]),
     "wsb_dir_extreme2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p,

@@ -218,6 +218,266 @@ int dispatch_tile(int kid, const TileArgs& a, hipStream_t s) {
     return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kid);
 }
 
+// ------------------------------------------------------------------------------------------------ several kernels
+// The scheme kernels cubic, w2, w4, w6 (any subset MASK, bit k = kernel id k) in one pass.  Per pair the chord
+// primitives of each support radius -- the clipped chord, the endpoint radii, the logarithm, the angle and the
+// differences dI_m -- are computed once (radius 1 for every kernel, radius 1/2 for the cubic's inner block) and
+// each kernel's six block sums are accumulated in the same loop over m; the channels of one kernel at a time then
+// go through LDS to the row sums (slot order, as tile12_kernel).  A workgroup of MTILE_T threads owns MTILE_R
+// rows; the row sums of every kernel stay in LDS.
+constexpr int MTILE_T = 256;
+constexpr int MTILE_R = 64;
+
+struct BlockSums {
+    double AP, VP, AF, VF, BP, BF;
+};
+
+template <int MASK>
+constexpr int mask_nm() {
+    int nm = 0;
+    if (MASK & 1) nm = KTab<WSB_CUBIC>::NMAXP > nm ? KTab<WSB_CUBIC>::NMAXP : nm;
+    if (MASK & 2) nm = KTab<WSB_W2>::NMAXP > nm ? KTab<WSB_W2>::NMAXP : nm;
+    if (MASK & 4) nm = KTab<WSB_W4>::NMAXP > nm ? KTab<WSB_W4>::NMAXP : nm;
+    if (MASK & 8) nm = KTab<WSB_W6>::NMAXP > nm ? KTab<WSB_W6>::NMAXP : nm;
+    return nm;
+}
+
+// one term m of block b of kernel KID: dIm = dI_m (m <= NMAXP of the pass), dR = rh^m - rl^m (m >= 2)
+template <int KID, int B>
+__device__ __forceinline__ void block_term(int m, double dIm, double dR, bool have_dI, BlockSums& s) {
+    using T = KTab<KID>;
+    if (have_dI && m < 16) {
+        if (T::P[B][m] != 0.0) s.AP += T::P[B][m] * dIm;
+        if (T::QP[B][m] != 0.0) s.VP += T::QP[B][m] * dIm;
+        if (T::F[B][m] != 0.0) s.AF += T::F[B][m] * dIm;
+        if (T::QF[B][m] != 0.0) s.VF += T::QF[B][m] * dIm;
+    }
+    if (m >= 2 && m - 2 < 16) {
+        if (T::QP[B][m - 2] != 0.0) s.BP += T::QP[B][m - 2] * dR;
+        if (T::QF[B][m - 2] != 0.0) s.BF += T::QF[B][m - 2] * dR;
+    }
+}
+
+// the channels of one block from its sums (as closed::pair_channels), added to c
+template <int KID, int B>
+__device__ __forceinline__ void block_channels(const BlockSums& s, double z, double th, double n0, double n1, double tx,
+                                               double ty, double c[wsb::NCH]) {
+    using T = KTab<KID>;
+    const double lam = z * s.VP - T::MRP[B] * th;
+    const double valF = z * s.VF - T::MRF[B] * th;
+    const double zn0 = z * n0, zn1 = z * n1;
+    const double YxP = zn0 * s.AP + tx * s.BP, YyP = zn1 * s.AP + ty * s.BP;
+    const double YxF = zn0 * s.AF + tx * s.BF, YyF = zn1 * s.AF + ty * s.BF;
+    c[0] += lam;
+    c[1] += n0 * s.AF;
+    c[2] += n1 * s.AF;
+    c[3] -= n0 * s.AP;
+    c[4] -= n1 * s.AP;
+    c[5] += lam - n0 * YxP;
+    c[6] -= n1 * YxP;
+    c[7] -= n0 * YyP;
+    c[8] += lam - n1 * YyP;
+    c[9] += n0 * YxF - valF;
+    c[10] += n0 * YyF;
+    c[11] += n1 * YyF - valF;
+}
+
+// the primitives of one support radius R for one pair; accumulates the sums of the kernels of MASK (SMALL: the
+// cubic's inner block only) and returns whether the chord is non-empty, with its angle in th
+template <int MASK, int FORM, bool SMALL>
+__device__ __forceinline__ bool radius_sums(double R, double z, double az, double zz, double s0, double s1, double r0sq,
+                                            double r1sq, double& r0, double& r1, double& th, BlockSums* S) {
+    if (!(az < R)) return false;
+    const bool clo = s0 < 0.0 && r0sq > R * R;
+    const bool chi = s1 > 0.0 && r1sq > R * R;
+    const double L = (clo || chi) ? closed::xsqrt<FORM>((R - az) * (R + az)) : 0.0;
+    const double lo = clo ? -L : s0, hi = chi ? L : s1;
+    if (!(lo < hi)) return false;
+    if (!clo && r0 < 0.0) r0 = closed::xsqrt<FORM>(r0sq);
+    if (!chi && r1 < 0.0) r1 = closed::xsqrt<FORM>(r1sq);
+    const double rl = clo ? R : r0, rh = chi ? R : r1;
+    double dA = 0.0;
+    if (z != 0.0) {
+        const double Nh = hi >= 0.0 ? hi + rh : zz, Dh = hi >= 0.0 ? 1.0 : rh - hi;
+        const double Nl = lo >= 0.0 ? lo + rl : zz, Dl = lo >= 0.0 ? 1.0 : rl - lo;
+        dA = log((Nh * Dl) / (Dh * Nl));
+    }
+    constexpr int NM = SMALL ? KTab<WSB_CUBIC>::NMAXP : mask_nm<MASK>();
+    double dIm2 = hi - lo, dIm1 = 0.5 * (hi * rh - lo * rl + zz * dA);   // dI_0, dI_1
+    double ph = rh, pl = rl;
+#pragma unroll
+    for (int m = 0; m <= NM + 2; ++m) {
+        if (m >= 2) {
+            ph *= rh;
+            pl *= rl;
+        }
+        double dIm = 0.0;
+        if (m == 0) dIm = dIm2;
+        else if (m == 1) dIm = dIm1;
+        else if (m <= NM) {
+            dIm = ((hi * ph - lo * pl) + (double)m * zz * dIm2) * (1.0 / (double)(m + 1));
+            dIm2 = dIm1;
+            dIm1 = dIm;
+        }
+        const double dR = ph - pl;
+        const bool have = m <= NM;
+        if (SMALL) {
+            block_term<WSB_CUBIC, 1>(m, dIm, dR, have, S[0]);
+        } else {
+            if (MASK & 1) block_term<WSB_CUBIC, 0>(m, dIm, dR, have && m <= KTab<WSB_CUBIC>::NMAXP, S[0]);
+            if (MASK & 2) block_term<WSB_W2, 0>(m, dIm, dR, have && m <= KTab<WSB_W2>::NMAXP, S[1]);
+            if (MASK & 4) block_term<WSB_W4, 0>(m, dIm, dR, have && m <= KTab<WSB_W4>::NMAXP, S[2]);
+            if (MASK & 8) block_term<WSB_W6, 0>(m, dIm, dR, have && m <= KTab<WSB_W6>::NMAXP, S[3]);
+        }
+    }
+    th = FORM == closed::MIXED ? closed::dangle_mixed(z, lo, hi) : closed::dangle_fp64(z, lo, hi);
+    return true;
+}
+
+struct MultiArgs {
+    TileArgs t;                 // t.out unused
+    double* out[4];             // per kernel id (nullptr: not requested)
+};
+
+template <int MASK, int FORM>
+__global__ __launch_bounds__(MTILE_T) void tile12_multi_kernel(MultiArgs ma) {
+    const TileArgs& a = ma.t;
+    using BlockScan = hipcub::BlockScan<int, MTILE_T>;
+    __shared__ typename BlockScan::TempStorage scan_tmp;
+    __shared__ int s_off[MTILE_T + 1];
+    __shared__ int s_row[MTILE_R];
+    __shared__ double s_px[MTILE_R], s_py[MTILE_R], s_h[MTILE_R], s_ih[MTILE_R];
+    __shared__ double s_res[wsb::NCH][MTILE_T];
+    __shared__ double s_acc[4][wsb::NCH][MTILE_R];
+    const int tid = threadIdx.x;
+    const int pos = blockIdx.x * MTILE_R + tid;
+    const bool live = tid < MTILE_R && pos < a.n;
+    const int row = live ? (a.order ? a.order[pos] : pos) : 0;
+    const int cnt = live ? min(max(a.counts[row], 0), a.K) : 0;
+    if (live) {
+        s_row[tid] = row;
+        s_px[tid] = a.lpos[2 * row];
+        s_py[tid] = a.lpos[2 * row + 1];
+        const double h = a.lsup[row];
+        s_h[tid] = h;
+        s_ih[tid] = 1.0 / h;
+    }
+    if (tid < MTILE_R) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < wsb::NCH; ++c) s_acc[k][c][tid] = 0.0;
+    }
+    int off, total;
+    BlockScan(scan_tmp).ExclusiveSum(cnt, off, total);
+    s_off[tid] = off;
+    if (tid == 0) s_off[MTILE_T] = total;
+    __syncthreads();
+    for (int base = 0; base < total; base += MTILE_T) {
+        const int j = base + tid;
+        BlockSums S[4] = {}, Sc = {};
+        bool on1 = false, on05 = false;
+        double th1 = 0.0, th05 = 0.0, z = 0.0, n0 = 0.0, n1 = 0.0, tx = 0.0, ty = 0.0;
+        if (j < total) {
+            int lo = 0, hi = MTILE_R;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            const int r = lo;
+            const int e = a.slots[(int64_t)s_row[r] * a.K + (j - s_off[r])];
+            if (e >= 0 && e < a.ne) {
+                const double x = s_px[r], y = s_py[r], h = s_h[r], ih = s_ih[r];
+                const double4 ab = a.erec[e];
+                const double4 tn = a.etan[e];
+                const double ux = ab.x - x, uy = ab.y - y, vx = ab.z - x, vy = ab.w - y;
+                tx = tn.x;
+                ty = tn.y;
+                const double iell = tn.z;
+                if (iell != 0.0) {
+                    double cr = wsb_cross2(ux, uy, vx, vy);
+                    const double mag = fabs(ux * vy) + fabs(uy * vx);
+                    if (!(mag > 1e-280 && fabs(cr) > 1.7763568394002505e-15 * mag))
+                        cr = wsb_cross2(ux / h, uy / h, vx / h, vy / h) * h * h;
+                    z = cr * ih * iell;
+                    const double px = ux * ih, py = uy * ih, qx = vx * ih, qy = vy * ih;
+                    const double s0 = tx * px + ty * py, s1 = tx * qx + ty * qy;
+                    n0 = ty;
+                    n1 = -tx;
+                    const double az = fabs(z), zz = z * z;
+                    const double r0sq = px * px + py * py, r1sq = qx * qx + qy * qy;
+                    double r0 = -1.0, r1 = -1.0;
+                    on1 = radius_sums<MASK, FORM, false>(1.0, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th1, S);
+                    if (MASK & 1) on05 = radius_sums<MASK, FORM, true>(0.5, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th05, &Sc);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!(MASK & (1 << k))) continue;
+            double c[wsb::NCH];
+#pragma unroll
+            for (int q = 0; q < wsb::NCH; ++q) c[q] = 0.0;
+            if (on1) {
+                if (k == 0) block_channels<WSB_CUBIC, 0>(S[0], z, th1, n0, n1, tx, ty, c);
+                if (k == 1) block_channels<WSB_W2, 0>(S[1], z, th1, n0, n1, tx, ty, c);
+                if (k == 2) block_channels<WSB_W4, 0>(S[2], z, th1, n0, n1, tx, ty, c);
+                if (k == 3) block_channels<WSB_W6, 0>(S[3], z, th1, n0, n1, tx, ty, c);
+            }
+            if (k == 0 && on05) block_channels<WSB_CUBIC, 1>(Sc, z, th05, n0, n1, tx, ty, c);
+#pragma unroll
+            for (int q = 0; q < wsb::NCH; ++q) s_res[q][tid] = c[q] * WSB_INV_PI;
+            __syncthreads();
+            if (tid < MTILE_R) {
+                const int b0 = max(off, base), b1 = min(off + cnt, base + MTILE_T);
+                double acc[wsb::NCH];
+#pragma unroll
+                for (int q = 0; q < wsb::NCH; ++q) acc[q] = s_acc[k][q][tid];
+                for (int jj = b0; jj < b1; ++jj) {
+#pragma unroll
+                    for (int q = 0; q < wsb::NCH; ++q) acc[q] += s_res[q][jj - base];
+                }
+#pragma unroll
+                for (int q = 0; q < wsb::NCH; ++q) s_acc[k][q][tid] = acc[q];
+            }
+            __syncthreads();
+        }
+    }
+    // finalise: thread (kernel k, row r) for the MTILE_R rows x 4 kernels
+    const int r = tid & (MTILE_R - 1), k = tid / MTILE_R;
+    const int p2 = blockIdx.x * MTILE_R + r;
+    if (p2 < a.n && (MASK & (1 << k))) {
+        const int rr = s_row[r];
+        double acc[wsb::NCH];
+#pragma unroll
+        for (int q = 0; q < wsb::NCH; ++q) acc[q] = s_acc[k][q][r];
+        const double* plane = k == 0 ? KTab<WSB_CUBIC>::PLANE : k == 1 ? KTab<WSB_W2>::PLANE : k == 2 ? KTab<WSB_W4>::PLANE
+                                                                                                      : KTab<WSB_W6>::PLANE;
+        finalize_row(acc, a.ind[rr], s_h[r], plane, a.rot4, a.flags, ma.out[k] + (int64_t)rr * wsb::NCH);
+    }
+}
+
+template <int MASK, int FORM>
+int launch_multi(const MultiArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((tile12_multi_kernel<MASK, FORM>), dim3((a.t.n + MTILE_R - 1) / MTILE_R), dim3(MTILE_T), 0, s, a);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+template <int FORM>
+int dispatch_multi(int mask, const MultiArgs& a, hipStream_t s) {
+    switch (mask) {
+#define WSB_MULTI_CASE(M) \
+    case M: return launch_multi<M, FORM>(a, s);
+        WSB_MULTI_CASE(1) WSB_MULTI_CASE(2) WSB_MULTI_CASE(3) WSB_MULTI_CASE(4) WSB_MULTI_CASE(5) WSB_MULTI_CASE(6)
+        WSB_MULTI_CASE(7) WSB_MULTI_CASE(8) WSB_MULTI_CASE(9) WSB_MULTI_CASE(10) WSB_MULTI_CASE(11) WSB_MULTI_CASE(12)
+        WSB_MULTI_CASE(13) WSB_MULTI_CASE(14) WSB_MULTI_CASE(15)
+#undef WSB_MULTI_CASE
+    }
+    return wsb::set_error(WSB_EINVAL, "no kernel requested");
+}
+
 // ------------------------------------------------------------------------------------------------ pair list path
 struct PairArgs {
     const int32_t* pq;
@@ -356,4 +616,25 @@ extern "C" int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form
     if (form == WSB_FORM_FP64) return dispatch_pairs<closed::FP64>(kernel_id, a, s);
     if (form == WSB_FORM_MIXED) return dispatch_pairs<closed::MIXED>(kernel_id, a, s);
     return wsb::set_error(WSB_ENOTSUP, "unknown form %d", form);
+}
+
+extern "C" int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t* order, const int32_t* slots, const int32_t* counts,
+                                   int K, const double* ind, const double* lpos, const double* lsup, int n, const double* rot4,
+                                   int flags, double* const* outs, void* stream) {
+    WSB_REQUIRE(g && outs, WSB_EINVAL, "null argument");
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes n=%d K=%d", n, K);
+    WSB_REQUIRE(!(flags & WSB_OUT_WORLD) || rot4, WSB_EINVAL, "WSB_OUT_WORLD needs rot4");
+    WSB_REQUIRE(form == WSB_FORM_FP64 || form == WSB_FORM_MIXED, WSB_ENOTSUP, "several kernels: the fp64 and mixed forms");
+    int mask = 0;
+    MultiArgs a{};
+    for (int k = 0; k < 4; ++k) {
+        a.out[k] = outs[k];
+        if (outs[k]) mask |= 1 << k;
+    }
+    WSB_REQUIRE(mask != 0, WSB_EINVAL, "no kernel requested");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(slots && counts && ind && lpos && lsup, WSB_EINVAL, "null buffer");
+    a.t = TileArgs{order, slots, counts, K, ind, lpos, lsup, n, g->erec, g->etan, g->ne, rot4, flags, nullptr};
+    hipStream_t s = wsb::as_stream(stream);
+    return form == WSB_FORM_FP64 ? dispatch_multi<closed::FP64>(mask, a, s) : dispatch_multi<closed::MIXED>(mask, a, s);
 }

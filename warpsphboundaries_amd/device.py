@@ -134,6 +134,30 @@ class Geom2D:
                                        L.stream_ptr(dev)))
         return out
 
+    def moments_multi(self, kernels, slots, counts, ind, lpos, lsup, form="fp64", rot4=None, flags=0, outs=None, order=None):
+        """{kernel: [N, 12]} for several of the scheme kernels (cubic, w2, w4, w6) in one pass: each pair's chord
+        primitives are shared by the kernels (``wsb_moments2d_multi``); the same values as ``moments`` per kernel."""
+        L.require_device(lpos, lsup, slots, counts, ind)
+        n = lpos.shape[0]
+        dev = lpos.device
+        ptrs = (ctypes.c_void_p * 4)()
+        res = {}
+        for k in kernels:
+            kid = KERNEL_ID[k]
+            if kid > 3:
+                raise NotImplementedError(f"{k}: several kernels in one pass are the scheme kernels cubic, w2, w4, w6")
+            o = outs[k] if outs is not None and k in outs else (
+                torch.zeros((n, 12), dtype=torch.float64, device=dev) if flags & L.OUT_ACCUMULATE
+                else torch.empty((n, 12), dtype=torch.float64, device=dev))
+            res[k] = o
+            ptrs[kid] = o.data_ptr()
+        lpos = lpos.to(torch.float64).contiguous()
+        lsup = lsup.to(torch.float64).contiguous()
+        L.check(L.load().wsb_moments2d_multi(self.handle, L.form_id(form), L.ptr(order), L.ptr(slots), L.ptr(counts), self.K,
+                                             L.ptr(ind), L.ptr(lpos), L.ptr(lsup), n, L.ptr(rot4), int(flags), ptrs,
+                                             L.stream_ptr(dev)))
+        return res
+
     def signed_distance(self, lpos, support_max=None):
         L.require_device(lpos)
         m = lpos.shape[0]
