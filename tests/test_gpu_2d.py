@@ -190,7 +190,7 @@ def test_golden_fixture_elements_on_device(form, kname):
 def test_moments_multi_matches_single_launches_and_oracle(form, kset):
     """wsb_moments2d_multi (several kernels in one pass sharing each pair's chord primitives) against the oracle
     (the single-launch gates) and against one launch per kernel (the same closed form, summed in the same slot
-    order; the shared primitives round as the single kernels' own within 1e-13 of the signal); rows in cell order
+    order; within 1e-13 of the signal, 5e-13 for w6, whose oracle gate is five times the others'); rows in cell order
     and in input order bitwise equal; WSB_OUT_ACCUMULATE doubles; physical units and a world rotation as the single
     launch."""
     from warpsphboundaries_amd import _lib as L
@@ -228,12 +228,14 @@ def test_moments_multi_matches_single_launches_and_oracle(form, kset):
         scale = np.maximum(np.abs(single).max(0), 1.0)
         d = (np.abs(got - single) / scale).max()
         print(form, kset, k, "oracle", err.max(), "multi vs single", d)
-        assert d <= 1e-13, (k, d)
+        # which product of a difference the compiler fuses is its own choice; w6's coefficients (to 1.3e4) amplify
+        # that rounding tenfold over the other kernels (its oracle gate is 5e-12 against 1e-12)
+        assert d <= (5e-13 if k == "w6" else 1e-13), (k, d)
         assert torch.equal(grouped[k], multi[k])
         assert torch.equal(acc[k], 2.0 * multi[k])
         sw = geo.moments(k, slots, counts, ind, lp, ls, form=form, rot4=rot4, flags=fl)
         dw = ((world[k] - sw).abs().amax(0) / sw.abs().amax(0).clamp(min=1.0)).max().item()
-        assert dw <= 1e-13, (k, dw)
+        assert dw <= (5e-13 if k == "w6" else 1e-13), (k, dw)
 
 
 def test_moments_multi_rejects_what_it_does_not_fuse():

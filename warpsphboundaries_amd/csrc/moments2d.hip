@@ -224,7 +224,9 @@ int dispatch_tile(int kid, const TileArgs& a, hipStream_t s) {
 // differences dI_m -- are computed once (radius 1 for every kernel, radius 1/2 for the cubic's inner block) and
 // each kernel's six block sums are accumulated in the same loop over m; the channels of one kernel at a time then
 // go through LDS to the row sums (slot order, as tile12_kernel).  A workgroup of MTILE_T threads owns MTILE_R
-// rows; the row sums of every kernel stay in LDS.
+// rows; thread t keeps the row sums of row t % MTILE_R, channels 3 (t / MTILE_R) .. +2, of every kernel in
+// registers (all four wavefronts reduce, and the row sums take no LDS), and stages them through LDS once at the end,
+// where the factor 1/pi of every channel is applied to the row sum.
 constexpr int MTILE_T = 256;
 constexpr int MTILE_R = 64;
 
@@ -258,50 +260,50 @@ __device__ __forceinline__ void block_term(int m, double dIm, double dR, bool ha
     }
 }
 
-// the channels of one block from its sums (as closed::pair_channels), added to c
-template <int KID, int B>
-__device__ __forceinline__ void block_channels(const BlockSums& s, double z, double th, double n0, double n1, double tx,
-                                               double ty, double c[wsb::NCH]) {
-    using T = KTab<KID>;
-    const double lam = z * s.VP - T::MRP[B] * th;
-    const double valF = z * s.VF - T::MRF[B] * th;
+// the twelve channels of one kernel from its block sums (as closed::pair_channels): mp = sum over blocks of
+// MRP[b] th_b, mf = sum of MRF[b] th_b (the angle terms; the cubic's two blocks share the sums' accumulators)
+__device__ __forceinline__ void kernel_channels(const BlockSums& s, double mp, double mf, double z, double n0, double n1,
+                                                double tx, double ty, double c[wsb::NCH]) {
+    const double lam = z * s.VP - mp;
+    const double valF = z * s.VF - mf;
     const double zn0 = z * n0, zn1 = z * n1;
     const double YxP = zn0 * s.AP + tx * s.BP, YyP = zn1 * s.AP + ty * s.BP;
     const double YxF = zn0 * s.AF + tx * s.BF, YyF = zn1 * s.AF + ty * s.BF;
-    c[0] += lam;
-    c[1] += n0 * s.AF;
-    c[2] += n1 * s.AF;
-    c[3] -= n0 * s.AP;
-    c[4] -= n1 * s.AP;
-    c[5] += lam - n0 * YxP;
-    c[6] -= n1 * YxP;
-    c[7] -= n0 * YyP;
-    c[8] += lam - n1 * YyP;
-    c[9] += n0 * YxF - valF;
-    c[10] += n0 * YyF;
-    c[11] += n1 * YyF - valF;
+    c[0] = lam;
+    c[1] = n0 * s.AF;
+    c[2] = n1 * s.AF;
+    c[3] = -(n0 * s.AP);
+    c[4] = -(n1 * s.AP);
+    c[5] = lam - n0 * YxP;
+    c[6] = -(n1 * YxP);
+    c[7] = -(n0 * YyP);
+    c[8] = lam - n1 * YyP;
+    c[9] = n0 * YxF - valF;
+    c[10] = n0 * YyF;
+    c[11] = n1 * YyF - valF;
 }
 
 // the primitives of one support radius R for one pair; accumulates the sums of the kernels of MASK (SMALL: the
-// cubic's inner block only) and returns whether the chord is non-empty, with its angle in th
+// cubic's inner block only, into the cubic's accumulator S[0]) and sets the chord's angle th.  Branch-free: a lane whose chord is empty (or that has
+// no pair, valid false) runs the same instructions on lo = hi = 0, rl = rh = 1, which make every dI_m, every
+// difference of powers and the angle exactly zero, so its sums stay zero and the block sums need no merge at a join.
 template <int MASK, int FORM, bool SMALL>
-__device__ __forceinline__ bool radius_sums(double R, double z, double az, double zz, double s0, double s1, double r0sq,
-                                            double r1sq, double& r0, double& r1, double& th, BlockSums* S) {
-    if (!(az < R)) return false;
+__device__ __forceinline__ void radius_sums(double R, bool valid, double z, double az, double zz, double s0, double s1,
+                                            double r0sq, double r1sq, double r0, double r1, double& th, BlockSums* S) {
     const bool clo = s0 < 0.0 && r0sq > R * R;
     const bool chi = s1 > 0.0 && r1sq > R * R;
-    const double L = (clo || chi) ? closed::xsqrt<FORM>((R - az) * (R + az)) : 0.0;
-    const double lo = clo ? -L : s0, hi = chi ? L : s1;
-    if (!(lo < hi)) return false;
-    if (!clo && r0 < 0.0) r0 = closed::xsqrt<FORM>(r0sq);
-    if (!chi && r1 < 0.0) r1 = closed::xsqrt<FORM>(r1sq);
-    const double rl = clo ? R : r0, rh = chi ? R : r1;
-    double dA = 0.0;
-    if (z != 0.0) {
-        const double Nh = hi >= 0.0 ? hi + rh : zz, Dh = hi >= 0.0 ? 1.0 : rh - hi;
-        const double Nl = lo >= 0.0 ? lo + rl : zz, Dl = lo >= 0.0 ? 1.0 : rl - lo;
-        dA = log((Nh * Dl) / (Dh * Nl));
-    }
+    const double L = closed::xsqrt<FORM>(fmax((R - az) * (R + az), 0.0));
+    double lo = clo ? -L : s0, hi = chi ? L : s1;
+    const bool on = valid && az < R && lo < hi;
+    double rl = clo ? R : r0, rh = chi ? R : r1;
+    lo = on ? lo : 0.0;
+    hi = on ? hi : 0.0;
+    rl = on ? rl : 1.0;
+    rh = on ? rh : 1.0;
+    const double Nh = hi >= 0.0 ? hi + rh : zz, Dh = hi >= 0.0 ? 1.0 : rh - hi;
+    const double Nl = lo >= 0.0 ? lo + rl : zz, Dl = lo >= 0.0 ? 1.0 : rl - lo;
+    const double lg = log((Nh * Dl) / (Dh * Nl));
+    const double dA = (on && z != 0.0) ? lg : 0.0;
     constexpr int NM = SMALL ? KTab<WSB_CUBIC>::NMAXP : mask_nm<MASK>();
     double dIm2 = hi - lo, dIm1 = 0.5 * (hi * rh - lo * rl + zz * dA);   // dI_0, dI_1
     double ph = rh, pl = rl;
@@ -330,8 +332,8 @@ __device__ __forceinline__ bool radius_sums(double R, double z, double az, doubl
             if (MASK & 8) block_term<WSB_W6, 0>(m, dIm, dR, have && m <= KTab<WSB_W6>::NMAXP, S[3]);
         }
     }
-    th = FORM == closed::MIXED ? closed::dangle_mixed(z, lo, hi) : closed::dangle_fp64(z, lo, hi);
-    return true;
+    const double t = FORM == closed::MIXED ? closed::dangle_mixed(z, lo, hi) : closed::dangle_fp64(z, lo, hi);
+    th = on ? t : 0.0;
 }
 
 struct MultiArgs {
@@ -340,15 +342,16 @@ struct MultiArgs {
 };
 
 template <int MASK, int FORM>
-__global__ __launch_bounds__(MTILE_T) void tile12_multi_kernel(MultiArgs ma) {
+__global__ __launch_bounds__(MTILE_T) __attribute__((amdgpu_waves_per_eu(FORM == closed::MIXED ? 4 : 3))) void tile12_multi_kernel(MultiArgs ma) {
     const TileArgs& a = ma.t;
     using BlockScan = hipcub::BlockScan<int, MTILE_T>;
     __shared__ typename BlockScan::TempStorage scan_tmp;
     __shared__ int s_off[MTILE_T + 1];
     __shared__ int s_row[MTILE_R];
     __shared__ double s_px[MTILE_R], s_py[MTILE_R], s_h[MTILE_R], s_ih[MTILE_R];
-    __shared__ double s_res[wsb::NCH][MTILE_T];
-    __shared__ double s_acc[4][wsb::NCH][MTILE_R];
+    __shared__ double s_res[wsb::NCH][MTILE_T];          // one kernel's pair channels; at the end the row sums
+    static_assert(4 * wsb::NCH * MTILE_R <= wsb::NCH * MTILE_T && wsb::NCH == 12 && MTILE_T == 4 * MTILE_R,
+                  "row sums of four kernels staged in s_res; four groups of three channels");
     const int tid = threadIdx.x;
     const int pos = blockIdx.x * MTILE_R + tid;
     const bool live = tid < MTILE_R && pos < a.n;
@@ -362,12 +365,13 @@ __global__ __launch_bounds__(MTILE_T) void tile12_multi_kernel(MultiArgs ma) {
         s_h[tid] = h;
         s_ih[tid] = 1.0 / h;
     }
-    if (tid < MTILE_R) {
+    // reduction role: row rr_ of the tile, channels 3 g_ .. 3 g_ + 2
+    const int rr_ = tid & (MTILE_R - 1), g_ = tid / MTILE_R;
+    double acc[4][3];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int c = 0; c < wsb::NCH; ++c) s_acc[k][c][tid] = 0.0;
-    }
+        for (int i = 0; i < 3; ++i) acc[k][i] = 0.0;
     int off, total;
     BlockScan(scan_tmp).ExclusiveSum(cnt, off, total);
     s_off[tid] = off;
@@ -375,83 +379,104 @@ __global__ __launch_bounds__(MTILE_T) void tile12_multi_kernel(MultiArgs ma) {
     __syncthreads();
     for (int base = 0; base < total; base += MTILE_T) {
         const int j = base + tid;
-        BlockSums S[4] = {}, Sc = {};
-        bool on1 = false, on05 = false;
+        BlockSums S[4] = {};
         double th1 = 0.0, th05 = 0.0, z = 0.0, n0 = 0.0, n1 = 0.0, tx = 0.0, ty = 0.0;
-        if (j < total) {
+        // a (64-lane) wavefront whose slots all lie past the tile's pairs skips the pair phase (its sums stay zero)
+        if (__builtin_amdgcn_readfirstlane(base + (tid & ~63) < total ? 1 : 0)) {
+            // the pair of slot j, branch-free: lanes past the tile's pairs (and invalid slots) take edge 0 and valid =
+            // false, which zeroes their sums (the geometry has at least one edge)
+            const int jc = min(j, total - 1);
             int lo = 0, hi = MTILE_R;
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
-                if (s_off[mid] <= j) lo = mid;
+                if (s_off[mid] <= jc) lo = mid;
                 else hi = mid;
             }
             const int r = lo;
-            const int e = a.slots[(int64_t)s_row[r] * a.K + (j - s_off[r])];
-            if (e >= 0 && e < a.ne) {
-                const double x = s_px[r], y = s_py[r], h = s_h[r], ih = s_ih[r];
-                const double4 ab = a.erec[e];
-                const double4 tn = a.etan[e];
-                const double ux = ab.x - x, uy = ab.y - y, vx = ab.z - x, vy = ab.w - y;
-                tx = tn.x;
-                ty = tn.y;
-                const double iell = tn.z;
-                if (iell != 0.0) {
-                    double cr = wsb_cross2(ux, uy, vx, vy);
-                    const double mag = fabs(ux * vy) + fabs(uy * vx);
-                    if (!(mag > 1e-280 && fabs(cr) > 1.7763568394002505e-15 * mag))
-                        cr = wsb_cross2(ux / h, uy / h, vx / h, vy / h) * h * h;
-                    z = cr * ih * iell;
-                    const double px = ux * ih, py = uy * ih, qx = vx * ih, qy = vy * ih;
-                    const double s0 = tx * px + ty * py, s1 = tx * qx + ty * qy;
-                    n0 = ty;
-                    n1 = -tx;
-                    const double az = fabs(z), zz = z * z;
-                    const double r0sq = px * px + py * py, r1sq = qx * qx + qy * qy;
-                    double r0 = -1.0, r1 = -1.0;
-                    on1 = radius_sums<MASK, FORM, false>(1.0, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th1, S);
-                    if (MASK & 1) on05 = radius_sums<MASK, FORM, true>(0.5, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th05, &Sc);
-                }
-            }
+            const int e0 = a.slots[(int64_t)s_row[r] * a.K + (jc - s_off[r])];
+            const bool ok = j < total && e0 >= 0 && e0 < a.ne;
+            const int e = ok ? e0 : 0;
+            const double x = s_px[r], y = s_py[r], h = s_h[r], ih = s_ih[r];
+            const double4 ab = a.erec[e];
+            const double4 tn = a.etan[e];
+            const double ux = ab.x - x, uy = ab.y - y, vx = ab.z - x, vy = ab.w - y;
+            tx = tn.x;
+            ty = tn.y;
+            const double iell = tn.z;
+            const bool valid = ok && iell != 0.0;
+            double cr = wsb_cross2(ux, uy, vx, vy);
+            const double mag = fabs(ux * vy) + fabs(uy * vx);
+            if (!(mag > 1e-280 && fabs(cr) > 1.7763568394002505e-15 * mag))
+                cr = wsb_cross2(ux / h, uy / h, vx / h, vy / h) * h * h;
+            z = cr * ih * iell;
+            const double px = ux * ih, py = uy * ih, qx = vx * ih, qy = vy * ih;
+            const double s0 = tx * px + ty * py, s1 = tx * qx + ty * qy;
+            n0 = ty;
+            n1 = -tx;
+            const double az = fabs(z), zz = z * z;
+            const double r0sq = px * px + py * py, r1sq = qx * qx + qy * qy;
+            const double r0 = closed::xsqrt<FORM>(r0sq), r1 = closed::xsqrt<FORM>(r1sq);
+            radius_sums<MASK, FORM, false>(1.0, valid, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th1, S);
+            if (MASK & 1) radius_sums<MASK, FORM, true>(0.5, valid, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th05, S);
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (!(MASK & (1 << k))) continue;
             double c[wsb::NCH];
+            if (k == 0)
+                kernel_channels(S[0], KTab<WSB_CUBIC>::MRP[0] * th1 + KTab<WSB_CUBIC>::MRP[1] * th05,
+                                KTab<WSB_CUBIC>::MRF[0] * th1 + KTab<WSB_CUBIC>::MRF[1] * th05, z, n0, n1, tx, ty, c);
+            if (k == 1) kernel_channels(S[1], KTab<WSB_W2>::MRP[0] * th1, KTab<WSB_W2>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+            if (k == 2) kernel_channels(S[2], KTab<WSB_W4>::MRP[0] * th1, KTab<WSB_W4>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+            if (k == 3) kernel_channels(S[3], KTab<WSB_W6>::MRP[0] * th1, KTab<WSB_W6>::MRF[0] * th1, z, n0, n1, tx, ty, c);
 #pragma unroll
-            for (int q = 0; q < wsb::NCH; ++q) c[q] = 0.0;
-            if (on1) {
-                if (k == 0) block_channels<WSB_CUBIC, 0>(S[0], z, th1, n0, n1, tx, ty, c);
-                if (k == 1) block_channels<WSB_W2, 0>(S[1], z, th1, n0, n1, tx, ty, c);
-                if (k == 2) block_channels<WSB_W4, 0>(S[2], z, th1, n0, n1, tx, ty, c);
-                if (k == 3) block_channels<WSB_W6, 0>(S[3], z, th1, n0, n1, tx, ty, c);
-            }
-            if (k == 0 && on05) block_channels<WSB_CUBIC, 1>(Sc, z, th05, n0, n1, tx, ty, c);
-#pragma unroll
-            for (int q = 0; q < wsb::NCH; ++q) s_res[q][tid] = c[q] * WSB_INV_PI;
+            for (int q = 0; q < wsb::NCH; ++q) s_res[q][tid] = c[q];
             __syncthreads();
-            if (tid < MTILE_R) {
-                const int b0 = max(off, base), b1 = min(off + cnt, base + MTILE_T);
-                double acc[wsb::NCH];
+            {
+                // slot order; four slots' loads issued before their adds
+                const int b0 = max(s_off[rr_], base) - base, b1 = min(s_off[rr_ + 1], base + MTILE_T) - base;
+                const double* p0 = s_res[3 * g_];
+                const double* p1 = s_res[3 * g_ + 1];
+                const double* p2 = s_res[3 * g_ + 2];
+                int jj = b0;
+                for (; jj + 4 <= b1; jj += 4) {
+                    double x0[4], x1[4], x2[4];
 #pragma unroll
-                for (int q = 0; q < wsb::NCH; ++q) acc[q] = s_acc[k][q][tid];
-                for (int jj = b0; jj < b1; ++jj) {
+                    for (int u = 0; u < 4; ++u) {
+                        x0[u] = p0[jj + u];
+                        x1[u] = p1[jj + u];
+                        x2[u] = p2[jj + u];
+                    }
 #pragma unroll
-                    for (int q = 0; q < wsb::NCH; ++q) acc[q] += s_res[q][jj - base];
+                    for (int u = 0; u < 4; ++u) {
+                        acc[k][0] += x0[u];
+                        acc[k][1] += x1[u];
+                        acc[k][2] += x2[u];
+                    }
                 }
-#pragma unroll
-                for (int q = 0; q < wsb::NCH; ++q) s_acc[k][q][tid] = acc[q];
+                for (; jj < b1; ++jj) {
+                    acc[k][0] += p0[jj];
+                    acc[k][1] += p1[jj];
+                    acc[k][2] += p2[jj];
+                }
             }
             __syncthreads();
         }
     }
-    // finalise: thread (kernel k, row r) for the MTILE_R rows x 4 kernels
+    // stage the row sums [kernel][channel][row] through s_res, then finalise: thread (kernel k, row r)
+    double* stage = &s_res[0][0];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) stage[(k * wsb::NCH + 3 * g_ + i) * MTILE_R + rr_] = acc[k][i];
+    __syncthreads();
     const int r = tid & (MTILE_R - 1), k = tid / MTILE_R;
     const int p2 = blockIdx.x * MTILE_R + r;
     if (p2 < a.n && (MASK & (1 << k))) {
         const int rr = s_row[r];
         double acc[wsb::NCH];
 #pragma unroll
-        for (int q = 0; q < wsb::NCH; ++q) acc[q] = s_acc[k][q][r];
+        for (int q = 0; q < wsb::NCH; ++q) acc[q] = stage[(k * wsb::NCH + q) * MTILE_R + r] * WSB_INV_PI;
         const double* plane = k == 0 ? KTab<WSB_CUBIC>::PLANE : k == 1 ? KTab<WSB_W2>::PLANE : k == 2 ? KTab<WSB_W4>::PLANE
                                                                                                       : KTab<WSB_W6>::PLANE;
         finalize_row(acc, a.ind[rr], s_h[r], plane, a.rot4, a.flags, ma.out[k] + (int64_t)rr * wsb::NCH);
