@@ -77,7 +77,9 @@ int wsb_plan_destroy(wsb_plan* plan);
 int wsb_geom2d_create(const double* verts, int nv, const int32_t* edges, int ne, int background,
                       double support_max, wsb_geom2d** out, int* K);
 int wsb_geom2d_destroy(wsb_geom2d* g);
-/* info[8]: K, nx, ny, nbins, ne, nv, ncell_items, device bytes */
+/* info[10]: K, nx, ny, nbins, ne, nv, ncell_items, device bytes, local winding (1: rows of cells with edges count
+ * the crossings of the segment from their cell's reference point; 0: every such row takes the ray over its y-bin),
+ * cells with edges that have no reference point (their rows take the ray) */
 int wsb_geom2d_info(const wsb_geom2d* g, int64_t* info);
 /* device pointers of the uploaded vertices [nv*2] f64 and edges [ne*2] i32 */
 int wsb_geom2d_arrays(const wsb_geom2d* g, const double** verts, const int32_t** edges);

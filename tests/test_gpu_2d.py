@@ -249,3 +249,46 @@ def test_moments_multi_rejects_what_it_does_not_fuse():
         geo.moments_multi(("w2", "lw2"), slots, counts, ind, lp, ls)
     with pytest.raises(NotImplementedError):
         geo.moments_multi(("w2",), slots, counts, ind, lp, ls, form="plain")
+
+
+def _line_points(loop, h, n, seed):
+    """points on the lines of the edges beyond their ends, at the vertices' heights, and within 1e-12 of vertices:
+    where a ray test or a segment-crossing test is closest to its rounding bound."""
+    g = WL.rng(seed)
+    lo, hi = loop.min(0) - 2 * h, loop.max(0) + 2 * h
+    p, q = loop, np.roll(loop, -1, 0)
+    k = g.integers(0, len(loop), n)
+    ext = p[k] + (1.0 + g.uniform(0.0, 3.0, (n, 1))) * (q[k] - p[k])
+    level = np.stack([g.uniform(lo[0], hi[0], n), p[k][:, 1]], 1)
+    near = p[k] + g.normal(0.0, 1e-12, (n, 2))
+    return np.concatenate([ext, level, near, g.uniform(lo, hi, (n, 2))])
+
+
+@pytest.mark.parametrize("name", ["star256", "notch", "box-outside", "nearly-horizontal"])
+def test_local_winding_indicator_bit_exact(name):
+    """the indicator of rows in cells with edges is the cell reference point's plus the crossings of the segment to
+    the row (geom2d_device.h local_winding_edge), with the reference's ray where a decision is uncertain: equal to
+    the oracle's ray cast on points placed on edge lines, at vertex heights and next to vertices.  A geometry with
+    an edge 1e-15 off horizontal takes the ray for every row (info local_winding 0)."""
+    if name == "nearly-horizontal":
+        loop, solid = np.array([[0.1, 0.2], [0.9, 0.2 + 1e-15], [0.7, 0.8], [0.2, 0.7]]), "inside"
+    else:
+        loop, solid = _geometries()[name]
+    rep = _rep(loop, solid)
+    h = 0.02
+    pts = np.concatenate([_line_points(rep.vertices.numpy(), h, 4000, 21), _particles(rep.vertices.numpy(), h, 2000, 22)])
+    sup = np.full(len(pts), h)
+    st = oadj.Static(rep.vertices.numpy(), rep.edges.numpy(), rep.background, h)
+    geo = rep.geom(h, DEV)
+    assert geo.info["local_winding"] == (0 if name == "nearly-horizontal" else 1)
+    assert geo.info["cells_without_reference"] == 0
+    lp = torch.tensor(pts, dtype=torch.float64, device=DEV)
+    ls = torch.tensor(sup, dtype=torch.float64, device=DEV)
+    slots, counts, ind = geo.adjacency(lp, ls, pad=True)
+    e_ref, c_ref, i_ref = oadj.fixed_topology(st, pts, sup)
+    assert np.array_equal(slots.cpu().numpy(), e_ref) and np.array_equal(counts.cpu().numpy(), c_ref)
+    got = ind.cpu().numpy()
+    frac = i_ref != np.round(i_ref)
+    print(name, "rows", len(pts), "boundary rows", int(frac.sum()))
+    assert np.array_equal(got[~frac], i_ref[~frac])
+    np.testing.assert_allclose(got[frac], i_ref[frac], rtol=0, atol=1e-14)

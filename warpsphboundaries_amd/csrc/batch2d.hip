@@ -29,6 +29,7 @@ struct GeoDesc2 {
     const int32_t* E;
     const double4* erec;
     const double4* etan;
+    const double4* cref;        // nullptr: no local winding (see adjacency2d_kernel)
 };
 
 namespace {
@@ -71,10 +72,17 @@ __global__ __launch_bounds__(256) void batch_adjacency2d_kernel(BatchAdjArgs a) 
     if (inside) {
         const int cid = (int)fx * d.ny + (int)fy;
         const double reach = h * (1.0 + 1e-9) + 1e-14 * (1.0 + fabs(px) + fabs(py));
+        const bool empty = d.cempty[cid] != 0;
+        double4 q = double4{0.0, 0.0, 0.0, 0.0};
+        if (!empty && d.cref) q = d.cref[cid];
+        const bool local = q.w != 0.0;
+        int dw = 0;
+        bool unsure = false;
         for (int i = d.cstart[cid]; i < d.cstart[cid + 1]; ++i) {
             const double4 rec = d.cxy[i];
             const double ax = rec.x, ay = rec.y;
             const double bx = rec.z, by = rec.w;
+            if (local) local_winding_edge(ax, ay, bx, by, px, py, q.x, q.y, dw, unsure);
             if (fmin(ax, bx) - px > reach || px - fmax(ax, bx) > reach || fmin(ay, by) - py > reach || py - fmax(ay, by) > reach)
                 continue;
             const double dx = bx - ax;
@@ -88,7 +96,8 @@ __global__ __launch_bounds__(256) void batch_adjacency2d_kernel(BatchAdjArgs a) 
                 ++n;
             }
         }
-        ind = d.cempty[cid] != 0 ? d.cind[cid]
+        ind = empty              ? d.cind[cid]
+              : local && !unsure ? q.z + (double)dw
                                  : winding_point(px, py, h, d.V, d.E, d.ne, d.bstart, d.bxy, d.y0, d.dy, d.nb, d.bg);
     } else {
         ind = d.bg;
@@ -371,6 +380,7 @@ extern "C" int wsb_geom2d_desc(const wsb_geom2d* g, void* out) {
     d.cxy = g->cxy;
     d.cind = g->cind;
     d.cempty = g->cempty;
+    d.cref = g->local ? g->cref : nullptr;
     d.bstart = g->bstart;
     d.bxy = g->bxy;
     d.V = g->verts;

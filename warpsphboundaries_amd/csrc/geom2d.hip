@@ -112,6 +112,41 @@ HostBins build_bins(const double* V, const int32_t* E, int ne) {
     return b;
 }
 
+// Reference points of the local winding count (local_winding_edge): for each cell with edges, the first of a
+// few trial points inside it whose ray decisions over its y-bin (winding_point's test) and whose orientations
+// against every edge of the cell's list are certain; (x, y, indicator, 1), or zeros when no trial point qualifies.
+// `local` is cleared when an edge outside a point's cell list could be uncertain for the reference's ray at it:
+// such an edge crosses the ray farther than smax from the point, so its line passes within 32 u D^2 / (len smax)
+// of the point (D the grid diagonal) only if |dy| <= 32 u D^2 / smax; exactly horizontal edges never count.
+double4 cell_reference(int k, const HostCells& c, const HostBins& b, const std::vector<double4>& cxy,
+                       const std::vector<double4>& bxy, const double* V, const int32_t* E, int ne, int background) {
+    static const double trial[][2] = {{0.5, 0.5},   {0.31, 0.67}, {0.69, 0.33}, {0.33, 0.31}, {0.67, 0.71},
+                                      {0.5, 0.19},  {0.19, 0.5},  {0.81, 0.5},  {0.5, 0.83},  {0.41, 0.57},
+                                      {0.23, 0.23}, {0.77, 0.21}, {0.24, 0.79}, {0.79, 0.77}, {0.57, 0.39}};
+    const int cx = k / c.ny, cy = k % c.ny;
+    for (const auto& f : trial) {
+        const double qx = c.lo0 + ((double)cx + f[0]) * c.cell;
+        const double qy = c.lo1 + ((double)cy + f[1]) * c.cell;
+        bool ok = true;
+        for (int i = c.start[k]; ok && i < c.start[k + 1]; ++i)
+            ok = orient_certain(cxy[i].x, cxy[i].y, cxy[i].z, cxy[i].w, qx, qy) != 0;
+        const double iyf = std::floor((qy - b.y0) / b.dy);
+        if (ok && iyf >= 0.0 && iyf <= (double)b.nb) {
+            const int iy = (int)iyf < b.nb - 1 ? (int)iyf : b.nb - 1;
+            for (int i = b.start[iy]; ok && i < b.start[iy + 1]; ++i) {
+                const double4 e = bxy[i];
+                if (e.x < qx && e.z < qx) continue;
+                ok = orient_certain(e.x, e.y, e.z, e.w, qx, qy) != 0;
+            }
+        }
+        if (!ok) continue;
+        const double w = winding_point(qx, qy, c.cell, V, E, ne, b.start.data(), bxy.data(), b.y0, b.dy, b.nb,
+                                       (double)background);
+        return double4{qx, qy, w, 1.0};
+    }
+    return double4{0.0, 0.0, 0.0, 0.0};
+}
+
 // ------------------------------------------------------------------------------------------------ device kernels
 struct AdjArgs {
     const double* lpos;
@@ -134,6 +169,7 @@ struct AdjArgs {
     const int32_t* bstart;
     const double4* bxy;
     const double4* cxy;
+    const double4* cref;        // nullptr: no local winding
     int32_t* slots;
     int32_t* counts;
     double* ind;
@@ -143,7 +179,10 @@ struct AdjArgs {
 // one thread per row (reference scene/fixedadj.py:67-147); with `order` (rows sorted by cell) thread i serves
 // row order[i], so the rows of one wavefront share their cell's item list.  A candidate whose segment box lies
 // farther than h (1 + 1e-9) from the point on either axis cannot pass the reference predicate (the rounded
-// distance then exceeds h), so it is rejected before the division.
+// distance then exceeds h), so it is rejected before the division.  The indicator of a row in a cell with edges is
+// the cell reference point's plus the crossings of the segment from it, counted over the same item list
+// (local_winding_edge); a row with an uncertain decision, or in a cell without a reference point, takes the
+// reference's ray over its y-bin (winding_point).
 __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
     const int i_ = blockIdx.x * blockDim.x + threadIdx.x;
     if (i_ >= a.n) return;
@@ -159,10 +198,17 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
         if (inside) {
             const int cid = (int)fx * a.ny + (int)fy;
             const double reach = h * (1.0 + 1e-9) + 1e-14 * (1.0 + fabs(px) + fabs(py));
+            const bool empty = a.cempty[cid] != 0;
+            double4 q = double4{0.0, 0.0, 0.0, 0.0};
+            if (!empty && a.cref) q = a.cref[cid];
+            const bool local = q.w != 0.0;
+            int dw = 0;
+            bool unsure = false;
             for (int i = a.cstart[cid]; i < a.cstart[cid + 1]; ++i) {
                 const double4 rec = a.cxy[i];
                 const double ax = rec.x, ay = rec.y;
                 const double bx = rec.z, by = rec.w;
+                if (local) local_winding_edge(ax, ay, bx, by, px, py, q.x, q.y, dw, unsure);
                 if (fmin(ax, bx) - px > reach || px - fmax(ax, bx) > reach || fmin(ay, by) - py > reach ||
                     py - fmax(ay, by) > reach)
                     continue;
@@ -177,8 +223,10 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
                     ++n;
                 }
             }
-            if (a.cempty[cid] != 0) {
+            if (empty) {
                 ind = a.cind[cid];
+            } else if (local && !unsure) {
+                ind = q.z + (double)dw;
             } else {
                 ind = winding_point(px, py, h, a.V, a.E, a.ne, a.bstart, a.bxy, a.y0, a.dy, a.nb, a.bg);
             }
@@ -378,10 +426,25 @@ extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edg
     const int ncells = c.nx * c.ny;
     std::vector<double> cind(ncells, 0.0);
     std::vector<int32_t> cempty(ncells, 0);
+    std::vector<double4> cref(ncells, double4{0.0, 0.0, 0.0, 0.0});
+    int local = 1;
+    int64_t noref = 0;
+    {
+        const double gx = c.nx * c.cell, gy = c.ny * c.cell;
+        const double thr = 64.0 * 1.1102230246251565e-16 * (gx * gx + gy * gy) / support_max;
+        for (int e = 0; e < ne && local; ++e) {
+            const double dyv = erec[e].w - erec[e].y;
+            if (dyv != 0.0 && std::fabs(dyv) <= thr) local = 0;
+        }
+    }
     int Kmax = 1;
     for (int k = 0; k < ncells; ++k) {
         const int cnt = c.start[k + 1] - c.start[k];
         Kmax = std::max(Kmax, cnt);
+        if (cnt != 0 && local) {
+            cref[k] = cell_reference(k, c, b, cxy, bxy, verts, edges, ne, background);
+            noref += cref[k].w == 0.0;
+        }
         if (cnt == 0) {
             cempty[k] = 1;
             const int cx = k / c.ny, cy = k % c.ny;
@@ -421,6 +484,9 @@ extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edg
     if (!st) st = upload(&g->bxy, bxy.data(), bxy.size(), &g->bytes);
     if (!st) st = upload(&g->erec, erec.data(), erec.size(), &g->bytes);
     if (!st) st = upload(&g->etan, etan.data(), etan.size(), &g->bytes);
+    if (!st) st = upload(&g->cref, cref.data(), cref.size(), &g->bytes);
+    g->local = local;
+    g->noref = noref;
     if (st) {
         wsb_geom2d_destroy(g);
         return st;
@@ -434,7 +500,7 @@ extern "C" int wsb_geom2d_destroy(wsb_geom2d* g) {
     if (!g) return WSB_OK;
     for (void* p : {(void*)g->verts, (void*)g->edges, (void*)g->cstart, (void*)g->citems, (void*)g->cind,
                     (void*)g->cempty, (void*)g->bstart, (void*)g->bitems, (void*)g->cxy, (void*)g->bxy, (void*)g->erec,
-                    (void*)g->etan})
+                    (void*)g->etan, (void*)g->cref})
         if (p) (void)hipFree(p);
     delete g;
     return WSB_OK;
@@ -450,6 +516,8 @@ extern "C" int wsb_geom2d_info(const wsb_geom2d* g, int64_t* info) {
     info[5] = g->nv;
     info[6] = g->ncell_items;
     info[7] = g->bytes;
+    info[8] = g->local;
+    info[9] = g->noref;
     return WSB_OK;
 }
 
@@ -502,7 +570,7 @@ extern "C" int wsb_adjacency2d(const wsb_geom2d* g, const double* lpos, const do
     WSB_REQUIRE(lpos && lsup && slots && ind, WSB_EINVAL, "null buffer");
     AdjArgs a{lpos, lsup, valid, order, n, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny,
               g->cstart, g->citems, g->K, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb,
-              g->bstart, g->bxy, g->cxy, slots, counts, ind, pad};
+              g->bstart, g->bxy, g->cxy, g->local ? g->cref : nullptr, slots, counts, ind, pad};
     hipLaunchKernelGGL(adjacency2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;

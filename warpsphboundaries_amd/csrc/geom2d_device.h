@@ -82,3 +82,36 @@ __host__ __device__ inline double winding_point(double px, double py, double h, 
     }
     return w + bg;
 }
+
+// Sign of the orientation (a, b, c) = (a - c) x (b - c) when the rounded value certainly has the sign of the exact
+// one (|value| above 16 u of the magnitude of its two products, the filter of winding_point), else 0.  With c = p
+// it is the ray test's cu of winding_point.
+__host__ __device__ inline int orient_certain(double ax, double ay, double bx, double by, double cx, double cy) {
+#pragma clang fp contract(off)
+    const double l = (ax - cx) * (by - cy), r = (ay - cy) * (bx - cx);
+    const double mag = fabs(l) + fabs(r), d = l - r;
+    if (!(mag > 1e-280 && fabs(d) > 1.7763568394002505e-15 * mag)) return 0;
+    return d > 0.0 ? 1 : -1;
+}
+
+// One edge (a, b) of a point's cell list in the local winding count: the winding number at p is the one at the
+// cell's reference point q plus the signed crossings of the segment q -> p (+1 from the right of a -> b to its
+// left).  Every edge that can cross the segment lies in the cell list.  unsure is set when the reference's own ray
+// decision at p for this edge is uncertain (p within rounding of the edge's line while the edge is not wholly left
+// of p: there the reference may differ from the topology) or when a crossing decision is uncertain; the caller
+// then evaluates winding_point.  Edges outside the cell list are never uncertain for the reference when every
+// non-horizontal edge of the geometry is steep enough (checked once per geometry, wsb_geom2d_create).
+__host__ __device__ inline void local_winding_edge(double ax, double ay, double bx, double by, double px, double py,
+                                                   double qx, double qy, int& dw, bool& unsure) {
+    const int sp = orient_certain(ax, ay, bx, by, px, py);
+    if (sp == 0 && !(ax < px && bx < px)) unsure = true;
+    if (fmax(ax, bx) < fmin(qx, px) || fmin(ax, bx) > fmax(qx, px) || fmax(ay, by) < fmin(qy, py) ||
+        fmin(ay, by) > fmax(qy, py))
+        return;
+    const int sq = orient_certain(ax, ay, bx, by, qx, qy);
+    const int sa = orient_certain(qx, qy, px, py, ax, ay);
+    const int sb = orient_certain(qx, qy, px, py, bx, by);
+    if (sq * sp < 0 && sa * sb < 0) dw += sp;
+    else if (!(sq * sp > 0 || sa * sb > 0)) unsure = true;
+}
+

@@ -67,6 +67,11 @@ struct wsb_geom2d {
     double4* bxy = nullptr;       // [nbin_items]  (ax, ay, bx, by) of each bin item
     double4* erec = nullptr;      // [ne] (ax, ay, bx, by)
     double4* etan = nullptr;      // [ne] (tx, ty, 1/len, len): unit tangent and length in the local frame
+    // local winding: per cell with edges a reference point and its indicator (x, y, indicator, 1), (0, 0, 0, 0)
+    // where none qualifies; `local` = 0 when some edge is too close to horizontal for the ray argument
+    double4* cref = nullptr;      // [nx*ny]
+    int local = 0;
+    int64_t noref = 0;            // cells with edges and no reference point
 };
 
 struct wsb_plan {

@@ -61,10 +61,10 @@ class Geom2D:
                                                float(support_max), ctypes.byref(h), ctypes.byref(K)))
         self.handle = h
         self.K = int(K.value)
-        info = (ctypes.c_int64 * 8)()
+        info = (ctypes.c_int64 * 10)()
         L.check(L.load().wsb_geom2d_info(self.handle, info))
         self.info = dict(K=info[0], nx=info[1], ny=info[2], nbins=info[3], ne=info[4], nv=info[5], cell_items=info[6],
-                         device_bytes=info[7])
+                         device_bytes=info[7], local_winding=info[8], cells_without_reference=info[9])
         pv, pe = ctypes.c_void_p(), ctypes.c_void_p()
         L.check(L.load().wsb_geom2d_arrays(self.handle, ctypes.byref(pv), ctypes.byref(pe)))
         self.verts_ptr, self.edges_ptr = pv, pe
