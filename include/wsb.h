@@ -84,8 +84,9 @@ int wsb_geom2d_info(const wsb_geom2d* g, int64_t* info);
 /* device pointers of the uploaded vertices [nv*2] f64 and edges [ne*2] i32 */
 int wsb_geom2d_arrays(const wsb_geom2d* g, const double** verts, const int32_t** edges);
 
-/* ---- particle rows sorted by static cell (device radix sort of the cell keys; no counterpart in the reference,
- * which walks rows in input order): order[n] is a permutation of 0..n-1 with the rows of one cell adjacent.
+/* ---- particle rows sorted by static cell (device counting sort of the cell keys; no counterpart in the reference,
+ * which walks rows in input order): order[n] is a permutation of 0..n-1 with the rows of one cell adjacent (in
+ * the order their atomic ranks were taken) and the rows outside the grid last.
  * Passing it to the adjacency / moment entry points changes which thread serves a row, never a result. */
 size_t wsb_sort2d_scratch_bytes(const wsb_geom2d* g, int n);
 int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_t* order, void* scratch, size_t scratch_bytes,

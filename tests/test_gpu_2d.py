@@ -87,6 +87,13 @@ def test_adjacency_bit_exact(name):
     # rows served in cell order give the same slots and indicator
     order = geo.sort(lp)
     assert sorted(order.cpu().numpy().tolist()) == list(range(len(pts)))
+    # grouped by cell key in ascending order, the rows outside the grid last
+    nx, ny = st.cl.dims
+    fx = np.floor((pts[:, 0] - st.cl.lo[0]) / st.cl.cell)
+    fy = np.floor((pts[:, 1] - st.cl.lo[1]) / st.cl.cell)
+    inside = (fx >= 0) & (fx < nx) & (fy >= 0) & (fy < ny)
+    key = np.where(inside, fx * ny + fy, nx * ny)
+    assert np.array_equal(key[order.cpu().numpy()], np.sort(key))
     s2, c2, i2 = geo.adjacency(lp, ls, valid=valid, pad=True, order=order)
     assert torch.equal(s2, slots) and torch.equal(c2, counts) and torch.equal(i2, ind)
     got = ind.cpu().numpy()

@@ -335,55 +335,59 @@ __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
 // cell depends on the atomics' timing; it only decides which thread serves a row, never a result.
 // Rows inside the grid are placed by per-cell atomics (a cell holds a few dozen rows); the rows outside the grid,
 // which share one key and may be most of the particles, are placed after them by an exclusive scan of their flags.
-__global__ __launch_bounds__(256) void cell_count_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
-                                                         int ny, int32_t* key, int32_t* outflag, int32_t* hist) {
+// counting sort by cell: the cell key of every row and its rank among the rows of its key, then each row to its
+// key's offset plus its rank.  A row inside the grid takes its rank from an atomic count of its cell; the rows
+// outside it (often most of a workload) take the key nbins + their 64-lane wavefront and their rank among that
+// wavefront's outside rows from a ballot, so they need no atomic (and land last, in input order).
+__global__ __launch_bounds__(256) void cell_rank_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
+                                                        int ny, int32_t* key, int32_t* rank, int32_t* hist) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const double fx = floor((lpos[2 * r] - clo0) / cell);
     const double fy = floor((lpos[2 * r + 1] - clo1) / cell);
     const bool inside = fx >= 0.0 && fx < (double)nx && fy >= 0.0 && fy < (double)ny;
-    const int k = inside ? (int)fx * ny + (int)fy : -1;
-    key[r] = k;
-    outflag[r] = inside ? 0 : 1;
-    if (inside) atomicAdd(hist + k, 1);
-}
-
-__global__ __launch_bounds__(256) void cell_scatter_kernel(const int32_t* key, const int32_t* outflag, const int32_t* outpos, int n,
-                                                           int32_t* cursor, int32_t* order) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int k = key[r];
-    if (k >= 0) {
-        order[atomicAdd(cursor + k, 1)] = r;
+    const uint64_t out = __ballot(!inside);
+    if (inside) {
+        const int k = (int)fx * ny + (int)fy;
+        key[r] = k;
+        rank[r] = atomicAdd(hist + k, 1);
     } else {
-        const int n_out = outpos[n - 1] + outflag[n - 1];
-        order[(n - n_out) + outpos[r]] = r;
+        const int k = nx * ny + (r >> 6);
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(out >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)out, 0u));
+        key[r] = k;
+        rank[r] = below;
+        if (below == 0) hist[k] = __popcll(out);
     }
 }
-
+__global__ __launch_bounds__(256) void cell_place_kernel(const int32_t* key, const int32_t* rank, const int32_t* start, int n,
+                                                         int32_t* order) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) order[start[key[r]] + rank[r]] = r;
+}
 __global__ void zero_kernel(int32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
 struct SortLayout {
-    size_t key, flag, opos, hist, cur, tmp, tmp_bytes, total;
+    size_t key, rank, hist, start, tmp, tmp_bytes, total;
 };
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// keys: the nbins cells, then one per 64-row wavefront for the rows outside the grid
+inline int sort_keys(int n, int nbins) { return nbins + (n + 63) / 64; }
+
 SortLayout sort_layout(int n, int nbins) {
     SortLayout L{};
-    size_t tmp = 0, tmp2 = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, nbins);
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, (const int32_t*)nullptr, (int32_t*)nullptr, n);
-    tmp = std::max(tmp, tmp2);
+    size_t tmp = 0;
+    const int nk = sort_keys(n, nbins);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, nk);
     L.key = 0;
-    L.flag = al256(L.key + sizeof(int32_t) * (size_t)n);
-    L.opos = al256(L.flag + sizeof(int32_t) * (size_t)n);
-    L.hist = al256(L.opos + sizeof(int32_t) * (size_t)n);
-    L.cur = al256(L.hist + sizeof(int32_t) * (size_t)nbins);
-    L.tmp = al256(L.cur + sizeof(int32_t) * (size_t)nbins);
+    L.rank = al256(L.key + sizeof(int32_t) * (size_t)n);
+    L.hist = al256(L.rank + sizeof(int32_t) * (size_t)n);
+    L.start = al256(L.hist + sizeof(int32_t) * (size_t)nk);
+    L.tmp = al256(L.start + sizeof(int32_t) * (size_t)nk);
     L.tmp_bytes = tmp;
     L.total = al256(L.tmp + tmp);
     return L;
@@ -544,20 +548,18 @@ extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_
     WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", L.total, scratch_bytes);
     char* base = static_cast<char*>(scratch);
     int32_t* key = reinterpret_cast<int32_t*>(base + L.key);
-    int32_t* flag = reinterpret_cast<int32_t*>(base + L.flag);
-    int32_t* opos = reinterpret_cast<int32_t*>(base + L.opos);
+    int32_t* rank = reinterpret_cast<int32_t*>(base + L.rank);
     int32_t* hist = reinterpret_cast<int32_t*>(base + L.hist);
-    int32_t* cur = reinterpret_cast<int32_t*>(base + L.cur);
+    int32_t* start = reinterpret_cast<int32_t*>(base + L.start);
     hipStream_t s = wsb::as_stream(stream);
-    hipLaunchKernelGGL(zero_kernel, dim3((nbins + 255) / 256), dim3(256), 0, s, hist, nbins);
-    hipLaunchKernelGGL(cell_count_kernel, dim3((n + 255) / 256), dim3(256), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx,
-                       g->ny, key, flag, hist);
+    const int nk = sort_keys(n, nbins);
+    hipLaunchKernelGGL(zero_kernel, dim3((nk + 255) / 256), dim3(256), 0, s, hist, nk);
+    hipLaunchKernelGGL(cell_rank_kernel, dim3((n + 255) / 256), dim3(256), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx,
+                       g->ny, key, rank, hist);
     WSB_HIP(hipGetLastError());
     size_t tmp = L.tmp_bytes;
-    WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, hist, cur, nbins, s));
-    tmp = L.tmp_bytes;
-    WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, flag, opos, n, s));
-    hipLaunchKernelGGL(cell_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key, flag, opos, n, cur, order);
+    WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, hist, start, nk, s));
+    hipLaunchKernelGGL(cell_place_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key, rank, start, n, order);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
