@@ -1,0 +1,77 @@
+"""Representations lowered to surface loops on the device (reference scene/scene.py:459-535, 558-596, 970-996 and
+its tests/scene/test_box_rep.py, test_fused_implicit.py): a BoxRep scene gives the results of the scene with its
+polygon bitwise (the reference's own gate is its table error, 1e-8 on lam and up to 3e-6 on the gradients; the
+lowering is exact), inside / signed distance equal; a disk cavity equals the reference's tier-2 polygon (area
+preserving, edges <= h/16) bitwise; a solid disk's lowering converges to the disk: full support deep inside the
+fluid, the exact signed distance, and the moments of the n-gon within 2e-4 of those of the 8n-gon."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from warpsphboundaries_amd.fused import FusedGroup, FusedWall, WallOutput
+from warpsphboundaries_amd.scene import (Body, BoxRep, DiskBody, ImplicitRep, ParticleState, Scene, SurfaceRep,
+                                         fixed_adjacency)
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+OUTS = (WallOutput("lam", 0, "lam"), WallOutput("g", 0, "g0"), WallOutput("cov", 0, "cov"))
+
+
+def _eval(rep, pos, h, kernel="w2", center=(0.05, -0.02), angle=0.3):
+    sc = Scene([Body(bodyId=0, reps=[rep], center=center, angle=angle)], DEV)
+    ps = ParticleState(torch.tensor(pos, device=DEV), torch.full((len(pos),), h, dtype=torch.float64, device=DEV))
+    return FusedWall(sc, fixed_adjacency(sc, ps, None, h), (FusedGroup(kernel),)).evaluate(OUTS), sc
+
+
+@pytest.mark.parametrize("solid", ["inside", "outside"])
+@pytest.mark.parametrize("kernel", ["w2", "w4", "cone"])
+def test_box_rep_equals_its_polygon(solid, kernel):
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-0.2, 1.2, (3000, 2))
+    h = 0.05
+    a, sa = _eval(BoxRep((0.1, 0.2), (0.8, 0.6), solid), pos, h, kernel)
+    b, sb = _eval(SurfaceRep.box((0.1, 0.2), (0.8, 0.6), solid), pos, h, kernel)
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
+    assert float(a["lam"].abs().max()) > 0.5
+    pts = torch.tensor(pos, device=DEV)
+    assert torch.equal(sa.inside(pts), sb.inside(pts))
+    da, na, _ = sa.signed_distance(pts, supportMax=h)
+    db, nb, _ = sb.signed_distance(pts, supportMax=h)
+    assert torch.equal(da, db) and torch.equal(na, nb)
+
+
+def test_disk_cavity_equals_the_tier2_polygon():
+    rng = np.random.default_rng(6)
+    R, h, c = 0.3, 0.02, (0.5, 0.5)
+    ang = rng.uniform(0, 2 * math.pi, 4000)
+    rad = R * np.sqrt(rng.uniform(0, 1, 4000))
+    pos = np.stack([c[0] + rad * np.cos(ang), c[1] + rad * np.sin(ang)], 1)
+    a, _ = _eval(ImplicitRep(DiskBody(c, R, "outside")), pos, h, center=(0.0, 0.0), angle=0.0)
+    n = max(24, math.ceil(2 * math.pi * R / (h / 16)))
+    b, _ = _eval(SurfaceRep.regularPolygon(c, R, n, solid="outside"), pos, h, center=(0.0, 0.0), angle=0.0)
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
+    deep = rad < R - 1.01 * h
+    assert torch.all(a["lam"][0][torch.tensor(deep, device=DEV)] == 0.0)      # the fluid inside: no wall within h
+
+
+def test_solid_disk_lowering_converges_and_signed_distance():
+    rng = np.random.default_rng(7)
+    R, h, c = 0.2, 0.02, (0.3, 0.4)
+    ang = rng.uniform(0, 2 * math.pi, 3000)
+    rad = R + rng.uniform(-1.5 * h, 1.5 * h, 3000)
+    pos = np.stack([c[0] + rad * np.cos(ang), c[1] + rad * np.sin(ang)], 1)
+    a, sa = _eval(ImplicitRep(DiskBody(c, R)), pos, h, center=(0.0, 0.0), angle=0.0)
+    n = max(24, math.ceil(2 * math.pi * R / (h / 16)))
+    fine, _ = _eval(SurfaceRep.regularPolygon(c, R, 8 * n), pos, h, center=(0.0, 0.0), angle=0.0)
+    for k in ("lam", "g", "cov"):
+        d = float((a[k] - fine[k]).abs().max())
+        assert d <= 2e-4 * max(float(fine[k].abs().max()), 1.0), (k, d)
+    inside = torch.tensor(rad < R - 1.01 * h, device=DEV)
+    assert torch.allclose(a["lam"][0][inside], torch.ones_like(a["lam"][0][inside]), rtol=0, atol=1e-12)
+    d, nrm, _ = sa.signed_distance(torch.tensor(pos, device=DEV))
+    np.testing.assert_allclose(d.cpu().numpy(), rad - R, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(nrm.cpu().numpy(), np.stack([np.cos(ang), np.sin(ang)], 1), rtol=0, atol=1e-12)

@@ -1,0 +1,49 @@
+"""BoxRep / ImplicitRep lowering on the host (reference scene/scene.py:459-535, 558-596, 970-996): argument checks,
+the tier-2 polygon's edge count n = max(24, ceil(2 pi R / (h_min / 16))) and area preservation, the unbounded flag
+of fluid-domain representations.  No device calls."""
+import math
+
+import numpy as np
+import pytest
+
+from warpsphboundaries_amd.scene import Body, BoxRep, DiskBody, ImplicitRep, SurfaceRep, TierPolicy
+
+
+def _area(P):
+    x, y = P[:, 0], P[:, 1]
+    return 0.5 * float(np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y))
+
+
+def test_box_rep_checks_and_polygon():
+    with pytest.raises(ValueError):
+        BoxRep((0, 0), (1, 1), "sideways")
+    with pytest.raises(ValueError):
+        BoxRep((0, 0), (0, 1))
+    b = BoxRep((0.1, 0.2), (0.7, 0.5))
+    P = b.surface().vertices.numpy()
+    assert sorted(map(tuple, P)) == sorted([(0.1, 0.2), (0.7, 0.2), (0.7, 0.5), (0.1, 0.5)])
+    assert _area(P) == pytest.approx(0.6 * 0.3, rel=1e-15)          # counter-clockwise around the solid
+    t = BoxRep((0, 0), (1, 1), "outside")
+    assert t.surface().background == 1 and _area(t.surface().vertices.numpy()) < 0
+    assert Body(reps=[t]).unbounded() and not Body(reps=[b]).unbounded()
+
+
+@pytest.mark.parametrize("R,hmin,n", [(0.3, 0.02, 1508), (0.003, 0.02, 24), (0.25, 0.01, 2514)])
+def test_disk_tier2_polygon(R, hmin, n):
+    rep = ImplicitRep(DiskBody((0.4, 0.5), R, "outside"))
+    assert math.ceil(2 * math.pi * R / (hmin / 16)) == n or n == 24
+    poly = rep.lowered(hmin, "cpu")
+    P = poly.vertices.numpy()
+    assert len(P) == n and poly.background == 1
+    assert abs(_area(P)) == pytest.approx(math.pi * R * R, rel=1e-13)             # area preserving
+    assert rep.lowered(hmin, "cpu") is poly                                         # cached per edge count
+    solid = ImplicitRep(DiskBody((0.4, 0.5), R)).lowered(hmin, "cpu")
+    assert solid.background == 0 and _area(solid.vertices.numpy()) > 0
+    assert Body(reps=[rep]).unbounded() and not Body(reps=[ImplicitRep(DiskBody((0, 0), R))]).unbounded()
+
+
+def test_implicit_rep_rejects_other_shapes_and_uses_the_policy():
+    with pytest.raises(NotImplementedError):
+        ImplicitRep(object())
+    coarse = ImplicitRep(DiskBody((0, 0), 0.3), TierPolicy(maxPolygonEdgeOverH=0.5))
+    assert len(coarse.lowered(0.02, "cpu").vertices) == math.ceil(2 * math.pi * 0.3 / 0.01)

@@ -25,7 +25,7 @@ import torch
 from . import _lib as L
 from . import precision, triangle
 from .kernels import P5_NAMES
-from .scene import FixedTopology, SurfaceRep, kernelName
+from .scene import BoxRep, FixedTopology, ImplicitRep, SurfaceRep, kernelName
 
 F64 = torch.float64
 KINDS = {"lam": (0, 1), "g0": (1, 2), "cov": (2, 4), "a1g1": (3, 2), "lap": (4, 1), "m1": (5, 2), "m2": (6, 4)}
@@ -181,4 +181,7 @@ class FusedWall(WallAggregate):
 
     @staticmethod
     def supported(scene, lowering=True):
-        return all(isinstance(r, SurfaceRep) for b in scene.bodies for r in b.reps)
+        """every representation evaluates exactly: surface loops, and (with ``lowering``) boxes and disks through
+        their polygons."""
+        kinds = (SurfaceRep, BoxRep, ImplicitRep) if lowering else (SurfaceRep,)
+        return all(isinstance(r, kinds) for b in scene.bodies for r in b.reps)

@@ -175,6 +175,125 @@ class SurfaceRep:
 
 
 @dataclass
+class BoxRep:
+    """an axis-aligned rectangle [lo, hi] of the body frame: the solid (``solid='inside'``) or the fluid domain
+    (``'outside'``: a tank).  The reference (scene/scene.py:459-535, scene/box.py) evaluates it from Chebyshev corner
+    tables (error to 3e-7 on the gradient channels); here it is lowered to its 4-edge polygon (``surface()``), whose
+    closed form is exact, so a box and its polygon give the same results."""
+    kind = "box"
+
+    def __init__(self, lo, hi, solid: str = "inside"):
+        if solid not in ("inside", "outside"):
+            raise ValueError("solid must be 'inside' or 'outside'")
+        self.lo = torch.as_tensor(lo, dtype=F64)
+        self.hi = torch.as_tensor(hi, dtype=F64)
+        self.lo_h = tuple(float(v) for v in self.lo)
+        self.hi_h = tuple(float(v) for v in self.hi)
+        if not (self.hi_h[0] > self.lo_h[0] and self.hi_h[1] > self.lo_h[1]):
+            raise ValueError("BoxRep needs hi > lo on both axes")
+        self.solid = solid
+        self._surface = None
+
+    def surface(self):
+        if self._surface is None:
+            self._surface = SurfaceRep.box(self.lo_h, self.hi_h, self.solid).to(self.lo.device)
+        return self._surface
+
+    def to(self, device):
+        self.lo, self.hi = self.lo.to(device), self.hi.to(device)
+        if self._surface is not None:
+            self._surface.to(device)
+        return self
+
+    def bounds(self):
+        return self.lo, self.hi
+
+    def lowered(self, hmin, device):
+        return self.surface().to(device)
+
+    def indicator(self, p, supports=None):
+        return self.surface().indicator(p, supports)
+
+
+@dataclass
+class TierPolicy:
+    """the reference's tier thresholds (scene/implicitBodies.py:292-297); only the tier-2 polygon's edge bound is used
+    here: implicit shapes are lowered to polygons whose edges are at most ``maxPolygonEdgeOverH`` times the smallest
+    support."""
+    tier4MaxRadius: float = 0.2
+    tier3MinRadius: float = 2.0
+    maxPolygonEdgeOverH: float = 1.0 / 16.0
+
+
+@dataclass
+class DiskBody:
+    """a disk: ``solid='inside'`` an obstacle, ``'outside'`` a cavity wall with the fluid inside the circle
+    (reference scene/implicitBodies.py:311-316)."""
+    center: tuple
+    radius: float
+    solid: str = "inside"
+    bodyId: int = 0
+
+
+class ImplicitRep:
+    """an analytic primitive of the body frame (reference scene/scene.py:558-596).  A disk is lowered to the
+    reference's tier-2 polygon: area preserving, ``n = max(24, ceil(2 pi R / (maxPolygonEdgeOverH hmin)))`` edges
+    (``fallbackSurface``, scene/scene.py:588-596).  The reference evaluates a SOLID disk from its disk table instead
+    (DiskArrayRep, edge/warpdisk.py), which this lowering approximates to the polygon's accuracy; other primitives
+    have no lowering and raise NotImplementedError."""
+    kind = "implicit"
+
+    def __init__(self, shape, policy=None):
+        if not isinstance(shape, DiskBody):
+            raise NotImplementedError("implicit representations: disks only (the half plane has no polygon)")
+        if shape.solid not in ("inside", "outside"):
+            raise ValueError("solid must be 'inside' or 'outside'")
+        self.shape = shape
+        self.policy = policy or TierPolicy()
+        self._fallback = {}
+        self._device = "cpu"
+
+    @property
+    def solid(self):
+        return self.shape.solid
+
+    def bounds(self):
+        c = torch.as_tensor(self.shape.center, dtype=F64)
+        return c - self.shape.radius, c + self.shape.radius
+
+    def to(self, device):
+        self._device = device
+        for rep in self._fallback.values():
+            rep.to(device)
+        return self
+
+    def fallbackSurface(self, hmin, device):
+        s = self.shape
+        n = max(24, int(np.ceil(2 * np.pi * s.radius / (self.policy.maxPolygonEdgeOverH * float(hmin)))))
+        if n not in self._fallback:
+            self._fallback[n] = SurfaceRep.regularPolygon(s.center, s.radius, n, solid=s.solid).to(device)
+        return self._fallback[n]
+
+    def lowered(self, hmin, device):
+        return self.fallbackSurface(hmin, device)
+
+    def signed(self, lpos):
+        """(d, n): exact signed distance of the circle (positive in the fluid) and the unit normal into the fluid,
+        body frame (reference scene/implicitBodies.py:317-330)."""
+        c = torch.as_tensor(self.shape.center, dtype=F64, device=lpos.device)
+        v = lpos.to(F64) - c
+        r = v.norm(dim=1).clamp(min=1e-300)
+        e = v / r[:, None]
+        if self.shape.solid == "inside":
+            return r - self.shape.radius, e
+        return self.shape.radius - r, -e
+
+    def indicator(self, p, supports=None):
+        """1 in the solid, 0 in the fluid (the circle itself: 1/2)."""
+        d, _ = self.signed(p)
+        return torch.where(d < 0, 1.0, torch.where(d > 0, 0.0, 0.5)).to(F64)
+
+
 class SurfaceTopology:
     qi: torch.Tensor
     e: torch.Tensor
@@ -224,7 +343,13 @@ class Body:
         return self.pose.toLocal(world)
 
     def unbounded(self):
-        return any(isinstance(r, SurfaceRep) and r.background for r in self.reps)
+        return any((isinstance(r, SurfaceRep) and r.background) or getattr(r, "solid", "inside") == "outside"
+                   for r in self.reps)
+
+    def fusedReps(self, hmin, device):
+        """the surface loops this body is evaluated with: a SurfaceRep as it is, a BoxRep its polygon, an
+        ImplicitRep its tier-2 polygon for the smallest support ``hmin`` (reference scene/scene.py:970-996)."""
+        return [r if isinstance(r, SurfaceRep) else r.lowered(hmin, device) for r in self.reps]
 
     def obb(self):
         los, his = zip(*[r.bounds() for r in self.reps])
@@ -293,10 +418,15 @@ class Scene:
                 continue
             lp = bd.toLocal(pts)
             for rep in bd.reps:
-                if not isinstance(rep, SurfaceRep):
+                if isinstance(rep, BoxRep):
+                    rep = rep.surface()
+                if isinstance(rep, ImplicitRep):
+                    d, n_loc = rep.signed(lp)
+                elif isinstance(rep, SurfaceRep):
+                    smax = float(supportMax) if supportMax is not None else _default_support(rep)
+                    d, n_loc, _ = rep.geom(smax, pts.device).signed_distance(lp, smax)
+                else:
                     continue
-                smax = float(supportMax) if supportMax is not None else _default_support(rep)
-                d, n_loc, _ = rep.geom(smax, pts.device).signed_distance(lp, smax)
                 n = bd.pose.vecToWorld(n_loc)
                 better = d < best
                 best = torch.where(better, d, best)
@@ -348,11 +478,13 @@ def fixed_adjacency(scene: Scene, queryParticles, operationProperties=None, supp
             d = (lo - lpos).clamp(min=0) + (lpos - hi).clamp(min=0)
             valid = (d.norm(dim=1) < sup) & allowed
         reps = []
-        for rep in body.reps:
+        hmin = float(sup.min()) if any(isinstance(r, ImplicitRep) for r in body.reps) and N else float(supportMax)
+        surfaces = body.fusedReps(hmin, dev)
+        for rep in surfaces:
             g = rep.geom(float(supportMax), dev)
             slots, counts, ind = g.adjacency(lpos, sup, valid=valid, pad=True)
             reps.append(FixedTopology(slots, counts, g.K, ind, g))
-        bodies.append(BodyAdjacency(body, cand, lpos, sup, reps, valid, list(body.reps)))
+        bodies.append(BodyAdjacency(body, cand, lpos, sup, reps, valid, surfaces))
     return SceneAdjacency(N, bodies, {"candidates": None}, queryParticles, float(supportMax))
 
 
