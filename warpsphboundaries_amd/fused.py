@@ -58,6 +58,30 @@ class WallAggregate:
     """marker base of the wall aggregates a scheme consumes (``.out``, ``evaluate``)."""
 
 
+def rep_channels(rep, topo, ba, kernel, form, R, out=None, accumulate=False):
+    """[N, 12] world-frame physical channels (device order) of one surface representation of a body over its
+    fixed-capacity adjacency: the closed form, or for a conditioned triangle its quadrature on the rows where the
+    triangle is small or a sliver at their support (reference scene/fused.py:556-564); added to ``out`` when
+    ``accumulate``."""
+    if not isinstance(rep, SurfaceRep) or not isinstance(topo, FixedTopology):
+        raise NotImplementedError("FusedWall: surface representations with the fixed-capacity adjacency only")
+    if rep._triangle is None:
+        flags = L.OUT_PHYSICAL | L.OUT_WORLD | (L.OUT_ACCUMULATE if accumulate else 0)
+        return topo.geom.moments(kernel, topo.e, topo.counts, topo.ind, ba.lpos, ba.lsup, form=form, rot4=R,
+                                 flags=flags, out=out)
+    flags = L.OUT_PHYSICAL | L.OUT_WORLD
+    own = topo.geom.moments(kernel, topo.e, topo.counts, topo.ind, ba.lpos, ba.lsup, form=form, rot4=R, flags=flags)
+    triangle.override(rep.vertices, rep._triangle, kernel, rep.background, ba.lpos, ba.lsup, valid=ba.valid, rot4=R,
+                      flags=flags, out=own)
+    if out is None:
+        return own
+    if accumulate:
+        out += own
+    else:
+        out.copy_(own)
+    return out
+
+
 class FusedWall(WallAggregate):
     def __init__(self, scene, adjacency, groups):
         self.scene = scene
@@ -75,22 +99,7 @@ class FusedWall(WallAggregate):
                 R = ba.body.pose.Rflat().to(dev, F64).contiguous()
                 reps = ba.repList if ba.repList is not None else ba.body.reps
                 for rep, topo in zip(reps, ba.reps):
-                    if not isinstance(rep, SurfaceRep) or not isinstance(topo, FixedTopology):
-                        raise NotImplementedError("FusedWall: surface representations with the fixed-capacity adjacency only")
-                    if rep._triangle is None:
-                        flags = L.OUT_PHYSICAL | L.OUT_WORLD | L.OUT_ACCUMULATE
-                        topo.geom.moments(grp.kernel, topo.e, topo.counts, topo.ind, ba.lpos, ba.lsup, form=grp.form,
-                                          rot4=R, flags=flags, out=self.channels[gi, bi])
-                    else:
-                        # a conditioned triangle: its rows where the triangle is small or a sliver at their support
-                        # take the quadrature in place of this representation's closed form (reference
-                        # scene/fused.py:556-564), then the representation accumulates like the others
-                        flags = L.OUT_PHYSICAL | L.OUT_WORLD
-                        own = topo.geom.moments(grp.kernel, topo.e, topo.counts, topo.ind, ba.lpos, ba.lsup,
-                                                form=grp.form, rot4=R, flags=flags)
-                        triangle.override(rep.vertices, rep._triangle, grp.kernel, rep.background, ba.lpos, ba.lsup,
-                                          valid=ba.valid, rot4=R, flags=flags, out=own)
-                        self.channels[gi, bi] += own
+                    rep_channels(rep, topo, ba, grp.kernel, grp.form, R, self.channels[gi, bi], accumulate=True)
             if grp.channels is not None:
                 keep = torch.zeros(12, dtype=F64, device=dev)
                 keep[list(grp.channels)] = 1.0

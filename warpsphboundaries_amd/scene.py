@@ -392,6 +392,19 @@ class Scene:
             for r in b.reps:
                 r.to(device)
 
+    def precompute(self, adjacency, operationProperties, channels=None, form=None):
+        """per-representation moments of one kernel (``pairmoments.PairMoments``; reference scene/scene.py:1418)."""
+        from .pairmoments import precompute
+        return precompute(self, adjacency, operationProperties, channels, form)
+
+    def pairMoments(self, queryParticles, operationProperties, channels=None, form=None):
+        from .pairmoments import pair_moments
+        return pair_moments(self, queryParticles, operationProperties, channels, form)
+
+    def moments(self, source, queryParticles, operationProperties, channels=None, form=None):
+        from .pairmoments import moments
+        return moments(self, source, queryParticles, operationProperties, channels, form)
+
     def inside(self, points, body=None):
         pts = points.to(self.device, F64)
         out = torch.zeros(len(pts), dtype=torch.bool, device=pts.device)
