@@ -57,6 +57,7 @@ enum wsb_out_flags {
 };
 
 typedef struct wsb_geom2d wsb_geom2d;
+typedef struct wsb_disks2d wsb_disks2d;
 typedef struct wsb_plan wsb_plan;
 
 int wsb_last_error(char* buf, size_t n);
@@ -164,6 +165,33 @@ int wsb_triangle2d(const double* tri, int background, int kernel_id, const doubl
 int wsb_triangle2d_host(const double* tri, int background, int kernel_id, const double* lpos, const double* lsup,
                         const uint8_t* valid, int n, double tolerance, int max_panels, double* values, double* estimate,
                         int32_t* status);
+
+/* ---- solid disks of one body frame: a fibre bundle (reference DiskArrayRep, scene/scene.py:808-888) or one solid
+ * disk (ImplicitRep of a DiskBody, scene/scene.py:576-587).  centres[m*2], radii[m] (host).  Builds the static cell
+ * list of the centres (cell = support_max + the largest radius; replaces DiskArrayRep.cells, scene/scene.py:869-888)
+ * and uploads it; *K = the most disks in any 3 x 3 block of cells (slot capacity per particle). */
+int wsb_disks2d_create(const double* centres, const double* radii, int m, double support_max, wsb_disks2d** out,
+                       int* K);
+int wsb_disks2d_destroy(wsb_disks2d* g);
+/* info[5]: K, nx, ny, m, device bytes */
+int wsb_disks2d_info(const wsb_disks2d* g, int64_t* info);
+/* slots[n*K]: the disks with |centre - x| < h + R in the 3 x 3 cells around the particle's cell, cell by cell (x outer,
+ * y inner) in cell-list order, -1 padded; counts[n] (nullable) the filled slots; valid[n] (nullable) 0 = no disks.
+ * Replaces fixed_disk_topology / _disk_slots_kernel (scene/fixedadj.py:199-246); bit-exact with it. */
+int wsb_disk_adjacency2d(const wsb_disks2d* g, const double* lpos, const double* lsup, const uint8_t* valid, int n,
+                         int32_t* slots, int32_t* counts, void* stream);
+/* out[n*12]: the twelve channels (device order, units of h unless WSB_OUT_PHYSICAL) of the solid disks of each row's
+ * slots, summed in slot order; each disk is complete (its solid, indicator included), so WSB_OUT_NO_INDICATOR is
+ * implied.  FP64 radial Gauss quadrature of the arc-angle form (every kernel id, W^5 included).  Replaces
+ * warpdisk.disk_channels_device + FusedWall._add_disks (edge/warpdisk.py:98-199, scene/fused.py:567-575), whose
+ * five table channels (lam, m1, g0, g1 along / across the centre direction) it extends by the second moments. */
+int wsb_disk_moments2d(const wsb_disks2d* g, int kernel_id, const int32_t* slots, const int32_t* counts, int K,
+                       const double* lpos, const double* lsup, int n, const double* rot4, int flags, double* out,
+                       void* stream);
+
+/* the seven channels of one disk (lam, m1 and g0 along the centre direction, g1 and m2 along / across it; units of h)
+ * at centre distances D[n] and radii R[n] in units of h, on the host (validation of the device arithmetic) */
+int wsb_disk_channels_host(int kernel_id, const double* D, const double* R, int n, double* out);
 
 /* ---- batched evaluation across many resident geometries (continuous batching of a boundary server; no counterpart
  * in the reference).  wsb_geom2d_desc writes wsb_geom2d_desc_bytes() bytes describing a geometry (grid parameters and

@@ -6,7 +6,7 @@ import math
 import numpy as np
 import pytest
 
-from warpsphboundaries_amd.scene import Body, BoxRep, DiskBody, ImplicitRep, SurfaceRep, TierPolicy
+from warpsphboundaries_amd.scene import Body, BoxRep, DiskArrayRep, DiskBody, ImplicitRep, SurfaceRep, TierPolicy
 
 
 def _area(P):
@@ -37,8 +37,12 @@ def test_disk_tier2_polygon(R, hmin, n):
     assert len(P) == n and poly.background == 1
     assert abs(_area(P)) == pytest.approx(math.pi * R * R, rel=1e-13)             # area preserving
     assert rep.lowered(hmin, "cpu") is poly                                         # cached per edge count
-    solid = ImplicitRep(DiskBody((0.4, 0.5), R)).lowered(hmin, "cpu")
-    assert solid.background == 0 and _area(solid.vertices.numpy()) > 0
+    solid_rep = ImplicitRep(DiskBody((0.4, 0.5), R))
+    solid = solid_rep.lowered(hmin, "cpu")                                         # a solid disk: the disk element
+    assert isinstance(solid, DiskArrayRep) and solid.background == 0 and solid_rep.lowered(hmin, "cpu") is solid
+    assert solid.centres.tolist() == [[0.4, 0.5]] and solid.radii.tolist() == [R]
+    fb = solid_rep.fallbackSurface(hmin, "cpu")
+    assert fb.background == 0 and _area(fb.vertices.numpy()) > 0
     assert Body(reps=[rep]).unbounded() and not Body(reps=[ImplicitRep(DiskBody((0, 0), R))]).unbounded()
 
 
@@ -46,4 +50,23 @@ def test_implicit_rep_rejects_other_shapes_and_uses_the_policy():
     with pytest.raises(NotImplementedError):
         ImplicitRep(object())
     coarse = ImplicitRep(DiskBody((0, 0), 0.3), TierPolicy(maxPolygonEdgeOverH=0.5))
-    assert len(coarse.lowered(0.02, "cpu").vertices) == math.ceil(2 * math.pi * 0.3 / 0.01)
+    assert len(coarse.fallbackSurface(0.02, "cpu").vertices) == math.ceil(2 * math.pi * 0.3 / 0.01)
+
+
+def test_disk_array_rep_host_side():
+    """DiskArrayRep argument checks, bounds, exact signed distance and indicator on the host (reference
+    scene/scene.py:808-867)."""
+    with pytest.raises(ValueError):
+        DiskArrayRep([(0, 0), (1, 1)], [0.1])
+    with pytest.raises(ValueError):
+        DiskArrayRep(np.zeros((0, 2)), [])
+    import torch
+    rep = DiskArrayRep([(0.0, 0.0), (1.0, 0.5)], [0.25, 0.5])
+    lo, hi = rep.bounds()
+    assert lo.tolist() == [-0.25, -0.25] and hi.tolist() == [1.5, 1.0]
+    p = torch.tensor([[0.0, 0.5], [1.0, 0.5], [0.6, 0.5], [0.25, 0.0]], dtype=torch.float64)
+    d, n = rep.signed(p)
+    assert d.tolist() == pytest.approx([0.25, -0.5, -0.1, 0.0], abs=1e-15)
+    assert n[0].tolist() == [0.0, 1.0] and n[2].tolist() == [-1.0, 0.0]
+    assert rep.indicator(p).tolist() == [0.0, 1.0, 1.0, 0.5]
+    assert not Body(reps=[rep]).unbounded() and Body(reps=[rep]).fusedReps(0.01, "cpu")[0] is rep

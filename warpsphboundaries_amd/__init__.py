@@ -4,7 +4,7 @@ closed polygonal walls at fluid particles, on hand-written HIP kernels for gfx95
 Public surface (the reference's names):
     AnalyticBoundary, BoundaryProvider                     provider protocol (scene/provider.py)
     Scene, Body, SurfaceRep, ParticleState, fixed_adjacency scene layer
-    BoxRep, ImplicitRep, DiskBody, TierPolicy              representations lowered to surface loops
+    BoxRep, ImplicitRep, DiskBody, TierPolicy, DiskArrayRep representations (boxes as polygons, solid disks as the disk element)
     FusedWall, FusedGroup, WallOutput                      fused evaluation
     edge_channels, second_channels, indicator_vector       pair-list evaluators
     value, gradient, moment, moment_gradient               polygon element API
@@ -21,7 +21,7 @@ def __getattr__(name):
     if name in ("AnalyticBoundary", "BoundaryProvider"):
         from . import provider
         return getattr(provider, name)
-    if name in ("Scene", "Body", "SurfaceRep", "ParticleState", "fixed_adjacency", "Pose", "BoxRep", "ImplicitRep",
+    if name in ("Scene", "Body", "SurfaceRep", "ParticleState", "fixed_adjacency", "Pose", "BoxRep", "ImplicitRep", "DiskArrayRep",
                 "DiskBody", "TierPolicy"):
         from . import scene
         return getattr(scene, name)

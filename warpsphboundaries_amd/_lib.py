@@ -62,6 +62,13 @@ SIGNATURES = {
     "wsb_gather_rows": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "wsb_scatter_rows": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "wsb_fill_rows_i32": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "wsb_disks2d_create": (c_int, [c_void_p, c_void_p, c_int, c_double, POINTER(c_void_p), POINTER(c_int)]),
+    "wsb_disks2d_destroy": (c_int, [c_void_p]),
+    "wsb_disks2d_info": (c_int, [c_void_p, POINTER(c_int64)]),
+    "wsb_disk_adjacency2d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "wsb_disk_moments2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int,
+                                   c_void_p, c_void_p]),
+    "wsb_disk_channels_host": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "wsb_mesh3d_create": (c_int, [c_void_p, c_int, c_void_p, c_int, c_double, c_int, POINTER(c_void_p), POINTER(c_int64)]),
     "wsb_mesh3d_destroy": (c_int, [c_void_p]),
     "wsb_mesh3d_info": (c_int, [c_void_p, POINTER(c_int64)]),

@@ -9,7 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["api.hip", "geom2d.hip", "moments2d.hip", "batch2d.hip", "tri2d.hip", "sphere3d.hip", "mesh3d.hip"]
+SOURCES = ["api.hip", "geom2d.hip", "moments2d.hip", "batch2d.hip", "tri2d.hip", "sphere3d.hip", "mesh3d.hip", "disk2d.hip"]
 OUT = os.path.join(HERE, "libwsb.so")
 OBJDIR = os.path.join(HERE, "_obj")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

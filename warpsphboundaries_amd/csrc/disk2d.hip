@@ -1,0 +1,448 @@
+// Solid disks of a body frame (a fibre bundle, or one solid disk): the static cell list over the disk centres, the
+// fixed-capacity slot fill of every particle, and the twelve channels of every (particle, disk) pair, FP64.
+//
+// Channels of one disk (h = 1, centre at distance D along the unit vector e from the particle, radius R, tangent t):
+// with y the offset from the particle, r = |y| and alpha(r) the half angle of the arc of the circle |y| = r inside the
+// disk (pi where the whole circle is inside, 0 where none of it is), the polar integrals reduce to one radial
+// integral each:
+//     lam  = int W                =  int_0^1 W(r) 2 alpha r dr
+//     m1   = int y W              =  e int_0^1 W(r) 2 sin(alpha) r^2 dr
+//     g0   = int grad_x W         = -e int_0^1 W'(r) 2 sin(alpha) r dr
+//     g1   = int y (x) grad_x W   = -int_0^1 W'(r) r^2 [(alpha + sin cos) e e^T + (alpha - sin cos) t t^T] dr
+//     m2   = int y (x) y W        =  int_0^1 W(r) r^3 [(alpha + sin cos) e e^T + (alpha - sin cos) t t^T] dr
+// cos(alpha) = (r^2 + D^2 - R^2) / (2 r D).  alpha has square-root ends at r = |D - R| and r = D + R, and varies on the
+// scale |D - R| next to the first: the radial range is cut at those radii and at the kernel knots, refined
+// geometrically (ratio 4) away from both square-root points, and each panel takes 16 Gauss-Legendre nodes, in t^2
+// (resp. (1 - t)^2) on a panel that ends at a square-root point; the W^5 kernels take panels of at most 1/16.  1 -+ cos(alpha) are formed from the products
+// (r - |D - R|)(D + R - r) etc. so that they keep their relative accuracy next to the ends.  A disk that covers the
+// support (D <= R - 1) gives the whole-support values, one out of reach (D >= 1 + R) nothing.
+//
+// Replaces the reference's table lookup of the disk element (edge/warpdisk.py:98-131, its tables built by
+// edge/disk.py:113-157 from the divergence theorem on the circle) and the slot fill of scene/fixedadj.py:199-246.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "edge2d_device.h"
+#include "wsb_internal.h"
+
+struct wsb_disks2d {
+    int device = 0;
+    int m = 0;
+    double support_max = 0, cell = 0, lo0 = 0, lo1 = 0;
+    int nx = 1, ny = 1, K = 1;
+    int64_t bytes = 0;
+    double4* rec = nullptr;      // [m] (cx, cy, R, 0) by disk index
+    int32_t* cstart = nullptr;   // [nx*ny+1]
+    int32_t* citems = nullptr;   // [m] disk indices by cell, ascending index within a cell
+};
+
+namespace {
+
+constexpr int DISK_NODES = 16;
+
+template <int KID>
+__host__ __device__ __forceinline__ void disk_profile(double r, double& P, double& dP) {
+    if constexpr (KID == WSB_W2P5 || KID == WSB_W4P5) {
+        double radial;
+        chord::p5_wr<KID>(r, P, radial);
+        dP = -radial * r;
+    } else {
+        // pi W = u^K q(r), u = HI - r, on the piece of r: P' = u^(K-1) (u q' - K q)
+        using T = DTab<KID>;
+        const int p = (T::NP == 2 && r >= T::HI[0]) ? 1 : 0;
+        double q = T::Q[p][T::DEG], dq = 0.0;
+#pragma unroll
+        for (int n = T::DEG - 1; n >= 0; --n) {
+            dq = dq * r + q;
+            q = q * r + T::Q[p][n];
+        }
+        const int K = T::K[p];
+        const double u = T::HI[p] - r;
+        double uk1 = 1.0;
+        for (int k = 1; k < K; ++k) uk1 *= u;
+        const double pv = (K > 0) ? uk1 * u * q : q;
+        const double dv = (K > 0) ? uk1 * (u * dq - (double)K * q) : dq;
+        P = (r < 1.0) ? pv : 0.0;
+        dP = (r < 1.0) ? dv : 0.0;
+    }
+}
+
+// v: lam, m1 (along e), g0 (along e), g1 (e e), g1 (t t), m2 (e e), m2 (t t) of one disk, units of h
+template <int KID>
+__host__ __device__ void disk_channels(double D, double R, double v[7]) {
+    using T = DTab<KID>;
+    // the W^5 kernels (degree 25 / 40, every power of u = 1 - r up to 30) take panels of at most 1/16
+    constexpr double MAXW = (KID == WSB_W2P5 || KID == WSB_W4P5) ? 0.0625 : 2.0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = 0.0;
+    if (!(D < 1.0 + R)) return;
+    if (D <= R - 1.0) {
+        v[0] = v[3] = v[4] = T::MASS;
+        v[5] = v[6] = T::SECOND;
+        return;
+    }
+    const double a = fabs(D - R), b = D + R;
+    const bool inside = D < R;
+    const double pi = 3.1415926535897932384626433832795;
+    // r < a inside the disk: the whole circle (alpha = pi); polynomial in r, exact with 16 nodes between knots
+    if (inside) {
+        const double end = fmin(a, 1.0);
+        double lo = 0.0;
+        for (int it = 0; it < 64 && lo < end; ++it) {
+            double hi = fmin(end, lo + MAXW);
+            if (T::NP == 2 && lo < T::HI[0]) hi = fmin(hi, T::HI[0]);
+            const double w = hi - lo;
+            for (int j = 0; j < DISK_NODES; ++j) {
+                const double r = lo + w * GLTab::X16[j], wg = w * GLTab::W16[j];
+                double P, dP;
+                disk_profile<KID>(r, P, dP);
+                v[0] += wg * P * 2.0 * pi * r;
+                const double g = -wg * dP * r * r * pi;
+                v[3] += g;
+                v[4] += g;
+                const double s2 = wg * P * r * r * r * pi;
+                v[5] += s2;
+                v[6] += s2;
+            }
+            lo = hi;
+        }
+    }
+    if (a < 1.0) {
+        const bool bend = b <= 1.0;                  // the outer square-root point lies inside the support
+        const double width = b - a;
+        const double send = fmin(b, 1.0) - a;        // the arc range in s = r - a
+        double knot = -1.0;                          // kernel knot inside the arc range
+        if (T::NP == 2 && T::HI[0] > a && T::HI[0] < fmin(b, 1.0)) knot = T::HI[0] - a;
+        const bool tiny = a <= 1e-15 * send;
+        double cur = 0.0;
+        for (int it = 0; it < 200 && cur < send; ++it) {
+            const double nh = (knot > cur) ? knot : send;
+            // a panel's other singular points stay at least a third of its width away (half of it for a panel in
+            // t^2 or (1 - t)^2, whose map halves the distance near its regular end)
+            double nxt;
+            int kind;
+            if (cur == 0.0) {
+                kind = 1;
+                nxt = fmin(tiny ? nh : fmin(nh, 2.0 * a), MAXW);
+                nxt = (bend && nxt == send) ? 0.5 * send : fmin(nxt, 0.5 * width);
+            } else {
+                kind = 0;
+                nxt = fmin(fmin(nh, 4.0 * cur), cur + MAXW);
+                if (bend && nxt == send && send <= 2.0 * cur) kind = 2;
+                else nxt = fmin(nxt, 0.25 * (cur + 3.0 * width));
+            }
+            if (!(nxt > cur) || it == 199) {         // rounding left no room: close the range in one panel
+                nxt = send;
+                kind = bend ? 2 : 0;
+            }
+            const double wd = nxt - cur;
+            for (int j = 0; j < DISK_NODES; ++j) {
+                const double t = GLTab::X16[j], wq = GLTab::W16[j];
+                double s, bmr, wg;
+                if (kind == 1) {
+                    s = wd * t * t;
+                    bmr = width - s;
+                    wg = 2.0 * wd * t * wq;
+                } else if (kind == 2) {
+                    const double u = 1.0 - t, du = wd * u * u;
+                    s = nxt - du;
+                    bmr = du;
+                    wg = 2.0 * wd * u * wq;
+                } else {
+                    s = cur + wd * t;
+                    bmr = width - s;
+                    wg = wd * wq;
+                }
+                const double r = a + s;
+                const double i2rD = 0.5 / (r * D);
+                double omc, opc;                    // 1 - cos(alpha), 1 + cos(alpha)
+                if (inside) {
+                    omc = bmr * (r + a) * i2rD;
+                    opc = s * (r + b) * i2rD;
+                } else {
+                    omc = s * bmr * i2rD;
+                    opc = (r + a) * (r + b) * i2rD;
+                }
+                const double al = 2.0 * atan2(sqrt(omc), sqrt(opc));
+                const double sc = sqrt(omc * opc) * 0.5 * (opc - omc);
+                const double sa = sqrt(omc * opc);
+                double P, dP;
+                disk_profile<KID>(r, P, dP);
+                const double wP = wg * P, wdP = wg * dP, rr = r * r;
+                v[0] += wP * 2.0 * al * r;
+                v[1] += wP * 2.0 * sa * rr;
+                v[2] -= wdP * 2.0 * sa * r;
+                v[3] -= wdP * rr * (al + sc);
+                v[4] -= wdP * rr * (al - sc);
+                v[5] += wP * rr * r * (al + sc);
+                v[6] += wP * rr * r * (al - sc);
+            }
+            cur = nxt;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] *= WSB_INV_PI;
+}
+
+// one thread per row: the disks of the 3 x 3 cells around the row's cell, cell by cell (x outer, y inner) and in
+// cell-list order, kept when |centre - x| < h + R (reference scene/fixedadj.py:199-222)
+__global__ void disk_slots_kernel(const double4* __restrict__ rec, const int32_t* __restrict__ cstart,
+                                  const int32_t* __restrict__ citems, double lo0, double lo1, double cell, int nx, int ny,
+                                  int K, const double* __restrict__ lpos, const double* __restrict__ lsup,
+                                  const uint8_t* __restrict__ valid, int n, int32_t* __restrict__ slots,
+                                  int32_t* __restrict__ counts) {
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int32_t* out = slots + (size_t)r * K;
+    int cnt = 0;
+    if (!valid || valid[r]) {
+        const double px = lpos[2 * r], py = lpos[2 * r + 1], h = lsup[r];
+        const double fx = floor((px - lo0) / cell), fy = floor((py - lo1) / cell);
+        if (fx >= -1.0 && fx <= (double)nx && fy >= -1.0 && fy <= (double)ny) {
+            const int cx = (int)fx, cy = (int)fy;
+            for (int ddx = -1; ddx <= 1; ++ddx) {
+                const int ix = cx + ddx;
+                if (ix < 0 || ix >= nx) continue;
+                for (int ddy = -1; ddy <= 1; ++ddy) {
+                    const int iy = cy + ddy;
+                    if (iy < 0 || iy >= ny) continue;
+                    const int cid = ix * ny + iy;
+                    for (int i = cstart[cid]; i < cstart[cid + 1]; ++i) {
+                        const int m = citems[i];
+                        const double4 c = rec[m];
+                        const double ex = px - c.x, ey = py - c.y;
+                        const double ex2 = ex * ex, ey2 = ey * ey;
+                        if (sqrt(ex2 + ey2) < h + c.z && cnt < K) out[cnt++] = m;
+                    }
+                }
+            }
+        }
+    }
+    if (counts) counts[r] = cnt;
+    for (int s = cnt; s < K; ++s) out[s] = -1;
+}
+
+// one thread per row: the row's disks in slot order, each disk's channels turned into the body frame, then the
+// physical scaling, the world rotation and the store of finalize_row (the disk is complete: no indicator term)
+template <int KID>
+__global__ void __launch_bounds__(256) disk_moments_kernel(const double4* __restrict__ rec,
+                                                           const int32_t* __restrict__ slots,
+                                                           const int32_t* __restrict__ counts, int K,
+                                                           const double* __restrict__ lpos,
+                                                           const double* __restrict__ lsup, int n,
+                                                           const double* __restrict__ rot4, int flags,
+                                                           double* __restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double c[wsb::NCH];
+#pragma unroll
+    for (int k = 0; k < wsb::NCH; ++k) c[k] = 0.0;
+    const double px = lpos[2 * r], py = lpos[2 * r + 1], h = lsup[r], ih = 1.0 / h;
+    const int cnt = counts ? min(counts[r], K) : K;
+    for (int s = 0; s < cnt; ++s) {
+        const int m = slots[(size_t)r * K + s];
+        if (m < 0) continue;
+        const double4 d = rec[m];
+        const double yx = (d.x - px) * ih, yy = (d.y - py) * ih;
+        const double D = sqrt(yx * yx + yy * yy);
+        double v[7];
+        disk_channels<KID>(D, d.z * ih, v);
+        double ex = 1.0, ey = 0.0;
+        if (D > 1e-12) {
+            ex = yx / D;
+            ey = yy / D;
+        }
+        const double tx = -ey, ty = ex;
+        c[0] += v[0];
+        c[1] += v[1] * ex;
+        c[2] += v[1] * ey;
+        c[3] += v[2] * ex;
+        c[4] += v[2] * ey;
+        const double gxy = v[3] * ex * ey + v[4] * tx * ty;
+        c[5] += v[3] * ex * ex + v[4] * tx * tx;
+        c[6] += gxy;
+        c[7] += gxy;
+        c[8] += v[3] * ey * ey + v[4] * ty * ty;
+        c[9] += v[5] * ex * ex + v[6] * tx * tx;
+        c[10] += v[5] * ex * ey + v[6] * tx * ty;
+        c[11] += v[5] * ey * ey + v[6] * ty * ty;
+    }
+    finalize_row(c, 0.0, h, nullptr, rot4, flags | WSB_OUT_NO_INDICATOR, out + (size_t)r * wsb::NCH);
+}
+
+template <int KID>
+void launch_moments(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K, const double* lpos,
+                    const double* lsup, int n, const double* rot4, int flags, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(disk_moments_kernel<KID>, dim3((n + 255) / 256), dim3(256), 0, st, g->rec, slots, counts, K,
+                       lpos, lsup, n, rot4, flags, out);
+}
+
+template <int KID>
+void host_channels(const double* D, const double* R, int n, double* out) {
+    for (int i = 0; i < n; ++i) disk_channels<KID>(D[i], R[i], out + 7 * (size_t)i);
+}
+
+template <class T>
+int upload(T** dst, const T* src, size_t count, int64_t* bytes) {
+    const size_t nb = std::max<size_t>(count, 1) * sizeof(T);
+    WSB_HIP(hipMalloc(dst, nb));
+    if (count) WSB_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    *bytes += (int64_t)nb;
+    return WSB_OK;
+}
+
+}  // namespace
+
+// the static cell list of the centres (reference scene/scene.py:869-888): cell = support_max + the largest radius over
+// the centres' box grown by one cell, disks by cell (stable, so ascending index within a cell), K = the most disks of
+// any 3 x 3 block of cells (at least 1)
+extern "C" int wsb_disks2d_create(const double* centres, const double* radii, int m, double support_max,
+                                  wsb_disks2d** out, int* K) {
+    WSB_REQUIRE(centres && radii && out, WSB_EINVAL, "null argument");
+    WSB_REQUIRE(m >= 1, WSB_EINVAL, "a disk array needs at least one disk (m=%d)", m);
+    WSB_REQUIRE(support_max > 0 && std::isfinite(support_max), WSB_EINVAL, "support_max must be positive and finite");
+    double rmax = 0.0, lo0 = centres[0], lo1 = centres[1], hi0 = centres[0], hi1 = centres[1];
+    for (int i = 0; i < m; ++i) {
+        WSB_REQUIRE(std::isfinite(centres[2 * i]) && std::isfinite(centres[2 * i + 1]), WSB_EINVAL, "non-finite centre");
+        WSB_REQUIRE(radii[i] > 0 && std::isfinite(radii[i]), WSB_EINVAL, "disk %d: radius must be positive and finite", i);
+        rmax = std::max(rmax, radii[i]);
+        lo0 = std::min(lo0, centres[2 * i]);
+        lo1 = std::min(lo1, centres[2 * i + 1]);
+        hi0 = std::max(hi0, centres[2 * i]);
+        hi1 = std::max(hi1, centres[2 * i + 1]);
+    }
+    const double cell = support_max + rmax;
+    lo0 -= cell;
+    lo1 -= cell;
+    hi0 += cell;
+    hi1 += cell;
+    const double dx = std::max(std::floor((hi0 - lo0) / cell) + 1.0, 1.0);
+    const double dy = std::max(std::floor((hi1 - lo1) / cell) + 1.0, 1.0);
+    WSB_REQUIRE(dx * dy <= (double)(1 << 28), WSB_EINVAL,
+                "disk cell grid of %.0f x %.0f cells (support_max %g, largest radius %g) exceeds 2^28 cells", dx, dy,
+                support_max, rmax);
+    const int nx = (int)dx, ny = (int)dy;
+    std::vector<int64_t> cid(m);
+    for (int i = 0; i < m; ++i) {
+        const int64_t cx = (int64_t)std::floor((centres[2 * i] - lo0) / cell);
+        const int64_t cy = (int64_t)std::floor((centres[2 * i + 1] - lo1) / cell);
+        cid[i] = cx * ny + cy;
+    }
+    std::vector<int32_t> order(m);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t p, int32_t q) { return cid[p] < cid[q]; });
+    const size_t ncell = (size_t)nx * ny;
+    std::vector<int32_t> count(ncell, 0), start(ncell + 1, 0);
+    for (int i = 0; i < m; ++i) count[cid[i]] += 1;
+    for (size_t k = 0; k < ncell; ++k) start[k + 1] = start[k] + count[k];
+    int Kmax = 1;
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iy = 0; iy < ny; ++iy) {
+            int tot = 0;
+            for (int a = std::max(ix - 1, 0); a <= std::min(ix + 1, nx - 1); ++a)
+                for (int b = std::max(iy - 1, 0); b <= std::min(iy + 1, ny - 1); ++b) tot += count[(size_t)a * ny + b];
+            Kmax = std::max(Kmax, tot);
+        }
+    std::vector<double4> rec(m);
+    for (int i = 0; i < m; ++i) rec[i] = double4{centres[2 * i], centres[2 * i + 1], radii[i], 0.0};
+    auto* g = new wsb_disks2d();
+    WSB_HIP(hipGetDevice(&g->device));
+    g->m = m;
+    g->support_max = support_max;
+    g->cell = cell;
+    g->lo0 = lo0;
+    g->lo1 = lo1;
+    g->nx = nx;
+    g->ny = ny;
+    g->K = Kmax;
+    int st = upload(&g->rec, rec.data(), rec.size(), &g->bytes);
+    if (!st) st = upload(&g->cstart, start.data(), start.size(), &g->bytes);
+    if (!st) st = upload(&g->citems, order.data(), order.size(), &g->bytes);
+    if (st) {
+        wsb_disks2d_destroy(g);
+        return st;
+    }
+    *out = g;
+    if (K) *K = Kmax;
+    return WSB_OK;
+}
+
+extern "C" int wsb_disks2d_destroy(wsb_disks2d* g) {
+    if (!g) return WSB_OK;
+    for (void* p : {(void*)g->rec, (void*)g->cstart, (void*)g->citems})
+        if (p) (void)hipFree(p);
+    delete g;
+    return WSB_OK;
+}
+
+extern "C" int wsb_disks2d_info(const wsb_disks2d* g, int64_t* info) {
+    WSB_REQUIRE(g && info, WSB_EINVAL, "null argument");
+    info[0] = g->K;
+    info[1] = g->nx;
+    info[2] = g->ny;
+    info[3] = g->m;
+    info[4] = g->bytes;
+    return WSB_OK;
+}
+
+extern "C" int wsb_disk_adjacency2d(const wsb_disks2d* g, const double* lpos, const double* lsup, const uint8_t* valid,
+                                    int n, int32_t* slots, int32_t* counts, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null disk array");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(lpos && lsup && slots, WSB_EINVAL, "null buffer");
+    hipLaunchKernelGGL(disk_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), g->rec,
+                       g->cstart, g->citems, g->lo0, g->lo1, g->cell, g->nx, g->ny, g->K, lpos, lsup, valid, n, slots,
+                       counts);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+extern "C" int wsb_disk_moments2d(const wsb_disks2d* g, int kernel_id, const int32_t* slots, const int32_t* counts,
+                                  int K, const double* lpos, const double* lsup, int n, const double* rot4, int flags,
+                                  double* out, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null disk array");
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes (n=%d K=%d)", n, K);
+    WSB_REQUIRE(!(flags & ~(WSB_OUT_ACCUMULATE | WSB_OUT_PHYSICAL | WSB_OUT_WORLD | WSB_OUT_NO_INDICATOR)), WSB_EINVAL,
+                "unknown flags %d", flags);
+    WSB_REQUIRE(!(flags & WSB_OUT_WORLD) || rot4, WSB_EINVAL, "WSB_OUT_WORLD needs rot4");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(slots && lpos && lsup && out, WSB_EINVAL, "null buffer");
+    const hipStream_t st = wsb::as_stream(stream);
+    switch (kernel_id) {
+        case WSB_CUBIC: launch_moments<WSB_CUBIC>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        case WSB_W2: launch_moments<WSB_W2>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        case WSB_W4: launch_moments<WSB_W4>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        case WSB_W6: launch_moments<WSB_W6>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        case WSB_CONE: launch_moments<WSB_CONE>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        case WSB_LW2: launch_moments<WSB_LW2>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        case WSB_LW4: launch_moments<WSB_LW4>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        case WSB_W2P5: launch_moments<WSB_W2P5>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        case WSB_W4P5: launch_moments<WSB_W4P5>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
+        default: return wsb::set_error(WSB_EINVAL, "unknown kernel id %d", kernel_id);
+    }
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+extern "C" int wsb_disk_channels_host(int kernel_id, const double* D, const double* R, int n, double* out) {
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative count");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(D && R && out, WSB_EINVAL, "null buffer");
+    switch (kernel_id) {
+        case WSB_CUBIC: host_channels<WSB_CUBIC>(D, R, n, out); break;
+        case WSB_W2: host_channels<WSB_W2>(D, R, n, out); break;
+        case WSB_W4: host_channels<WSB_W4>(D, R, n, out); break;
+        case WSB_W6: host_channels<WSB_W6>(D, R, n, out); break;
+        case WSB_CONE: host_channels<WSB_CONE>(D, R, n, out); break;
+        case WSB_LW2: host_channels<WSB_LW2>(D, R, n, out); break;
+        case WSB_LW4: host_channels<WSB_LW4>(D, R, n, out); break;
+        case WSB_W2P5: host_channels<WSB_W2P5>(D, R, n, out); break;
+        case WSB_W4P5: host_channels<WSB_W4P5>(D, R, n, out); break;
+        default: return wsb::set_error(WSB_EINVAL, "unknown kernel id %d", kernel_id);
+    }
+    return WSB_OK;
+}

@@ -132,6 +132,45 @@ def render_triangle():
         out.append(f"  static constexpr double C = {float(kernel(name).c2_pi)!r};")
         out.append("};")
     out.append("")
+    # the kernel as pieces (disk element): pi W = (HI[p] - r)^K[p] sum_n Q[p][n] r^n on [HI[p-1], HI[p]) with the
+    # largest K that divides the piece exactly (the Wendland factor (1 - r)^k: Q then has positive coefficients and the
+    # evaluation no cancellation), whole-support mass and int y_x^2 W; the W^5 kernels are evaluated from their
+    # Wendland shape (DEG = -1)
+    from warpsphboundaries_amd.kernels import plane_mass, plane_second_moment
+    out.append("template <int KID> struct DTab;")
+    for name in ALL_NAMES:
+        k = kernel(name)
+        p5 = name in P5_NAMES
+        pcs = k.pieces
+        rows, Ks, deg = [], [], -1
+        for lo, hi, c in pcs:
+            q, K = list(c), 0
+            while not p5 and len(q) > 1:
+                # divide by (hi - r): synthetic division at r = hi, exact in rationals
+                quo, acc = [F(0)] * (len(q) - 1), F(0)
+                for n in range(len(q) - 1, 0, -1):
+                    acc = acc * hi + q[n]
+                    quo[n - 1] = acc
+                rem = acc * hi + q[0]
+                if rem != 0:
+                    break
+                q, K = [-v for v in quo], K + 1
+            rows.append(_arrn([] if p5 else q, 12))
+            Ks.append(K)
+            deg = -1 if p5 else max(deg, len(q) - 1)
+        if len(pcs) == 1:
+            rows.append(_arrn([], 12))
+            Ks.append(0)
+        out.append(f"template <> struct DTab<{KERNEL_ID[name]}> {{   // {name}")
+        out.append(f"  static constexpr int NP = {len(pcs)};")
+        out.append(f"  static constexpr int DEG = {deg};")
+        out.append(f"  static constexpr double HI[2] = {{{', '.join(repr(float(hi)) for _, hi, _ in pcs)}{', 1.0' if len(pcs) == 1 else ''}}};")
+        out.append(f"  static constexpr int K[2] = {{{Ks[0]}, {Ks[1]}}};")
+        out.append(f"  static constexpr double Q[2][12] = {{{', '.join(rows)}}};")
+        out.append(f"  static constexpr double MASS = {float(plane_mass(name))!r};")
+        out.append(f"  static constexpr double SECOND = {float(plane_second_moment(name))!r};")
+        out.append("};")
+    out.append("")
     out.append("struct GLTab {")
     for n in (8, 16, 24, 32):
         x, w = np.polynomial.legendre.leggauss(n)

@@ -171,6 +171,76 @@ class Geom2D:
         return d, nrm, edge
 
 
+class Disks2D:
+    """static device structures of a set of solid disks for one support_max (the cell list over the centres)."""
+
+    def __init__(self, centres: np.ndarray, radii: np.ndarray, support_max: float, device_index: int):
+        c = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 2)
+        r = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+        if len(c) != len(r):
+            raise ValueError("DiskArrayRep: centres and radii differ in length")
+        self.device_index = device_index
+        self.support_max = float(support_max)
+        h = ctypes.c_void_p()
+        K = ctypes.c_int()
+        with torch.cuda.device(device_index):
+            L.check(L.load().wsb_disks2d_create(c.ctypes.data, r.ctypes.data, len(r), float(support_max), ctypes.byref(h),
+                                                ctypes.byref(K)))
+        self.handle = h
+        self.K = int(K.value)
+        info = (ctypes.c_int64 * 5)()
+        L.check(L.load().wsb_disks2d_info(self.handle, info))
+        self.info = dict(K=info[0], nx=info[1], ny=info[2], m=info[3], device_bytes=info[4])
+
+    def __del__(self):
+        try:
+            if self.handle:
+                L.load().wsb_disks2d_destroy(self.handle)
+        except Exception:
+            pass
+
+    def adjacency(self, lpos, lsup, valid=None, slots=None, counts=None):
+        """(slots [N*K] int32 (-1 padded), counts [N] int32): the disks within one support plus radius of every row."""
+        L.require_device(lpos, lsup)
+        n = lpos.shape[0]
+        dev = lpos.device
+        lpos = lpos.to(torch.float64).contiguous()
+        lsup = lsup.to(torch.float64).contiguous()
+        if slots is None:
+            slots = torch.empty(n * self.K, dtype=torch.int32, device=dev)
+        if counts is None:
+            counts = torch.empty(n, dtype=torch.int32, device=dev)
+        v = None if valid is None else valid.to(torch.uint8).contiguous()
+        L.check(L.load().wsb_disk_adjacency2d(self.handle, L.ptr(lpos), L.ptr(lsup), L.ptr(v), n, L.ptr(slots),
+                                              L.ptr(counts), L.stream_ptr(dev)))
+        return slots, counts
+
+    def moments(self, kernel, slots, counts, lpos, lsup, rot4=None, flags=0, out=None):
+        """[N, 12] twelve channels per row (device order) of the row's disks, see include/wsb.h for the flags."""
+        L.require_device(lpos, lsup, slots)
+        n = lpos.shape[0]
+        dev = lpos.device
+        if out is None:
+            out = torch.zeros((n, 12), dtype=torch.float64, device=dev) if flags & L.OUT_ACCUMULATE \
+                else torch.empty((n, 12), dtype=torch.float64, device=dev)
+        lpos = lpos.to(torch.float64).contiguous()
+        lsup = lsup.to(torch.float64).contiguous()
+        L.check(L.load().wsb_disk_moments2d(self.handle, KERNEL_ID[kernel], L.ptr(slots), L.ptr(counts), self.K,
+                                            L.ptr(lpos), L.ptr(lsup), n, L.ptr(rot4), int(flags), L.ptr(out),
+                                            L.stream_ptr(dev)))
+        return out
+
+
+def disk_channels_host(kernel, D, R):
+    """[n, 7] (lam, m1, g0 along the centre direction; g1, m2 along / across it) of single disks, units of h, computed
+    on the host by the library's own arithmetic (validation)."""
+    D = np.ascontiguousarray(D, dtype=np.float64).reshape(-1)
+    R = np.ascontiguousarray(np.broadcast_to(R, D.shape), dtype=np.float64)
+    out = np.zeros((len(D), 7))
+    L.check(L.load().wsb_disk_channels_host(KERNEL_ID[kernel], L.nptr(D), L.nptr(R), len(D), L.nptr(out)))
+    return out
+
+
 def edge_channels12(pair_q, pair_e, positions, supports, vertices, edges, kernel, form="fp64"):
     """[P, 12] edge-local channels (units of h) of an explicit pair list on the device."""
     L.require_device(pair_q, pair_e, positions, supports, vertices, edges)

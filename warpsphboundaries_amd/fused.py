@@ -25,7 +25,7 @@ import torch
 from . import _lib as L
 from . import precision, triangle
 from .kernels import P5_NAMES
-from .scene import BoxRep, FixedTopology, ImplicitRep, SurfaceRep, kernelName
+from .scene import BoxRep, DiskArrayRep, FixedTopology, ImplicitRep, SurfaceRep, kernelName
 
 F64 = torch.float64
 KINDS = {"lam": (0, 1), "g0": (1, 2), "cov": (2, 4), "a1g1": (3, 2), "lap": (4, 1), "m1": (5, 2), "m2": (6, 4)}
@@ -59,10 +59,14 @@ class WallAggregate:
 
 
 def rep_channels(rep, topo, ba, kernel, form, R, out=None, accumulate=False):
-    """[N, 12] world-frame physical channels (device order) of one surface representation of a body over its
-    fixed-capacity adjacency: the closed form, or for a conditioned triangle its quadrature on the rows where the
-    triangle is small or a sliver at their support (reference scene/fused.py:556-564); added to ``out`` when
+    """[N, 12] world-frame physical channels (device order) of one representation of a body over its fixed-capacity
+    adjacency: for a surface loop the closed form, or for a conditioned triangle its quadrature on the rows where the
+    triangle is small or a sliver at their support (reference scene/fused.py:556-564); for solid disks the disk
+    element of every slot (FP64 whatever the form; reference scene/fused.py:567-575); added to ``out`` when
     ``accumulate``."""
+    if isinstance(rep, DiskArrayRep) and isinstance(topo, FixedTopology):
+        flags = L.OUT_PHYSICAL | L.OUT_WORLD | (L.OUT_ACCUMULATE if accumulate else 0)
+        return topo.geom.moments(kernel, topo.e, topo.counts, ba.lpos, ba.lsup, rot4=R, flags=flags, out=out)
     if not isinstance(rep, SurfaceRep) or not isinstance(topo, FixedTopology):
         raise NotImplementedError("FusedWall: surface representations with the fixed-capacity adjacency only")
     if rep._triangle is None:
@@ -80,6 +84,11 @@ def rep_channels(rep, topo, ba, kernel, form, R, out=None, accumulate=False):
     else:
         out.copy_(own)
     return out
+
+
+def _surface_only(rep, what):
+    if isinstance(rep, DiskArrayRep):
+        raise NotImplementedError(f"FusedWall.{what}: surface and box representations only (no disk element form)")
 
 
 class FusedWall(WallAggregate):
@@ -159,6 +168,7 @@ class FusedWall(WallAggregate):
         ax = axes.to(dev, F64).contiguous()
         lib = L.load()
         for bi, ba, rep, topo, c, s in self._reps():
+            _surface_only(rep, "cone_area")
             L.check(lib.wsb_cone_area2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K, L.ptr(topo.ind),
                                         L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(ax), c, s, float(half_angle),
                                         L.ptr(out), L.stream_ptr(dev)))
@@ -180,6 +190,7 @@ class FusedWall(WallAggregate):
         v = vecs.to(dev, F64).contiguous()
         lib = L.load()
         for bi, ba, rep, topo, c, s in self._reps():
+            _surface_only(rep, "dir_extreme")
             seg = out[bi * N:(bi + 1) * N]
             L.check(lib.wsb_dir_extreme2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K, L.ptr(topo.ind),
                                           L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(v[bi]), c, s, L.ptr(seg),
@@ -190,7 +201,7 @@ class FusedWall(WallAggregate):
 
     @staticmethod
     def supported(scene, lowering=True):
-        """every representation evaluates exactly: surface loops, and (with ``lowering``) boxes and disks through
-        their polygons."""
-        kinds = (SurfaceRep, BoxRep, ImplicitRep) if lowering else (SurfaceRep,)
+        """every representation evaluates exactly: surface loops and disk arrays, and (with ``lowering``) boxes through
+        their polygons and disks through the disk element (a cavity through its polygon)."""
+        kinds = (SurfaceRep, DiskArrayRep, BoxRep, ImplicitRep) if lowering else (SurfaceRep, DiskArrayRep)
         return all(isinstance(r, kinds) for b in scene.bodies for r in b.reps)

@@ -17,7 +17,7 @@ import torch
 
 from . import _lib as L
 from . import triangle
-from .device import Geom2D
+from .device import Disks2D, Geom2D
 from .kernels import KERNEL_ID, plane_channels
 
 F64 = torch.float64
@@ -236,11 +236,10 @@ class DiskBody:
 
 
 class ImplicitRep:
-    """an analytic primitive of the body frame (reference scene/scene.py:558-596).  A disk is lowered to the
+    """an analytic primitive of the body frame (reference scene/scene.py:558-596).  A SOLID disk is evaluated as a
+    one-disk ``DiskArrayRep`` (the disk element, any radius; reference scene/scene.py:576-587), a cavity as the
     reference's tier-2 polygon: area preserving, ``n = max(24, ceil(2 pi R / (maxPolygonEdgeOverH hmin)))`` edges
-    (``fallbackSurface``, scene/scene.py:588-596).  The reference evaluates a SOLID disk from its disk table instead
-    (DiskArrayRep, edge/warpdisk.py), which this lowering approximates to the polygon's accuracy; other primitives
-    have no lowering and raise NotImplementedError."""
+    (``fallbackSurface``, scene/scene.py:588-596); other primitives have no lowering and raise NotImplementedError."""
     kind = "implicit"
 
     def __init__(self, shape, policy=None):
@@ -251,6 +250,7 @@ class ImplicitRep:
         self.shape = shape
         self.policy = policy or TierPolicy()
         self._fallback = {}
+        self._disks = None
         self._device = "cpu"
 
     @property
@@ -265,7 +265,15 @@ class ImplicitRep:
         self._device = device
         for rep in self._fallback.values():
             rep.to(device)
+        if self._disks is not None:
+            self._disks.to(device)
         return self
+
+    def disks(self):
+        """a solid disk as a one-disk ``DiskArrayRep`` (cached)."""
+        if self._disks is None:
+            self._disks = DiskArrayRep([self.shape.center], [self.shape.radius]).to(self._device)
+        return self._disks
 
     def fallbackSurface(self, hmin, device):
         s = self.shape
@@ -275,6 +283,8 @@ class ImplicitRep:
         return self._fallback[n]
 
     def lowered(self, hmin, device):
+        if self.shape.solid == "inside":
+            return self.disks().to(device)
         return self.fallbackSurface(hmin, device)
 
     def signed(self, lpos):
@@ -292,6 +302,72 @@ class ImplicitRep:
         """1 in the solid, 0 in the fluid (the circle itself: 1/2)."""
         d, _ = self.signed(p)
         return torch.where(d < 0, 1.0, torch.where(d > 0, 0.0, 0.5)).to(F64)
+
+
+class DiskArrayRep:
+    """a set of solid disks (fibres) of the body frame: centres [M, 2], radii [M]; the disks must not overlap (the
+    solid is their union and the integrals add).  Every (particle, disk) pair is one closed evaluation of the disk
+    element (``Disks2D.moments``; no polygon), the disks near a particle come from a cell list over the centres with a
+    fixed capacity (``Disks2D.adjacency``) (reference scene/scene.py:808-888, scene/fixedadj.py:199-246)."""
+    kind = "disks"
+
+    def __init__(self, centres, radii):
+        self.centres = torch.as_tensor(centres, dtype=F64).reshape(-1, 2)
+        self.radii = torch.as_tensor(radii, dtype=F64).reshape(-1)
+        if len(self.centres) != len(self.radii):
+            raise ValueError("DiskArrayRep: centres and radii differ in length")
+        if len(self.radii) == 0:
+            raise ValueError("DiskArrayRep: no disks")
+        self._geoms = {}
+        self._bounds = None
+
+    solid = "inside"
+    background = 0
+
+    def to(self, device):
+        self.centres, self.radii = self.centres.to(device), self.radii.to(device)
+        self._bounds = None
+        return self
+
+    def bounds(self):
+        if self._bounds is None:
+            r = self.radii[:, None]
+            self._bounds = ((self.centres - r).amin(0), (self.centres + r).amax(0))
+        return self._bounds
+
+    def geom(self, support_max: float, device) -> Disks2D:
+        dev = torch.device(device)
+        key = (float(support_max), dev.index or 0)
+        if key not in self._geoms:
+            self._geoms[key] = Disks2D(self.centres.cpu().numpy(), self.radii.cpu().numpy(), float(support_max),
+                                       dev.index or 0)
+        return self._geoms[key]
+
+    def signed(self, pos):
+        """(d, n): signed distance (positive in the fluid) to the nearest circle and the unit normal into the fluid
+        there, body frame (reference scene/scene.py:853-867), in chunks of 4096 points."""
+        N = len(pos)
+        out_d = torch.full((N,), float("inf"), dtype=F64, device=pos.device)
+        out_n = torch.zeros((N, 2), dtype=F64, device=pos.device)
+        cen = self.centres.to(pos.device)
+        rad = self.radii.to(pos.device)
+        for lo in range(0, N, 4096):
+            v = pos[lo:lo + 4096, None, :].to(F64) - cen[None]
+            dist = v.norm(dim=2)
+            d = dist - rad[None]
+            k = d.argmin(1)
+            ar = torch.arange(len(d), device=pos.device)
+            out_d[lo:lo + 4096] = d[ar, k]
+            out_n[lo:lo + 4096] = v[ar, k] / dist[ar, k].clamp(min=1e-300)[:, None]
+        return out_d, out_n
+
+    def indicator(self, p, supports=None):
+        """1 in a disk, 0 in the fluid (on a circle: 1/2)."""
+        d, _ = self.signed(p)
+        return torch.where(d < 0, 1.0, torch.where(d > 0, 0.0, 0.5)).to(F64)
+
+    def lowered(self, hmin, device):
+        return self
 
 
 class SurfaceTopology:
@@ -347,8 +423,9 @@ class Body:
                    for r in self.reps)
 
     def fusedReps(self, hmin, device):
-        """the surface loops this body is evaluated with: a SurfaceRep as it is, a BoxRep its polygon, an
-        ImplicitRep its tier-2 polygon for the smallest support ``hmin`` (reference scene/scene.py:970-996)."""
+        """the representations this body is evaluated with: a SurfaceRep or DiskArrayRep as it is, a BoxRep its
+        polygon, a solid ImplicitRep disk a one-disk DiskArrayRep, a cavity its tier-2 polygon for the smallest support
+        ``hmin`` (reference scene/scene.py:970-996)."""
         return [r if isinstance(r, SurfaceRep) else r.lowered(hmin, device) for r in self.reps]
 
     def obb(self):
@@ -433,7 +510,7 @@ class Scene:
             for rep in bd.reps:
                 if isinstance(rep, BoxRep):
                     rep = rep.surface()
-                if isinstance(rep, ImplicitRep):
+                if isinstance(rep, (ImplicitRep, DiskArrayRep)):
                     d, n_loc = rep.signed(lp)
                 elif isinstance(rep, SurfaceRep):
                     smax = float(supportMax) if supportMax is not None else _default_support(rep)
@@ -495,7 +572,11 @@ def fixed_adjacency(scene: Scene, queryParticles, operationProperties=None, supp
         surfaces = body.fusedReps(hmin, dev)
         for rep in surfaces:
             g = rep.geom(float(supportMax), dev)
-            slots, counts, ind = g.adjacency(lpos, sup, valid=valid, pad=True)
+            if isinstance(rep, DiskArrayRep):
+                slots, counts = g.adjacency(lpos, sup, valid=valid)
+                ind = torch.zeros(N, dtype=F64, device=dev)            # the disk element is complete
+            else:
+                slots, counts, ind = g.adjacency(lpos, sup, valid=valid, pad=True)
             reps.append(FixedTopology(slots, counts, g.K, ind, g))
         bodies.append(BodyAdjacency(body, cand, lpos, sup, reps, valid, surfaces))
     return SceneAdjacency(N, bodies, {"candidates": None}, queryParticles, float(supportMax))
