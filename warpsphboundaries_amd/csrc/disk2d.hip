@@ -69,9 +69,10 @@ __host__ __device__ __forceinline__ void disk_profile(double r, double& P, doubl
     }
 }
 
-// v: lam, m1 (along e), g0 (along e), g1 (e e), g1 (t t), m2 (e e), m2 (t t) of one disk, units of h
+// v: lam, m1 (along e), g0 (along e), g1 (e e), g1 (t t), m2 (e e), m2 (t t) of one disk, units of h, from the
+// Gauss nodes j0, j0 + jstep, ... of every panel (the host takes them all, a 16-lane group one node per lane)
 template <int KID>
-__host__ __device__ void disk_channels(double D, double R, double v[7]) {
+__host__ __device__ void disk_channels_part(double D, double R, int j0, int jstep, double v[7]) {
     using T = DTab<KID>;
     // the W^5 kernels (degree 25 / 40, every power of u = 1 - r up to 30) take panels of at most 1/16
     constexpr double MAXW = (KID == WSB_W2P5 || KID == WSB_W4P5) ? 0.0625 : 2.0;
@@ -79,9 +80,11 @@ __host__ __device__ void disk_channels(double D, double R, double v[7]) {
     for (int k = 0; k < 7; ++k) v[k] = 0.0;
     if (!(D < 1.0 + R)) return;
     if (D <= R - 1.0) {
-        v[0] = v[3] = v[4] = T::MASS;
-        v[5] = v[6] = T::SECOND;
-        return;
+        if (j0 == 0) {
+            v[0] = v[3] = v[4] = T::MASS * 3.1415926535897932384626433832795;
+            v[5] = v[6] = T::SECOND * 3.1415926535897932384626433832795;
+        }
+        return;                                      // (the caller's 1/pi gives the whole-support values)
     }
     const double a = fabs(D - R), b = D + R;
     const bool inside = D < R;
@@ -94,7 +97,7 @@ __host__ __device__ void disk_channels(double D, double R, double v[7]) {
             double hi = fmin(end, lo + MAXW);
             if (T::NP == 2 && lo < T::HI[0]) hi = fmin(hi, T::HI[0]);
             const double w = hi - lo;
-            for (int j = 0; j < DISK_NODES; ++j) {
+            for (int j = j0; j < DISK_NODES; j += jstep) {
                 const double r = lo + w * GLTab::X16[j], wg = w * GLTab::W16[j];
                 double P, dP;
                 disk_profile<KID>(r, P, dP);
@@ -138,7 +141,7 @@ __host__ __device__ void disk_channels(double D, double R, double v[7]) {
                 kind = bend ? 2 : 0;
             }
             const double wd = nxt - cur;
-            for (int j = 0; j < DISK_NODES; ++j) {
+            for (int j = j0; j < DISK_NODES; j += jstep) {
                 const double t = GLTab::X16[j], wq = GLTab::W16[j];
                 double s, bmr, wg;
                 if (kind == 1) {
@@ -182,6 +185,11 @@ __host__ __device__ void disk_channels(double D, double R, double v[7]) {
             cur = nxt;
         }
     }
+}
+
+template <int KID>
+__host__ __device__ void disk_channels(double D, double R, double v[7]) {
+    disk_channels_part<KID>(D, R, 0, 1, v);
 #pragma unroll
     for (int k = 0; k < 7; ++k) v[k] *= WSB_INV_PI;
 }
@@ -225,8 +233,9 @@ __global__ void disk_slots_kernel(const double4* __restrict__ rec, const int32_t
     for (int s = cnt; s < K; ++s) out[s] = -1;
 }
 
-// one thread per row: the row's disks in slot order, each disk's channels turned into the body frame, then the
-// physical scaling, the world rotation and the store of finalize_row (the disk is complete: no indicator term)
+// a group of 16 lanes per row: the row's disks in slot order, each disk's panels with one Gauss node per lane and a
+// fixed butterfly sum over the group, the disk's channels turned into the body frame, then the physical scaling, the
+// world rotation and the store of finalize_row by the group's first lane (the disk is complete: no indicator term)
 template <int KID>
 __global__ void __launch_bounds__(256) disk_moments_kernel(const double4* __restrict__ rec,
                                                            const int32_t* __restrict__ slots,
@@ -235,8 +244,9 @@ __global__ void __launch_bounds__(256) disk_moments_kernel(const double4* __rest
                                                            const double* __restrict__ lsup, int n,
                                                            const double* __restrict__ rot4, int flags,
                                                            double* __restrict__ out) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    const int lane = threadIdx.x & (DISK_NODES - 1);
+    const int r = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / DISK_NODES);
+    if (r >= n) return;                              // whole groups leave together
     double c[wsb::NCH];
 #pragma unroll
     for (int k = 0; k < wsb::NCH; ++k) c[k] = 0.0;
@@ -249,7 +259,13 @@ __global__ void __launch_bounds__(256) disk_moments_kernel(const double4* __rest
         const double yx = (d.x - px) * ih, yy = (d.y - py) * ih;
         const double D = sqrt(yx * yx + yy * yy);
         double v[7];
-        disk_channels<KID>(D, d.z * ih, v);
+        disk_channels_part<KID>(D, d.z * ih, lane, DISK_NODES, v);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+#pragma unroll
+            for (int o = DISK_NODES / 2; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, DISK_NODES);
+            v[k] *= WSB_INV_PI;
+        }
         double ex = 1.0, ey = 0.0;
         if (D > 1e-12) {
             ex = yx / D;
@@ -270,14 +286,15 @@ __global__ void __launch_bounds__(256) disk_moments_kernel(const double4* __rest
         c[10] += v[5] * ex * ey + v[6] * tx * ty;
         c[11] += v[5] * ey * ey + v[6] * ty * ty;
     }
-    finalize_row(c, 0.0, h, nullptr, rot4, flags | WSB_OUT_NO_INDICATOR, out + (size_t)r * wsb::NCH);
+    if (lane == 0) finalize_row(c, 0.0, h, nullptr, rot4, flags | WSB_OUT_NO_INDICATOR, out + (size_t)r * wsb::NCH);
 }
 
 template <int KID>
 void launch_moments(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K, const double* lpos,
                     const double* lsup, int n, const double* rot4, int flags, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(disk_moments_kernel<KID>, dim3((n + 255) / 256), dim3(256), 0, st, g->rec, slots, counts, K,
-                       lpos, lsup, n, rot4, flags, out);
+    const size_t threads = (size_t)n * DISK_NODES;
+    hipLaunchKernelGGL(disk_moments_kernel<KID>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, g->rec,
+                       slots, counts, K, lpos, lsup, n, rot4, flags, out);
 }
 
 template <int KID>
@@ -406,6 +423,7 @@ extern "C" int wsb_disk_moments2d(const wsb_disks2d* g, int kernel_id, const int
                                   double* out, void* stream) {
     WSB_REQUIRE(g, WSB_EINVAL, "null disk array");
     WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes (n=%d K=%d)", n, K);
+    WSB_REQUIRE((size_t)n * DISK_NODES / 256 < (size_t)1 << 31, WSB_EINVAL, "too many rows (%d)", n);
     WSB_REQUIRE(!(flags & ~(WSB_OUT_ACCUMULATE | WSB_OUT_PHYSICAL | WSB_OUT_WORLD | WSB_OUT_NO_INDICATOR)), WSB_EINVAL,
                 "unknown flags %d", flags);
     WSB_REQUIRE(!(flags & WSB_OUT_WORLD) || rot4, WSB_EINVAL, "WSB_OUT_WORLD needs rot4");
