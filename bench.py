@@ -276,6 +276,62 @@ def mesh3d_measure(torch, dev, steps, warmup):
     }
 
 
+def disks_measure(torch, dev, steps, warmup):
+    """the disk element on a fibre bundle (DiskArrayRep): 4,096 solid fibres, 1,048,576 particles outside them,
+    h = 0.006; step = disk slot list + twelve moments for each of cubic, w2, w4, w6 (FP64)."""
+    from warpsphboundaries_amd import workloads as WL
+    from warpsphboundaries_amd.scene import DiskArrayRep
+    log("generating the fibre bundle workload ...")
+    w = WL.fibre_bundle()
+    N = len(w.positions)
+    g = DiskArrayRep(w.centres, w.radii).to(dev).geom(w.h, dev)
+    lpos = torch.tensor(w.positions, dtype=torch.float64, device=dev)
+    lsup = torch.tensor(w.supports, dtype=torch.float64, device=dev)
+    slots = torch.empty(N * g.K, dtype=torch.int32, device=dev)
+    counts = torch.empty(N, dtype=torch.int32, device=dev)
+    outs = {k: torch.empty((N, 12), dtype=torch.float64, device=dev) for k in w.kernels}
+
+    def step():
+        g.adjacency(lpos, lsup, slots=slots, counts=counts)
+        for k in w.kernels:
+            g.moments(k, slots, counts, lpos, lsup, out=outs[k])
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / steps
+    stream = torch.cuda.current_stream()
+    per = {}
+    for name, fn in [("adjacency", lambda: g.adjacency(lpos, lsup, slots=slots, counts=counts))] + \
+            [("moments_" + k, (lambda k=k: g.moments(k, slots, counts, lpos, lsup, out=outs[k]))) for k in w.kernels]:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        per[name] = e0.elapsed_time(e1) / 1e3 / steps
+    pairs = int(counts.sum())
+    evals = 12 * N * len(w.kernels)
+    # algorithmic bytes of one moments launch: position, support, count and slots in, twelve float64 out, one 32-byte
+    # disk record per pair
+    bytes_mom = N * (16 + 8 + 4 + 4 * g.K + 96) + pairs * 32
+    return {
+        "workload": f"{w.name}: {len(w.radii)} solid fibres (radii 0.2-0.3 of the 1/64 lattice, jittered), {N:,} "
+                    f"particles outside them, h = {w.h}; step = disk slot list + twelve moments for cubic, w2, w4, w6",
+        "N": N, "disks": int(len(w.radii)), "pairs": pairs, "K": g.K,
+        "ms_per_step": t_step * 1e3, "evals_per_s": evals / t_step,
+        "launch_ms": {k: v * 1e3 for k, v in per.items()},
+        "pairs_per_s_w4": pairs / per["moments_w4"],
+        "moments_w4_hbm_GBps_algorithmic": bytes_mom / per["moments_w4"] / 1e9,
+        "bound": "fp64 VALU issue (16-node Gauss panels of the arc-angle form per pair)",
+    }
+
+
 def _traffic(kernel):
     """HBM bytes per launch of the mixed-form fused twelve-moment pass on the star workload, from the committed
     FETCH_SIZE / WRITE_SIZE counter runs scaled on a copy kernel of known size (None when the summary is absent)."""
@@ -299,6 +355,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--only-3d", action="store_true", help="run the 3-D section alone and print it")
+    ap.add_argument("--no-disks", action="store_true")
+    ap.add_argument("--only-disks", action="store_true", help="run the fibre-bundle disk section alone and print it")
     ap.add_argument("--probe", action="store_true",
                     help="counter probe: one mixed step of the star workload and a 512 MiB device copy, then exit")
     args = ap.parse_args()
@@ -315,6 +373,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.only_3d:
         print(json.dumps(mesh3d_measure(torch, dev, args.steps, args.warmup)), flush=True)
+        return
+    if args.only_disks:
+        print(json.dumps(disks_measure(torch, dev, args.steps, args.warmup)), flush=True)
         return
     log("generating the star workload ...")
     work = WL.config2(n=args.n)
@@ -488,6 +549,9 @@ def main():
     if not args.no_3d:
         line["mesh3d"] = mesh3d_measure(torch, dev, args.steps, args.warmup)
         log("3-D:", line["mesh3d"])
+    if not args.no_disks:
+        line["disks"] = disks_measure(torch, dev, args.steps, args.warmup)
+        log("disks:", line["disks"])
     if not args.no_serve:
         log("served workload ...")
         # headline: the simulations' particle arrays are resident on this GPU (device-tensor requests and results)

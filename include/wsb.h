@@ -189,6 +189,15 @@ int wsb_disk_moments2d(const wsb_disks2d* g, int kernel_id, const int32_t* slots
                        const double* lpos, const double* lsup, int n, const double* rot4, int flags, double* out,
                        void* stream);
 
+/* the geometric queries of the solid disks within one support, as wsb_cone_area2d / wsb_dir_extreme2d for a surface
+ * (replace _disk_cone_area_kernel / _disk_dir_extreme_kernel, scene/fused.py:312-470): per row the swept areas of its
+ * slots' disks (closed form per range of ray angles) and the largest |cos| of the directions to their points within h */
+int wsb_disk_cone_area2d(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K, const double* lpos,
+                         const double* lsup, int n, const double* axes, double pose_cos, double pose_sin,
+                         double half_angle, double* out, void* stream);
+int wsb_disk_dir_extreme2d(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K, const double* lpos,
+                           const double* lsup, int n, const double* vecs, double pose_cos, double pose_sin, double* out,
+                           void* stream);
 /* the seven channels of one disk (lam, m1 and g0 along the centre direction, g1 and m2 along / across it; units of h)
  * at centre distances D[n] and radii R[n] in units of h, on the host (validation of the device arithmetic) */
 int wsb_disk_channels_host(int kernel_id, const double* D, const double* R, int n, double* out);

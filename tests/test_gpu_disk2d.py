@@ -228,13 +228,79 @@ def test_adjacency_and_moments_replay_in_a_captured_graph():
         assert torch.equal(out, eager), rep
 
 
-def test_disk_arrays_reject_cone_area():
-    sc = Scene([Body(bodyId=0, reps=[DiskArrayRep([(0.0, 0.0)], [0.1])])], DEV)
-    ps = _ps(_queries(10))
-    fw = FusedWall(sc, fixed_adjacency(sc, ps, None, H), (FusedGroup("w2"),))
-    with pytest.raises(NotImplementedError):
-        fw.cone_area(torch.ones((10, 2), dtype=F64, device=DEV), 0.5)
+def _numeric_cone_areas(centres, radii, x, axis, half, H, nth=200000):
+    """area of (disks cap disk(x, H) cap wedge(x, axis, half)) and of (disks cap disk(x, H)) by integrating the ray
+    lengths over the angle (midpoint rule on the wedge's own angle interval and on the full circle; independent of the
+    closed form)."""
+    def sweep(th, dth):
+        d = np.stack([np.cos(th), np.sin(th)], 1)
+        tot = 0.0
+        for c, r in zip(centres, radii):
+            v = np.asarray(c) - x
+            b = d @ v
+            disc = b * b - (v @ v - r * r)
+            hit = disc > 0
+            sq = np.sqrt(np.where(hit, disc, 0.0))
+            lo, hi = np.clip(b - sq, 0, H), np.clip(b + sq, 0, H)
+            tot += np.where(hit, 0.5 * (hi ** 2 - lo ** 2), 0.0).sum() * dth
+        return tot
+    full = sweep((np.arange(nth) + 0.5) * (2 * np.pi / nth) - np.pi, 2 * np.pi / nth)
+    if half >= np.pi:
+        return full, full
+    a0 = np.arctan2(axis[1], axis[0]) - half
+    return sweep(a0 + (np.arange(nth) + 0.5) * (2 * half / nth), 2 * half / nth), full
+
+
+@pytest.mark.parametrize("half", [math.pi / 6, math.pi / 2, 2.5, math.pi])
+def test_cone_area_closed_form_equals_the_numerical_angular_integral(half):
+    cs, rs = [(0.0, 0.0), (0.45, 0.1), (-0.3, 0.5)], [0.15, 0.25, 0.4]
+    sc = Scene([Body(bodyId=0, center=(0.1, 0.2), angle=0.7, reps=[DiskArrayRep(cs, rs)])], DEV)
+    rng = np.random.default_rng(3)
+    lpos_h = rng.uniform(-0.6, 0.9, (120, 2))
+    b = sc.bodies[0]
+    pos = b.pose.toWorld(torch.as_tensor(lpos_h, dtype=F64, device=DEV))
+    ax = torch.as_tensor(rng.normal(size=(120, 2)), dtype=F64, device=DEV)
+    fw = FusedWall(sc, fixed_adjacency(sc, _ps(pos), None, H), (FusedGroup("w2"),))
+    got = fw.cone_area(ax, half).cpu().numpy()
+    lp = b.pose.toLocal(pos).cpu().numpy()
+    la = (ax @ b.pose.R.to(ax.device)).cpu().numpy()                      # world axis -> body frame (R^T a)
+    worst = 0.0
+    for i in range(0, 120, 4):
+        w, f = _numeric_cone_areas(cs, rs, lp[i], la[i], half, H)
+        worst = max(worst, abs(got[0, i] - w), abs(got[1, i] - f))
+    assert worst < 1e-7 * H * H * 100, worst
+    assert float(got[1].max()) > 0.1 * H * H
+
+
+def test_dir_extreme_equals_a_sampled_maximum():
+    cs, rs = [(0.0, 0.0), (0.45, 0.1), (-0.3, 0.5)], [0.15, 0.25, 0.4]
+    sc = Scene([Body(bodyId=0, center=(0.1, 0.2), angle=0.7, reps=[DiskArrayRep(cs, rs)])], DEV)
+    rng = np.random.default_rng(12)
+    b = sc.bodies[0]
+    pos = b.pose.toWorld(torch.as_tensor(rng.uniform(-0.6, 0.9, (80, 2)), dtype=F64, device=DEV))
+    vec = torch.as_tensor(rng.normal(size=(1, 80, 2)), dtype=F64, device=DEV)
+    fw = FusedWall(sc, fixed_adjacency(sc, _ps(pos), None, H), (FusedGroup("w2"),))
+    got = fw.dir_extreme(vec)[0].cpu().numpy()
+    lp = b.pose.toLocal(pos).cpu().numpy()
+    lv = (vec[0] @ b.pose.R.to(DEV)).cpu().numpy()
+    # sampled: points of every disk on a fine polar grid, kept within H of the query
+    ref = np.zeros(80)
+    for c, r in zip(cs, rs):
+        rr, tt = np.meshgrid(np.linspace(0, r, 400), np.linspace(0, 2 * np.pi, 1600, endpoint=False))
+        pts = np.asarray(c) + np.stack([rr.ravel() * np.cos(tt.ravel()), rr.ravel() * np.sin(tt.ravel())], 1)
+        for i in range(80):
+            v = pts - lp[i]
+            dist = np.linalg.norm(v, axis=1)
+            ok = (dist < H) & (dist > 0)
+            if ok.any():
+                cosv = np.abs(v[ok] @ lv[i]) / (dist[ok] * np.linalg.norm(lv[i]))
+                ref[i] = max(ref[i], cosv.max())
+    assert np.all(got >= ref - 1e-12) and np.abs(got - ref).max() < 5e-3, np.abs(got - ref).max()
+    assert (got > 0).sum() > 20
+
+
+def test_bad_disk_array_is_rejected():
+    import ctypes
     with pytest.raises(ValueError):
         L.check(L.load().wsb_disks2d_create(np.zeros(2).ctypes.data, np.array([-1.0]).ctypes.data, 1, 0.1,
-                                            __import__("ctypes").byref(__import__("ctypes").c_void_p()),
-                                            __import__("ctypes").byref(__import__("ctypes").c_int())))
+                                            ctypes.byref(ctypes.c_void_p()), ctypes.byref(ctypes.c_int())))

@@ -69,6 +69,10 @@ SIGNATURES = {
     "wsb_disk_moments2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int,
                                    c_void_p, c_void_p]),
     "wsb_disk_channels_host": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "wsb_disk_cone_area2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_double,
+                                     c_double, c_double, c_void_p, c_void_p]),
+    "wsb_disk_dir_extreme2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                       c_double, c_double, c_void_p, c_void_p]),
     "wsb_mesh3d_create": (c_int, [c_void_p, c_int, c_void_p, c_int, c_double, c_int, POINTER(c_void_p), POINTER(c_int64)]),
     "wsb_mesh3d_destroy": (c_int, [c_void_p]),
     "wsb_mesh3d_info": (c_int, [c_void_p, POINTER(c_int64)]),

@@ -289,6 +289,153 @@ __global__ void __launch_bounds__(256) disk_moments_kernel(const double4* __rest
     if (lane == 0) finalize_row(c, 0.0, h, nullptr, rot4, flags | WSB_OUT_NO_INDICATOR, out + (size_t)r * wsb::NCH);
 }
 
+// ------------------------------------------------------------------------------------ cone area, direction extreme
+// The rays from the query at angle psi from the centre direction meet the disk (centre distance D, radius R, support 1)
+// between t-+ = D cos psi -+ q, q = sqrt(R^2 - D^2 sin^2 psi); the area swept between angles 0 and psi is
+// I(psi) = int (1/2)(hi^2 - lo^2), hi / lo = t+ / t- clipped to [0, 1].  On a range of psi where neither t+ nor t-
+// crosses 0 or 1 the integrand is a constant, 1/2 or (1/2) t-+^2, whose antiderivative is
+// (1/2) [R^2 psi + (D^2 / 2) sin 2 psi -+ (s q + R^2 asin(s / R))], s = D sin psi.  The ranges are cut at the tangent
+// ray (sin psi = R / D) and at the rays through the intersections of the circle with the support circle.
+__device__ inline double disk_sweep_prim(double D, double R, double psi, double sgn) {
+    const double s = fmin(fmax(D * sin(psi), -R), R);
+    const double q = sqrt(fmax(R * R - s * s, 0.0));
+    return 0.5 * (R * R * psi + 0.5 * D * D * sin(2.0 * psi) + sgn * (s * q + R * R * asin(s / R)));
+}
+
+// I(psi) for psi in [0, pi]
+__device__ inline double disk_sweep(double D, double R, double psi) {
+    double cut[4] = {0.0, psi, psi, psi};
+    int nc = 1;
+    if (D > R) {
+        const double a = asin(fmin(R / D, 1.0));
+        if (a > 0.0 && a < psi) cut[nc++] = a;
+    }
+    if (D > 1e-12) {
+        const double c = (D * D + 1.0 - R * R) / (2.0 * D);
+        if (fabs(c) <= 1.0) {
+            const double a = acos(c);
+            if (a > 0.0 && a < psi) cut[nc++] = a;
+        }
+    }
+    if (nc == 3 && cut[2] < cut[1]) {
+        const double t = cut[1];
+        cut[1] = cut[2];
+        cut[2] = t;
+    }
+    cut[nc] = psi;
+    double total = 0.0;
+    for (int k = 0; k < nc; ++k) {
+        const double lo = cut[k], hi = cut[k + 1];
+        if (!(hi > lo)) continue;
+        const double m = 0.5 * (lo + hi), sm = D * sin(m);
+        if (!(fabs(sm) < R)) continue;                 // these rays miss the disk
+        const double q = sqrt(R * R - sm * sm), tp = D * cos(m) + q, tm = D * cos(m) - q;
+        if (!(tp > 0.0)) continue;                     // the disk lies behind the query on these rays
+        // hi part: 1/2 (clipped at the support) or 1/2 t+^2; lo part: 1/2 (disk beyond the support), 1/2 t-^2, or 0
+        const double hi_part = tp >= 1.0 ? 0.5 * (hi - lo) : disk_sweep_prim(D, R, hi, 1.0) - disk_sweep_prim(D, R, lo, 1.0);
+        double lo_part = 0.0;
+        if (tm >= 1.0) lo_part = 0.5 * (hi - lo);
+        else if (tm > 0.0) lo_part = disk_sweep_prim(D, R, hi, -1.0) - disk_sweep_prim(D, R, lo, -1.0);
+        total += hi_part - lo_part;
+    }
+    return total;
+}
+
+__device__ inline double wrap_angle(double a) {
+    const double two_pi = 6.283185307179586476925286766559, pi = 3.1415926535897932384626433832795;
+    double r = a + pi;
+    r = r - two_pi * floor(r / two_pi);
+    return r - pi;
+}
+
+// I at any angle: odd, and 2 pi periodic up to the whole area 2 I(pi)
+__device__ inline double disk_sweep_any(double D, double R, double psi) {
+    const double two_pi = 6.283185307179586476925286766559, pi = 3.1415926535897932384626433832795;
+    const double p0 = wrap_angle(psi);
+    const double k = floor((psi - p0) / two_pi + 0.5);
+    double v = disk_sweep(D, R, fabs(p0));
+    if (p0 < 0.0) v = -v;
+    if (k != 0.0) v += k * 2.0 * disk_sweep(D, R, pi);
+    return v;
+}
+
+// one thread per row: area(disks cap disk(p, H) cap wedge(p, axis, half_angle)) added to out[q], the full area to
+// out[n + q] (reference scene/fused.py:312-433, the same quantities as the surface form)
+__global__ void __launch_bounds__(256) disk_cone_area_kernel(const double4* __restrict__ rec,
+                                                             const int32_t* __restrict__ slots,
+                                                             const int32_t* __restrict__ counts, int K,
+                                                             const double* __restrict__ lpos,
+                                                             const double* __restrict__ lsup, int n,
+                                                             const double* __restrict__ axes, double pc, double ps,
+                                                             double half_angle, double* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double pi = 3.1415926535897932384626433832795;
+    const double H = lsup[q], px = lpos[2 * q], py = lpos[2 * q + 1];
+    const double wx = axes[2 * q], wy = axes[2 * q + 1];
+    const double th = atan2(-ps * wx + pc * wy, pc * wx + ps * wy);       // the axis in the body frame
+    const bool wedge_on = half_angle < pi;
+    double a_wedge = 0.0, a_full = 0.0;
+    const int cnt = counts ? min(counts[q], K) : K;
+    for (int i = 0; i < cnt; ++i) {
+        const int m = slots[(size_t)q * K + i];
+        if (m < 0) continue;
+        const double4 d = rec[m];
+        const double cx = d.x - px, cy = d.y - py;
+        const double D = sqrt(cx * cx + cy * cy) / H, R = d.z / H;
+        const double full = 2.0 * disk_sweep(D, R, pi);
+        a_full += full;
+        if (wedge_on) {
+            const double g = wrap_angle(th - atan2(cy, cx));
+            a_wedge += disk_sweep_any(D, R, g + half_angle) - disk_sweep_any(D, R, g - half_angle);
+        } else {
+            a_wedge += full;
+        }
+    }
+    out[q] += a_wedge * H * H;
+    out[n + q] += a_full * H * H;
+}
+
+// one thread per row: the largest |cos| between the world vector and a direction from p to a point of a disk within H,
+// max-accumulated into out[q].  The directions to the disk points within H form [-b, b] about the centre direction:
+// b = asin(R / d) when the tangent points are within H, else the angle where the circle meets the support circle;
+// psi = the angle between the line of the vector and the centre direction: cos(max(0, psi - b)); 1 inside a disk.
+__global__ void __launch_bounds__(256) disk_dir_extreme_kernel(const double4* __restrict__ rec,
+                                                               const int32_t* __restrict__ slots,
+                                                               const int32_t* __restrict__ counts, int K,
+                                                               const double* __restrict__ lpos,
+                                                               const double* __restrict__ lsup, int n,
+                                                               const double* __restrict__ vecs, double pc, double ps,
+                                                               double* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double H = lsup[q], px = lpos[2 * q], py = lpos[2 * q + 1];
+    const double wx = vecs[2 * q], wy = vecs[2 * q + 1];
+    const double bx = pc * wx + ps * wy, by = -ps * wx + pc * wy;
+    const double nb = sqrt(bx * bx + by * by);
+    if (!(nb > 0.0)) return;
+    double best = 0.0;
+    const int cnt = counts ? min(counts[q], K) : K;
+    for (int i = 0; i < cnt; ++i) {
+        const int m = slots[(size_t)q * K + i];
+        if (m < 0) continue;
+        const double4 dk = rec[m];
+        const double cx = dk.x - px, cy = dk.y - py, R = dk.z;
+        const double d = sqrt(cx * cx + cy * cy);
+        double f = 0.0;
+        if (d <= R) {
+            f = 1.0;
+        } else if (d - R < H) {
+            const double b = (d * d - R * R <= H * H) ? asin(fmin(R / d, 1.0))
+                                                      : acos(fmin(fmax((d * d + H * H - R * R) / (2.0 * d * H), -1.0), 1.0));
+            const double psi = acos(fmin(fabs(cx * bx + cy * by) / (d * nb), 1.0));
+            f = cos(fmax(0.0, psi - b));
+        }
+        best = fmax(best, f);
+    }
+    out[q] = fmax(out[q], best);
+}
+
 template <int KID>
 void launch_moments(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K, const double* lpos,
                     const double* lsup, int n, const double* rot4, int flags, double* out, hipStream_t st) {
@@ -462,5 +609,32 @@ extern "C" int wsb_disk_channels_host(int kernel_id, const double* D, const doub
         case WSB_W4P5: host_channels<WSB_W4P5>(D, R, n, out); break;
         default: return wsb::set_error(WSB_EINVAL, "unknown kernel id %d", kernel_id);
     }
+    return WSB_OK;
+}
+
+extern "C" int wsb_disk_cone_area2d(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K,
+                                    const double* lpos, const double* lsup, int n, const double* axes, double pose_cos,
+                                    double pose_sin, double half_angle, double* out, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null disk array");
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes (n=%d K=%d)", n, K);
+    WSB_REQUIRE(half_angle >= 0.0, WSB_EINVAL, "negative half angle");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(slots && lpos && lsup && axes && out, WSB_EINVAL, "null buffer");
+    hipLaunchKernelGGL(disk_cone_area_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), g->rec,
+                       slots, counts, K, lpos, lsup, n, axes, pose_cos, pose_sin, half_angle, out);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+extern "C" int wsb_disk_dir_extreme2d(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K,
+                                      const double* lpos, const double* lsup, int n, const double* vecs, double pose_cos,
+                                      double pose_sin, double* out, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null disk array");
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes (n=%d K=%d)", n, K);
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(slots && lpos && lsup && vecs && out, WSB_EINVAL, "null buffer");
+    hipLaunchKernelGGL(disk_dir_extreme_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), g->rec,
+                       slots, counts, K, lpos, lsup, n, vecs, pose_cos, pose_sin, out);
+    WSB_HIP(hipGetLastError());
     return WSB_OK;
 }

@@ -86,11 +86,6 @@ def rep_channels(rep, topo, ba, kernel, form, R, out=None, accumulate=False):
     return out
 
 
-def _surface_only(rep, what):
-    if isinstance(rep, DiskArrayRep):
-        raise NotImplementedError(f"FusedWall.{what}: surface and box representations only (no disk element form)")
-
-
 class FusedWall(WallAggregate):
     def __init__(self, scene, adjacency, groups):
         self.scene = scene
@@ -168,7 +163,11 @@ class FusedWall(WallAggregate):
         ax = axes.to(dev, F64).contiguous()
         lib = L.load()
         for bi, ba, rep, topo, c, s in self._reps():
-            _surface_only(rep, "cone_area")
+            if isinstance(rep, DiskArrayRep):
+                L.check(lib.wsb_disk_cone_area2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K,
+                                                 L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(ax), c, s, float(half_angle),
+                                                 L.ptr(out), L.stream_ptr(dev)))
+                continue
             L.check(lib.wsb_cone_area2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K, L.ptr(topo.ind),
                                         L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(ax), c, s, float(half_angle),
                                         L.ptr(out), L.stream_ptr(dev)))
@@ -190,8 +189,12 @@ class FusedWall(WallAggregate):
         v = vecs.to(dev, F64).contiguous()
         lib = L.load()
         for bi, ba, rep, topo, c, s in self._reps():
-            _surface_only(rep, "dir_extreme")
             seg = out[bi * N:(bi + 1) * N]
+            if isinstance(rep, DiskArrayRep):
+                L.check(lib.wsb_disk_dir_extreme2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K,
+                                                   L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(v[bi]), c, s, L.ptr(seg),
+                                                   L.stream_ptr(dev)))
+                continue
             L.check(lib.wsb_dir_extreme2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K, L.ptr(topo.ind),
                                           L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(v[bi]), c, s, L.ptr(seg),
                                           L.stream_ptr(dev)))

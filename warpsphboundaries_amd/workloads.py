@@ -400,3 +400,40 @@ def jitter(positions, h, seed):
 def poisson_arrivals(n, qps, seed=6):
     g = rng(seed)
     return np.cumsum(g.exponential(1.0 / qps, n))
+
+
+@dataclass
+class FibreBundle:
+    name: str
+    centres: np.ndarray
+    radii: np.ndarray
+    positions: np.ndarray
+    supports: np.ndarray
+    h: float
+    kernels: tuple
+
+
+def fibre_bundle(n=1 << 20, side=64, seed=8):
+    """a bundle of side^2 solid fibres (disks) on a jittered lattice in the unit square, radii 0.2-0.3 of the lattice
+    spacing, n particles uniform in the square outside the fibres, h = 0.006 (about 0.38 lattice spacings): the
+    disk-element workload of a DiskArrayRep body (reference scene/scene.py:808-888)."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side), indexing="ij"), -1).reshape(-1, 2)
+    centres = (g + 0.5 + rng.uniform(-0.15, 0.15, g.shape)) / side
+    radii = rng.uniform(0.2, 0.3, len(g)) / side
+    pos = np.empty((0, 2))
+    while len(pos) < n:
+        p = rng.uniform(0.0, 1.0, (n, 2))
+        cell = np.clip(np.floor(p * side).astype(np.int64), 0, side - 1)
+        near = np.full(len(p), np.inf)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                cx, cy = cell[:, 0] + dx, cell[:, 1] + dy
+                ok = (cx >= 0) & (cx < side) & (cy >= 0) & (cy < side)
+                k = np.where(ok, cx * side + cy, 0)
+                d = np.linalg.norm(p - centres[k], axis=1) - radii[k]
+                near = np.minimum(near, np.where(ok, d, np.inf))
+        pos = np.concatenate([pos, p[near > 0]])
+    h = 0.006
+    return FibreBundle(f"fibre bundle {side * side}", centres, radii, pos[:n], np.full(n, h), h,
+                       ("cubic", "w2", "w4", "w6"))
