@@ -16,6 +16,7 @@ Contents (each function cites the reference file:line it restates):
 * ``disk2d``    the disk element (circle identity of ``edge/disk.py`` with second moments) and the disk cell list /
                 slot fill (``scene/scene.py:869-888``, ``scene/fixedadj.py:199-222``).
 * ``cone2d``, ``triangle2d``  the cone area / direction extreme and the conditioned triangle quadrature.
+* ``contour2d`` the marching-squares zero level set of a sampled distance (``scene/scene.py:762-806``).
 * ``arith``     the arithmetic the restatements run in: float64 (default), x86 long double, or mpmath.
 
 How each part is pinned (tests/test_oracle_*.py, tests/test_mesh3d_cpu.py): the reference's 40-digit 2-D golden

@@ -4,7 +4,8 @@ closed polygonal walls at fluid particles, on hand-written HIP kernels for gfx95
 Public surface (the reference's names):
     AnalyticBoundary, BoundaryProvider                     provider protocol (scene/provider.py)
     Scene, Body, SurfaceRep, ParticleState, fixed_adjacency scene layer
-    BoxRep, ImplicitRep, DiskBody, TierPolicy, DiskArrayRep representations (boxes as polygons, solid disks as the disk element)
+    BoxRep, ImplicitRep, DiskBody, TierPolicy, DiskArrayRep, SdfRep  representations (boxes and sampled distance
+                                                           fields as polygons, solid disks as the disk element)
     FusedWall, FusedGroup, WallOutput                      fused evaluation
     edge_channels, second_channels, indicator_vector       pair-list evaluators
     value, gradient, moment, moment_gradient               polygon element API
@@ -21,7 +22,7 @@ def __getattr__(name):
     if name in ("AnalyticBoundary", "BoundaryProvider"):
         from . import provider
         return getattr(provider, name)
-    if name in ("Scene", "Body", "SurfaceRep", "ParticleState", "fixed_adjacency", "Pose", "BoxRep", "ImplicitRep", "DiskArrayRep",
+    if name in ("Scene", "Body", "SurfaceRep", "ParticleState", "fixed_adjacency", "Pose", "BoxRep", "ImplicitRep", "DiskArrayRep", "SdfRep",
                 "DiskBody", "TierPolicy"):
         from . import scene
         return getattr(scene, name)

@@ -25,7 +25,7 @@ import torch
 from . import _lib as L
 from . import precision, triangle
 from .kernels import P5_NAMES
-from .scene import BoxRep, DiskArrayRep, FixedTopology, ImplicitRep, SurfaceRep, kernelName
+from .scene import BoxRep, DiskArrayRep, FixedTopology, ImplicitRep, SdfRep, SurfaceRep, kernelName
 
 F64 = torch.float64
 KINDS = {"lam": (0, 1), "g0": (1, 2), "cov": (2, 4), "a1g1": (3, 2), "lap": (4, 1), "m1": (5, 2), "m2": (6, 4)}
@@ -204,7 +204,8 @@ class FusedWall(WallAggregate):
 
     @staticmethod
     def supported(scene, lowering=True):
-        """every representation evaluates exactly: surface loops and disk arrays, and (with ``lowering``) boxes through
-        their polygons and disks through the disk element (a cavity through its polygon)."""
-        kinds = (SurfaceRep, DiskArrayRep, BoxRep, ImplicitRep) if lowering else (SurfaceRep, DiskArrayRep)
+        """every representation evaluates exactly: surface loops and disk arrays, and (with ``lowering``) boxes and
+        sampled distance fields through their polygons and disks through the disk element (a cavity through its
+        polygon)."""
+        kinds = (SurfaceRep, DiskArrayRep, BoxRep, ImplicitRep, SdfRep) if lowering else (SurfaceRep, DiskArrayRep)
         return all(isinstance(r, kinds) for b in scene.bodies for r in b.reps)

@@ -304,6 +304,127 @@ class ImplicitRep:
         return torch.where(d < 0, 1.0, torch.where(d > 0, 0.0, 0.5)).to(F64)
 
 
+class SdfRep:
+    """a sampled signed distance of the body frame: node values ``values`` [nx, ny] (positive in the fluid) on the grid
+    ``origin + spacing * (i, j)`` (reference scene/scene.py:599-670).  The fused evaluation integrates it exactly as
+    a surface: its ``fallback`` SurfaceRep when given, else the zero level set of the samples (``contour``, marching
+    squares).  ``signed`` is the bilinear interpolant with the gradient of the bilinear cell as normal direction."""
+    kind = "sdf"
+
+    def __init__(self, values, origin, spacing, fallback: Optional[SurfaceRep] = None, **policy):
+        self.d = torch.as_tensor(values, dtype=F64)
+        if self.d.dim() != 2 or min(self.d.shape) < 2:
+            raise ValueError("SdfRep: values must be a [nx, ny] grid with nx, ny >= 2")
+        self.origin = torch.as_tensor(origin, dtype=F64).reshape(2)
+        self.spacing = float(spacing)
+        if not self.spacing > 0:
+            raise ValueError("SdfRep: spacing must be positive")
+        self.fallback = fallback
+        self.policy = dict(policy)          # the reference's tier-3 guard thresholds; the fused path does not use them
+        self._contour = None
+
+    @staticmethod
+    def fromFunction(fn, lo, hi, spacing, **kw):
+        """``fn(points [M, 2]) -> d`` sampled on the grid covering [lo, hi] (reference scene/scene.py:636-642)."""
+        nx = int(np.ceil((hi[0] - lo[0]) / spacing)) + 1
+        ny = int(np.ceil((hi[1] - lo[1]) / spacing)) + 1
+        X, Y = np.meshgrid(lo[0] + spacing * np.arange(nx), lo[1] + spacing * np.arange(ny), indexing="ij")
+        pts = torch.as_tensor(np.stack([X.ravel(), Y.ravel()], 1), dtype=F64)
+        return SdfRep(torch.as_tensor(fn(pts), dtype=F64).reshape(nx, ny), lo, spacing, **kw)
+
+    @property
+    def solid(self):
+        return "outside" if self.contour().background else "inside"
+
+    def to(self, device):
+        self.d, self.origin = self.d.to(device), self.origin.to(device)
+        if self.fallback is not None:
+            self.fallback.to(device)
+        if self._contour is not None:
+            self._contour.to(device)
+        return self
+
+    def bounds(self):
+        ext = torch.as_tensor(self.d.shape, dtype=F64, device=self.d.device) - 1
+        return self.origin, self.origin + ext * self.spacing
+
+    def contour(self):
+        """the zero level set of the samples as a SurfaceRep: on every cell edge whose end nodes differ in sign
+        (solid = d < 0) a vertex at the linear zero, joined cell by cell with the solid on the left of every edge; a
+        cell with four crossings joins them around its solid corners, or across its centre when the mean of its four
+        nodes is solid.  The border nodes must be all fluid (an enclosed solid, background 0) or all solid (a tank,
+        background 1).  Vertices and edges are numbered in the order the cells are visited (i outer, j inner,
+        corners counter-clockwise), as the reference's marching squares (scene/scene.py:762-806)."""
+        if self._contour is None:
+            self._contour = _contour_of(self.d.cpu().numpy(), self.origin.cpu().numpy(), self.spacing).to(self.d.device)
+        return self._contour
+
+    def lowered(self, hmin, device):
+        return (self.fallback if self.fallback is not None else self.contour()).to(device)
+
+    def signed(self, pos):
+        """(d, n): the bilinear interpolant of the samples (clamped to the grid) and the unit normal into the fluid
+        (its normalised gradient), body frame."""
+        d = self.d.to(pos.device)
+        nx, ny = d.shape
+        u = (pos.to(F64) - self.origin.to(pos.device)) / self.spacing
+        uc = torch.stack([u[:, 0].clamp(0, nx - 1), u[:, 1].clamp(0, ny - 1)], 1)
+        i0 = torch.stack([uc[:, 0].floor().clamp(0, nx - 2), uc[:, 1].floor().clamp(0, ny - 2)], 1).long()
+        t = uc - i0
+        a, b = i0[:, 0], i0[:, 1]
+        d00, d10, d01, d11 = d[a, b], d[a + 1, b], d[a, b + 1], d[a + 1, b + 1]
+        tx, ty = t[:, 0], t[:, 1]
+        val = (1 - tx) * (1 - ty) * d00 + tx * (1 - ty) * d10 + (1 - tx) * ty * d01 + tx * ty * d11
+        gx = ((1 - ty) * (d10 - d00) + ty * (d11 - d01)) / self.spacing
+        gy = ((1 - tx) * (d01 - d00) + tx * (d11 - d10)) / self.spacing
+        g = torch.stack([gx, gy], 1)
+        return val, g / g.norm(dim=1, keepdim=True).clamp(min=1e-300)
+
+    def indicator(self, p, supports=None):
+        d, _ = self.signed(p)
+        return torch.where(d < 0, 1.0, torch.where(d > 0, 0.0, 0.5)).to(F64)
+
+
+def _contour_of(d, origin, spacing):
+    """the SurfaceRep of the zero level set of node values d [nx, ny] (see ``SdfRep.contour``)."""
+    nx, ny = d.shape
+    solid = d < 0
+    border = np.concatenate([solid[0], solid[-1], solid[1:-1, 0], solid[1:-1, -1]])
+    if border.any() and not border.all():
+        raise ValueError("SdfRep.contour: the sampled grid must cover the body with a margin (all border nodes fluid, "
+                         "or all solid)")
+    background = 1 if border.all() else 0
+    index, verts, edges = {}, [], []
+
+    def vertex(p, q):
+        if (p, q) not in index:
+            t = d[p] / (d[p] - d[q])
+            index[(p, q)] = len(verts)
+            verts.append((origin[0] + spacing * (p[0] + t * (q[0] - p[0])), origin[1] + spacing * (p[1] + t * (q[1] - p[1]))))
+        return index[(p, q)]
+
+    mixed = solid[:-1, :-1].astype(np.int8) + solid[1:, :-1] + solid[1:, 1:] + solid[:-1, 1:]
+    for i, j in zip(*np.nonzero((mixed > 0) & (mixed < 4))):
+        i, j = int(i), int(j)
+        ring = ((i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1))
+        sol = [bool(solid[c]) for c in ring]
+        cross = []                                  # (leaving the solid?, vertex) counter-clockwise along the cell
+        for k in range(4):
+            if sol[k] != sol[(k + 1) % 4]:
+                cross.append((sol[k], vertex(ring[k], ring[(k + 1) % 4])))
+        m = len(cross)
+        step = -1 if (m == 4 and float(d[i:i + 2, j:j + 2].mean()) < 0) else 1
+        for k in range(m):
+            if cross[k][0]:
+                q = (k + step) % m
+                while cross[q][0]:
+                    q = (q + step) % m
+                edges.append((cross[k][1], cross[q][1]))
+    if not edges:
+        raise ValueError("SdfRep.contour: no zero level set in the sampled grid")
+    return SurfaceRep(np.asarray(verts, dtype=float), np.asarray(edges, dtype=np.int32), background=background)
+
+
 class DiskArrayRep:
     """a set of solid disks (fibres) of the body frame: centres [M, 2], radii [M]; the disks must not overlap (the
     solid is their union and the integrals add).  Every (particle, disk) pair is one closed evaluation of the disk
@@ -510,7 +631,7 @@ class Scene:
             for rep in bd.reps:
                 if isinstance(rep, BoxRep):
                     rep = rep.surface()
-                if isinstance(rep, (ImplicitRep, DiskArrayRep)):
+                if isinstance(rep, (ImplicitRep, DiskArrayRep, SdfRep)):
                     d, n_loc = rep.signed(lp)
                 elif isinstance(rep, SurfaceRep):
                     smax = float(supportMax) if supportMax is not None else _default_support(rep)
