@@ -75,3 +75,25 @@ def test_solid_disk_lowering_converges_and_signed_distance():
     d, nrm, _ = sa.signed_distance(torch.tensor(pos, device=DEV))
     np.testing.assert_allclose(d.cpu().numpy(), rad - R, rtol=0, atol=1e-14)
     np.testing.assert_allclose(nrm.cpu().numpy(), np.stack([np.cos(ang), np.sin(ang)], 1), rtol=0, atol=1e-12)
+
+
+def test_sdf_rep_is_its_contour_and_converges_to_the_disk():
+    """a sampled disk distance: the fused evaluation of the SdfRep equals that of its marching-squares contour
+    bitwise, and approaches the exact disk element as the spacing shrinks (polygon error O(spacing^2))."""
+    from warpsphboundaries_amd.scene import DiskArrayRep, SdfRep
+    c, R, h = (0.3, 0.4), 0.15, 0.02
+    rng = np.random.default_rng(21)
+    ang = rng.uniform(0, 2 * math.pi, 1500)
+    rad = R + rng.uniform(-1.2 * h, 1.2 * h, 1500)
+    pos = np.stack([c[0] + rad * np.cos(ang), c[1] + rad * np.sin(ang)], 1)
+    fn = lambda p: (p - torch.as_tensor(c, dtype=torch.float64)).norm(dim=1) - R
+    exact, _ = _eval(DiskArrayRep([c], [R]), pos, h)
+    errs = []
+    for spacing in (0.004, 0.001):
+        rep = SdfRep.fromFunction(fn, (0.1, 0.2), (0.5, 0.6), spacing)
+        a, sa = _eval(rep, pos, h)
+        b, _ = _eval(rep.contour(), pos, h)
+        for k in a:
+            assert torch.equal(a[k], b[k]), k
+        errs.append(max(float((a[k] - exact[k]).abs().max()) / float(exact[k].abs().max()) for k in a))
+    assert errs[1] < 0.2 * errs[0] and errs[1] < 1e-3, errs
