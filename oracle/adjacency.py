@@ -86,23 +86,21 @@ def build_ybins(verts, edges):
 
 def _winding_row(px, py, h, verts, edges, bin_items, background, normalise=True):
     """one point: y-bin crossing count, boundary angle sum (scene/fixedadj.py:98-146)."""
-    w = 0.0
-    boundary = False
-    for e in bin_items:
-        v0, v1 = edges[e]
-        ax, ay = verts[v0]
-        cxx, cyy = verts[v1]
-        if normalise:
-            arx, ary, crx, cry = (ax - px) / h, (ay - py) / h, (cxx - px) / h, (cyy - py) / h
-        else:
-            arx, ary, crx, cry = ax - px, ay - py, cxx - px, cyy - py
-        cross = _cross2(arx, ary, crx, cry)
-        if ay <= py and py < cyy and cross > 0.0:
-            w += 1.0
-        if cyy <= py and py < ay and cross < 0.0:
-            w -= 1.0
-        if cross == 0.0 and arx * crx + ary * cry <= 0.0:
-            boundary = True
+    # the crossing count is an integer, so the per-item decisions are evaluated elementwise (numpy rounds every
+    # product and difference separately, as the scalar form) and counted in any order
+    its = np.asarray(bin_items, dtype=np.int64)
+    a = verts[edges[its, 0]]
+    c = verts[edges[its, 1]]
+    ay, cyy = a[:, 1], c[:, 1]
+    if normalise:
+        arx, ary, crx, cry = (a[:, 0] - px) / h, (ay - py) / h, (c[:, 0] - px) / h, (cyy - py) / h
+    else:
+        arx, ary, crx, cry = a[:, 0] - px, ay - py, c[:, 0] - px, cyy - py
+    cross = _cross2(arx, ary, crx, cry)
+    up = (ay <= py) & (py < cyy) & (cross > 0.0)
+    down = (cyy <= py) & (py < ay) & (cross < 0.0)
+    w = float(int(up.sum()) - int(down.sum()))
+    boundary = bool(((cross == 0.0) & (arx * crx + ary * cry <= 0.0)).any())
     if boundary:
         angle = 0.0
         for e in range(len(edges)):
