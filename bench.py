@@ -392,14 +392,22 @@ def main():
     ind = torch.empty(N, dtype=torch.float64, device=dev)
     order = torch.empty(N, dtype=torch.int32, device=dev)
     sort_scratch = torch.empty(int(L.load().wsb_sort2d_scratch_bytes(geo.handle, N)), dtype=torch.uint8, device=dev)
+    corder = torch.empty(N, dtype=torch.int32, device=dev)
+    nwork = torch.empty(1, dtype=torch.int32, device=dev)
+    corder_scratch = torch.empty(int(L.load().wsb_count_order2d_scratch_bytes()), dtype=torch.uint8, device=dev)
     outs = {k: torch.empty((N, 12), dtype=torch.float64, device=dev) for k in work.kernels}
 
     def evaluate(form, scratch=None):
         if form == "plain":                 # the plan interpreter: one launch per kernel
             for k in work.kernels:
                 geo.moments(k, slots, counts, ind, lpos, lsup, form=form, out=outs[k], scratch=scratch)
-        else:                               # the closed form: the four kernels in one pass sharing the chords
-            geo.moments_multi(work.kernels, slots, counts, ind, lpos, lsup, form=form, outs=outs)
+        else:                               # the closed form: the four kernels in one pass sharing the chords,
+            # one row per thread, rows served in cell order
+            geo.moments_multi(work.kernels, slots, counts, ind, lpos, lsup, form=form, outs=outs, order=order)
+
+    def evaluate_by_count(form):
+        """the same pass in the count order (rows with slots grouped by count, indicator terms stored apart)"""
+        geo.moments_multi(work.kernels, slots, counts, ind, lpos, lsup, form=form, outs=outs, order=corder, nwork=nwork)
 
     def step(form, scratch=None):
         geo.sort(lpos, order=order, scratch=sort_scratch)
@@ -455,20 +463,22 @@ def main():
     # ---- evaluate-only timing (events on the launch stream): the fused pass, and each kernel's own launch beside it
     geo.sort(lpos, order=order, scratch=sort_scratch)
     geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind, order=order)
+    geo.count_order(counts, base=order, order=corder, nwork=nwork, scratch=corder_scratch)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     reps = max(5, args.steps)
-    t_multi = {}
+    t_multi, t_multi_count = {}, {}
     for form in ("mixed", "fp64"):
-        for _ in range(2):
-            evaluate(form)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            evaluate(form)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        t_multi[form] = e0.elapsed_time(e1) / 1e3 / reps
+        for fn, dst in ((evaluate, t_multi), (evaluate_by_count, t_multi_count)):
+            for _ in range(2):
+                fn(form)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn(form)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            dst[form] = e0.elapsed_time(e1) / 1e3 / reps
     per_kernel = {}
     for form in ("mixed", "fp64"):
         for k in work.kernels:
@@ -500,9 +510,16 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     t_adj_unsorted = e0.elapsed_time(e1) / 1e3 / args.steps
-    log("fused-pass seconds:", t_multi, "single-kernel launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()},
+    geo.adjacency(lpos, lsup, pad=False, slots=slots, counts=counts, ind=ind, order=order)
+    e0.record(stream)
+    for _ in range(args.steps):
+        geo.count_order(counts, base=order, order=corder, nwork=nwork, scratch=corder_scratch)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_corder = e0.elapsed_time(e1) / 1e3 / args.steps
+    log("fused-pass seconds:", t_multi, "in count order:", t_multi_count, "single-kernel launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()},
         "adjacency", t_adj, "sort", t_sort,
-        "adjacency in input order", t_adj_unsorted)
+        "adjacency in input order", t_adj_unsorted, "count order", t_corder)
 
     # ---- roofline of the dominant kernel: the mixed-form fused pass over the four kernels
     npairs, active = active_radius_counts(torch, rep.vertices.to(dev), rep.edges.to(dev).long(), slots, counts, lpos,
@@ -512,7 +529,7 @@ def main():
     t_dom = t_multi["mixed"]
     t_roof = max(bytes_alg / PEAK_HBM, flops / PEAK_FP64)
     compute_bound = flops / PEAK_FP64 >= bytes_alg / PEAK_HBM
-    kname = "tile12_multi_kernel<15, 2>"
+    kname = "rows12_multi_kernel<15, 2>"
     roof = {"bound": "mfma" if compute_bound else "hbm",
             "pipe": "FP64 VALU (no MFMA: transcendental gather-reduce; the MI355X FP64 vector peak equals its FP64 matrix peak)",
             "kernel": kname + " (cubic, w2, w4, w6 in one pass, mixed form)",
@@ -538,10 +555,17 @@ def main():
                   "plain_fp64": {"evals_per_s": evals / t_plain, "ms_per_step": t_plain * 1e3}},
         "speedup_mixed_vs_plain": t_plain / t_mixed,
         "evaluate_only_evals_per_s": {"mixed": evals / t_multi["mixed"], "fp64": evals / t_multi["fp64"]},
+        "evaluate_only_count_order_evals_per_s": {"mixed": evals / t_multi_count["mixed"],
+                                                  "fp64": evals / t_multi_count["fp64"],
+                                                  "note": "rows with slots grouped by count (wsb_count_order2d, "
+                                                          "count_order_ms apart), indicator terms stored by a separate "
+                                                          "contiguous pass; pays off when one adjacency is evaluated "
+                                                          "more than once"},
         "evaluate_only_single_kernel_launches_evals_per_s": {
             "mixed": evals / sum(per_kernel[("mixed", k)] for k in work.kernels),
             "fp64": evals / sum(per_kernel[("fp64", k)] for k in work.kernels)},
         "adjacency_ms": t_adj * 1e3, "cell_sort_ms": t_sort * 1e3, "adjacency_input_order_ms": t_adj_unsorted * 1e3,
+        "count_order_ms": t_corder * 1e3,
         "max_signal_relative_error_vs_fp64": err,
         "served_latency": None,
         "roofline": roof,

@@ -127,11 +127,25 @@ int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double
 /* ---- several scheme kernels in one pass (the reference's fused stage 1 evaluates the kernel groups of a layout in one
  * launch family, scene/fused.py:521-564): outs[4] (a host array, indexed by kernel id WSB_CUBIC..WSB_W6; nullptr =
  * not requested) receive what wsb_moments2d gives for each kernel, with the chord primitives of every pair (endpoint
- * radii, logarithm, angle, I_m recurrence) computed once per support radius for all of them.  Forms WSB_FORM_FP64
- * and WSB_FORM_MIXED; the other arguments as wsb_moments2d. */
-int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t* order, const int32_t* slots, const int32_t* counts,
-                        int K, const double* ind, const double* lpos, const double* lsup, int n, const double* rot4,
-                        int flags, double* const* outs, void* stream);
+ * radii, logarithm, angle, I_m recurrence) computed once per support radius for all of them.  One thread serves one
+ * row (its slots in order, the row sums in registers).  order (nullable): the serving order; nwork (device, nullable,
+ * needs order): only the first *nwork positions of order hold rows with slots (wsb_count_order2d writes both), and
+ * the indicator terms of every row are first stored by a pass over the rows in input order (contiguous stores).  Any
+ * order gives the same results.  Forms WSB_FORM_FP64 and WSB_FORM_MIXED; the other arguments as wsb_moments2d. */
+int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t* order, const int32_t* nwork, const int32_t* slots,
+                        const int32_t* counts, int K, const double* ind, const double* lpos, const double* lsup, int n,
+                        const double* rot4, int flags, double* const* outs, void* stream);
+
+/* ---- rows grouped by slot count, the largest count first (device counting sort; no counterpart in the reference,
+ * which evaluates rows in input order): order[n] is a permutation of base[0..n) (of 0..n-1 when base is NULL) along
+ * which the slot counts never increase (counts at or above 1023 share one bin), the rows with no slot last; *nwork
+ * (device, nullable) receives the number of rows with at least one slot.  Passed as `order` and `nwork` to
+ * wsb_moments2d_multi, whose threads each serve one row, it gives every wavefront rows of one slot count; it changes
+ * which thread serves a row, never a result.  scratch: wsb_count_order2d_scratch_bytes() bytes (counters, cleared by
+ * a kernel: no memset). */
+size_t wsb_count_order2d_scratch_bytes(void);
+int wsb_count_order2d(const int32_t* counts, const int32_t* base, int n, int K, int32_t* order, int32_t* nwork,
+                      void* scratch, size_t scratch_bytes, void* stream);
 
 /* ---- wedge areas and directional extremes of the solid within one support (reference scene/fused.py:157-309
  * `_cone_area_kernel`, `_dir_extreme_kernel`, used by `FusedWall.cone_area` / `dir_extreme`, :741-835), FP64, over

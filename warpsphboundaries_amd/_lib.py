@@ -29,6 +29,8 @@ SIGNATURES = {
     "wsb_geom2d_arrays": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p)]),
     "wsb_sort2d_scratch_bytes": (c_size_t, [c_void_p, c_int]),
     "wsb_sort2d": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "wsb_count_order2d_scratch_bytes": (c_size_t, []),
+    "wsb_count_order2d": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_adjacency2d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                 c_void_p]),
     "wsb_moments2d_scratch_bytes": (c_size_t, [c_int, c_int]),
@@ -38,8 +40,8 @@ SIGNATURES = {
                                     c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_signed_distance2d": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wsb_sphere_closed_form": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
-    "wsb_moments2d_multi": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
-                                    c_void_p, c_int, c_void_p, c_void_p]),
+    "wsb_moments2d_multi": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                    c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "wsb_cone_area2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double,
                                 c_double, c_double, c_void_p, c_void_p]),
     "wsb_dir_extreme2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p,

@@ -339,6 +339,7 @@ __device__ __forceinline__ void radius_sums(double R, bool valid, double z, doub
 struct MultiArgs {
     TileArgs t;                 // t.out unused
     double* out[4];             // per kernel id (nullptr: not requested)
+    const int32_t* nwork;       // nullable (device): nwork[0] = the first positions of t.order that are rows with slots
 };
 
 template <int MASK, int FORM>
@@ -483,9 +484,152 @@ __global__ __launch_bounds__(MTILE_T) __attribute__((amdgpu_waves_per_eu(FORM ==
     }
 }
 
+// ------------------------------------------------------------------------------------------------ one row per lane
+// The same closed form with one thread per row: the thread walks its row's slots in order and adds each pair's
+// channels of every kernel of MASK to its own row sums in registers (the same sums, in the same slot order, as the
+// tile kernel: no LDS, no barrier, no pair-to-row search).  A wavefront costs the largest slot count among its 64
+// rows, so the rows are served in the order of wsb_count_order2d (grouped by slot count, largest first): every
+// wavefront then has rows of one count, and the rows with no pair (only the indicator term) come last.  The edge
+// records of the next slot are loaded while the current pair is evaluated.
+constexpr int RROW_T = 256;
+constexpr int RHEAVY_T = 256;           // four wavefronts: two workgroups share a compute unit at two waves per SIMD
+
+template <int MASK, int FORM>
+__device__ __forceinline__ void serve_row(const MultiArgs& ma, int pos) {
+    const TileArgs& a = ma.t;
+    const int row = a.order ? a.order[pos] : pos;
+    const int cnt = min(max(a.counts[row], 0), a.K);
+    const double x = a.lpos[2 * row], y = a.lpos[2 * row + 1], h = a.lsup[row];
+    const double ih = 1.0 / h;
+    double acc[4][wsb::NCH];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int q = 0; q < wsb::NCH; ++q) acc[k][q] = 0.0;
+    const int32_t* sl = a.slots + (int64_t)row * a.K;
+    int e_next = cnt > 0 ? sl[0] : 0;
+    bool ok_next = e_next >= 0 && e_next < a.ne;
+    double4 ab_next = a.erec[ok_next ? e_next : 0];
+    double4 tn_next = a.etan[ok_next ? e_next : 0];
+    for (int s = 0; s < cnt; ++s) {
+        const bool ok = ok_next;
+        const double4 ab = ab_next, tn = tn_next;
+        if (s + 1 < cnt) {
+            e_next = sl[s + 1];
+            ok_next = e_next >= 0 && e_next < a.ne;
+            const int en = ok_next ? e_next : 0;
+            ab_next = a.erec[en];
+            tn_next = a.etan[en];
+        }
+        const double ux = ab.x - x, uy = ab.y - y, vx = ab.z - x, vy = ab.w - y;
+        const double tx = tn.x, ty = tn.y, iell = tn.z;
+        const bool valid = ok && iell != 0.0;
+        double cr = wsb_cross2(ux, uy, vx, vy);
+        const double mag = fabs(ux * vy) + fabs(uy * vx);
+        if (!(mag > 1e-280 && fabs(cr) > 1.7763568394002505e-15 * mag)) cr = wsb_cross2(ux / h, uy / h, vx / h, vy / h) * h * h;
+        const double z = cr * ih * iell;
+        const double px = ux * ih, py = uy * ih, qx = vx * ih, qy = vy * ih;
+        const double s0 = tx * px + ty * py, s1 = tx * qx + ty * qy;
+        const double n0 = ty, n1 = -tx;
+        const double az = fabs(z), zz = z * z;
+        const double r0sq = px * px + py * py, r1sq = qx * qx + qy * qy;
+        const double r0 = closed::xsqrt<FORM>(r0sq), r1 = closed::xsqrt<FORM>(r1sq);
+        BlockSums S[4] = {};
+        double th1 = 0.0, th05 = 0.0;
+        radius_sums<MASK, FORM, false>(1.0, valid, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th1, S);
+        double c[wsb::NCH];
+        // the single-radius kernels first, so that only the cubic's sums live through its second radius
+        if (MASK & 2) {
+            kernel_channels(S[1], KTab<WSB_W2>::MRP[0] * th1, KTab<WSB_W2>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+#pragma unroll
+            for (int q = 0; q < wsb::NCH; ++q) acc[1][q] += c[q];
+        }
+        if (MASK & 4) {
+            kernel_channels(S[2], KTab<WSB_W4>::MRP[0] * th1, KTab<WSB_W4>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+#pragma unroll
+            for (int q = 0; q < wsb::NCH; ++q) acc[2][q] += c[q];
+        }
+        if (MASK & 8) {
+            kernel_channels(S[3], KTab<WSB_W6>::MRP[0] * th1, KTab<WSB_W6>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+#pragma unroll
+            for (int q = 0; q < wsb::NCH; ++q) acc[3][q] += c[q];
+        }
+        if (MASK & 1) {
+            radius_sums<MASK, FORM, true>(0.5, valid, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th05, S);
+            kernel_channels(S[0], KTab<WSB_CUBIC>::MRP[0] * th1 + KTab<WSB_CUBIC>::MRP[1] * th05,
+                            KTab<WSB_CUBIC>::MRF[0] * th1 + KTab<WSB_CUBIC>::MRF[1] * th05, z, n0, n1, tx, ty, c);
+#pragma unroll
+            for (int q = 0; q < wsb::NCH; ++q) acc[0][q] += c[q];
+        }
+    }
+    const double ind = a.ind[row];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!(MASK & (1 << k))) continue;
+#pragma unroll
+        for (int q = 0; q < wsb::NCH; ++q) acc[k][q] *= WSB_INV_PI;
+        const double* plane = k == 0 ? KTab<WSB_CUBIC>::PLANE : k == 1 ? KTab<WSB_W2>::PLANE : k == 2 ? KTab<WSB_W4>::PLANE
+                                                                                                      : KTab<WSB_W6>::PLANE;
+        finalize_row(acc[k], ind, h, plane, a.rot4, a.flags, ma.out[k] + (int64_t)row * wsb::NCH);
+    }
+}
+
+// one position per thread.  With nwork only the first nwork[0] positions are served (the rows with slots of a count
+// order); a workgroup past them ends at its first instruction.
+template <int MASK, int FORM>
+__global__ __launch_bounds__(RHEAVY_T) __attribute__((amdgpu_waves_per_eu(2))) void rows12_multi_kernel(MultiArgs ma) {
+    const int npos = ma.nwork ? min(ma.t.n, ma.nwork[0]) : ma.t.n;
+    if ((int64_t)blockIdx.x * RHEAVY_T >= npos) return;
+    const int64_t pos = (int64_t)blockIdx.x * RHEAVY_T + threadIdx.x;
+    if (pos < npos) serve_row<MASK, FORM>(ma, (int)pos);
+}
+
+// with nwork: the indicator term of every row in input order (consecutive threads store consecutive rows); the rows
+// with slots are then written again, complete, by rows12_multi_kernel.  Under WSB_OUT_ACCUMULATE only the rows with no
+// slot are added here (each row receives exactly one contribution).  A kernel of few registers, so that enough
+// wavefronts are resident to keep the stores at HBM speed.
+template <int MASK>
+__global__ __launch_bounds__(RROW_T) void rows12_indicator_kernel(MultiArgs ma) {
+    const TileArgs& a = ma.t;
+    __shared__ double stage[RROW_T * wsb::NCH];
+    const int64_t r0 = (int64_t)blockIdx.x * RROW_T;
+    const int row = (int)r0 + threadIdx.x;
+    const int rows = (int)min((int64_t)RROW_T, a.n - r0);
+    const bool acc = (a.flags & WSB_OUT_ACCUMULATE) != 0;
+    const bool live = row < a.n;
+    const double ind = live ? a.ind[row] : 0.0, h = live ? a.lsup[row] : 1.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!(MASK & (1 << k))) continue;
+        if (live) {
+            double c[wsb::NCH];
+#pragma unroll
+            for (int q = 0; q < wsb::NCH; ++q) c[q] = 0.0 * WSB_INV_PI;      // the empty row sum, as rows12_multi_kernel
+            const double* plane = k == 0 ? KTab<WSB_CUBIC>::PLANE : k == 1 ? KTab<WSB_W2>::PLANE : k == 2 ? KTab<WSB_W4>::PLANE
+                                                                                                          : KTab<WSB_W6>::PLANE;
+            finalize_row(c, ind, h, plane, a.rot4, a.flags & ~WSB_OUT_ACCUMULATE, stage + threadIdx.x * wsb::NCH);
+        }
+        __syncthreads();
+        double* o = ma.out[k] + r0 * wsb::NCH;
+        if (!acc) {
+            // the workgroup's rows are one contiguous block of the output: 16-byte stores, consecutive lanes adjacent
+            const double2* st2 = reinterpret_cast<const double2*>(stage);
+            double2* o2 = reinterpret_cast<double2*>(o);
+            for (int j = threadIdx.x; j < rows * wsb::NCH / 2; j += RROW_T) o2[j] = st2[j];
+        } else {
+            for (int j = threadIdx.x; j < rows * wsb::NCH; j += RROW_T) {
+                const int r = (int)r0 + j / wsb::NCH;
+                if (min(max(a.counts[r], 0), a.K) == 0) o[j] += stage[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <int MASK, int FORM>
 int launch_multi(const MultiArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((tile12_multi_kernel<MASK, FORM>), dim3((a.t.n + MTILE_R - 1) / MTILE_R), dim3(MTILE_T), 0, s, a);
+    if (a.nwork) hipLaunchKernelGGL((rows12_indicator_kernel<MASK>), dim3((a.t.n + RROW_T - 1) / RROW_T), dim3(RROW_T), 0, s, a);
+    hipLaunchKernelGGL((rows12_multi_kernel<MASK, FORM>), dim3((a.t.n + RHEAVY_T - 1) / RHEAVY_T), dim3(RHEAVY_T), 0, s, a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
@@ -643,9 +787,10 @@ extern "C" int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form
     return wsb::set_error(WSB_ENOTSUP, "unknown form %d", form);
 }
 
-extern "C" int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t* order, const int32_t* slots, const int32_t* counts,
-                                   int K, const double* ind, const double* lpos, const double* lsup, int n, const double* rot4,
-                                   int flags, double* const* outs, void* stream) {
+extern "C" int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t* order, const int32_t* nwork,
+                                   const int32_t* slots, const int32_t* counts, int K, const double* ind, const double* lpos,
+                                   const double* lsup, int n, const double* rot4, int flags, double* const* outs,
+                                   void* stream) {
     WSB_REQUIRE(g && outs, WSB_EINVAL, "null argument");
     WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes n=%d K=%d", n, K);
     WSB_REQUIRE(!(flags & WSB_OUT_WORLD) || rot4, WSB_EINVAL, "WSB_OUT_WORLD needs rot4");
@@ -659,7 +804,133 @@ extern "C" int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t*
     WSB_REQUIRE(mask != 0, WSB_EINVAL, "no kernel requested");
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(slots && counts && ind && lpos && lsup, WSB_EINVAL, "null buffer");
+    WSB_REQUIRE(!nwork || order, WSB_EINVAL, "nwork needs the order it counts (wsb_count_order2d)");
     a.t = TileArgs{order, slots, counts, K, ind, lpos, lsup, n, g->erec, g->etan, g->ne, rot4, flags, nullptr};
+    a.nwork = nwork;
     hipStream_t s = wsb::as_stream(stream);
     return form == WSB_FORM_FP64 ? dispatch_multi<closed::FP64>(mask, a, s) : dispatch_multi<closed::MIXED>(mask, a, s);
+}
+
+// ------------------------------------------------------------------------------------------------ count order
+// Rows grouped by slot count, the largest count first (no counterpart in the reference: it decides which thread serves
+// a row, never a result); the rows with no slot come last and their number is left on the device (nwork = the rows
+// with slots).  Counting sort over CORD_NB bins (counts at or above CORD_NB - 1 share the last bin), each workgroup
+// over a chunk of CORD_CH rows: a histogram (rows with no slot counted per wavefront by a ballot, the others by LDS
+// atomics, one global atomic per bin and workgroup), one workgroup's exclusive scan in descending count order, and a
+// scatter in which each workgroup reserves one block of positions per bin and places its rows by their ranks.  The
+// counters are cleared by a kernel (no memset node when a caller captures the launches).
+namespace {
+
+constexpr int CORD_NB = 1024;
+constexpr int CORD_T = 256;
+constexpr int CORD_CH = 16 * CORD_T;
+
+__device__ __forceinline__ int cord_bin(const int32_t* counts, const int32_t* base, int p, int K) {
+    const int r = base ? base[p] : p;
+    return min(max(min(counts[r], K), 0), CORD_NB - 1);
+}
+
+__global__ __launch_bounds__(CORD_T) void cord_clear_kernel(int32_t* hist, int32_t* cursor) {
+    for (int b = threadIdx.x; b < CORD_NB; b += CORD_T) {
+        hist[b] = 0;
+        cursor[b] = 0;
+    }
+}
+
+// LDS histogram of the workgroup's chunk
+__device__ __forceinline__ void cord_chunk_hist(const int32_t* counts, const int32_t* base, int n, int K, int32_t* lh) {
+    for (int b = threadIdx.x; b < CORD_NB; b += CORD_T) lh[b] = 0;
+    __syncthreads();
+    const int p0 = blockIdx.x * CORD_CH;
+    for (int it = 0; it < CORD_CH / CORD_T; ++it) {
+        const int p = p0 + it * CORD_T + threadIdx.x;
+        const int bin = p < n ? cord_bin(counts, base, p, K) : -1;
+        const uint64_t zero = __ballot(bin == 0);
+        if ((threadIdx.x & 63) == 0 && zero) atomicAdd(&lh[0], (int)__popcll(zero));
+        if (bin > 0) atomicAdd(&lh[bin], 1);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(CORD_T) void cord_hist_kernel(const int32_t* counts, const int32_t* base, int n, int K,
+                                                           int32_t* hist) {
+    __shared__ int32_t lh[CORD_NB];
+    cord_chunk_hist(counts, base, n, K, lh);
+    for (int b = threadIdx.x; b < CORD_NB; b += CORD_T)
+        if (lh[b]) atomicAdd(&hist[b], lh[b]);
+}
+
+// off[b] = number of rows in bins above b (descending count order); off[CORD_NB] = rows with at least one slot
+__global__ __launch_bounds__(CORD_T) void cord_scan_kernel(const int32_t* hist, int32_t* off, int32_t* nwork) {
+    using BlockScan = hipcub::BlockScan<int, CORD_T>;
+    __shared__ typename BlockScan::TempStorage tmp;
+    constexpr int PER = CORD_NB / CORD_T;
+    int v[PER];
+    // thread t owns the bins NB-1-(PER t + i): the scan runs from the largest count down
+#pragma unroll
+    for (int i = 0; i < PER; ++i) v[i] = hist[CORD_NB - 1 - (PER * threadIdx.x + i)];
+    int ex[PER];
+    BlockScan(tmp).ExclusiveSum(v, ex);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) off[CORD_NB - 1 - (PER * threadIdx.x + i)] = ex[i];
+    if (threadIdx.x == CORD_T - 1) {
+        off[CORD_NB] = ex[PER - 1];      // the exclusive sum at bin 0: the rows with at least one slot
+        if (nwork) *nwork = ex[PER - 1];
+    }
+}
+
+__global__ __launch_bounds__(CORD_T) void cord_scatter_kernel(const int32_t* counts, const int32_t* base, int n, int K,
+                                                              const int32_t* off, int32_t* cursor, int32_t* order) {
+    __shared__ int32_t lh[CORD_NB];
+    __shared__ int32_t lbase[CORD_NB];
+    cord_chunk_hist(counts, base, n, K, lh);
+    for (int b = threadIdx.x; b < CORD_NB; b += CORD_T) {
+        lbase[b] = lh[b] ? off[b] + atomicAdd(&cursor[b], lh[b]) : 0;
+        lh[b] = 0;
+    }
+    __syncthreads();
+    const int p0 = blockIdx.x * CORD_CH;
+    const int lane = threadIdx.x & 63;
+    for (int it = 0; it < CORD_CH / CORD_T; ++it) {
+        const int p = p0 + it * CORD_T + threadIdx.x;
+        const int bin = p < n ? cord_bin(counts, base, p, K) : -1;
+        // rows with no slot: the wavefront's block (one LDS atomic) + the set lanes below in the ballot, so they keep
+        // their order inside a wavefront
+        const uint64_t zero = __ballot(bin == 0);
+        int wbase = 0;
+        if (lane == 0 && zero) wbase = atomicAdd(&lh[0], (int)__popcll(zero));
+        wbase = __shfl(wbase, 0);
+        int rank = 0;
+        if (bin == 0) rank = wbase + (int)__popcll(zero & ((1ull << lane) - 1ull));
+        else if (bin > 0) rank = atomicAdd(&lh[bin], 1);
+        if (bin >= 0) order[lbase[bin] + rank] = base ? base[p] : p;
+    }
+}
+
+}  // namespace
+
+extern "C" size_t wsb_count_order2d_scratch_bytes(void) { return (3 * CORD_NB + 1) * sizeof(int32_t); }
+
+extern "C" int wsb_count_order2d(const int32_t* counts, const int32_t* base, int n, int K, int32_t* order, int32_t* nwork,
+                                 void* scratch, size_t scratch_bytes, void* stream) {
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes n=%d K=%d", n, K);
+    WSB_REQUIRE(scratch_bytes >= wsb_count_order2d_scratch_bytes(), WSB_EINVAL, "scratch too small: %zu < %zu",
+                scratch_bytes, wsb_count_order2d_scratch_bytes());
+    WSB_REQUIRE(scratch, WSB_EINVAL, "null scratch");
+    int32_t* hist = static_cast<int32_t*>(scratch);
+    int32_t* cursor = hist + CORD_NB;
+    int32_t* off = cursor + CORD_NB;       // CORD_NB + 1 entries
+    hipStream_t s = wsb::as_stream(stream);
+    hipLaunchKernelGGL(cord_clear_kernel, dim3(1), dim3(CORD_T), 0, s, hist, cursor);
+    if (n > 0) {
+        WSB_REQUIRE(counts && order, WSB_EINVAL, "null buffer");
+        const int nb = (n + CORD_CH - 1) / CORD_CH;
+        hipLaunchKernelGGL(cord_hist_kernel, dim3(nb), dim3(CORD_T), 0, s, counts, base, n, K, hist);
+        hipLaunchKernelGGL(cord_scan_kernel, dim3(1), dim3(CORD_T), 0, s, hist, off, nwork);
+        hipLaunchKernelGGL(cord_scatter_kernel, dim3(nb), dim3(CORD_T), 0, s, counts, base, n, K, off, cursor, order);
+    } else {
+        hipLaunchKernelGGL(cord_scan_kernel, dim3(1), dim3(CORD_T), 0, s, hist, off, nwork);
+    }
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
 }

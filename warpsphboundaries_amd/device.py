@@ -92,6 +92,11 @@ class Geom2D:
         L.check(L.load().wsb_sort2d(self.handle, L.ptr(lpos), n, L.ptr(order), L.ptr(scratch), nbytes, L.stream_ptr(dev)))
         return order
 
+    def count_order(self, counts, base=None, order=None, nwork=None, scratch=None):
+        """(order [N] int32, nwork [1] int32): the rows grouped by slot count, largest first, and the number of rows
+        with slots (``count_order``): the serving order of ``moments_multi``, one row per thread."""
+        return count_order(counts, self.K, base, order, nwork, scratch)
+
     def adjacency(self, lpos, lsup, valid=None, pad=True, slots=None, counts=None, ind=None, order=None):
         """(slots [N*K] int32, counts [N] int32, ind [N] f64) -- the bit-exact fixed-capacity adjacency."""
         L.require_device(lpos, lsup)
@@ -134,9 +139,11 @@ class Geom2D:
                                        L.stream_ptr(dev)))
         return out
 
-    def moments_multi(self, kernels, slots, counts, ind, lpos, lsup, form="fp64", rot4=None, flags=0, outs=None, order=None):
+    def moments_multi(self, kernels, slots, counts, ind, lpos, lsup, form="fp64", rot4=None, flags=0, outs=None, order=None,
+                      nwork=None):
         """{kernel: [N, 12]} for several of the scheme kernels (cubic, w2, w4, w6) in one pass: each pair's chord
-        primitives are shared by the kernels (``wsb_moments2d_multi``); the same values as ``moments`` per kernel."""
+        primitives are shared by the kernels (``wsb_moments2d_multi``); the same values as ``moments`` per kernel.
+        ``order`` / ``nwork``: the serving order and its count of rows with slots (``count_order``)."""
         L.require_device(lpos, lsup, slots, counts, ind)
         n = lpos.shape[0]
         dev = lpos.device
@@ -153,7 +160,9 @@ class Geom2D:
             ptrs[kid] = o.data_ptr()
         lpos = lpos.to(torch.float64).contiguous()
         lsup = lsup.to(torch.float64).contiguous()
-        L.check(L.load().wsb_moments2d_multi(self.handle, L.form_id(form), L.ptr(order), L.ptr(slots), L.ptr(counts), self.K,
+        if nwork is not None and (order is None or nwork.dtype != torch.int32 or nwork.numel() < 1):
+            raise ValueError("nwork: the int32 device count of count_order, with the order it counts")
+        L.check(L.load().wsb_moments2d_multi(self.handle, L.form_id(form), L.ptr(order), L.ptr(nwork), L.ptr(slots), L.ptr(counts), self.K,
                                              L.ptr(ind), L.ptr(lpos), L.ptr(lsup), n, L.ptr(rot4), int(flags), ptrs,
                                              L.stream_ptr(dev)))
         return res
@@ -229,6 +238,28 @@ class Disks2D:
                                             L.ptr(lpos), L.ptr(lsup), n, L.ptr(rot4), int(flags), L.ptr(out),
                                             L.stream_ptr(dev)))
         return out
+
+
+def count_order(counts, K, base=None, order=None, nwork=None, scratch=None):
+    """(order [N] int32, nwork [1] int32): the rows (of ``base`` [N] when given, else 0..N-1) grouped by slot count
+    (counts clamped to K), largest first, and the number of rows with at least one slot (``wsb_count_order2d``)."""
+    L.require_device(counts)
+    n = counts.shape[0]
+    dev = counts.device
+    if counts.dtype != torch.int32 or not counts.is_contiguous():
+        raise ValueError("counts: a contiguous int32 device tensor")
+    if base is not None and (base.dtype != torch.int32 or base.numel() != n or base.device != dev):
+        raise ValueError("base: an int32 permutation of the N rows on the device of counts")
+    if order is None:
+        order = torch.empty(n, dtype=torch.int32, device=dev)
+    if nwork is None:
+        nwork = torch.empty(1, dtype=torch.int32, device=dev)
+    nbytes = int(L.load().wsb_count_order2d_scratch_bytes())
+    if scratch is None or scratch.numel() < nbytes:
+        scratch = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    L.check(L.load().wsb_count_order2d(L.ptr(counts), L.ptr(base), n, int(K), L.ptr(order), L.ptr(nwork), L.ptr(scratch),
+                                       nbytes, L.stream_ptr(dev)))
+    return order, nwork
 
 
 def disk_channels_host(kernel, D, R):
