@@ -17,6 +17,7 @@ Contents (each function cites the reference file:line it restates):
                 slot fill (``scene/scene.py:869-888``, ``scene/fixedadj.py:199-222``).
 * ``cone2d``, ``triangle2d``  the cone area / direction extreme and the conditioned triangle quadrature.
 * ``contour2d`` the marching-squares zero level set of a sampled distance (``scene/scene.py:762-806``).
+* ``sdf2d``     the bilinear signed distance of a sampled field with node-blended jets (``scene/scene.py:665-706``).
 * ``arith``     the arithmetic the restatements run in: float64 (default), x86 long double, or mpmath.
 
 How each part is pinned (tests/test_oracle_*.py, tests/test_mesh3d_cpu.py): the reference's 40-digit 2-D golden

@@ -3,7 +3,9 @@
 INFRASTRUCTURE ONLY.
 
 Every cell (i outer, j inner) whose four nodes are not all of one sign gets, on each of its edges whose end nodes
-differ (solid = d < 0), the vertex at the linear zero d[a] / (d[a] - d[b]) (shared with the neighbouring cell); the
+differ (solid = d < 0), the vertex at the linear zero d[a] / (d[a] - d[b]); a vertex is keyed by the directed cell
+edge, so the two cells that share a grid edge each number their own copy of its crossing (the loops close by
+position, not by index); the
 crossings are taken counter-clockwise around the cell and each solid -> fluid crossing is joined to the next
 fluid -> solid crossing (the previous one, for a four-crossing cell whose node mean is solid), so the solid lies on
 the left of every edge.  Border nodes all fluid: background 0; all solid: background 1; otherwise an error.

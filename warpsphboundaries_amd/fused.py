@@ -100,10 +100,14 @@ class FusedWall(WallAggregate):
         self._scratch = None
         for gi, grp in enumerate(self.groups):
             for bi, ba in enumerate(adjacency.bodies):
-                R = ba.body.pose.Rflat().to(dev, F64).contiguous()
+                R = ba.body.rot4(dev)
                 reps = ba.repList if ba.repList is not None else ba.body.reps
+                # the rows of a compacted adjacency are its candidates: evaluated apart, then put at their places
+                dst = torch.zeros((len(ba.cand), 12), dtype=F64, device=dev) if ba.compact else self.channels[gi, bi]
                 for rep, topo in zip(reps, ba.reps):
-                    rep_channels(rep, topo, ba, grp.kernel, grp.form, R, self.channels[gi, bi], accumulate=True)
+                    rep_channels(rep, topo, ba, grp.kernel, grp.form, R, dst, accumulate=True)
+                if ba.compact:
+                    self.channels[gi, bi].index_copy_(0, ba.cand, dst)
             if grp.channels is not None:
                 keep = torch.zeros(12, dtype=F64, device=dev)
                 keep[list(grp.channels)] = 1.0
@@ -163,21 +167,40 @@ class FusedWall(WallAggregate):
         ax = axes.to(dev, F64).contiguous()
         lib = L.load()
         for bi, ba, rep, topo, c, s in self._reps():
+            n = len(ba.cand)
+            part = torch.zeros(2 * n, dtype=F64, device=dev) if ba.compact else out
+            axp = ax[ba.cand].contiguous() if ba.compact else ax
             if isinstance(rep, DiskArrayRep):
                 L.check(lib.wsb_disk_cone_area2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K,
-                                                 L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(ax), c, s, float(half_angle),
-                                                 L.ptr(out), L.stream_ptr(dev)))
-                continue
-            L.check(lib.wsb_cone_area2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K, L.ptr(topo.ind),
-                                        L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(ax), c, s, float(half_angle),
-                                        L.ptr(out), L.stream_ptr(dev)))
-            if rep.background and ba.valid is not None:
+                                                 L.ptr(ba.lpos), L.ptr(ba.lsup), n, L.ptr(axp), c, s, float(half_angle),
+                                                 L.ptr(part), L.stream_ptr(dev)))
+            else:
+                L.check(lib.wsb_cone_area2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K,
+                                            L.ptr(topo.ind), L.ptr(ba.lpos), L.ptr(ba.lsup), n, L.ptr(axp), c, s,
+                                            float(half_angle), L.ptr(part), L.stream_ptr(dev)))
+            if ba.compact:
+                out[:N].index_add_(0, ba.cand, part[:n])
+                out[N:].index_add_(0, ba.cand, part[n:])
+            excluded = self._excluded(ba)
+            if getattr(rep, "background", 0) and excluded is not None:
                 # a query the adjacency left out of a background representation is deep in its solid: the full sector
-                out_of = (~ba.valid).to(F64)
+                out_of = excluded.to(F64)
+                h2 = self._supports() ** 2
                 wedge = 2.0 * float(half_angle) if float(half_angle) < math.pi else 2.0 * math.pi
-                out[:N] += out_of * 0.5 * ba.lsup * ba.lsup * wedge
-                out[N:] += out_of * math.pi * ba.lsup * ba.lsup
+                out[:N] += out_of * 0.5 * h2 * wedge
+                out[N:] += out_of * math.pi * h2
         return out.reshape(2, N)
+
+    def _supports(self):
+        return self.adj.queryParticles.supports.to(self.dev, F64)
+
+    def _excluded(self, ba):
+        """bool [N]: the queries a body's adjacency left out (None: none)."""
+        if ba.compact:
+            kept = torch.zeros(self.N, dtype=torch.bool, device=self.dev)
+            kept[ba.cand] = True
+            return ~kept
+        return None if ba.valid is None else ~ba.valid
 
     def dir_extreme(self, vecs):
         """[B, N] float64: for the world vectors ``vecs`` [B, N, 2] the largest |cos| between the vector and a
@@ -190,16 +213,22 @@ class FusedWall(WallAggregate):
         lib = L.load()
         for bi, ba, rep, topo, c, s in self._reps():
             seg = out[bi * N:(bi + 1) * N]
+            n = len(ba.cand)
+            part = torch.zeros(n, dtype=F64, device=dev) if ba.compact else seg
+            vp = v[bi][ba.cand].contiguous() if ba.compact else v[bi]
             if isinstance(rep, DiskArrayRep):
                 L.check(lib.wsb_disk_dir_extreme2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K,
-                                                   L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(v[bi]), c, s, L.ptr(seg),
+                                                   L.ptr(ba.lpos), L.ptr(ba.lsup), n, L.ptr(vp), c, s, L.ptr(part),
                                                    L.stream_ptr(dev)))
-                continue
-            L.check(lib.wsb_dir_extreme2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K, L.ptr(topo.ind),
-                                          L.ptr(ba.lpos), L.ptr(ba.lsup), N, L.ptr(v[bi]), c, s, L.ptr(seg),
-                                          L.stream_ptr(dev)))
-            if rep.background and ba.valid is not None:
-                seg[(~ba.valid) & (v[bi].norm(dim=1) > 0)] = 1.0
+            else:
+                L.check(lib.wsb_dir_extreme2d(topo.geom.handle, L.ptr(topo.e), L.ptr(topo.counts), topo.K,
+                                              L.ptr(topo.ind), L.ptr(ba.lpos), L.ptr(ba.lsup), n, L.ptr(vp), c, s,
+                                              L.ptr(part), L.stream_ptr(dev)))
+            if ba.compact:
+                seg.index_copy_(0, ba.cand, torch.maximum(seg[ba.cand], part))
+            excluded = self._excluded(ba)
+            if getattr(rep, "background", 0) and excluded is not None:
+                seg[excluded & (v[bi].norm(dim=1) > 0)] = 1.0
         return out.reshape(B, N)
 
     @staticmethod

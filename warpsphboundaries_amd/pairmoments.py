@@ -15,7 +15,7 @@ from typing import Optional
 
 import torch
 
-from .scene import SceneAdjacency, fixed_adjacency, kernelName
+from .scene import SceneAdjacency, kernelName
 
 F64 = torch.float64
 
@@ -60,7 +60,7 @@ def precompute(scene, adjacency: SceneAdjacency, operationProperties, channels=N
     entries, pairs = [], 0
     for ba in adjacency.bodies:
         ent = {"body": ba.body, "cand": ba.cand, "surface": [], "implicit": [], "volume": []}
-        R = ba.body.pose.Rflat().to(dev, F64).contiguous()
+        R = ba.body.rot4(dev)
         reps = ba.repList if ba.repList is not None else ba.body.reps
         for rep, topo in zip(reps, ba.reps):
             c = rep_channels(rep, topo, ba, name, form, R)
@@ -83,8 +83,8 @@ def precompute(scene, adjacency: SceneAdjacency, operationProperties, channels=N
 
 
 def pair_moments(scene, queryParticles, operationProperties, channels=None, form=None) -> PairMoments:
-    """``Scene.pairMoments``: adjacency + precompute in one call."""
-    return precompute(scene, fixed_adjacency(scene, queryParticles), operationProperties, channels, form)
+    """``Scene.pairMoments``: the broadphase adjacency (``Scene.adjacency``) + precompute in one call."""
+    return precompute(scene, scene.adjacency(queryParticles, operationProperties), operationProperties, channels, form)
 
 
 def moments(scene, source, queryParticles, operationProperties, channels=None, form=None) -> PairMoments:

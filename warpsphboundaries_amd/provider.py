@@ -1,12 +1,15 @@
-"""The analytic boundary as a PROVIDER for an SPH scheme (reference ``scene/provider.py``).
+"""Boundary provider over the analytic bodies of a scene (the protocol of the reference's ``scene/provider.py``).
 
-    aggregate(ps, support, ...)   the integrals of every body at the particle positions of ``ps``: a ``FusedWall``
-                                  whose ``.out`` holds lam [B,N], G [B,N,2], Cov [B,N,2,2] (world frame, float64)
-                                  and, in the full layout, cover [B,N,2], lap [B,N], tens [B,N,2] (raw, see
-                                  ``wallops`` for the factors)
-    signed_distance(x, ...)       (d, n, hit[, body]) of points to the nearest body: d > 0 in the fluid, n into the fluid
-    bodies                        the rigid bodies of the scene
-    supported(...)                whether the fused path serves the scene
+A particle scheme hands the provider its particle state once per position set and reads back, per body and particle,
+the wall integrals it needs; the physics of the boundary conditions stays in the scheme.
+
+    aggregate(ps, support, kernel, laplacian, fixedAdjacency, lean)
+        -> a FusedWall; ``.out`` maps  lam [B,N] (int W),  G [B,N,2] (int grad W),  Cov [B,N,2,2] (int y (x) grad W)
+           and, unless ``lean``, the raw cover [B,N,2], lap [B,N] (with ``laplacian``) and tens [B,N,2] integrals
+           (``wallops`` holds their factors); further outputs through ``evaluate``, ``cone_area``, ``dir_extreme``
+    signed_distance(x, body, supportMax, want_body)
+        -> (d, n, hit[, body]): distance to the nearest solid, positive in the fluid, normal pointing into the fluid
+    bodies, supported(fixedAdjacency)
 """
 from __future__ import annotations
 
@@ -29,7 +32,8 @@ class BoundaryProvider(Protocol):
 
 
 class AnalyticBoundary:
-    """the analytic bodies of a ``Scene`` as a boundary provider (exact edge-reduction integrals on the MI355X)."""
+    """serves a ``Scene`` through the ``BoundaryProvider`` protocol: every integral is the exact closed form of the
+    bodies' edges, evaluated by the HIP library in the arithmetic ``form`` (the process precision when None)."""
 
     def __init__(self, scene: Scene, form: str = None):
         self.scene = scene
@@ -41,7 +45,8 @@ class AnalyticBoundary:
         return self.scene.bodies
 
     def family(self, kernel):
-        """Wendland family of the exact wall operations (tensile, wall Laplacian): C2 and C4 only."""
+        """the Wendland family ('w2' or 'w4') whose wall Laplacian and tensile kernels exist; other kernels raise
+        NotImplementedError."""
         fam = _FAMILY.get(kernelName(kernel))
         if fam is None:
             raise NotImplementedError("exact wall operations: Wendland C2 and C4 only")
@@ -54,10 +59,10 @@ class AnalyticBoundary:
         return {"kernel": kernelName(kernel), "operation": operation, "operationMode": "BoundaryToFluid"}
 
     def layout(self, kernel, laplacian, fixedAdjacency=True):
-        """(groups, roles): the fused kernel groups the scheme needs and the group index of each role (reference
-        ``scene/provider.py:73-89``): the scheme kernel 'w', the cover kernel 'cone' (gradient channels), the wall
-        Laplacian 'lap' (``l<fam>``) when ``laplacian``, and the tensile kernel 'tens' (``<fam>p5``, gradient
-        channels); built once per configuration."""
+        """(groups, roles) of the full layout (reference ``scene/provider.py:73-89``): a tuple of FusedGroups and the
+        group index of each role -- 'w' the scheme kernel, 'cone' the cover kernel's gradient, 'lap' the wall Laplacian
+        kernel ``l<fam>`` (only with ``laplacian``), 'tens' the gradient of the tensile kernel ``<fam>p5`` -- cached per
+        (kernel, laplacian, fixedAdjacency)."""
         key = (kernelName(kernel), bool(laplacian), bool(fixedAdjacency))
         if key not in self._layouts:
             fam = self.family(kernel)
@@ -72,17 +77,16 @@ class AnalyticBoundary:
         return self._layouts[key]
 
     def aggregate(self, ps, support, kernel="w2", laplacian=False, fixedAdjacency=True, lean=False):
-        """the fused wall evaluation at the positions of ``ps``; ``support``: host float >= every support of ``ps``.
-        ``.out``: lam, G, Cov of the scheme kernel, and unless ``lean`` the raw cover (gradient of the cone kernel),
-        lap (2 lam - tr Cov of ``l<fam>``, with ``laplacian``) and tens (gradient of ``<fam>p5``): the scheme applies
-        the factors (``wallops.cover_factor``, ``lap_factor``, ``tensile_factor``)."""
-        if not fixedAdjacency:
-            raise NotImplementedError("only the fixed-capacity (graph-capturable) adjacency is built")
+        """a FusedWall at the positions of ``ps`` (``support``: a host float no smaller than any support of ``ps``).
+        The adjacency is the fixed-capacity one (graph-capturable) or, with ``fixedAdjacency=False``, the broadphase
+        candidates of ``Scene.adjacency``; both give the same integrals.  ``.out`` holds lam, G, Cov of the scheme
+        kernel and, unless ``lean``, the raw cover, lap (2 lam - tr Cov of ``l<fam>``, with ``laplacian``) and tens
+        integrals, whose factors ``wallops`` supplies."""
         if lean:
             groups, idx = (FusedGroup(kernelName(kernel), None, self.form),), {"w": 0}
         else:
             groups, idx = self.layout(kernel, laplacian, fixedAdjacency)
-        adj = fixed_adjacency(self.scene, ps, None, support)
+        adj = fixed_adjacency(self.scene, ps, None, support) if fixedAdjacency else self.scene.adjacency(ps, None, support)
         fw = FusedWall(self.scene, adj, groups)
         outs = [WallOutput("lam", 0, "lam"), WallOutput("G", 0, "g0"), WallOutput("Cov", 0, "cov")]
         if not lean:

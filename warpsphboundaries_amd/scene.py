@@ -71,13 +71,15 @@ class Pose:
         d = p - self.center
         return torch.stack([d[:, 0] * c + d[:, 1] * s, d[:, 1] * c - d[:, 0] * s], 1)
 
-    def toWorld(self, p):
-        c, s = self._cs()
-        return torch.stack([p[:, 0] * c - p[:, 1] * s + self.center[0], p[:, 0] * s + p[:, 1] * c + self.center[1]], 1)
-
     def vecToWorld(self, v):
+        """R v of body-frame vectors v [N, 2]."""
         c, s = self._cs()
-        return torch.stack([v[:, 0] * c - v[:, 1] * s, v[:, 0] * s + v[:, 1] * c], 1)
+        x, y = v[:, 0], v[:, 1]
+        return torch.stack((c * x - s * y, s * x + c * y), dim=1)
+
+    def toWorld(self, p):
+        """body-frame points [N, 2] to the world: R p + center."""
+        return self.vecToWorld(p) + self.center
 
 
 # ----------------------------------------------------------------------------------------------------------------------- surfaces
@@ -275,12 +277,20 @@ class ImplicitRep:
             self._disks = DiskArrayRep([self.shape.center], [self.shape.radius]).to(self._device)
         return self._disks
 
+    def tier2Edges(self, hmin):
+        """edges of the tier-2 polygon: at least 24, and no edge longer than maxPolygonEdgeOverH * hmin along the
+        circle."""
+        circumference = 2 * np.pi * self.shape.radius
+        return max(24, int(np.ceil(circumference / (self.policy.maxPolygonEdgeOverH * float(hmin)))))
+
     def fallbackSurface(self, hmin, device):
-        s = self.shape
-        n = max(24, int(np.ceil(2 * np.pi * s.radius / (self.policy.maxPolygonEdgeOverH * float(hmin)))))
-        if n not in self._fallback:
-            self._fallback[n] = SurfaceRep.regularPolygon(s.center, s.radius, n, solid=s.solid).to(device)
-        return self._fallback[n]
+        """the area-preserving regular polygon of the disk at the resolution of ``hmin`` (one per edge count)."""
+        n = self.tier2Edges(hmin)
+        rep = self._fallback.get(n)
+        if rep is None:
+            rep = self._fallback[n] = SurfaceRep.regularPolygon(self.shape.center, self.shape.radius, n,
+                                                                solid=self.shape.solid)
+        return rep.to(device)
 
     def lowered(self, hmin, device):
         if self.shape.solid == "inside":
@@ -306,12 +316,22 @@ class ImplicitRep:
 
 class SdfRep:
     """a sampled signed distance of the body frame: node values ``values`` [nx, ny] (positive in the fluid) on the grid
-    ``origin + spacing * (i, j)`` (reference scene/scene.py:599-670).  The fused evaluation integrates it exactly as
-    a surface: its ``fallback`` SurfaceRep when given, else the zero level set of the samples (``contour``, marching
-    squares).  ``signed`` is the bilinear interpolant with the gradient of the bilinear cell as normal direction."""
+    ``origin + spacing * (i, j)`` (reference scene/scene.py:599-670).  The fused evaluation integrates it exactly as a
+    surface: its ``fallback`` SurfaceRep when given, else the zero level set of the samples (``contour``, marching
+    squares).  ``signed`` blends node jets (value, central-difference gradient and Laplacian) bilinearly.  The keyword
+    arguments are the reference's tier-3 guard thresholds, validated as there; the tier-3 local model itself is not
+    built, so the fused path does not use them."""
     kind = "sdf"
 
-    def __init__(self, values, origin, spacing, fallback: Optional[SurfaceRep] = None, **policy):
+    def __init__(self, values, origin, spacing, fallback: Optional[SurfaceRep] = None, maxKappaH: float = 0.5,
+                 gradTol: float = 0.1, minCurvatureDenominator: float = 0.25, maxCurvatureVariationH2: float = 0.5,
+                 backend: str = "warp"):
+        if backend not in ("warp", "torch"):
+            raise ValueError(f"SdfRep backend {backend!r}: 'warp' or 'torch'")
+        if not 0.0 < float(minCurvatureDenominator) <= 1.0:
+            raise ValueError("SdfRep: minCurvatureDenominator must lie in (0, 1]")
+        if not (math.isfinite(float(maxCurvatureVariationH2)) and float(maxCurvatureVariationH2) > 0.0):
+            raise ValueError("SdfRep: maxCurvatureVariationH2 must be finite and positive")
         self.d = torch.as_tensor(values, dtype=F64)
         if self.d.dim() != 2 or min(self.d.shape) < 2:
             raise ValueError("SdfRep: values must be a [nx, ny] grid with nx, ny >= 2")
@@ -320,21 +340,29 @@ class SdfRep:
         if not self.spacing > 0:
             raise ValueError("SdfRep: spacing must be positive")
         self.fallback = fallback
-        self.policy = dict(policy)          # the reference's tier-3 guard thresholds; the fused path does not use them
+        self.backend = backend
+        self.maxKappaH, self.gradTol = float(maxKappaH), float(gradTol)
+        self.minCurvatureDenominator = float(minCurvatureDenominator)
+        self.maxCurvatureVariationH2 = float(maxCurvatureVariationH2)
         self._contour = None
+        self._jets = None
 
     @staticmethod
     def fromFunction(fn, lo, hi, spacing, **kw):
-        """``fn(points [M, 2]) -> d`` sampled on the grid covering [lo, hi] (reference scene/scene.py:636-642)."""
-        nx = int(np.ceil((hi[0] - lo[0]) / spacing)) + 1
-        ny = int(np.ceil((hi[1] - lo[1]) / spacing)) + 1
-        X, Y = np.meshgrid(lo[0] + spacing * np.arange(nx), lo[1] + spacing * np.arange(ny), indexing="ij")
-        pts = torch.as_tensor(np.stack([X.ravel(), Y.ravel()], 1), dtype=F64)
-        return SdfRep(torch.as_tensor(fn(pts), dtype=F64).reshape(nx, ny), lo, spacing, **kw)
+        """the SdfRep of ``fn(points [M, 2]) -> d`` on the smallest grid of the given spacing that starts at ``lo`` and
+        reaches ``hi`` (reference scene/scene.py:636-642)."""
+        counts = [int(np.ceil((b - a) / spacing)) + 1 for a, b in zip(lo, hi)]
+        axes = [a + spacing * np.arange(n) for a, n in zip(lo, counts)]
+        nodes = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 2)
+        values = torch.as_tensor(fn(torch.as_tensor(nodes, dtype=F64)), dtype=F64).reshape(counts)
+        return SdfRep(values, lo, spacing, **kw)
 
     @property
     def solid(self):
-        return "outside" if self.contour().background else "inside"
+        """'outside' when the solid is unbounded (a tank): decided by the fallback when one is given, else by the
+        border of the samples."""
+        background = self.fallback.background if self.fallback is not None else self.contour().background
+        return "outside" if background else "inside"
 
     def to(self, device):
         self.d, self.origin = self.d.to(device), self.origin.to(device)
@@ -342,6 +370,7 @@ class SdfRep:
             self.fallback.to(device)
         if self._contour is not None:
             self._contour.to(device)
+        self._jets = None
         return self
 
     def bounds(self):
@@ -349,12 +378,7 @@ class SdfRep:
         return self.origin, self.origin + ext * self.spacing
 
     def contour(self):
-        """the zero level set of the samples as a SurfaceRep: on every cell edge whose end nodes differ in sign
-        (solid = d < 0) a vertex at the linear zero, joined cell by cell with the solid on the left of every edge; a
-        cell with four crossings joins them around its solid corners, or across its centre when the mean of its four
-        nodes is solid.  The border nodes must be all fluid (an enclosed solid, background 0) or all solid (a tank,
-        background 1).  Vertices and edges are numbered in the order the cells are visited (i outer, j inner,
-        corners counter-clockwise), as the reference's marching squares (scene/scene.py:762-806)."""
+        """the zero level set of the samples as a SurfaceRep (see ``_contour_of``)."""
         if self._contour is None:
             self._contour = _contour_of(self.d.cpu().numpy(), self.origin.cpu().numpy(), self.spacing).to(self.d.device)
         return self._contour
@@ -362,67 +386,91 @@ class SdfRep:
     def lowered(self, hmin, device):
         return (self.fallback if self.fallback is not None else self.contour()).to(device)
 
+    def _node_jets(self):
+        """[4, nx, ny]: node value, central-difference gradient and five-point Laplacian; at the border the missing
+        neighbour is the node itself (the half one-sided difference there)."""
+        if self._jets is None:
+            d, s = self.d, self.spacing
+            xp, xm = torch.cat((d[1:], d[-1:]), 0), torch.cat((d[:1], d[:-1]), 0)
+            yp, ym = torch.cat((d[:, 1:], d[:, -1:]), 1), torch.cat((d[:, :1], d[:, :-1]), 1)
+            self._jets = torch.stack((d, (xp - xm) / (2 * s), (yp - ym) / (2 * s), (xp + xm + yp + ym - 4 * d) / (s * s)))
+        return self._jets
+
     def signed(self, pos):
-        """(d, n): the bilinear interpolant of the samples (clamped to the grid) and the unit normal into the fluid
-        (its normalised gradient), body frame."""
-        d = self.d.to(pos.device)
-        nx, ny = d.shape
+        """(d, n, kappa, gradNorm) at body-frame points [M, 2]: the node jets blended bilinearly over the cell of the
+        point clamped to the grid, d grown by the distance from the grid beyond it, n the unit gradient (into the
+        fluid), kappa = lap / (1 - d lap) (reference scene/scene.py:680-706)."""
+        jets = self._node_jets().to(pos.device)
+        nx, ny = self.d.shape
         u = (pos.to(F64) - self.origin.to(pos.device)) / self.spacing
-        uc = torch.stack([u[:, 0].clamp(0, nx - 1), u[:, 1].clamp(0, ny - 1)], 1)
-        i0 = torch.stack([uc[:, 0].floor().clamp(0, nx - 2), uc[:, 1].floor().clamp(0, ny - 2)], 1).long()
-        t = uc - i0
-        a, b = i0[:, 0], i0[:, 1]
-        d00, d10, d01, d11 = d[a, b], d[a + 1, b], d[a, b + 1], d[a + 1, b + 1]
-        tx, ty = t[:, 0], t[:, 1]
-        val = (1 - tx) * (1 - ty) * d00 + tx * (1 - ty) * d10 + (1 - tx) * ty * d01 + tx * ty * d11
-        gx = ((1 - ty) * (d10 - d00) + ty * (d11 - d01)) / self.spacing
-        gy = ((1 - tx) * (d01 - d00) + tx * (d11 - d10)) / self.spacing
-        g = torch.stack([gx, gy], 1)
-        return val, g / g.norm(dim=1, keepdim=True).clamp(min=1e-300)
+        uc = torch.stack((u[:, 0].clamp(0, nx - 1), u[:, 1].clamp(0, ny - 1)), 1)
+        cell = torch.stack((uc[:, 0].floor().clamp(0, nx - 2), uc[:, 1].floor().clamp(0, ny - 2)), 1)
+        f = uc - cell
+        ci = cell.long()
+        w0 = (1 - f[:, 0], f[:, 0])
+        w1 = (1 - f[:, 1], f[:, 1])
+        blend = 0
+        for a in (0, 1):
+            for b in (0, 1):
+                blend = blend + (w0[a] * w1[b])[None] * jets[:, ci[:, 0] + a, ci[:, 1] + b]
+        d, gx, gy, lap = blend
+        d = d + ((u - uc) * self.spacing).norm(dim=1)
+        gn = torch.sqrt(gx * gx + gy * gy).clamp(min=1e-300)
+        return d, torch.stack((gx, gy), 1) / gn[:, None], lap / (1.0 - d * lap), gn
 
     def indicator(self, p, supports=None):
-        d, _ = self.signed(p)
+        d = self.signed(p)[0]
         return torch.where(d < 0, 1.0, torch.where(d > 0, 0.0, 0.5)).to(F64)
 
 
+# the four corners of a grid cell counter-clockwise, as offsets from its lower-left node
+_CELL_CORNERS = np.array([(0, 0), (1, 0), (1, 1), (0, 1)])
+
+
 def _contour_of(d, origin, spacing):
-    """the SurfaceRep of the zero level set of node values d [nx, ny] (see ``SdfRep.contour``)."""
+    """the SurfaceRep of the zero level set of node values d [nx, ny] (solid = d < 0), vectorised over the cells.
+
+    A cell with both kinds of corner has a crossing on every side whose two corners differ (the linear zero of the
+    node values along it); each side of each cell is one vertex, numbered cell by cell (i outer, j inner) and side by
+    side counter-clockwise: the two cells that share a grid edge number its crossing once each, as the reference's
+    marching squares does (scene/scene.py:762-806), so the loops close by position.  A side that leaves the solid
+    (its first corner solid) is joined to the next side that enters it: the neighbouring crossing counter-clockwise,
+    or clockwise in a saddle cell whose four nodes average below zero.  The border nodes must be all fluid (an enclosed
+    solid, background 0) or all solid (a tank, background 1)."""
+    d = np.asarray(d, dtype=np.float64)
     nx, ny = d.shape
     solid = d < 0
-    border = np.concatenate([solid[0], solid[-1], solid[1:-1, 0], solid[1:-1, -1]])
-    if border.any() and not border.all():
-        raise ValueError("SdfRep.contour: the sampled grid must cover the body with a margin (all border nodes fluid, "
-                         "or all solid)")
-    background = 1 if border.all() else 0
-    index, verts, edges = {}, [], []
-
-    def vertex(p, q):
-        if (p, q) not in index:
-            t = d[p] / (d[p] - d[q])
-            index[(p, q)] = len(verts)
-            verts.append((origin[0] + spacing * (p[0] + t * (q[0] - p[0])), origin[1] + spacing * (p[1] + t * (q[1] - p[1]))))
-        return index[(p, q)]
-
-    mixed = solid[:-1, :-1].astype(np.int8) + solid[1:, :-1] + solid[1:, 1:] + solid[:-1, 1:]
-    for i, j in zip(*np.nonzero((mixed > 0) & (mixed < 4))):
-        i, j = int(i), int(j)
-        ring = ((i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1))
-        sol = [bool(solid[c]) for c in ring]
-        cross = []                                  # (leaving the solid?, vertex) counter-clockwise along the cell
-        for k in range(4):
-            if sol[k] != sol[(k + 1) % 4]:
-                cross.append((sol[k], vertex(ring[k], ring[(k + 1) % 4])))
-        m = len(cross)
-        step = -1 if (m == 4 and float(d[i:i + 2, j:j + 2].mean()) < 0) else 1
-        for k in range(m):
-            if cross[k][0]:
-                q = (k + step) % m
-                while cross[q][0]:
-                    q = (q + step) % m
-                edges.append((cross[k][1], cross[q][1]))
-    if not edges:
-        raise ValueError("SdfRep.contour: no zero level set in the sampled grid")
-    return SurfaceRep(np.asarray(verts, dtype=float), np.asarray(edges, dtype=np.int32), background=background)
+    ring = np.concatenate((solid[0], solid[-1], solid[1:-1, 0], solid[1:-1, -1]))
+    if ring.any() and not ring.all():
+        raise ValueError("SdfRep.contour: the border nodes of the grid are partly solid; sample the body with a margin "
+                         "of fluid (or, for a tank, of solid) all round")
+    background = int(ring.all())
+    ci, cj = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), indexing="ij")
+    ci, cj = ci.ravel(), cj.ravel()
+    corner = [solid[ci + di, cj + dj] for di, dj in _CELL_CORNERS]             # 4 x [cells]
+    cuts = np.stack([corner[k] != corner[(k + 1) % 4] for k in range(4)], 1)  # [cells, 4]
+    first = np.cumsum(cuts.ravel()) - 1                                       # vertex number of each (cell, side)
+    vid = np.where(cuts.ravel(), first, -1).reshape(-1, 4)
+    cell_k, side_k = np.nonzero(cuts)
+    a = _CELL_CORNERS[side_k] + np.stack((ci[cell_k], cj[cell_k]), 1)
+    b = _CELL_CORNERS[(side_k + 1) % 4] + np.stack((ci[cell_k], cj[cell_k]), 1)
+    da, db = d[a[:, 0], a[:, 1]], d[b[:, 0], b[:, 1]]
+    t = da / (da - db)
+    verts = np.stack([origin[ax] + spacing * (a[:, ax] + t * (b[:, ax] - a[:, ax])) for ax in (0, 1)], 1)
+    # joins: every side leaving the solid, in cell order and counter-clockwise within a cell
+    leave_c, leave_k = np.nonzero(cuts & np.stack(corner, 1))
+    saddle = cuts.sum(1) == 4
+    mean_solid = (d[ci, cj] + d[ci + 1, cj] + d[ci + 1, cj + 1] + d[ci, cj + 1]) / 4 < 0
+    step = np.where(saddle[leave_c] & mean_solid[leave_c], -1, 1)
+    target = (leave_k + step) % 4
+    two = ~saddle[leave_c]
+    # two crossings: the other one, wherever it sits on the cell
+    other = np.argmax(cuts[leave_c] & (np.arange(4)[None] != leave_k[:, None]), axis=1)
+    target = np.where(two, other, target)
+    edges = np.stack((vid[leave_c, leave_k], vid[leave_c, target]), 1)
+    if len(edges) == 0:
+        raise ValueError("SdfRep.contour: the samples have no zero level set")
+    return SurfaceRep(verts, edges.astype(np.int32), background=background)
 
 
 class DiskArrayRep:
@@ -465,22 +513,24 @@ class DiskArrayRep:
         return self._geoms[key]
 
     def signed(self, pos):
-        """(d, n): signed distance (positive in the fluid) to the nearest circle and the unit normal into the fluid
-        there, body frame (reference scene/scene.py:853-867), in chunks of 4096 points."""
-        N = len(pos)
-        out_d = torch.full((N,), float("inf"), dtype=F64, device=pos.device)
-        out_n = torch.zeros((N, 2), dtype=F64, device=pos.device)
+        """(d, n): the smallest |x - c| - R over the disks (positive in the fluid; the first disk on a tie) and the
+        unit vector from that disk's centre, body frame (reference scene/scene.py:853-867).  The points go in blocks
+        sized so that a block's distance table holds about 4 M entries; no host synchronisation."""
         cen = self.centres.to(pos.device)
         rad = self.radii.to(pos.device)
-        for lo in range(0, N, 4096):
-            v = pos[lo:lo + 4096, None, :].to(F64) - cen[None]
-            dist = v.norm(dim=2)
-            d = dist - rad[None]
-            k = d.argmin(1)
-            ar = torch.arange(len(d), device=pos.device)
-            out_d[lo:lo + 4096] = d[ar, k]
-            out_n[lo:lo + 4096] = v[ar, k] / dist[ar, k].clamp(min=1e-300)[:, None]
-        return out_d, out_n
+        p = pos.to(F64)
+        block = max(1, (1 << 22) // max(len(rad), 1))
+        d_out, n_out = [], []
+        for s in range(0, len(p), block):
+            q = p[s:s + block]
+            gap = torch.linalg.vector_norm(q[:, None, :] - cen[None], dim=2) - rad[None]
+            d, k = gap.min(dim=1)
+            v = q - cen[k]
+            d_out.append(d)
+            n_out.append(v / torch.linalg.vector_norm(v, dim=1, keepdim=True).clamp(min=1e-300))
+        if not d_out:
+            return torch.zeros(0, dtype=F64, device=pos.device), torch.zeros((0, 2), dtype=F64, device=pos.device)
+        return torch.cat(d_out), torch.cat(n_out)
 
     def indicator(self, p, supports=None):
         """1 in a disk, 0 in the fluid (on a circle: 1/2)."""
@@ -540,8 +590,29 @@ class Body:
         return self.pose.toLocal(world)
 
     def unbounded(self):
-        return any((isinstance(r, SurfaceRep) and r.background) or getattr(r, "solid", "inside") == "outside"
-                   for r in self.reps)
+        """whether the solid of the body reaches infinity (its broadphase keeps every query): a surface loop with
+        background 1, a tank box, a cavity disk, or a sampled field whose fallback surface has background 1 (the
+        reference's convention, scene/scene.py:1039-1051: a field without a fallback counts as bounded)."""
+        for r in self.reps:
+            if isinstance(r, SurfaceRep):
+                hit = bool(r.background)
+            elif isinstance(r, SdfRep):
+                hit = r.fallback is not None and bool(r.fallback.background)
+            else:
+                hit = getattr(r, "solid", "inside") == "outside"
+            if hit:
+                return True
+        return False
+
+    def rot4(self, device):
+        """R as a row-major [4] float64 device tensor, built once per (angle, device): the fused evaluation of a body
+        whose pose does not change makes no host-to-device copy (it can be captured in a graph)."""
+        key = (float(self.angle), str(torch.device(device)))
+        cache = self.__dict__.setdefault("_rot4", {})
+        if key not in cache:
+            cache.clear()
+            cache[key] = self.pose.Rflat().to(device, F64).contiguous()
+        return cache[key]
 
     def fusedReps(self, hmin, device):
         """the representations this body is evaluated with: a SurfaceRep or DiskArrayRep as it is, a BoxRep its
@@ -562,6 +633,9 @@ class Body:
 
 @dataclass
 class BodyAdjacency:
+    """the queries of one body: ``cand`` their global rows (all N rows for the fixed-capacity adjacency, the
+    broadphase candidates when ``compact``), their body-frame positions and supports, one topology per representation
+    (``repList``), and ``valid`` the broadphase of the fixed-capacity rows."""
     body: object
     cand: torch.Tensor
     lpos: torch.Tensor
@@ -569,6 +643,7 @@ class BodyAdjacency:
     reps: list
     valid: Optional[torch.Tensor] = None
     repList: Optional[list] = None
+    compact: bool = False
 
 
 @dataclass
@@ -590,6 +665,29 @@ class Scene:
             for r in b.reps:
                 r.to(device)
 
+    def adjacency(self, queryParticles, operationProperties=None, supportMax: float = None):
+        """`SceneAdjacency` over the broadphase candidates of every body (reference scene/scene.py:1400-1416): the
+        fluid queries whose support circle meets the body box (every fluid query of an unbounded solid), compacted
+        (this reads the candidate counts on the host), and per representation the fixed-capacity topology of the
+        candidates.  The integrals over it equal those of ``fixed_adjacency`` row for row; only the fixed-capacity
+        adjacency avoids the host synchronisation (and can be captured in a graph)."""
+        dev = torch.device(self.device)
+        pos = queryParticles.positions.to(dev, F64)
+        sup = queryParticles.supports.to(dev, F64)
+        if pos.shape[1] != 2:
+            raise NotImplementedError("2D only")
+        smax = float(sup.max()) if supportMax is None else float(supportMax)
+        fluid = queryAllowed(queryParticles, dev)
+        bodies, counts = [], []
+        for b in self.bodies:
+            full = _body_adjacency(b, pos, sup, fluid, torch.arange(len(pos), device=dev), smax, dev)
+            cand = torch.nonzero(full.valid).flatten()
+            lpos, lsup = full.lpos[cand].contiguous(), sup[cand].contiguous()
+            topos = [_rep_topology(r, lpos, lsup, None, smax, dev) for r in full.repList]
+            bodies.append(BodyAdjacency(b, cand, lpos, lsup, topos, None, full.repList, compact=True))
+            counts.append(int(len(cand)))
+        return SceneAdjacency(len(pos), bodies, {"candidates": counts}, queryParticles, smax)
+
     def precompute(self, adjacency, operationProperties, channels=None, form=None):
         """per-representation moments of one kernel (``pairmoments.PairMoments``; reference scene/scene.py:1418)."""
         from .pairmoments import precompute
@@ -603,50 +701,70 @@ class Scene:
         from .pairmoments import moments
         return moments(self, source, queryParticles, operationProperties, channels, form)
 
+    def _chosen(self, body):
+        return [(bi, bd) for bi, bd in enumerate(self.bodies) if body is None or bi == body]
+
     def inside(self, points, body=None):
+        """True where a world point lies in the solid of a body (of ``body`` only when given): the winding indicator
+        above 1/2 for a surface loop, a negative signed distance for the other representations (reference
+        scene/scene.py:1279-1310)."""
         pts = points.to(self.device, F64)
-        out = torch.zeros(len(pts), dtype=torch.bool, device=pts.device)
-        for bi, bd in enumerate(self.bodies):
-            if body is not None and bi != body:
-                continue
-            lp = bd.toLocal(pts)
-            for rep in bd.reps:
-                out |= rep.indicator(lp) > 0.5
-        return out
+        flags = [_rep_inside(rep, bd.toLocal(pts)) for _, bd in self._chosen(body) for rep in bd.reps]
+        if not flags:
+            return torch.zeros(len(pts), dtype=torch.bool, device=pts.device)
+        return torch.stack(flags).any(dim=0)
 
     def signed_distance(self, points, body=None, supportMax=None, want_body=False):
-        """(d [M], n [M,2], hit [M][, body [M]]): d > 0 in the fluid, n the unit normal into the fluid (world frame);
-        the nearest edge of every surface loop, sign by the winding number at ``supportMax`` normalisation
-        (reference scene/scene.py:1312-1372)."""
+        """(d [M], n [M,2], hit [M][, body [M]]): the smallest signed distance over the representations of the
+        bodies (the first representation on a tie), positive in the fluid, with its unit normal into the fluid in the
+        world frame and the index of its body (reference scene/scene.py:1312-1372; surface loops: the nearest edge,
+        the sign from the winding number normalised by ``supportMax``)."""
         pts = points.to(self.device, F64)
         M = len(pts)
-        bidx = torch.full((M,), -1, dtype=torch.long, device=pts.device)
-        best = torch.full((M,), float("inf"), dtype=F64, device=pts.device)
-        normal = torch.zeros((M, 2), dtype=F64, device=pts.device)
-        hit = torch.zeros(M, dtype=torch.bool, device=pts.device)
-        for bi, bd in enumerate(self.bodies):
-            if body is not None and bi != body:
-                continue
+        dist, normals, owner = [], [], []
+        for bi, bd in self._chosen(body):
             lp = bd.toLocal(pts)
             for rep in bd.reps:
-                if isinstance(rep, BoxRep):
-                    rep = rep.surface()
-                if isinstance(rep, (ImplicitRep, DiskArrayRep, SdfRep)):
-                    d, n_loc = rep.signed(lp)
-                elif isinstance(rep, SurfaceRep):
-                    smax = float(supportMax) if supportMax is not None else _default_support(rep)
-                    d, n_loc, _ = rep.geom(smax, pts.device).signed_distance(lp, smax)
-                else:
-                    continue
-                n = bd.pose.vecToWorld(n_loc)
-                better = d < best
-                best = torch.where(better, d, best)
-                normal = torch.where(better[:, None], n, normal)
-                bidx = torch.where(better, torch.full_like(bidx, bi), bidx)
-                hit |= True
+                d, n_loc = _rep_signed(rep, lp, supportMax)
+                dist.append(torch.nan_to_num(d, nan=float("inf")))
+                normals.append(bd.pose.vecToWorld(n_loc))
+                owner.append(torch.full((M,), bi, dtype=torch.long, device=pts.device))
+        if not dist:
+            best = torch.full((M,), float("inf"), dtype=F64, device=pts.device)
+            nrm = torch.zeros((M, 2), dtype=F64, device=pts.device)
+            bidx = torch.full((M,), -1, dtype=torch.long, device=pts.device)
+            hit = torch.zeros(M, dtype=torch.bool, device=pts.device)
+        else:
+            D = torch.stack(dist)
+            k = D.argmin(dim=0)                                  # the first minimum: the earliest representation
+            rows = torch.arange(M, device=pts.device)
+            best = D[k, rows]
+            nrm = torch.stack(normals)[k, rows]
+            bidx = torch.stack(owner)[k, rows]
+            hit = torch.ones(M, dtype=torch.bool, device=pts.device)
         if want_body:
-            return best, normal, hit, bidx
-        return best, normal, hit
+            return best, nrm, hit, bidx
+        return best, nrm, hit
+
+
+def _rep_inside(rep, lp):
+    if isinstance(rep, SurfaceRep):
+        return rep.indicator(lp) > 0.5
+    if isinstance(rep, BoxRep):
+        return rep.surface().indicator(lp) > 0.5
+    return rep.signed(lp)[0] < 0
+
+
+def _rep_signed(rep, lp, supportMax):
+    """(d, n) of one representation in its body frame."""
+    if isinstance(rep, BoxRep):
+        rep = rep.surface()
+    if isinstance(rep, SurfaceRep):
+        smax = float(supportMax) if supportMax is not None else _default_support(rep)
+        d, n, _ = rep.geom(smax, lp.device).signed_distance(lp, smax)
+        return d, n
+    d, n = rep.signed(lp)[:2]
+    return d, n
 
 
 def _default_support(rep: SurfaceRep) -> float:
@@ -666,41 +784,44 @@ def queryAllowed(queryParticles, dev):
 
 
 def fixed_adjacency(scene: Scene, queryParticles, operationProperties=None, supportMax: float = None):
-    """`SceneAdjacency` of every query against every body with fixed shapes: rows = queries, `valid` = broadphase
-    (support circle against the body box in the body frame; every row for an exterior solid) and fluid kind;
-    one adjacency launch per (body, representation), no host synchronisation (reference scene/fixedadj.py:249-270)."""
+    """`SceneAdjacency` of the queries against every body with fixed shapes (reference scene/fixedadj.py:249-270): per
+    body the queries in its frame, the broadphase (support circle against the body box; every fluid query of an
+    unbounded solid) and one fixed-capacity adjacency launch per representation it is evaluated with (implicit and
+    sampled shapes through their polygons at the resolution of ``supportMax``).  With ``supportMax`` given (a host
+    float >= every support) no step synchronises with the host."""
     dev = torch.device(scene.device)
     pos = queryParticles.positions.to(dev, F64)
     sup = queryParticles.supports.to(dev, F64)
     if pos.shape[1] != 2:
         raise NotImplementedError("2D only")
-    if supportMax is None:
-        supportMax = float(sup.max())
-    allowed = queryAllowed(queryParticles, dev)
-    N = len(pos)
-    cand = torch.arange(N, device=dev)
-    bodies = []
-    for body in scene.bodies:
-        lpos = body.toLocal(pos)
-        if body.unbounded():
-            valid = allowed
-        else:
-            lo, hi = body.obb()
-            d = (lo - lpos).clamp(min=0) + (lpos - hi).clamp(min=0)
-            valid = (d.norm(dim=1) < sup) & allowed
-        reps = []
-        hmin = float(sup.min()) if any(isinstance(r, ImplicitRep) for r in body.reps) and N else float(supportMax)
-        surfaces = body.fusedReps(hmin, dev)
-        for rep in surfaces:
-            g = rep.geom(float(supportMax), dev)
-            if isinstance(rep, DiskArrayRep):
-                slots, counts = g.adjacency(lpos, sup, valid=valid)
-                ind = torch.zeros(N, dtype=F64, device=dev)            # the disk element is complete
-            else:
-                slots, counts, ind = g.adjacency(lpos, sup, valid=valid, pad=True)
-            reps.append(FixedTopology(slots, counts, g.K, ind, g))
-        bodies.append(BodyAdjacency(body, cand, lpos, sup, reps, valid, surfaces))
-    return SceneAdjacency(N, bodies, {"candidates": None}, queryParticles, float(supportMax))
+    smax = float(sup.max()) if supportMax is None else float(supportMax)
+    fluid = queryAllowed(queryParticles, dev)
+    rows = torch.arange(len(pos), device=dev)
+    bodies = [_body_adjacency(b, pos, sup, fluid, rows, smax, dev) for b in scene.bodies]
+    return SceneAdjacency(len(pos), bodies, {"candidates": None}, queryParticles, smax)
+
+
+def _body_adjacency(body, pos, sup, fluid, rows, smax, dev):
+    lpos = body.toLocal(pos)
+    if body.unbounded():
+        valid = fluid
+    else:
+        lo, hi = body.obb()
+        gap = torch.maximum(lo - lpos, lpos - hi).clamp(min=0)          # per axis, outside the box only
+        valid = fluid & (gap.norm(dim=1) < sup)
+    reps = body.fusedReps(smax, dev)
+    return BodyAdjacency(body, rows, lpos, sup, [_rep_topology(r, lpos, sup, valid, smax, dev) for r in reps], valid,
+                         reps)
+
+
+def _rep_topology(rep, lpos, sup, valid, smax, dev):
+    """the fixed-capacity slots (and the winding indicator of a surface loop) of one representation."""
+    g = rep.geom(smax, dev)
+    if isinstance(rep, DiskArrayRep):
+        slots, counts = g.adjacency(lpos, sup, valid=valid)
+        return FixedTopology(slots, counts, g.K, torch.zeros(len(lpos), dtype=F64, device=dev), g)
+    slots, counts, ind = g.adjacency(lpos, sup, valid=valid, pad=True)
+    return FixedTopology(slots, counts, g.K, ind, g)
 
 
 def plane_vector(kernel: str, device) -> torch.Tensor:
