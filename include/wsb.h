@@ -307,6 +307,17 @@ int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* o
                   int64_t capacity, const double* chi, const double* pos, const double* sup, int n, int flags,
                   double* out, void* scratch, size_t scratch_bytes, void* stream);
 
+/* ---- wsb_moments3d without allocation or scratch argument.  No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The list of the
+ * particles with more than 64 faces lives in a file-scope device array of 8 MiB + 256 bytes per GPU (held until the
+ * process ends), so a call serves at most wsb_moments3d_static_max_rows() = 2,097,152 particles (more: WSB_EINVAL).
+ * Every call on one GPU shares that GPU's array (its counter is cleared by a kernel at the start of each call): calls
+ * that overlap on one GPU race.  Arguments and results as wsb_moments3d. */
+int64_t wsb_moments3d_static_max_rows(void);
+int wsb_moments3d_static(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets, const int32_t* faces,
+                         int64_t capacity, const double* chi, const double* pos, const double* sup, int n, int flags,
+                         double* out, void* stream);
+
 /* ---- host evaluations of the same arithmetic (validation; every pointer is host memory) */
 int wsb_adjacency3d_host(const wsb_mesh3d* m, const double* pos, const double* sup, int n, int64_t* offsets,
                          int32_t* faces, int64_t capacity, double* chi);

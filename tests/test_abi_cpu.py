@@ -14,7 +14,7 @@ def declared():
         if fn.endswith(".h"):
             text = open(os.path.join(ROOT, "include", fn)).read()
             text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-            names |= set(re.findall(r"^\s*(?:int|size_t|void|double)\s+\**(wsb_\w+)\s*\(", text, flags=re.M))
+            names |= set(re.findall(r"^\s*(?:int64_t|int|size_t|void|double)\s+\**(wsb_\w+)\s*\(", text, flags=re.M))
     return names
 
 

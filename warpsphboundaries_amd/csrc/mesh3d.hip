@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <queue>
 #include <vector>
 
@@ -1194,6 +1195,61 @@ extern "C" int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const
         case WSB_FORM_MIXED: return dispatch22<m3::MIXED>(kernel_id, a, s);
     }
     return wsb::set_error(WSB_ENOTSUP, "3-D moments support the fp64 and mixed forms (form %d)", form);
+}
+
+// ------------------------------------------------------------------------------------------------ no allocation
+// The heavy-row list of wsb_moments3d_static lives in a file-scope device array: its count in the first 256 bytes,
+// then one row index per particle.  Each GPU holds its own copy, at its own address, until the process ends; the host
+// takes that address once per device (hipGetSymbolAddress with the device current) and passes it to the kernels as
+// the caller-supplied buffer of wsb_moments3d.
+namespace {
+
+constexpr int64_t kStaticRows = int64_t(1) << 21;             // 2,097,152 particles: 8 MiB + 256 bytes per GPU
+constexpr int kStaticHead = 256 / (int)sizeof(int32_t);
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) int32_t scratch[kStaticHead + kStaticRows];
+
+std::mutex g_scratch_lock;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on `device` (looked up once per device)
+int static_scratch(int device, void** ptr) {
+    WSB_REQUIRE(device >= 0 && device < kMaxDevices, WSB_EINVAL, "device %d out of range", device);
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (!g_scratch_ptr[device]) {
+        int current = 0;
+        WSB_HIP(hipGetDevice(&current));
+        if (current != device) WSB_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (current != device) (void)hipSetDevice(current);
+        if (e != hipSuccess)
+            return wsb::set_error(WSB_EDEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                                  hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return WSB_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t wsb_moments3d_static_max_rows(void) { return kStaticRows; }
+
+extern "C" int wsb_moments3d_static(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets,
+                                    const int32_t* faces, int64_t capacity, const double* chi, const double* pos,
+                                    const double* sup, int n, int flags, double* out, void* stream) {
+    WSB_REQUIRE(m, WSB_EINVAL, "null mesh");
+    WSB_REQUIRE(n >= 0 && n <= kStaticRows, WSB_EINVAL,
+                "wsb_moments3d_static serves at most %lld particles per call (got %d); use wsb_moments3d with a "
+                "scratch buffer", (long long)kStaticRows, n);
+    if (n == 0) return WSB_OK;
+    void* ptr = nullptr;
+    const int rc = static_scratch(m->device, &ptr);
+    if (rc != WSB_OK) return rc;
+    return wsb_moments3d(m, kernel_id, form, offsets, faces, capacity, chi, pos, sup, n, flags, out, ptr,
+                         sizeof(int32_t) * (size_t)(kStaticHead + kStaticRows), stream);
 }
 
 // ------------------------------------------------------------------------------------------------ host evaluation
