@@ -69,58 +69,51 @@ def active_radius_counts(torch, verts, edges, slots, counts, lpos, lsup, K, knam
     return npairs, res
 
 
-def cpu_baseline(work, n_sample=131072, seed=7):
-    """the CPU oracle (numpy FP64 restatement of the reference's edge reductions) on a stratified subsample:
-    pair list from the oracle's cell-list slot fill, twelve channels per pair, row sums; single thread."""
-    from oracle import adjacency as oadj
-    from oracle import edge2d
+def cpu_baseline(work, n_sample=65536, seed=7, runs=5):
+    """the reference's CPU edge backend restated on torch CPU tensors (``oracle/torch2d.py``: FP64, every host thread
+    this process may use): slot fill of the static cell list, winding indicator, twelve channels per pair and row
+    sums.  The star workload on a stratified subsample that keeps its near-wall fraction (four kernels), and the
+    box + 64-gon scene in full (w4, two representations); one warm-up, then the median of ``runs`` runs of each."""
+    import torch
+    from oracle import torch2d as OT
     from warpsphboundaries_amd import workloads as WL
     from warpsphboundaries_amd.scene import SurfaceRep
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+
+    def prepared(loops, h):
+        out = []
+        for loop, solid in loops:
+            rep = SurfaceRep.polygon(loop, solid)
+            out.append(OT.Scene2D(rep.vertices.numpy(), rep.edges.numpy(), rep.background, h))
+        return out
+
+    def timed(scenes, pos, sup, kernels):
+        def run():
+            for sc in scenes:
+                OT.twelve(sc, pos, sup, kernels)
+        run()
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            run()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times))
+
     sub = WL.subsample(work, n_sample, seed)
-    loop, solid = sub.loops[0]
-    rep = SurfaceRep.polygon(loop, solid)
-    V, E = rep.vertices.numpy(), rep.edges.numpy()
-    st = oadj.Static(V, E, rep.background, sub.h)
-    pts, sup = sub.positions, sub.supports
-    # slots without the per-row indicator loop (the indicator is a table lookup for most rows on every path)
-    cl = st.cl
-    nx, ny = cl.dims
-
-    def run():
-        cx = np.floor((pts[:, 0] - cl.lo[0]) / cl.cell).astype(np.int64)
-        cy = np.floor((pts[:, 1] - cl.lo[1]) / cl.cell).astype(np.int64)
-        inside = (cx >= 0) & (cx < nx) & (cy >= 0) & (cy < ny)
-        cid = np.where(inside, cx * ny + cy, 0)
-        nc = np.where(inside, cl.start[cid + 1] - cl.start[cid], 0)
-        rows = np.repeat(np.arange(len(pts)), nc)
-        off = np.arange(int(nc.sum())) - np.repeat(np.cumsum(nc) - nc, nc)
-        e = cl.items[cl.start[cid[rows]] + off]
-        ax, ay = V[E[e, 0], 0], V[E[e, 0], 1]
-        dx, dy = V[E[e, 1], 0] - ax, V[E[e, 1], 1] - ay
-        qx, qy = pts[rows, 0], pts[rows, 1]
-        t = np.clip(((qx - ax) * dx + (qy - ay) * dy) / np.maximum(dx * dx + dy * dy, 1e-300), 0, 1)
-        keep = np.sqrt((qx - (ax + t * dx)) ** 2 + (qy - (ay + t * dy)) ** 2) < sup[rows]
-        r, e = rows[keep], e[keep]
-        P = (V[E[e, 0]] - pts[r]) / sup[r, None]
-        Q = (V[E[e, 1]] - pts[r]) / sup[r, None]
-        for k in sub.kernels:
-            c = edge2d.edge_channels12(P, Q, k)
-            out = np.zeros((len(pts), 12))
-            np.add.at(out, r, c)
-        return len(r)
-
-    run()
-    times = []
-    for _ in range(5):
-        t0 = time.perf_counter()
-        npairs = run()
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    evals = 12 * len(sub.kernels) * len(pts)
-    return {"value": evals / t, "unit": "particle-moment evals/s", "cores": 1, "kind": "port",
-            "sample": f"{len(pts)}-particle stratified subsample of the star workload ({int(sub.near.sum())} near-wall, "
-                      f"{npairs} pairs), slot fill + twelve channels + row sums for 4 kernels, numpy FP64 oracle, "
-                      f"median of 5 runs of {t:.2f} s"}
+    pos, sup = torch.tensor(sub.positions), torch.tensor(sub.supports)
+    t_star = timed(prepared(sub.loops, sub.h), pos, sup, sub.kernels)
+    star = 12 * len(sub.kernels) * len(pos) / t_star
+    c1 = WL.config1()
+    t_box = timed(prepared(c1.loops, c1.h), torch.tensor(c1.positions), torch.tensor(c1.supports), c1.kernels)
+    box = 12 * len(c1.kernels) * len(c1.positions) / t_box
+    return {"value": star, "unit": "particle-moment evals/s", "cores": cores, "kind": "port",
+            "sample": f"{len(pos)}-particle stratified subsample of the star workload ({int(sub.near.sum())} near-wall), "
+                      f"slot fill + winding indicator + twelve channels + row sums for cubic/w2/w4/w6, torch-CPU FP64 "
+                      f"restatement of the reference's CPU edge backend on {cores} threads, median of {runs} runs of "
+                      f"{t_star:.2f} s",
+            "box_64gon_full": {"value": box, "particles": len(c1.positions), "seconds": t_box,
+                               "note": "16,384 particles, w4, both representations, same restatement and threads"}}
 
 
 _SERVED = {}
@@ -335,12 +328,69 @@ def disks_measure(torch, dev, steps, warmup):
 def _traffic(kernel):
     """HBM bytes per launch of the mixed-form fused twelve-moment pass on the star workload, from the committed
     FETCH_SIZE / WRITE_SIZE counter runs scaled on a copy kernel of known size (None when the summary is absent)."""
-    fn = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_tile12_multi_star4096_1M.json")
+    fn = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_rows12_multi_star4096_1M.json")
     try:
         with open(fn) as f:
             return json.load(f)["kernels"][kernel]["traffic_bytes"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def _valu(kernel):
+    """VALU busy and issue fractions of ``kernel`` from the committed SQ counter runs (None when absent)."""
+    fn = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_valu_hot_kernels.json")
+    try:
+        with open(fn) as f:
+            d = json.load(f)["kernels"][kernel]
+        return {"valu_busy": d["valu_busy"], "valu_issue": d["valu_issue"], "fp64_share_of_valu": d["fp64_share_of_valu"],
+                "wait_share": d["wait_share"], "source": "profiles/pmc_valu_hot_kernels.json (rocprofv3 --pmc, "
+                                                         "SQ_ACTIVE_INST_VALU x 4 / (1024 SIMDs x GRBM_GUI_ACTIVE / 8))"}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+CH2 = ["m00", "m10", "m01", "Gx", "Gy", "g1xx", "g1xy", "g1yx", "g1yy", "m20", "m11", "m02"]
+
+
+def precision_mp(torch, dev):
+    """Per-moment error of every form on the 4,096-particle wedge subsample whose values were computed with mpmath
+    at 50 digits (tests/golden/stress2d_mp.npz): signal-relative (max |v - r| / max |r| over one wedge's particles,
+    then the maximum over wedges and kernels) and absolute in units of h."""
+    from warpsphboundaries_amd.scene import SurfaceRep
+    fn = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "stress2d_mp.npz")
+    d = np.load(fn)
+    off, poly, pts, h = d["verts_offsets"], d["poly"], d["positions"], float(d["h"])
+    kernels = [str(k) for k in d["kernels"]]
+    forms = ("plain", "fp64", "mixed")
+    got = {(f, k): np.zeros((len(pts), 12)) for f in forms for k in kernels}
+    for i in range(len(off) - 1):
+        sel = np.nonzero(poly == i)[0]
+        if len(sel) == 0:
+            continue
+        geo = SurfaceRep.polygon(d["verts"][off[i]:off[i + 1]], "inside").geom(h, dev)
+        lp = torch.tensor(pts[sel], dtype=torch.float64, device=dev)
+        ls = torch.full((len(sel),), h, dtype=torch.float64, device=dev)
+        slots, counts, ind = geo.adjacency(lp, ls)
+        for f in forms:
+            for k in kernels:
+                got[(f, k)][sel] = geo.moments(k, slots, counts, ind, lp, ls, form=f).cpu().numpy()
+    out = {"particles": int(len(pts)), "wedges": int(len(off) - 1), "channels": CH2, "reference": "mpmath, 50 digits"}
+    for f in forms:
+        sig, ab, per_k = np.zeros(12), np.zeros(12), {}
+        for ki, k in enumerate(kernels):
+            ref, v = d["values"][ki], got[(f, k)]
+            worst = np.zeros(12)
+            for c in np.unique(poly):
+                m = poly == c
+                e = np.abs(v[m] - ref[m]).max(0)
+                s = np.abs(ref[m]).max(0)
+                worst = np.maximum(worst, np.where(s > 0, e / np.where(s > 0, s, 1.0), e))
+            sig = np.maximum(sig, worst)
+            ab = np.maximum(ab, np.abs(v - ref).max(0))
+            per_k[k] = float(worst.max())
+        out[f] = {"signal_relative": [float(x) for x in sig], "absolute_units_of_h": [float(x) for x in ab],
+                  "worst_signal_relative_per_kernel": per_k}
+    return out
 
 
 def main():
@@ -433,7 +483,7 @@ def main():
         dst.copy_(src)                      # a plain copy of known size: 512 MiB read, 512 MiB written
         torch.cuda.synchronize()
         print(json.dumps({"probe": "star", "N": N,
-                          "tile_algorithmic_bytes": N * opmodel.bytes_per_particle_2d(len(work.kernels)),
+                          "fused_pass_algorithmic_bytes": N * opmodel.bytes_per_particle_2d(len(work.kernels)),
                           "copy_bytes": src.numel() * 8}), flush=True)
         return
 
@@ -529,14 +579,14 @@ def main():
     t_dom = t_multi["mixed"]
     t_roof = max(bytes_alg / PEAK_HBM, flops / PEAK_FP64)
     compute_bound = flops / PEAK_FP64 >= bytes_alg / PEAK_HBM
-    kname = "rows12_multi_kernel<15, 2>"
+    kname = "rows12_multi_kernel<15, 2, false>"
     roof = {"bound": "mfma" if compute_bound else "hbm",
             "pipe": "FP64 VALU (no MFMA: transcendental gather-reduce; the MI355X FP64 vector peak equals its FP64 matrix peak)",
             "kernel": kname + " (cubic, w2, w4, w6 in one pass, mixed form)",
             "achieved": (flops / t_dom / 1e12) if compute_bound else (bytes_alg / t_dom / 1e9),
             "peak": PEAK_FP64 / 1e12 if compute_bound else PEAK_HBM / 1e9,
             "unit": "TFLOP/s" if compute_bound else "GB/s",
-            "frac": t_roof / t_dom, "traffic": _traffic(kname),
+            "frac": t_roof / t_dom, "traffic": _traffic(kname), "valu": _valu(kname),
             "algorithmic_flops": flops, "algorithmic_bytes": bytes_alg, "pairs": npairs,
             "active_chords_by_radius": {str(r): a for r, a in sorted(active.items())},
             "launch_seconds": t_dom, "hbm_GBps_algorithmic": bytes_alg / t_dom / 1e9}
@@ -570,6 +620,9 @@ def main():
         "served_latency": None,
         "roofline": roof,
     }
+    line["precision_mp_wedges"] = precision_mp(torch, dev)
+    log("per-moment error against mpmath:", {f: max(line["precision_mp_wedges"][f]["signal_relative"])
+                                             for f in ("plain", "fp64", "mixed")})
     if not args.no_3d:
         line["mesh3d"] = mesh3d_measure(torch, dev, args.steps, args.warmup)
         log("3-D:", line["mesh3d"])

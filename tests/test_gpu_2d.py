@@ -198,7 +198,8 @@ def test_moments_multi_matches_single_launches_and_oracle(form, kset):
     """wsb_moments2d_multi (several kernels in one pass sharing each pair's chord primitives) against the oracle
     (the single-launch gates) and against one launch per kernel (the same closed form, summed in the same slot
     order; within 1e-13 of the signal, 1e-12 for w6, whose oracle gate is five times the others'); rows in cell order
-    and in input order bitwise equal; WSB_OUT_ACCUMULATE doubles; physical units and a world rotation as the single
+    and in input order bitwise equal, in count order too for the mixed form (the FP64 form's count order runs another
+    kernel: within the multi-vs-single gate); WSB_OUT_ACCUMULATE doubles; physical units and a world rotation as the single
     launch."""
     from warpsphboundaries_amd import _lib as L
     loop, solid = _geometries()["star256"]
@@ -245,8 +246,14 @@ def test_moments_multi_matches_single_launches_and_oracle(form, kset):
         # that rounding tenfold over the other kernels (its oracle gate is 5e-12 against 1e-12)
         assert d <= (1e-12 if k == "w6" else 1e-13), (k, d)
         assert torch.equal(grouped[k], multi[k])
-        assert torch.equal(by_count[k], multi[k])
-        assert torch.equal(acc_count[k], 2.0 * multi[k])
+        if form == "fp64":
+            # cell and input order run the FP64 form through the workgroup tile kernel, count order through the
+            # row-per-thread kernel: the same slot-order sums, each with the contractions its compiler chose
+            dc = ((by_count[k] - multi[k]).abs().amax(0) / multi[k].abs().amax(0).clamp(min=1.0)).max().item()
+            assert dc <= (1e-12 if k == "w6" else 1e-13), (k, dc)
+        else:
+            assert torch.equal(by_count[k], multi[k])
+        assert torch.equal(acc_count[k], 2.0 * by_count[k])
         assert torch.equal(acc[k], 2.0 * multi[k])
         sw = geo.moments(k, slots, counts, ind, lp, ls, form=form, rot4=rot4, flags=fl)
         dw = ((world[k] - sw).abs().amax(0) / sw.abs().amax(0).clamp(min=1.0)).max().item()

@@ -222,11 +222,12 @@ int dispatch_tile(int kid, const TileArgs& a, hipStream_t s) {
 // The scheme kernels cubic, w2, w4, w6 (any subset MASK, bit k = kernel id k) in one pass.  Per pair the chord
 // primitives of each support radius -- the clipped chord, the endpoint radii, the logarithm, the angle and the
 // differences dI_m -- are computed once (radius 1 for every kernel, radius 1/2 for the cubic's inner block) and
-// each kernel's six block sums are accumulated in the same loop over m; the channels of one kernel at a time then
-// go through LDS to the row sums (slot order, as tile12_kernel).  A workgroup of MTILE_T threads owns MTILE_R
-// rows; thread t keeps the row sums of row t % MTILE_R, channels 3 (t / MTILE_R) .. +2, of every kernel in
-// registers (all four wavefronts reduce, and the row sums take no LDS), and stages them through LDS once at the end,
-// where the factor 1/pi of every channel is applied to the row sum.
+// each kernel's six block sums are accumulated in the same loop over m (rows12_multi_kernel below).
+// The FP64 form uses tile12_multi_kernel: a workgroup of MTILE_T threads owns MTILE_R rows, their pairs are packed
+// densely over the lanes, each kernel's channels go through LDS to the row sums (slot order, as tile12_kernel); thread
+// t keeps the row sums of row t % MTILE_R, channels 3 (t / MTILE_R) .. +2, of every kernel in registers and stages
+// them through LDS once at the end, where the factor 1/pi of every channel is applied to the row sum.  (The FP64
+// arithmetic needs more registers than one row per thread leaves: the mixed form takes rows12_multi_kernel.)
 constexpr int MTILE_T = 256;
 constexpr int MTILE_R = 64;
 
@@ -574,11 +575,12 @@ __device__ __forceinline__ void serve_row(const MultiArgs& ma, int pos) {
     }
 }
 
-// one position per thread.  With nwork only the first nwork[0] positions are served (the rows with slots of a count
-// order); a workgroup past them ends at its first instruction.
-template <int MASK, int FORM>
+// one position per thread.  COUNTED (a launch with nwork, its own instantiation so that a profile tells the two
+// launches apart): only the first nwork[0] positions are served (the rows with slots of a count order); a workgroup
+// past them ends at its first instruction.
+template <int MASK, int FORM, bool COUNTED>
 __global__ __launch_bounds__(RHEAVY_T) __attribute__((amdgpu_waves_per_eu(2))) void rows12_multi_kernel(MultiArgs ma) {
-    const int npos = ma.nwork ? min(ma.t.n, ma.nwork[0]) : ma.t.n;
+    const int npos = COUNTED ? min(ma.t.n, ma.nwork[0]) : ma.t.n;
     if ((int64_t)blockIdx.x * RHEAVY_T >= npos) return;
     const int64_t pos = (int64_t)blockIdx.x * RHEAVY_T + threadIdx.x;
     if (pos < npos) serve_row<MASK, FORM>(ma, (int)pos);
@@ -628,8 +630,18 @@ __global__ __launch_bounds__(RROW_T) void rows12_indicator_kernel(MultiArgs ma) 
 
 template <int MASK, int FORM>
 int launch_multi(const MultiArgs& a, hipStream_t s) {
-    if (a.nwork) hipLaunchKernelGGL((rows12_indicator_kernel<MASK>), dim3((a.t.n + RROW_T - 1) / RROW_T), dim3(RROW_T), 0, s, a);
-    hipLaunchKernelGGL((rows12_multi_kernel<MASK, FORM>), dim3((a.t.n + RHEAVY_T - 1) / RHEAVY_T), dim3(RHEAVY_T), 0, s, a);
+    if (FORM == closed::FP64 && !a.nwork) {
+        hipLaunchKernelGGL((tile12_multi_kernel<MASK, FORM>), dim3((a.t.n + MTILE_R - 1) / MTILE_R), dim3(MTILE_T), 0, s, a);
+        WSB_HIP(hipGetLastError());
+        return WSB_OK;
+    }
+    const dim3 grid((a.t.n + RHEAVY_T - 1) / RHEAVY_T);
+    if (a.nwork) {
+        hipLaunchKernelGGL((rows12_indicator_kernel<MASK>), dim3((a.t.n + RROW_T - 1) / RROW_T), dim3(RROW_T), 0, s, a);
+        hipLaunchKernelGGL((rows12_multi_kernel<MASK, FORM, true>), grid, dim3(RHEAVY_T), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((rows12_multi_kernel<MASK, FORM, false>), grid, dim3(RHEAVY_T), 0, s, a);
+    }
     WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
