@@ -567,6 +567,15 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     t_corder = e0.elapsed_time(e1) / 1e3 / args.steps
+    # signed distance of every particle (70 % of them farther than one support from the wall: the edge tree)
+    sd = geo.signed_distance(lpos, work.h)
+    e0.record(stream)
+    for _ in range(args.steps):
+        sd = geo.signed_distance(lpos, work.h)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_sd = e0.elapsed_time(e1) / 1e3 / args.steps
+    log("signed distance", t_sd)
     log("fused-pass seconds:", t_multi, "in count order:", t_multi_count, "single-kernel launch seconds:", {f"{a}/{b}": v for (a, b), v in per_kernel.items()},
         "adjacency", t_adj, "sort", t_sort,
         "adjacency in input order", t_adj_unsorted, "count order", t_corder)
@@ -616,6 +625,7 @@ def main():
             "fp64": evals / sum(per_kernel[("fp64", k)] for k in work.kernels)},
         "adjacency_ms": t_adj * 1e3, "cell_sort_ms": t_sort * 1e3, "adjacency_input_order_ms": t_adj_unsorted * 1e3,
         "count_order_ms": t_corder * 1e3,
+        "signed_distance_ms": t_sd * 1e3,
         "max_signal_relative_error_vs_fp64": err,
         "served_latency": None,
         "roofline": roof,

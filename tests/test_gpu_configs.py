@@ -174,6 +174,66 @@ def test_config2_star_subsample_all_kernels(star_subsample, form):
         _gate(got[k], oracle[k][3], form, k, "star4096 65,536-particle subsample")
 
 
+def _nearest_edges(V, E, pts, chunk=1024):
+    """(distance, edge) of the nearest edge of every point by a scan over all edges (float64, lowest id on a tie),
+    and the distance to the second nearest edge."""
+    a, b = V[E[:, 0]], V[E[:, 1]]
+    e = b - a
+    ee = (e * e).sum(1)
+    dist = np.empty(len(pts))
+    arg = np.empty(len(pts), dtype=np.int64)
+    second = np.empty(len(pts))
+    for s in range(0, len(pts), chunk):
+        p = pts[s:s + chunk]
+        t = np.clip(((p[:, None] - a[None]) * e[None]).sum(2) / ee[None], 0.0, 1.0)
+        dd = np.linalg.norm(p[:, None] - (a[None] + t[..., None] * e[None]), axis=2)
+        k = dd.argmin(1)
+        dist[s:s + chunk] = dd[np.arange(len(p)), k]
+        arg[s:s + chunk] = k
+        dd[np.arange(len(p)), k] = np.inf
+        second[s:s + chunk] = dd.min(1)
+    return dist, arg, second
+
+
+def test_config2_signed_distance_edge_tree(star_subsample):
+    """signed distance on the 4,096-edge star at h = 0.004 for the 65,536-particle subsample (70 % of it farther than
+    one support from every edge, where the edge tree finds the nearest edge) and 8,192 points over a box twice the
+    domain (inside the solid, and outside the cell grid): the distance within 1e-14 of a scan over all edges, the
+    scan's edge wherever its nearest edge is unique by more than 1e-12, the sign from the even-odd rule, and the
+    normal of the reference's rule."""
+    w, rep, _ = star_subsample
+    g = WL.rng(31)
+    pts = np.concatenate([w.positions, g.uniform(-0.5, 1.5, (8192, 2))])
+    geo = rep.geom(w.h, DEV)
+    d, n, edge = geo.signed_distance(torch.tensor(pts, device=DEV), w.h)
+    d, n, edge = d.cpu().numpy(), n.cpu().numpy(), edge.cpu().numpy()
+    V, E = rep.vertices.numpy(), rep.edges.numpy().astype(np.int64)
+    dist, arg, second = _nearest_edges(V, E, pts)
+    far = dist > w.h
+    assert far.sum() > 40000
+    np.testing.assert_allclose(np.abs(d), dist, rtol=0, atol=1e-14)
+    unique = second - dist > 1e-12           # the others are nearest to a vertex shared by two edges
+    assert unique.sum() > 20000
+    assert np.array_equal(edge[unique], arg[unique])
+    # the device's edge is a nearest edge wherever the scan's is not unique
+    a, b = V[E[edge, 0]], V[E[edge, 1]]
+    ev = b - a
+    t = np.clip(((pts - a) * ev).sum(1) / (ev * ev).sum(1), 0.0, 1.0)
+    cp = a + t[:, None] * ev
+    np.testing.assert_allclose(np.linalg.norm(pts - cp, axis=1), dist, rtol=0, atol=1e-14)
+    clear = np.abs(dist) > 1e-9
+    inside = WL.inside_polygon(pts, V)
+    assert np.array_equal(d[clear] < 0, inside[clear])
+    # the normal: the edge's right normal inside the solid, else the direction from the closest point
+    el = ev / np.linalg.norm(ev, axis=1)[:, None]
+    nref = np.stack([el[:, 1], -el[:, 0]], 1)
+    dirv = pts - cp
+    dn = np.linalg.norm(dirv, axis=1)
+    out = (dn > 1e-12) & (d > 0)
+    nref[out] = dirv[out] / dn[out, None]
+    np.testing.assert_allclose(n, nref, rtol=0, atol=1e-12)
+
+
 # ------------------------------------------------------------------------------------------------ icosphere cavity
 def _config3_particles(n=4096, seed=12):
     w = WL.config3()

@@ -112,6 +112,67 @@ HostBins build_bins(const double* V, const int32_t* E, int ne) {
     return b;
 }
 
+// Bounding-volume tree over the edges for the nearest-edge search of points that no cell list reaches: node i holds
+// the box (min x, min y, max x, max y) of its edges' endpoints and link (right, 0) for an inner node, whose left child
+// is i + 1, or (-(first + 1), count) for a leaf of items[first .. first + count).  Median split of the edge centres on
+// the longer axis of their bounds; leaves of at most kLeaf edges, so the depth is at most ceil(log2(ne / kLeaf)) + 1.
+constexpr int kLeaf = 4;
+constexpr int kTreeStack = 64;
+
+struct HostTree {
+    std::vector<double4> box;
+    std::vector<int2> link;
+    std::vector<int32_t> items;
+    int depth = 0;
+};
+
+static void tree_node(HostTree& t, const double* V, const int32_t* E, const std::vector<double>& cx,
+                      const std::vector<double>& cy, int first, int count, int depth) {
+    const int me = (int)t.box.size();
+    double4 b{INFINITY, INFINITY, -INFINITY, -INFINITY};
+    double lx = INFINITY, ly = INFINITY, hx = -INFINITY, hy = -INFINITY;
+    for (int i = first; i < first + count; ++i) {
+        const int e = t.items[i];
+        for (int j = 0; j < 2; ++j) {
+            const double x = V[2 * E[2 * e + j]], y = V[2 * E[2 * e + j] + 1];
+            b.x = std::min(b.x, x); b.y = std::min(b.y, y); b.z = std::max(b.z, x); b.w = std::max(b.w, y);
+        }
+        lx = std::min(lx, cx[e]); hx = std::max(hx, cx[e]); ly = std::min(ly, cy[e]); hy = std::max(hy, cy[e]);
+    }
+    t.box.push_back(b);
+    t.link.push_back(int2{0, 0});
+    t.depth = std::max(t.depth, depth);
+    if (count <= kLeaf) {
+        t.link[me] = int2{-(first + 1), count};
+        return;
+    }
+    const bool by_x = (hx - lx) >= (hy - ly);
+    const int half = count / 2;
+    auto beg = t.items.begin() + first;
+    std::nth_element(beg, beg + half, beg + count, [&](int32_t a, int32_t c) {
+        const double ka = by_x ? cx[a] : cy[a], kc = by_x ? cx[c] : cy[c];
+        return ka < kc || (ka == kc && a < c);
+    });
+    tree_node(t, V, E, cx, cy, first, half, depth + 1);
+    t.link[me].x = (int)t.box.size();
+    tree_node(t, V, E, cx, cy, first + half, count - half, depth + 1);
+}
+
+HostTree build_tree(const double* V, const int32_t* E, int ne) {
+    HostTree t;
+    std::vector<double> cx(ne), cy(ne);
+    for (int e = 0; e < ne; ++e) {
+        cx[e] = 0.5 * (V[2 * E[2 * e]] + V[2 * E[2 * e + 1]]);
+        cy[e] = 0.5 * (V[2 * E[2 * e] + 1] + V[2 * E[2 * e + 1] + 1]);
+    }
+    t.items.resize(ne);
+    for (int e = 0; e < ne; ++e) t.items[e] = e;
+    t.box.reserve(2 * (size_t)ne / kLeaf + 2);
+    t.link.reserve(2 * (size_t)ne / kLeaf + 2);
+    tree_node(t, V, E, cx, cy, 0, ne, 1);
+    return t;
+}
+
 // Reference points of the local winding count (local_winding_edge): for each cell with edges, the first of a
 // few trial points inside it whose ray decisions over its y-bin (winding_point's test) and whose orientations
 // against every edge of the cell's list are certain; (x, y, indicator, 1), or zeros when no trial point qualifies.
@@ -257,6 +318,10 @@ struct SdArgs {
     int nb;
     const int32_t* bstart;
     const double4* bxy;
+    const double4* tbox;
+    const int2* tlink;
+    const int32_t* titems;
+    double extent;
     double* d;
     double* nrm;
     int32_t* edge;
@@ -273,9 +338,57 @@ __device__ inline double seg_dist(double px, double py, const double* V, const i
     return sqrt(dx * dx + dy * dy);
 }
 
+// distance from (px, py) to the box (x0, y0, x1, y1), 0 inside
+__device__ inline double box_dist(double px, double py, double4 b) {
+    const double dx = fmax(fmax(b.x - px, px - b.z), 0.0);
+    const double dy = fmax(fmax(b.y - py, py - b.w), 0.0);
+    return sqrt(dx * dx + dy * dy);
+}
+
+// the nearest edge over the tree (the lowest edge id among equal distances), starting from the candidate (best, k):
+// a subtree is skipped only when its box is farther than best by more than the rounding of seg_dist can account for
+// (slack), so every edge whose computed distance could equal or beat best is evaluated
+__device__ inline void tree_nearest(const SdArgs& a, double px, double py, double& best, int& k, double& bx, double& by) {
+    const double slack = 256.0 * 1.1102230246251565e-16 * (fabs(px) + fabs(py) + a.extent);
+    int stack[kTreeStack];
+    int sp = 0, node = 0;
+    for (;;) {
+        const int2 l = a.tlink[node];
+        if (l.x < 0) {
+            const int first = -l.x - 1;
+            for (int i = first; i < first + l.y; ++i) {
+                const int e = a.titems[i];
+                double qx, qy;
+                const double dd = seg_dist(px, py, a.V, a.E, e, &qx, &qy);
+                if (dd < best || (dd == best && e < k)) { best = dd; k = e; bx = qx; by = qy; }
+            }
+        } else {
+            const int lc = node + 1, rc = l.x;
+            const double dl = box_dist(px, py, a.tbox[lc]), dr = box_dist(px, py, a.tbox[rc]);
+            const bool gl = dl <= best + slack, gr = dr <= best + slack;
+            if (gl && gr) {                 // nearer child first, the other on the stack
+                stack[sp++] = dl <= dr ? rc : lc;
+                node = dl <= dr ? lc : rc;
+                continue;
+            }
+            if (gl || gr) {
+                node = gl ? lc : rc;
+                continue;
+            }
+        }
+        // pop the next subtree that can still hold a candidate
+        bool found = false;
+        while (sp > 0) {
+            const int c = stack[--sp];
+            if (box_dist(px, py, a.tbox[c]) <= best + slack) { node = c; found = true; break; }
+        }
+        if (!found) return;
+    }
+}
+
 // nearest edge, sign from the winding number at support normalisation `smax` (reference scene/scene.py:1312-1372
 // with the sync-free indicator of scene/fixedadj.py:273-276); the cell list gives the exact nearest edge whenever
-// it is closer than smax, else every edge is scanned
+// it is closer than smax, else the edge tree is searched from the cell list's candidate
 __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.m) return;
@@ -294,15 +407,7 @@ __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
             if (dd < best) { best = dd; k = e; bx = qx; by = qy; }
         }
     }
-    if (!(best < a.cell)) {             // cell = the support the static list was inflated by
-        best = INFINITY;
-        k = -1;
-        for (int e = 0; e < a.ne; ++e) {
-            double qx, qy;
-            const double dd = seg_dist(px, py, a.V, a.E, e, &qx, &qy);
-            if (dd < best) { best = dd; k = e; bx = qx; by = qy; }
-        }
-    }
+    if (!(best < a.cell)) tree_nearest(a, px, py, best, k, bx, by);     // cell = the support the list was inflated by
     double ind;
     if (inside) {
         ind = a.cempty[cid] ? a.cind[cid]
@@ -414,6 +519,8 @@ extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edg
     for (int i = 0; i < 2 * nv; ++i) WSB_REQUIRE(std::isfinite(verts[i]), WSB_EINVAL, "non-finite vertex");
     HostCells c = build_cells(verts, edges, ne, support_max);
     HostBins b = build_bins(verts, edges, ne);
+    HostTree tree = build_tree(verts, edges, ne);
+    WSB_REQUIRE(tree.depth < kTreeStack, WSB_EINVAL, "edge tree depth %d exceeds %d", tree.depth, kTreeStack);
     auto rec_of = [&](int e) {
         return double4{verts[2 * edges[2 * e]], verts[2 * edges[2 * e] + 1], verts[2 * edges[2 * e + 1]],
                        verts[2 * edges[2 * e + 1] + 1]};
@@ -489,6 +596,15 @@ extern "C" int wsb_geom2d_create(const double* verts, int nv, const int32_t* edg
     if (!st) st = upload(&g->erec, erec.data(), erec.size(), &g->bytes);
     if (!st) st = upload(&g->etan, etan.data(), etan.size(), &g->bytes);
     if (!st) st = upload(&g->cref, cref.data(), cref.size(), &g->bytes);
+    if (!st) st = upload(&g->tbox, tree.box.data(), tree.box.size(), &g->bytes);
+    if (!st) st = upload(&g->tlink, tree.link.data(), tree.link.size(), &g->bytes);
+    if (!st) st = upload(&g->titems, tree.items.data(), tree.items.size(), &g->bytes);
+    {
+        double ext = 0.0;
+        for (int i = 0; i < 2 * nv; ++i) ext = std::max(ext, std::fabs(verts[i]));
+        g->extent = ext;
+        g->tnodes = (int)tree.box.size();
+    }
     g->local = local;
     g->noref = noref;
     if (st) {
@@ -504,7 +620,7 @@ extern "C" int wsb_geom2d_destroy(wsb_geom2d* g) {
     if (!g) return WSB_OK;
     for (void* p : {(void*)g->verts, (void*)g->edges, (void*)g->cstart, (void*)g->citems, (void*)g->cind,
                     (void*)g->cempty, (void*)g->bstart, (void*)g->bitems, (void*)g->cxy, (void*)g->bxy, (void*)g->erec,
-                    (void*)g->etan, (void*)g->cref})
+                    (void*)g->etan, (void*)g->cref, (void*)g->tbox, (void*)g->tlink, (void*)g->titems})
         if (p) (void)hipFree(p);
     delete g;
     return WSB_OK;
@@ -756,7 +872,7 @@ extern "C" int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, in
     WSB_REQUIRE(support_max > 0, WSB_EINVAL, "support_max must be positive");
     SdArgs a{lpos, m, support_max, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny, g->cstart,
              g->citems, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb, g->bstart, g->bxy,
-             d, nrm, edge};
+             g->tbox, g->tlink, g->titems, g->extent, d, nrm, edge};
     hipLaunchKernelGGL(signed_distance2d_kernel, dim3((m + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;

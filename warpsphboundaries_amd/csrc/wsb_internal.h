@@ -70,6 +70,13 @@ struct wsb_geom2d {
     // local winding: per cell with edges a reference point and its indicator (x, y, indicator, 1), (0, 0, 0, 0)
     // where none qualifies; `local` = 0 when some edge is too close to horizontal for the ray argument
     double4* cref = nullptr;      // [nx*ny]
+    // edge tree of the nearest-edge search: node boxes, links (inner: right child, left = next node; leaf:
+    // (-(first + 1), count)) and the edge ids its leaves range over; extent = the largest |coordinate| of a vertex
+    double4* tbox = nullptr;      // [tnodes]
+    int2* tlink = nullptr;        // [tnodes]
+    int32_t* titems = nullptr;    // [ne]
+    int tnodes = 0;
+    double extent = 0;
     int local = 0;
     int64_t noref = 0;            // cells with edges and no reference point
 };
