@@ -18,6 +18,9 @@ Contents (each function cites the reference file:line it restates):
 * ``cone2d``, ``triangle2d``  the cone area / direction extreme and the conditioned triangle quadrature.
 * ``contour2d`` the marching-squares zero level set of a sampled distance (``scene/scene.py:762-806``).
 * ``sdf2d``     the bilinear signed distance of a sampled field with node-blended jets (``scene/scene.py:665-706``).
+* ``sceneop``   the boundary operations over per-query moments (``scene/scene.py:1640-1707``), pinned by quadrature.
+* ``torch2d``   a torch-CPU restatement of the stage-1 reductions, the adjacency and the indicator (the timed CPU
+                baseline of ``bench.py``).
 * ``arith``     the arithmetic the restatements run in: float64 (default), x86 long double, or mpmath.
 
 How each part is pinned (tests/test_oracle_*.py, tests/test_mesh3d_cpu.py): the reference's 40-digit 2-D golden

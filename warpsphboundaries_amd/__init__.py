@@ -7,6 +7,8 @@ Public surface (the reference's names):
     BoxRep, ImplicitRep, DiskBody, TierPolicy, DiskArrayRep, SdfRep  representations (boxes and sampled distance
                                                            fields as polygons, solid disks as the disk element)
     FusedWall, FusedGroup, WallOutput                      fused evaluation
+    sceneOperation, BodyField, SceneReaction,              operations over PairMoments (Density, Interpolate,
+    OperationProperties, WarpOperation, GradientScheme     Gradient, Divergence, Curl, Covariance; the reaction)
     edge_channels, second_channels, indicator_vector       pair-list evaluators
     value, gradient, moment, moment_gradient               polygon element API
     KERNELS                                                exact kernel table
@@ -26,6 +28,9 @@ def __getattr__(name):
                 "DiskBody", "TierPolicy"):
         from . import scene
         return getattr(scene, name)
+    if name in ("sceneOperation", "BodyField", "SceneReaction", "OperationProperties", "WarpOperation", "GradientScheme"):
+        from . import operation
+        return getattr(operation, name)
     if name in ("FusedWall", "FusedGroup", "WallOutput"):
         from . import fused
         return getattr(fused, name)
