@@ -1,0 +1,61 @@
+"""The torch-CPU restatement (oracle/torch2d.py, the timed CPU baseline of bench.py) against the numpy oracle it
+restates: slot lists and indicator equal to oracle.adjacency.fixed_topology (the indicator bit for bit away from the
+boundary), the twelve channels of every kernel within 1e-12 of oracle.edge2d's row sums, on the 256-edge star and on
+the box + 64-gon scene (both solids)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjacency as oadj
+from oracle import edge2d
+from oracle import torch2d as OT
+from warpsphboundaries_amd import workloads as WL
+from warpsphboundaries_amd.scene import SurfaceRep
+
+
+def _oracle(rep, h, pts, sup, kernels):
+    st = oadj.Static(rep.vertices.numpy(), rep.edges.numpy(), rep.background, h)
+    e, c, ind = oadj.fixed_topology(st, pts, sup)
+    K = st.K
+    rows = np.repeat(np.arange(len(pts)), c)
+    sl = np.concatenate([e[r * K:r * K + k] for r, k in enumerate(c)])
+    P = (st.verts[st.edges[sl, 0]] - pts[rows]) / sup[rows, None]
+    Q = (st.verts[st.edges[sl, 1]] - pts[rows]) / sup[rows, None]
+    out = {}
+    for k in kernels:
+        acc = np.zeros((len(pts), 12))
+        np.add.at(acc, rows, edge2d.edge_channels12(P, Q, k))
+        out[k] = acc + ind[:, None] * edge2d.plane_vector(k)[None]
+    return rows, sl, ind, out
+
+
+def _cases():
+    g = WL.rng(21)
+    star = WL.star_polygon(256, g=WL.rng(3))
+    pts = g.uniform(0.0, 1.0, (3000, 2))
+    yield "star256", [(star, "inside")], 0.01, pts, np.where(g.uniform(0, 1, 3000) < 0.3, 0.007, 0.01)
+    w = WL.config1()
+    idx = WL.rng(5).choice(len(w.positions), 3000, replace=False)
+    yield "box+64gon", w.loops, w.h, w.positions[idx], w.supports[idx]
+
+
+@pytest.mark.parametrize("case", list(_cases()), ids=lambda c: c[0])
+def test_torch_restatement_matches_the_oracle(case):
+    name, loops, h, pts, sup = case
+    kernels = ("cubic", "w2", "w4", "w6")
+    for loop, solid in loops:
+        rep = SurfaceRep.polygon(loop, solid)
+        rows, sl, ind, ref = _oracle(rep, h, pts, sup, kernels)
+        sc = OT.Scene2D(rep.vertices.numpy(), rep.edges.numpy(), rep.background, h)
+        tp, ts = torch.tensor(pts), torch.tensor(sup)
+        r2, e2, cid, inside = sc.slots(tp, ts)
+        assert np.array_equal(r2.numpy(), rows) and np.array_equal(e2.numpy(), sl)
+        i2 = sc.indicator(tp, ts, cid, inside).numpy()
+        frac = ind != np.round(ind)
+        assert np.array_equal(i2[~frac], ind[~frac])
+        np.testing.assert_allclose(i2[frac], ind[frac], rtol=0, atol=1e-14)
+        got = OT.twelve(sc, tp, ts, kernels)
+        for k in kernels:
+            err = np.abs(got[k].numpy() - ref[k]).max()
+            assert err <= (5e-12 if k == "w6" else 1e-12), (name, k, err)
+        assert len(rows) > 500
