@@ -307,6 +307,15 @@ int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* o
                   int64_t capacity, const double* chi, const double* pos, const double* sup, int n, int flags,
                   double* out, void* scratch, size_t scratch_bytes, void* stream);
 
+/* ---- the six Hessian entries of baseline(3)[22:28] (reference fluxbridge/contract.py:41-46, the request (0, (i, j)),
+ * i <= j; mesh3d/reference.py:296-305 kernel_hessian at alpha = 0): out[n*6] = int_V d_i d_j W dV in the order (0,0)
+ * (0,1) (0,2) (1,1) (1,2) (2,2), units of h^-2 of the support-unit kernel unless WSB_OUT_PHYSICAL (h^-2 applied),
+ * from the face lists of wsb_adjacency3d (no winding term: the whole ball's Hessian is zero).  Flags
+ * WSB_OUT_PHYSICAL, WSB_OUT_ACCUMULATE; forms WSB_FORM_FP64 and WSB_FORM_MIXED; scratch as wsb_moments3d. */
+int wsb_hessian3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets, const int32_t* faces,
+                  int64_t capacity, const double* pos, const double* sup, int n, int flags, double* out,
+                  void* scratch, size_t scratch_bytes, void* stream);
+
 /* ---- wsb_moments3d without allocation or scratch argument.  No counterpart in the reference: for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The list of the
  * particles with more than 64 faces lives in a file-scope device array of 8 MiB + 256 bytes per GPU (held until the
@@ -326,6 +335,9 @@ int wsb_winding3d_exact_host(const wsb_mesh3d* m, const double* pos, int n, doub
  * WSB_FORM_MIXED evaluates the single-precision angles and square-root seeds of the mixed form on the host */
 int wsb_face_terms3d_host(const wsb_mesh3d* m, int kernel_id, int form, const int32_t* pair_face, const double* pair_pos,
                           const double* pair_sup, int P, double* out);
+/* out[P*6]: the Hessian face accumulators of each (query, face) pair (before the factor 1/pi and h^-2) */
+int wsb_face_hessian3d_host(const wsb_mesh3d* m, int kernel_id, int form, const int32_t* pair_face,
+                            const double* pair_pos, const double* pair_sup, int P, double* out);
 int wsb_finalize22_host(int kernel_id, const double* acc, const double* chi, const double* sup, int n, int flags,
                         double* out);
 

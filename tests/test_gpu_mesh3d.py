@@ -280,3 +280,89 @@ def test_wide_candidate_cells_bitexact():
         np.testing.assert_allclose(adj.chi.cpu().numpy(), hchi, rtol=0, atol=1e-14)
     short = m.adjacency(Q, HH, capacity=100)              # a face list too short at first: counted, then refilled
     assert short.count == len(hfaces) and np.array_equal(short.faces[: short.count].cpu().numpy(), hfaces)
+
+
+# ------------------------------------------------------------------------------------------------ Hessian entries
+def test_hessian_fixtures_fp64_and_mixed():
+    """baseline(3)[22:28] on the device against the 70-digit fixtures: FP64 within 1e-12 (1 + |ref|) (w6: 5e-12, its
+    kernel polynomial's coefficients enter the Hessian undivided), mixed within 1e-6 of the signal."""
+    worst = 0.0
+    for c in _fixture_cases():
+        V = np.array([[float(x) for x in v] for v in c["vertices"]])
+        h = float(c["h"])
+        m = MM.Mesh3D(V, OM.tetrahedron_faces(V), h, device=DEV)
+        Q = torch.tensor([[float(x) for x in c["query"]]], dtype=torch.float64, device=DEV)
+        H = torch.full((1,), h, dtype=torch.float64, device=DEV)
+        adj = m.adjacency(Q, H)
+        ref = np.array([float(v) for v in c["values"]])[22:28]
+        got = m.hessian(adj, Q, H, c["kernel"], "fp64").cpu().numpy()[0]
+        err = np.abs(got - ref) / (1 + np.abs(ref))
+        worst = max(worst, err.max())
+        assert err.max() <= (5e-12 if c["kernel"] == "w6" else 1e-12), (c.get("label", c["query"]), c["kernel"], err.max())
+        mix = m.hessian(adj, Q, H, c["kernel"], "mixed").cpu().numpy()[0]
+        assert (np.abs(mix - ref) <= 1e-6 * max(np.abs(ref).max(), 1e-300)).all(), (c.get("label"), c["kernel"])
+    print("Hessian fixture worst", worst)
+
+
+@pytest.mark.parametrize("k", KERNELS)
+def test_hessian_tiles_and_heavy_rows_match_host_and_oracle(k):
+    """the tile and heavy-row launches of the Hessian against the host arithmetic (summed in order) and the oracle's
+    kernel_hessian on a fine sphere in a box (hundreds of faces per particle near the sphere); the trace is the
+    oracle's laplacian; physical units (h^-2) against units of h."""
+    V, F = MM.box_surface((0, 0, 0), (1, 1, 1))
+    Vs, Fs = MM.icosphere(3, 0.25, (0.5, 0.5, 0.5))
+    V, F = np.vstack([V, Vs]), np.vstack([F, (Fs + 8)[:, ::-1]])
+    h = 0.12
+    rng = np.random.default_rng(9)
+    d = rng.normal(size=(150, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    P = np.vstack([0.5 + d * (0.25 + rng.uniform(-0.05, 0.1, 150))[:, None], rng.uniform(-0.05, 1.05, (90, 3))])
+    H = np.full(len(P), h)
+    H[::4] *= 0.7
+    m = MM.Mesh3D(V, F, h, device=DEV)
+    Q, HH = torch.tensor(P, device=DEV), torch.tensor(H, device=DEV)
+    adj = m.adjacency(Q, HH)
+    counts = np.diff(adj.offsets.cpu().numpy())
+    assert (counts > 64).sum() > 20 and (counts <= 64).sum() > 20
+    got = m.hessian(adj, Q, HH, k, "fp64").cpu().numpy()
+    host = MM.Mesh3D(V, F, h, host_only=True).hessian_host(P, H, k)
+    scale = np.abs(host).max()
+    # the host build contracts other products than the device: w6's undivided coefficients amplify that fivefold
+    assert np.abs(got - host).max() <= (5e-12 if k == "w6" else 1e-12) * scale, (k, np.abs(got - host).max() / scale)
+    units = m.hessian(adj, Q, HH, k, "fp64", physical=False).cpu().numpy()
+    np.testing.assert_allclose(units / (H * H)[:, None], got, rtol=1e-14, atol=1e-14 * scale)
+    cs = OM.ClosedSurfacePairs(V, F, P)
+    Hs = [cs.kernel_hessian(k, (0, 0, 0), hh) for hh in sorted(set(H))]
+    ref = np.zeros_like(got)
+    for hh, Hm in zip(sorted(set(H)), Hs):
+        sel = H == hh
+        ref[sel] = np.stack([Hm[i][j] for i, j in MM._HESSIAN], 1)[sel]
+    assert np.abs(got - ref).max() <= (1e-11 if k == "w6" else 2e-12) * scale, (k, np.abs(got - ref).max() / scale)
+    lap = np.zeros(len(P))
+    for hh in sorted(set(H)):
+        sel = H == hh
+        lap[sel] = cs.laplacian(k, (0, 0, 0), hh)[sel]
+    np.testing.assert_allclose(got[:, 0] + got[:, 3] + got[:, 5], lap, rtol=0, atol=1e-11 * scale)
+    # accumulate adds
+    acc = m.hessian(adj, Q, HH, k, "fp64", out=torch.tensor(got, device=DEV), accumulate=True).cpu().numpy()
+    assert np.array_equal(acc, 2 * got)
+
+
+def test_hessian_element_api_and_full_baseline_evaluator():
+    c = _fixture_cases()[4]
+    V = np.array([[float(x) for x in v] for v in c["vertices"]])
+    q = [float(x) for x in c["query"]]
+    h = float(c["h"])
+    ref = np.array([float(v) for v in c["values"]])
+    ev = MM.Evaluator(MM.Geometry([V]), kernel=c["kernel"])
+    assert len(ev.requests) == 28
+    got = ev.evaluate([q], h)[0]
+    gate = 5e-12 if c["kernel"] == "w6" else 1e-12
+    assert (np.abs(got - ref[:28]) <= gate * (1 + np.abs(ref[:28]))).all()
+    tet = MM.Tetrahedron(V, q)
+    Hm = np.array(tet.kernel_hessian(c["kernel"], (0, 0, 0), h))
+    assert np.allclose(Hm, Hm.T, rtol=0, atol=0)
+    assert np.allclose([Hm[i, j] for i, j in MM._HESSIAN], ref[22:28], rtol=0, atol=gate * (1 + np.abs(ref[22:28]).max()))
+    assert abs(tet.laplacian(c["kernel"], (0, 0, 0), h) - np.trace(Hm)) <= 1e-13 * (1 + abs(np.trace(Hm)))
+    with pytest.raises(NotImplementedError):
+        tet.kernel_hessian(c["kernel"], (1, 0, 0), h)

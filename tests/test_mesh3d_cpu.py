@@ -270,3 +270,47 @@ def test_surface_validation():
     soup = MM.Mesh3D(V, F[:3], 0.1, host_only=True, soup=True)
     off, faces, chi = soup.adjacency_host(np.array([[0.05, 0.5, 0.5]]), 0.1)
     assert (chi == 0).all() and len(faces) >= 1
+
+
+# ------------------------------------------------------------------------------------------------ Hessian entries
+@pytest.mark.parametrize("form,tol", [("fp64", 1e-12), ("mixed", 1e-10)])
+def test_host_hessian_matches_fixtures(form, tol):
+    """baseline(3)[22:28] (int_V d_i d_j W, i <= j) of the device arithmetic run on the host against the 70-digit
+    fixture values of every tetrahedron case (fluxbridge-v1 and the queries 1e-16 from faces, edges and vertices).
+    The Hessian's radial function W'/r carries the kernel polynomial's coefficients undivided (the gradient channels
+    see them divided by m + 2), so w6's alternating coefficients (to 2.5e4) cost it a few ulps more: its FP64 gate
+    is five times the others', as for w6 in the 2-D gates (reference tests/edge/test_warp2d.py:35)."""
+    worst, n = 0.0, 0
+    for c in _fixture_cases():
+        req = _req(c)
+        assert req[22:28] == [((0, 0, 0), d) for d in MM._HESSIAN]
+        V, q = _vq(c)
+        m = MM.Mesh3D(V, OM.tetrahedron_faces(V), float(c["h"]), host_only=True)
+        got = m.hessian_host(q[None], float(c["h"]), c["kernel"], form=form)[0]
+        ref = _ref(c)[22:28]
+        err = np.abs(got - ref) / (1 + np.abs(ref))
+        worst = max(worst, err.max())
+        n += 1
+        gate = 5 * tol if (c["kernel"] == "w6" and form == "fp64") else tol
+        assert err.max() <= gate, (c.get("label", c["query"]), c["kernel"], err.max())
+    assert n > 50 and worst > 0
+
+
+def test_host_hessian_matches_the_oracle_on_the_cavity():
+    """the same arithmetic against the oracle's kernel_hessian (pinned to the fixtures above) on the box with an
+    icosphere cavity, points near the sphere, the walls and everywhere; trace = the oracle's laplacian; symmetric."""
+    V, F = _cavity(2)
+    P = _points(40, 3)
+    h = 0.12
+    m = MM.Mesh3D(V, F, h, host_only=True)
+    for kernel in KERNELS:
+        got = m.hessian_host(P, h, kernel)
+        cs = OM.ClosedSurfacePairs(V, F, P)
+        H = cs.kernel_hessian(kernel, (0, 0, 0), h)
+        ref = np.stack([H[i][j] for i, j in MM._HESSIAN], 1)
+        scale = max(1.0, float(np.abs(ref).max()))
+        assert np.abs(got - ref).max() <= 1e-11 * scale, (kernel, np.abs(got - ref).max())
+        np.testing.assert_allclose(H[0][1], H[1][0], rtol=0, atol=1e-9 * scale)
+        lap = cs.laplacian(kernel, (0, 0, 0), h)
+        np.testing.assert_allclose(got[:, 0] + got[:, 3] + got[:, 5], lap, rtol=0, atol=1e-11 * scale)
+        assert np.abs(ref).max() > 1.0

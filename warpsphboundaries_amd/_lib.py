@@ -87,12 +87,15 @@ SIGNATURES = {
     "wsb_moments3d_scratch_bytes": (c_size_t, [c_int]),
     "wsb_moments3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int,
                               c_void_p, c_void_p, c_size_t, c_void_p]),
+    "wsb_hessian3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int,
+                              c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_moments3d_static_max_rows": (c_int64, []),
     "wsb_moments3d_static": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int,
                                      c_int, c_void_p, c_void_p]),
     "wsb_adjacency3d_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
     "wsb_winding3d_exact_host": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     "wsb_face_terms3d_host": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "wsb_face_hessian3d_host": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "wsb_finalize22_host": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
 }
 

@@ -215,6 +215,13 @@ def render3d():
                 out.append(f"  static constexpr double Q{key}[NB][{NM3}] = {{{', '.join(_arr3(r) for r in rows)}}};")
                 out.append(f"  static constexpr double GR{key}[NB] = "
                            f"{{{', '.join(repr(float(GR(getattr(b, attr), b.R))) for b in bl)}}};")
+        # Hessian: the radial function W'(r)/r = sum_k k P_k r^(k-2) has the K-chain weights (k P_k)/k = P_k at index
+        # k = m + 2 (k >= 1), G(R) = sum_(k>=1) P_k R^k; the in-plane edge integrand is W(r) - W(R) without its
+        # constant term: dI_k weights P_k (k >= 1) and -G(R) at k = 0.  HW holds both (index k, -G(R) at 0).
+        grl = [sum(c * b.R ** k for k, c in b.P.items() if k >= 1) for b in bl]
+        hw = [[(-grl[i] if k == 0 else b.P.get(k, 0)) for k in range(NM3)] for i, b in enumerate(bl)]
+        out.append(f"  static constexpr double HW[NB][{NM3}] = {{{', '.join(_arr3(r) for r in hw)}}};")
+        out.append(f"  static constexpr double GRL[NB] = {{{', '.join(repr(float(g)) for g in grl)}}};")
         out.append(f"  static constexpr double MR[NB] = {{{', '.join(repr(float(b.mR)) for b in bl)}}};")
         out.append(f"  static constexpr double MRF[NB] = {{{', '.join(repr(float(b.mRF)) for b in bl)}}};")
         out.append(f"  static constexpr double MRSUM = {float(sum(b.mR for b in bl))!r};")

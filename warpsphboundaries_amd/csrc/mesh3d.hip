@@ -756,14 +756,41 @@ struct Mom3Args {
 // pairs, never on the packing or on the other particles of its batch.
 constexpr int HEAVY3 = 64;
 
-template <int KID, int FORM>
+// MODE 0: the 22 stored channels (23 face accumulators, winding term); MODE 1: the six Hessian entries (six
+// accumulators, no winding term: the whole ball's Hessian is zero)
+template <int MODE> struct Mode3;
+template <> struct Mode3<0> {
+    static constexpr int NA = m3::NACC, NO = m3::NOUT;
+    template <int KID, int FORM>
+    __device__ static void pair(const double* fr, double px, double py, double pz, double ih, double* acc) {
+        m3::face_terms<KID, FORM>(fr, px, py, pz, ih, acc);
+    }
+    template <int KID>
+    __device__ static void finalize(const double* acc, double chi, double h, bool physical, double* out) {
+        m3::finalize22<KID>(acc, chi, h, physical, out);
+    }
+};
+template <> struct Mode3<1> {
+    static constexpr int NA = m3::NHESS, NO = m3::NHESS;
+    template <int KID, int FORM>
+    __device__ static void pair(const double* fr, double px, double py, double pz, double ih, double* acc) {
+        m3::face_hessian<KID, FORM>(fr, px, py, pz, ih, acc);
+    }
+    template <int KID>
+    __device__ static void finalize(const double* acc, double, double h, bool physical, double* out) {
+        m3::finalize_hessian(acc, h, physical, out);
+    }
+};
+
+template <int KID, int FORM, int MODE>
 __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
+    using M = Mode3<MODE>;
     using BlockScan = hipcub::BlockScan<int, TILE3>;
     __shared__ typename BlockScan::TempStorage scan_tmp;
     __shared__ int s_off[TILE3 + 1];
     __shared__ int s_first[TILE3];
     __shared__ double s_p[4][TILE3];
-    __shared__ double s_res[m3::NACC][TILE3];
+    __shared__ double s_res[M::NA][TILE3];
     const int tid = threadIdx.x;
     const int64_t r0 = blockIdx.x * (int64_t)a.rows_per_tile;
     const int64_t row = r0 + tid;
@@ -790,14 +817,14 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
     s_off[tid] = off;
     if (tid == 0) s_off[TILE3] = total;
     const bool mapped = __syncthreads_or(heavy);
-    double acc[m3::NACC];
+    double acc[M::NA];
 #pragma unroll
-    for (int k = 0; k < m3::NACC; ++k) acc[k] = 0.0;
+    for (int k = 0; k < M::NA; ++k) acc[k] = 0.0;
     for (int base = 0; base < total; base += TILE3) {
         const int j = base + tid;
-        double c[m3::NACC];
+        double c[M::NA];
 #pragma unroll
-        for (int k = 0; k < m3::NACC; ++k) c[k] = 0.0;
+        for (int k = 0; k < M::NA; ++k) c[k] = 0.0;
         if (j < total) {
             int lo = 0, hi = TILE3;
             while (hi - lo > 1) {
@@ -809,32 +836,32 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
             const int f = slot < a.capacity ? a.faces[slot] : -1;
             const double ihl = s_p[3][lo];
             if (f >= 0 && f < a.nf && ihl > 0.0)
-                m3::face_terms<KID, FORM>(a.frec + m3::FREC * (int64_t)f, s_p[0][lo], s_p[1][lo], s_p[2][lo], ihl, c);
+                M::template pair<KID, FORM>(a.frec + m3::FREC * (int64_t)f, s_p[0][lo], s_p[1][lo], s_p[2][lo], ihl, c);
         }
 #pragma unroll
-        for (int k = 0; k < m3::NACC; ++k) s_res[k][tid] = c[k];
+        for (int k = 0; k < M::NA; ++k) s_res[k][tid] = c[k];
         __syncthreads();
         const int b0 = max(off, base), b1 = heavy ? b0 : min(off + cnt, base + TILE3);   // off + 0 for heavy
         for (int jj = b0; jj < b1; ++jj) {
 #pragma unroll
-            for (int k = 0; k < m3::NACC; ++k) acc[k] += s_res[k][jj - base];
+            for (int k = 0; k < M::NA; ++k) acc[k] += s_res[k][jj - base];
         }
         __syncthreads();
     }
     // finalise into LDS, then store the tile's rows as one contiguous block (coalesced across the lanes)
     if (live) {
-        double o[m3::NOUT];
-        const double chi = (a.flags & WSB_OUT_NO_INDICATOR) ? 0.0 : a.chi[row];
-        m3::finalize22<KID>(acc, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, o);
+        double o[M::NO];
+        const double chi = (MODE != 0 || (a.flags & WSB_OUT_NO_INDICATOR)) ? 0.0 : a.chi[row];
+        M::template finalize<KID>(acc, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, o);
 #pragma unroll
-        for (int k = 0; k < m3::NOUT; ++k) s_res[k][tid] = o[k];
+        for (int k = 0; k < M::NO; ++k) s_res[k][tid] = o[k];
     }
     __syncthreads();
     const int nrows = (int)min((int64_t)a.rows_per_tile, (int64_t)a.n - r0);
-    double* dst = a.out + r0 * m3::NOUT;
+    double* dst = a.out + r0 * M::NO;
     const bool accumulate = (a.flags & WSB_OUT_ACCUMULATE) != 0;
-    for (int idx = tid; idx < nrows * m3::NOUT; idx += TILE3) {
-        const int rr = idx / m3::NOUT, k = idx - rr * m3::NOUT;
+    for (int idx = tid; idx < nrows * M::NO; idx += TILE3) {
+        const int rr = idx / M::NO, k = idx - rr * M::NO;
         if (s_p[3][rr] < 0.0) continue;                // stored by heavy22_kernel
         const double v = s_res[k][rr];
         if (accumulate) dst[idx] += v;
@@ -846,9 +873,10 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
 // fixed grid of workgroups taking them in turn): pair j of a particle goes to thread j % 256, each thread sums its
 // pairs in order, a butterfly adds the 64 sums of each wavefront, and the four wavefront sums are added in wavefront
 // order -- the same order whatever else is in the batch
-template <int KID, int FORM>
+template <int KID, int FORM, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void heavy22_kernel(Mom3Args a) {
-    __shared__ double s_acc[4][m3::NACC];
+    using M = Mode3<MODE>;
+    __shared__ double s_acc[4][M::NA];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const bool accumulate = (a.flags & WSB_OUT_ACCUMULATE) != 0;
     const int nheavy = *a.heavy_count;
@@ -859,36 +887,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void h
         const double px = a.pos[3 * r], py = a.pos[3 * r + 1], pz = a.pos[3 * r + 2];
         const double h = a.sup[r];
         const double ih = 1.0 / h;
-        double acc[m3::NACC];
+        double acc[M::NA];
 #pragma unroll
-        for (int k = 0; k < m3::NACC; ++k) acc[k] = 0.0;
+        for (int k = 0; k < M::NA; ++k) acc[k] = 0.0;
         for (int j = tid; j < rc; j += 256) {
             const int64_t slot = first + j;
             const int f = slot < a.capacity ? a.faces[slot] : -1;
-            if (f >= 0 && f < a.nf) m3::face_terms<KID, FORM>(a.frec + m3::FREC * (int64_t)f, px, py, pz, ih, acc);
+            if (f >= 0 && f < a.nf) M::template pair<KID, FORM>(a.frec + m3::FREC * (int64_t)f, px, py, pz, ih, acc);
         }
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
 #pragma unroll
-            for (int k = 0; k < m3::NACC; ++k) acc[k] += __shfl_xor(acc[k], o);
+            for (int k = 0; k < M::NA; ++k) acc[k] += __shfl_xor(acc[k], o);
         }
         if (lane == 0) {
 #pragma unroll
-            for (int k = 0; k < m3::NACC; ++k) s_acc[wv][k] = acc[k];
+            for (int k = 0; k < M::NA; ++k) s_acc[wv][k] = acc[k];
         }
         __syncthreads();
         if (wv == 0) {
-            double tot[m3::NACC];
+            double tot[M::NA];
 #pragma unroll
-            for (int k = 0; k < m3::NACC; ++k) tot[k] = ((s_acc[0][k] + s_acc[1][k]) + s_acc[2][k]) + s_acc[3][k];
-            double out[m3::NOUT];
-            const double chi = (a.flags & WSB_OUT_NO_INDICATOR) ? 0.0 : a.chi[r];
-            m3::finalize22<KID>(tot, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, out);
+            for (int k = 0; k < M::NA; ++k) tot[k] = ((s_acc[0][k] + s_acc[1][k]) + s_acc[2][k]) + s_acc[3][k];
+            double out[M::NO];
+            const double chi = (MODE != 0 || (a.flags & WSB_OUT_NO_INDICATOR)) ? 0.0 : a.chi[r];
+            M::template finalize<KID>(tot, chi, h, (a.flags & WSB_OUT_PHYSICAL) != 0, out);
             double v = 0.0;
 #pragma unroll
-            for (int k = 0; k < m3::NOUT; ++k) v = (k == lane) ? out[k] : v;
-            if (lane < m3::NOUT) {
-                double* dst = a.out + r * m3::NOUT + lane;
+            for (int k = 0; k < M::NO; ++k) v = (k == lane) ? out[k] : v;
+            if (lane < M::NO) {
+                double* dst = a.out + r * M::NO + lane;
                 if (accumulate) *dst += v;
                 else *dst = v;
             }
@@ -904,30 +932,30 @@ __global__ void zero3_counts_kernel(int32_t* c) {
     c[2] = 0;
 }
 
-template <int KID, int FORM>
+template <int KID, int FORM, int MODE>
 int launch22(const Mom3Args& a, hipStream_t s) {
     hipLaunchKernelGGL(zero_count3_kernel, dim3(1), dim3(1), 0, s, a.heavy_count);
     // rows per tile: TILE3, halved (down to 16) while the launch has fewer than 2,048 workgroups
     Mom3Args t = a;
     t.rows_per_tile = TILE3;
     while (t.rows_per_tile > 16 && ((int64_t)a.n + t.rows_per_tile - 1) / t.rows_per_tile < 2048) t.rows_per_tile >>= 1;
-    hipLaunchKernelGGL((tile22_kernel<KID, FORM>), dim3((unsigned)(((int64_t)a.n + t.rows_per_tile - 1) / t.rows_per_tile)),
+    hipLaunchKernelGGL((tile22_kernel<KID, FORM, MODE>), dim3((unsigned)(((int64_t)a.n + t.rows_per_tile - 1) / t.rows_per_tile)),
                        dim3(TILE3), 0, s, t);
     WSB_HIP(hipGetLastError());
     // up to 2,048 workgroups (8 per CU), one per listed particle at a time
     const int blocks = (int)std::min<int64_t>((int64_t)a.n, 2048);
-    hipLaunchKernelGGL((heavy22_kernel<KID, FORM>), dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((heavy22_kernel<KID, FORM, MODE>), dim3(blocks), dim3(256), 0, s, a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
 
-template <int FORM>
+template <int FORM, int MODE = 0>
 int dispatch22(int kid, const Mom3Args& a, hipStream_t s) {
     switch (kid) {
-        case WSB_CUBIC: return launch22<WSB_CUBIC, FORM>(a, s);
-        case WSB_W2: return launch22<WSB_W2, FORM>(a, s);
-        case WSB_W4: return launch22<WSB_W4, FORM>(a, s);
-        case WSB_W6: return launch22<WSB_W6, FORM>(a, s);
+        case WSB_CUBIC: return launch22<WSB_CUBIC, FORM, MODE>(a, s);
+        case WSB_W2: return launch22<WSB_W2, FORM, MODE>(a, s);
+        case WSB_W4: return launch22<WSB_W4, FORM, MODE>(a, s);
+        case WSB_W6: return launch22<WSB_W6, FORM, MODE>(a, s);
     }
     return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kid);
 }
@@ -939,6 +967,16 @@ void host_pairs(const double* frec, const int32_t* pf, const double* pq, const d
         for (int k = 0; k < m3::NACC; ++k) c[k] = 0.0;
         m3::face_terms<KID, FORM>(frec + m3::FREC * (int64_t)pf[j], pq[3 * j], pq[3 * j + 1], pq[3 * j + 2], 1.0 / ph[j], c);
         for (int k = 0; k < m3::NACC; ++k) out[(int64_t)j * m3::NACC + k] = c[k];
+    }
+}
+
+template <int KID, int FORM>
+void host_hessian_pairs(const double* frec, const int32_t* pf, const double* pq, const double* ph, int P, double* out) {
+    for (int j = 0; j < P; ++j) {
+        double c[m3::NHESS];
+        for (int k = 0; k < m3::NHESS; ++k) c[k] = 0.0;
+        m3::face_hessian<KID, FORM>(frec + m3::FREC * (int64_t)pf[j], pq[3 * j], pq[3 * j + 1], pq[3 * j + 2], 1.0 / ph[j], c);
+        for (int k = 0; k < m3::NHESS; ++k) out[(int64_t)j * m3::NHESS + k] = c[k];
     }
 }
 
@@ -1197,6 +1235,29 @@ extern "C" int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const
     return wsb::set_error(WSB_ENOTSUP, "3-D moments support the fp64 and mixed forms (form %d)", form);
 }
 
+extern "C" int wsb_hessian3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets, const int32_t* faces,
+                             int64_t capacity, const double* pos, const double* sup, int n, int flags, double* out,
+                             void* scratch, size_t scratch_bytes, void* stream) {
+    WSB_REQUIRE(m, WSB_EINVAL, "null mesh");
+    WSB_REQUIRE(!m->host_only, WSB_EINVAL, "host-only mesh has no device tables");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(offsets && pos && sup && out, WSB_EINVAL, "null buffer");
+    WSB_REQUIRE(!(flags & ~(WSB_OUT_PHYSICAL | WSB_OUT_ACCUMULATE)), WSB_EINVAL,
+                "3-D Hessian flags: WSB_OUT_PHYSICAL, WSB_OUT_ACCUMULATE (got %d)", flags);
+    WSB_REQUIRE(scratch && scratch_bytes >= wsb_moments3d_scratch_bytes(n), WSB_ENOMEM,
+                "3-D Hessian needs %zu scratch bytes (got %zu)", wsb_moments3d_scratch_bytes(n), scratch_bytes);
+    int32_t* hc = static_cast<int32_t*>(scratch);
+    int32_t* hl = reinterpret_cast<int32_t*>(static_cast<char*>(scratch) + 256);
+    Mom3Args a{offsets, faces, faces ? capacity : 0, nullptr, pos, sup, n, m->d_frec, m->nf, flags, out, hc, hl};
+    hipStream_t s = wsb::as_stream(stream);
+    switch (form) {
+        case WSB_FORM_FP64: return dispatch22<m3::FP64, 1>(kernel_id, a, s);
+        case WSB_FORM_MIXED: return dispatch22<m3::MIXED, 1>(kernel_id, a, s);
+    }
+    return wsb::set_error(WSB_ENOTSUP, "3-D Hessian supports the fp64 and mixed forms (form %d)", form);
+}
+
 // ------------------------------------------------------------------------------------------------ no allocation
 // The heavy-row list of wsb_moments3d_static lives in a file-scope device array: its count in the first 256 bytes,
 // then one row index per particle.  Each GPU holds its own copy, at its own address, until the process ends; the host
@@ -1306,6 +1367,26 @@ extern "C" int wsb_face_terms3d_host(const wsb_mesh3d* m, int kernel_id, int for
     if (form == WSB_FORM_FP64) return host_pairs_form<m3::FP64>(kernel_id, m->hg.frec, pair_face, pair_pos, pair_sup, P, out);
     if (form == WSB_FORM_MIXED) return host_pairs_form<m3::MIXED>(kernel_id, m->hg.frec, pair_face, pair_pos, pair_sup, P, out);
     return wsb::set_error(WSB_ENOTSUP, "3-D face terms support the fp64 and mixed forms (form %d)", form);
+}
+
+extern "C" int wsb_face_hessian3d_host(const wsb_mesh3d* m, int kernel_id, int form, const int32_t* pair_face,
+                                       const double* pair_pos, const double* pair_sup, int P, double* out) {
+    WSB_REQUIRE(m && pair_face && pair_pos && pair_sup && out, WSB_EINVAL, "null argument");
+    for (int j = 0; j < P; ++j) WSB_REQUIRE(pair_face[j] >= 0 && pair_face[j] < m->nf, WSB_EINVAL, "face index out of range");
+    WSB_REQUIRE(form == WSB_FORM_FP64 || form == WSB_FORM_MIXED, WSB_ENOTSUP, "3-D Hessian supports the fp64 and mixed forms (form %d)", form);
+    const double* fr = m->hg.frec;
+    const bool mixed = form == WSB_FORM_MIXED;
+    switch (kernel_id) {
+        case WSB_CUBIC: mixed ? host_hessian_pairs<WSB_CUBIC, m3::MIXED>(fr, pair_face, pair_pos, pair_sup, P, out)
+                              : host_hessian_pairs<WSB_CUBIC, m3::FP64>(fr, pair_face, pair_pos, pair_sup, P, out); return WSB_OK;
+        case WSB_W2: mixed ? host_hessian_pairs<WSB_W2, m3::MIXED>(fr, pair_face, pair_pos, pair_sup, P, out)
+                           : host_hessian_pairs<WSB_W2, m3::FP64>(fr, pair_face, pair_pos, pair_sup, P, out); return WSB_OK;
+        case WSB_W4: mixed ? host_hessian_pairs<WSB_W4, m3::MIXED>(fr, pair_face, pair_pos, pair_sup, P, out)
+                           : host_hessian_pairs<WSB_W4, m3::FP64>(fr, pair_face, pair_pos, pair_sup, P, out); return WSB_OK;
+        case WSB_W6: mixed ? host_hessian_pairs<WSB_W6, m3::MIXED>(fr, pair_face, pair_pos, pair_sup, P, out)
+                           : host_hessian_pairs<WSB_W6, m3::FP64>(fr, pair_face, pair_pos, pair_sup, P, out); return WSB_OK;
+    }
+    return wsb::set_error(WSB_ENOTSUP, "unknown kernel id %d", kernel_id);
 }
 
 extern "C" int wsb_finalize22_host(int kernel_id, const double* acc, const double* chi, const double* sup, int n, int flags,

@@ -525,6 +525,134 @@ __host__ __device__ inline void face_terms(const double* __restrict__ fr, double
         for (int j = 0; j < 3; ++j) acc[k++] -= n[j] * (z * n[d] * sP + l1[d]);
 }
 
+// The six Hessian entries of baseline(3)[22:28] (fluxbridge/contract.py:41-46; mesh3d/reference.py:296-305 at
+// alpha = 0): int_V d_i d_j W = sum_f n_i int_f y_j L(r) dS with L = W'(r)/r (the sphere part vanishes: W' = 0 at
+// every block radius).  On a face y = z n + rho, so int_f y_j L = z n_j S_f[L] + sum_e mu_e,j int_e (W - W(R)) ds
+// over the in-ball chord (the in-plane divergence theorem; W - W(R) vanishes on the arc).  S_f[L] is the face
+// closed form above with the K-chain weights HW[k] at index k = m + 2 (L has the power r^-1 where W has r^1, whose
+// K_1 = I_-1 + a^2 K_-1 the chain already holds); the edge part weights the same chord integrals dI_k with HW[k]
+// (-G(R) at k = 0).  acc[0..5] += the face's (i, j) terms for (0,0) (0,1) (0,2) (1,1) (1,2) (2,2), units of h.
+constexpr int NHESS = 6;
+
+template <int KID, int FORM>
+__host__ __device__ inline void face_hessian(const double* __restrict__ fr, double qx, double qy, double qz, double ih,
+                                             double acc[NHESS]) {
+    using T = K3Tab<KID>;
+    constexpr int DG = T::DEG;
+    PairGeo g;
+    pair_geo(fr, qx, qy, qz, ih, g);
+    const double z = g.z, a = fabs(z), sz = sgn(z);
+    if (!(a < T::R[0])) return;
+    double rv[3];
+    vertex_radii<FORM>(g, rv);
+    const double Om = omega_face<FORM>(g, rv);
+    double apow[DG + 3];
+    apow[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k <= DG + 2; ++k) apow[k] = apow[k - 1] * a;
+    double sL = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;
+#pragma unroll
+    for (int b = 0; b < T::NB; ++b) {
+        const double R = T::R[b];
+        if (!(a < R)) continue;
+        double tL = 0.0, thO = 0.0, phO = 0.0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const double w = g.w[e], lo = g.lo[e], hi = g.hi[e];
+            const double c2 = w * w + a * a;
+            const double bw = fabs(w), sw = sgn(w);
+            bool chord = false, clo = false, chi = false;
+            double L = 0.0, l = 0.0, u = 0.0;
+            if (c2 < R * R) {
+                L = xsqrt<FORM>(R * R - c2);
+                clo = lo < -L;
+                chi = hi > L;
+                l = clo ? -L : lo;
+                u = chi ? L : hi;
+                chord = l < u;
+            }
+            const double rlo = rv[e], rhi = rv[(e + 1) % 3];
+            if (bw > 0.0) {
+                if (!chord) {
+                    thO += sw * piece_theta<FORM>(bw, lo, hi);
+                    if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, lo, rlo, hi, rhi);
+                } else {
+                    if (clo) {
+                        thO += sw * piece_theta<FORM>(bw, lo, -L);
+                        if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, lo, rlo, -L, R);
+                    }
+                    if (chi) {
+                        thO += sw * piece_theta<FORM>(bw, L, hi);
+                        if (a > 0.0) phO += sw * piece_phi<FORM>(a, bw, c2, L, R, hi, rhi);
+                    }
+                }
+            }
+            if (!chord) continue;
+            const double rl = clo ? R : rlo;
+            const double ru = chi ? R : rhi;
+            double dIm1 = 0.0;
+            if (c2 > 0.0) {
+                const double Nu = u >= 0.0 ? u + ru : c2, Du = u >= 0.0 ? 1.0 : ru - u;
+                const double Nl = l >= 0.0 ? l + rl : c2, Dl = l >= 0.0 ? 1.0 : rl - l;
+                dIm1 = log((Nu * Dl) / (Du * Nl));
+            }
+            double dI[DG + 1];
+            double wJ[DG + 1];
+            dI[0] = u - l;
+            double pu = u, pl = l;
+            wJ[0] = 0.0;
+            if (DG >= 1) {
+                pu *= ru;
+                pl *= rl;
+                dI[1] = 0.5 * ((pu - pl) + c2 * dIm1);
+                wJ[1] = w * dIm1;
+            }
+#pragma unroll
+            for (int k = 2; k <= DG; ++k) {
+                pu *= ru;
+                pl *= rl;
+                dI[k] = ((pu - pl) + (double)k * c2 * dI[k - 2]) * (1.0 / (double)(k + 1));
+                wJ[k] = w * dI[k - 2] + apow[2] * wJ[k - 2];
+            }
+            double eL = 0.0, lW = 0.0;
+#pragma unroll
+            for (int k = 0; k <= DG; ++k) {
+                if (T::HW[b][k] != 0.0) {
+                    if (k >= 1) eL += T::HW[b][k] * wJ[k];
+                    lW += T::HW[b][k] * dI[k];
+                }
+            }
+            tL += eL;
+            const double* mu = fr + FR_MU + 3 * e;
+            hx += mu[0] * lW;
+            hy += mu[1] * lW;
+            hz += mu[2] * lW;
+        }
+        double eL2 = 0.0, oL = 0.0;
+#pragma unroll
+        for (int k = 1; k <= DG; ++k) {
+            if (T::HW[b][k] == 0.0) continue;
+            if (k % 2 == 0) eL2 += T::HW[b][k] * apow[k];
+            else oL += T::HW[b][k] * apow[k];
+        }
+        sL += tL + (T::GRL[b] - eL2) * thO - oL * (sz * Om + phO);
+    }
+    const double n[3] = {fr[FR_N], fr[FR_N + 1], fr[FR_N + 2]};
+    const double hl[3] = {hx, hy, hz};
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) acc[k++] += n[i] * (z * n[j] * sL + hl[j]);
+}
+
+// Hessian accumulators -> the six entries (x 1/pi of the tables), physical units (h^-2) when `physical`
+__host__ __device__ inline void finalize_hessian(const double acc[NHESS], double h, bool physical, double out[NHESS]) {
+    const double s = physical ? INV_PI / (h * h) : INV_PI;
+#pragma unroll
+    for (int k = 0; k < NHESS; ++k) out[k] = acc[k] * s;
+}
+
 // accumulators + winding number -> the 22 channels of baseline(3)[:22] (fluxbridge/contract.py:41-46), physical
 // units when `physical` (h^(|alpha| - |derivative|), fluxbridge/evaluate.py:76-78)
 template <int KID>

@@ -186,8 +186,33 @@ class Mesh3D:
                                   L.ptr(scratch), scratch.numel(), L.stream_ptr(pos.device)))
         return out
 
-    def evaluate(self, pos, sup, kernel="w2", form="fp64", physical=True):
-        return self.moments(self.adjacency(pos, sup), pos, sup, kernel, form, physical)
+    def hessian(self, adj: Adjacency3D, pos: torch.Tensor, sup: torch.Tensor, kernel="w2", form="fp64",
+                physical: bool = True, out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
+        """[N, 6] Hessian entries int_V d_i d_j W (i <= j: xx, xy, xz, yy, yz, zz), the requests baseline(3)[22:28]."""
+        self._check_inputs(pos, sup)
+        n = pos.shape[0]
+        if out is None:
+            out = torch.zeros((n, 6), dtype=torch.float64, device=pos.device)
+        elif out.shape != (n, 6) or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 [N, 6] tensor")
+        flags = (L.OUT_PHYSICAL if physical else 0) | (L.OUT_ACCUMULATE if accumulate else 0)
+        fid = L.form_id(form)
+        if fid == 0:
+            raise NotImplementedError("the 3-D path provides the fp64 and mixed forms")
+        lib = L.load()
+        scratch = self._buf("_mom_scratch", lib.wsb_moments3d_scratch_bytes(n), pos.device)
+        L.check(lib.wsb_hessian3d(self._h, KERNEL_ID[_kernel_name(kernel)], fid, L.ptr(adj.offsets), L.ptr(adj.faces),
+                                  adj.faces.numel(), L.ptr(pos), L.ptr(sup), n, flags, L.ptr(out), L.ptr(scratch),
+                                  scratch.numel(), L.stream_ptr(pos.device)))
+        return out
+
+    def evaluate(self, pos, sup, kernel="w2", form="fp64", physical=True, hessian=False):
+        """[N, 22] stored moments, [N, 28] with the Hessian entries (all of baseline(3))."""
+        adj = self.adjacency(pos, sup)
+        m = self.moments(adj, pos, sup, kernel, form, physical)
+        if not hessian:
+            return m
+        return torch.cat([m, self.hessian(adj, pos, sup, kernel, form, physical)], 1)
 
     # ------------------------------------------------------------------------------------------ host evaluation
     def adjacency_host(self, pos, sup):
@@ -217,6 +242,27 @@ class Mesh3D:
         L.check(L.load().wsb_face_terms3d_host(self._h, KERNEL_ID[_kernel_name(kernel)], L.form_id(form), L.nptr(f),
                                                L.nptr(P), L.nptr(H), len(f), L.nptr(out)))
         return out
+
+    def face_hessian_host(self, kernel, pair_face, pair_pos, pair_sup, form="fp64"):
+        f = np.ascontiguousarray(np.asarray(pair_face, dtype=np.int32))
+        P = np.ascontiguousarray(np.asarray(pair_pos, dtype=np.float64).reshape(-1, 3))
+        H = np.ascontiguousarray(np.broadcast_to(np.asarray(pair_sup, dtype=np.float64), (len(f),)))
+        out = np.zeros((len(f), 6))
+        L.check(L.load().wsb_face_hessian3d_host(self._h, KERNEL_ID[_kernel_name(kernel)], L.form_id(form), L.nptr(f),
+                                                 L.nptr(P), L.nptr(H), len(f), L.nptr(out)))
+        return out
+
+    def hessian_host(self, pos, sup, kernel="w2", physical=True, form="fp64"):
+        """the device Hessian arithmetic run on the host: [N, 6]."""
+        P = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+        H = np.ascontiguousarray(np.broadcast_to(np.asarray(sup, dtype=np.float64), (len(P),)))
+        off, faces, _ = self.adjacency_host(P, H)
+        q = np.repeat(np.arange(len(P)), np.diff(off))
+        acc = np.zeros((len(P), 6))
+        if len(faces):
+            np.add.at(acc, q, self.face_hessian_host(kernel, faces, P[q], H[q], form))
+        acc /= np.pi
+        return acc / (H * H)[:, None] if physical else acc
 
     def evaluate_host(self, pos, sup, kernel="w2", physical=True, form="fp64"):
         """the device arithmetic run on the host: [N, 22]."""
@@ -278,6 +324,9 @@ def baseline(dimension):
 
 
 _COLUMN = {Request(a, d): k for k, (a, d) in enumerate(REQUESTS22)}
+# the Hessian requests of baseline(3), columns 22..27 of a 28-column evaluation
+_HESSIAN = tuple((i, j) for i in range(3) for j in range(i, 3))
+_COLUMN.update({Request((0, 0, 0), d): NCH3 + k for k, d in enumerate(_HESSIAN)})
 
 
 def _tet_faces(V):
@@ -323,10 +372,11 @@ class Evaluator:
 
     def __init__(self, geometry: Geometry, requests: Sequence[Request] = None, *, kernel="w2", precision="float64",
                  device="cuda:0", compact=True):
-        requests = tuple(baseline(3)[:22] if requests is None else requests)
+        requests = tuple(baseline(3) if requests is None else requests)
         for r in requests:
             if r not in _COLUMN:
-                raise NotImplementedError(f"request {r} is outside the stored moments baseline(3)[:22]")
+                raise NotImplementedError(f"request {r} is outside baseline(3) (stored moments, first gradient "
+                                          "moments, the Hessian at alpha = 0)")
         if precision not in _PRECISION:
             raise NotImplementedError(f"precision {precision!r}: expected float64 or mixed")
         self.geometry, self.requests = geometry, requests
@@ -334,6 +384,7 @@ class Evaluator:
         self.form = _PRECISION[precision]
         self.device = torch.device(device)
         self.columns = [_COLUMN[r] for r in requests]
+        self.hessian = any(c >= NCH3 for c in self.columns)
         self._meshes = {}
 
     def _mesh(self, part, smax):
@@ -346,14 +397,16 @@ class Evaluator:
 
     def evaluate_device(self, queries: torch.Tensor, supports: torch.Tensor) -> torch.Tensor:
         n = queries.shape[0]
-        total = torch.zeros((n, NCH3), dtype=torch.float64, device=queries.device)
+        total = torch.zeros((n, NCH3 + (6 if self.hessian else 0)), dtype=torch.float64, device=queries.device)
         if n == 0:
             return total
         smax = float(supports.max())
         for part in self.geometry.parts:
             m = self._mesh(part, smax)
             adj = m.adjacency(queries, supports)
-            total += m.moments(adj, queries, supports, self.kernel, self.form, physical=True)
+            total[:, :NCH3] += m.moments(adj, queries, supports, self.kernel, self.form, physical=True)
+            if self.hessian:
+                total[:, NCH3:] += m.hessian(adj, queries, supports, self.kernel, self.form, physical=True)
         return total
 
     def evaluate(self, queries, supports):
@@ -380,7 +433,7 @@ def _alpha(alpha):
 
 class ClosedSurface:
     """reference mesh3d/reference.py:219-325 signatures on the device: moment / moment_gradient / kernel_gradient for
-    the stored-moment degrees (|alpha| <= 2 and |alpha| <= 1)."""
+    the stored-moment degrees (|alpha| <= 2 and |alpha| <= 1), kernel_hessian and laplacian at alpha = 0."""
 
     def __init__(self, vertices, faces, query=(0, 0, 0), device="cuda:0"):
         self.vertices = np.asarray(vertices, dtype=np.float64).reshape(-1, 3)
@@ -396,9 +449,26 @@ class ClosedSurface:
             m = Mesh3D(self.vertices, self.indices, float(h), device=self.device)
             q = torch.as_tensor(self.query[None], device=m.device)
             H = torch.full((1,), float(h), dtype=torch.float64, device=m.device)
-            v = m.evaluate(q, H, key[0], "fp64", physical=True)[0].cpu().numpy()
+            v = m.evaluate(q, H, key[0], "fp64", physical=True, hessian=True)[0].cpu().numpy()
             self._cache[key] = v
         return v
+
+    def kernel_hessian(self, name="w2", alpha=(0, 0, 0), h=1):
+        """int_V y^alpha Hessian_query W_h (mesh3d/reference.py:296-305) at alpha = 0, the 3 x 3 symmetric matrix."""
+        if any(_alpha(alpha)):
+            raise NotImplementedError("device kernel Hessians cover alpha = 0")
+        row = self._row(name, h)
+        H = [[0.0] * 3 for _ in range(3)]
+        for k, (i, j) in enumerate(_HESSIAN):
+            H[i][j] = H[j][i] = float(row[NCH3 + k])
+        return tuple(tuple(r) for r in H)
+
+    def laplacian(self, name="w2", alpha=(0, 0, 0), h=1):
+        """int_V y^alpha Delta W_h (mesh3d/reference.py:288-294) at alpha = 0: the trace of the kernel Hessian."""
+        if any(_alpha(alpha)):
+            raise NotImplementedError("device Laplacians cover alpha = 0")
+        H = self.kernel_hessian(name, alpha, h)
+        return H[0][0] + H[1][1] + H[2][2]
 
     def moment(self, name="w2", alpha=(0, 0, 0), h=1):
         alpha = _alpha(alpha)
@@ -477,6 +547,12 @@ class TetrahedralMesh:
 
     def moment_gradient(self, name="w2", alpha=(0, 0, 0), h=1):
         return self._boundary.moment_gradient(name, alpha, h)
+
+    def kernel_hessian(self, name="w2", alpha=(0, 0, 0), h=1):
+        return self._boundary.kernel_hessian(name, alpha, h)
+
+    def laplacian(self, name="w2", alpha=(0, 0, 0), h=1):
+        return self._boundary.laplacian(name, alpha, h)
 
 
 # ---------------------------------------------------------------------------------------------- test geometries
