@@ -180,6 +180,32 @@ class Geom2D:
         return d, nrm, edge
 
 
+def _require_buffer(t, name, dtype, numel, dev):
+    """a caller-supplied buffer the kernels write or read ``numel`` elements of: dtype, size, contiguity and device
+    checked on the host, so that a wrong buffer is an error instead of an out-of-bounds device access."""
+    if not torch.is_tensor(t):
+        raise ValueError(f"{name}: expected a torch tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if t.numel() < numel:
+        raise ValueError(f"{name}: {t.numel()} elements, at least {numel} needed")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if t.device != dev:
+        raise ValueError(f"{name}: on {t.device}, the rows are on {dev}")
+
+
+def _rows(lpos, lsup):
+    if lpos.dim() != 2 or lpos.shape[1] != 2:
+        raise ValueError(f"positions: expected [N, 2], got {tuple(lpos.shape)}")
+    n = lpos.shape[0]
+    if lsup.shape != (n,):
+        raise ValueError(f"supports: expected [{n}], got {tuple(lsup.shape)}")
+    if lsup.device != lpos.device:
+        raise ValueError("positions and supports on different devices")
+    return n
+
+
 class Disks2D:
     """static device structures of a set of solid disks for one support_max (the cell list over the centres)."""
 
@@ -211,7 +237,7 @@ class Disks2D:
     def adjacency(self, lpos, lsup, valid=None, slots=None, counts=None):
         """(slots [N*K] int32 (-1 padded), counts [N] int32): the disks within one support plus radius of every row."""
         L.require_device(lpos, lsup)
-        n = lpos.shape[0]
+        n = _rows(lpos, lsup)
         dev = lpos.device
         lpos = lpos.to(torch.float64).contiguous()
         lsup = lsup.to(torch.float64).contiguous()
@@ -219,6 +245,10 @@ class Disks2D:
             slots = torch.empty(n * self.K, dtype=torch.int32, device=dev)
         if counts is None:
             counts = torch.empty(n, dtype=torch.int32, device=dev)
+        _require_buffer(slots, "slots", torch.int32, n * self.K, dev)
+        _require_buffer(counts, "counts", torch.int32, n, dev)
+        if valid is not None and (valid.numel() != n or valid.device != dev):
+            raise ValueError(f"valid: expected [{n}] on {dev}")
         v = None if valid is None else valid.to(torch.uint8).contiguous()
         L.check(L.load().wsb_disk_adjacency2d(self.handle, L.ptr(lpos), L.ptr(lsup), L.ptr(v), n, L.ptr(slots),
                                               L.ptr(counts), L.stream_ptr(dev)))
@@ -227,11 +257,16 @@ class Disks2D:
     def moments(self, kernel, slots, counts, lpos, lsup, rot4=None, flags=0, out=None):
         """[N, 12] twelve channels per row (device order) of the row's disks, see include/wsb.h for the flags."""
         L.require_device(lpos, lsup, slots)
-        n = lpos.shape[0]
+        n = _rows(lpos, lsup)
         dev = lpos.device
         if out is None:
             out = torch.zeros((n, 12), dtype=torch.float64, device=dev) if flags & L.OUT_ACCUMULATE \
                 else torch.empty((n, 12), dtype=torch.float64, device=dev)
+        _require_buffer(slots, "slots", torch.int32, n * self.K, dev)
+        _require_buffer(counts, "counts", torch.int32, n, dev)
+        _require_buffer(out, "out", torch.float64, n * 12, dev)
+        if rot4 is not None:
+            _require_buffer(rot4, "rot4", torch.float64, 4, dev)
         lpos = lpos.to(torch.float64).contiguous()
         lsup = lsup.to(torch.float64).contiguous()
         L.check(L.load().wsb_disk_moments2d(self.handle, KERNEL_ID[kernel], L.ptr(slots), L.ptr(counts), self.K,
