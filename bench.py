@@ -163,36 +163,45 @@ def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, see
             work.append((sc.name, pos, sc.h, sc.kernel, 12 if sc.dim == 2 else 22))
     order = WL.rng(seed + 1).permutation(len(work))
     work = [work[i] for i in order]
-    # warm pass, not timed: the same requests all at once (per-stream buffers of every row bucket, captured graphs,
-    # first kernel loads), then the timed pass with Poisson arrivals
+    arrivals = WL.poisson_arrivals(len(work), qps, seed + 2)
+
+    def arrival_pass():
+        t0 = time.perf_counter()
+        reqs = []
+        i = 0
+        while i < len(work) or srv.pending or any(b is not None for b in srv.busy):
+            now = time.perf_counter() - t0
+            while i < len(work) and arrivals[i] <= now:
+                name, pos, h, k, _ = work[i]
+                r = srv.submit(name, pos, h, k)
+                r.arrival = t0 + arrivals[i]
+                reqs.append(r)
+                i += 1
+            srv.poll()
+        return reqs, time.perf_counter() - t0
+
+    # warm pass, not timed: the same requests all at once (per-stream buffers of every row bucket, first kernel
+    # loads), then once with the same arrival schedule (the batches a server in steady state has captured), then the
+    # timed pass
     for name, pos, h, k, _ in work:
         srv.submit(name, pos, h, k)
     srv.run()
+    arrival_pass()
     torch.cuda.synchronize()
     srv.completed.clear()
     srv.launched_batches = 0
     srv.relaunched_batches = 0
+    srv.captured_batches = 0
+    srv.replayed_batches = 0
     srv.host_seconds = {k: 0.0 for k in srv.host_seconds}
-    arrivals = WL.poisson_arrivals(len(work), qps, seed + 2)
-    t0 = time.perf_counter()
-    reqs = []
-    i = 0
-    while i < len(work) or srv.pending or any(b is not None for b in srv.busy):
-        now = time.perf_counter() - t0
-        while i < len(work) and arrivals[i] <= now:
-            name, pos, h, k, _ = work[i]
-            r = srv.submit(name, pos, h, k)
-            r.arrival = t0 + arrivals[i]
-            reqs.append(r)
-            i += 1
-        srv.poll()
-    wall = time.perf_counter() - t0
+    reqs, wall = arrival_pass()
     rows = sum(r.n for r in reqs)
     width = {name: wd for name, _, _, _, wd in work}
     evals = sum(r.n * width[r.scene] for r in reqs)
     rows3 = sum(r.n for r in reqs if width[r.scene] == 22)
     st = srv.latency_stats(reqs)
     st.update({"form": form, "qps_offered": qps, "requests": len(reqs), "rows": rows, "rows_3d": rows3, "wall_s": wall,
+               "captured_batches": srv.captured_batches, "replayed_batches": srv.replayed_batches,
                "evals_per_s": evals / wall, "batches": srv.launched_batches, "relaunched_3d_batches": srv.relaunched_batches,
                "streams": streams, "graphs": graphs, "cross_scene_batches": srv.batched,
                "device_resident_requests": bool(device_requests), "max_batch_rows": max_batch_rows,
@@ -663,8 +672,13 @@ def main():
         log("served mixed, host requests:", mixed_host)
         mixed_host_sat = serve_measure("mixed", sat)
         log("served mixed, host requests, saturation:", mixed_host_sat)
-        graph_q = serve_measure("mixed", args.qps, graphs=True, steps=2)
+        graph_q = serve_measure("mixed", args.qps, graphs=True, batched=False, steps=2)
         log("served mixed, per-scene captured graphs, host requests:", graph_q)
+        # captured cross-scene batches: the same workloads as the eager headline, one graph launch per batch
+        mixed_sat_g = serve_measure("mixed", sat, device_requests=True, graphs=True)
+        log("served mixed, saturation, captured cross-scene batches:", mixed_sat_g)
+        mixed_q_g = serve_measure("mixed", args.qps, device_requests=True, graphs=True)
+        log("served mixed at", args.qps, "QPS, captured cross-scene batches:", mixed_q_g)
         line["served_latency"] = {
             "p50_ms": mixed_q.get("p50_ms"), "p99_ms": mixed_q.get("p99_ms"), "qps": args.qps,
             "saturation_evals_per_s": {"mixed": mixed_sat["evals_per_s"], "plain_fp64": plain_sat["evals_per_s"],
@@ -674,6 +688,15 @@ def main():
             "fp64_saturation": fp64_sat, "mixed_host_requests_pcie": mixed_host,
             "mixed_host_requests_pcie_saturation": mixed_host_sat, "mixed_per_scene_graphs_host_requests": graph_q,
             "mixed_saturation_first_server_of_process": cold,
+            "mixed_saturation_captured": mixed_sat_g, "mixed_captured": mixed_q_g,
+            "host_launch_seconds": {"eager_saturation": mixed_sat["host_seconds"]["launch"],
+                                    "captured_saturation": mixed_sat_g["host_seconds"]["launch"],
+                                    "eager_at_qps": mixed_q["host_seconds"]["launch"],
+                                    "captured_at_qps": mixed_q_g["host_seconds"]["launch"]},
+            "plain_denominator_note": "the served plain FP64 form is the row-parallel plan interpreter (one thread per "
+                                      "row summing its pairs in slot order; batch2d.hip), and its 3-D scenes run the "
+                                      "FP64 closed form; the per-pair plain path of the headline (plain_pairs_kernel) "
+                                      "is timed in forms.plain_fp64",
             "workload": "256 scenes: 192 2-D star polygons (16-8192 edges) and 64 3-D closed meshes (box around an "
                         "icosphere cavity of 5,120 or 20,480 faces, or a box subdivided into 1,452-76,800 faces), "
                         "1e3-2e5 particles each, ~40 neighbours, w2/w4/w6, one jittered timestep per scene, Poisson "

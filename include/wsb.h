@@ -222,24 +222,32 @@ int wsb_disk_channels_host(int kernel_id, const double* D, const double* R, int 
  * names each row's geometry (index into that table; a negative index skips the row: no slots, indicator 0, its
  * output untouched).  slots[n*Kstride] with Kstride >= the K of every geometry in the
  * batch.  Each row's slots, indicator and moments are bitwise those of wsb_adjacency2d / wsb_moments2d (unsorted,
- * local frame) on its own geometry. */
+ * local frame) on its own geometry.  nrows (device, nullable; also for the cell sort and the plain moments): when
+ * given, only the first min(n, *nrows) rows are served while the launch shapes follow n, so a batch of any size up to
+ * n can be replayed from one captured graph. */
 size_t wsb_geom2d_desc_bytes(void);
 int wsb_geom2d_desc(const wsb_geom2d* g, void* out);
 int wsb_batch2d_adjacency(const void* descs, const int32_t* row_desc, const int32_t* order, const double* lpos,
-                          const double* lsup, int n, int Kstride, int32_t* slots, int32_t* counts, double* ind, void* stream);
-/* rows grouped by (geometry, cell): order[n] for wsb_batch2d_adjacency (device radix sort; the schedule only) */
-size_t wsb_batch2d_sort_scratch_bytes(int n);
-int wsb_batch2d_sort(const void* descs, const int32_t* row_desc, const double* lpos, int n, int ndesc, int32_t* order,
-                     void* scratch, size_t scratch_bytes, void* stream);
+                          const double* lsup, int n, int Kstride, int32_t* slots, int32_t* counts, double* ind,
+                          const int32_t* nrows, void* stream);
+/* rows grouped by (geometry, cell): order[n] for wsb_batch2d_adjacency (the schedule only, never a result), by a
+ * counting sort over the bins of every geometry of the table (wsb_desc2d_cell_bases numbers them, nbins in all; rows
+ * outside their grid or without a geometry share one last bin).  Its counters are cleared by a kernel and its launch
+ * shapes depend on (n, nbins) only: the call can be captured in a HIP graph.  No reference counterpart (the
+ * reference sorts nothing for batches of scenes). */
+int wsb_desc2d_cell_bases(void* descs_host, int ndesc, int32_t* nbins);
+size_t wsb_batch2d_cellsort_scratch_bytes(int n, int nbins);
+int wsb_batch2d_cellsort(const void* descs, const int32_t* row_desc, const double* lpos, int n, int nbins, int32_t* order,
+                         void* scratch, size_t scratch_bytes, const int32_t* nrows, void* stream);
 int wsb_batch2d_moments(const void* descs, const int32_t* row_desc, int kernel_id, int form, const int32_t* slots,
                         const int32_t* counts, int Kstride, const double* ind, const double* lpos, const double* lsup,
-                        int n, int flags, double* out, void* stream);
+                        int n, int flags, double* out, const int32_t* nrows, void* stream);
 
 /* the plain FP64 form of the batched moments: the plan interpreter of the per-scene plain path, one thread per row
  * summing its pairs in slot order (bitwise the per-scene plain rows); the plan's kernel is the batch's kernel */
 int wsb_batch2d_plain_moments(const void* descs, const int32_t* row_desc, const wsb_plan* plan, const int32_t* slots,
                               const int32_t* counts, int Kstride, const double* ind, const double* lpos, const double* lsup,
-                              int n, int flags, double* out, void* stream);
+                              int n, int flags, double* out, const int32_t* nrows, void* stream);
 
 /* rows of device-resident requests into a batch and back: segs is a device array of nseg records of
  * wsb_row_segment_bytes() bytes, each {const double* src; double* dst; int64_t first; int64_t count; int64_t value}

@@ -43,13 +43,53 @@ def test_results_equal_alone_and_in_the_mix():
 
 
 def test_graph_replays_equal_eager_launches():
+    """per-scene captured batches and captured cross-scene batches (host and device requests, with and without the
+    cell sort) equal the eager launches bitwise on every replay"""
     scenes = _scenes(3, seed=22)
     reqs = _requests(scenes, steps=4)
     _, eager = _serve(scenes, reqs, streams=2)
-    srv, graphed = _serve(scenes, reqs, streams=2, graphs=True)
+    srv, graphed = _serve(scenes, reqs, streams=2, graphs=True, batched=False)
     for e, g in zip(eager, graphed):
         assert np.array_equal(e.result, g.result)
     assert any(len(s.graphs) for pool in srv.slots for s in pool.values())
+    # several requests per batch (16,384 rows), and one request per batch (many replays of each captured batch)
+    for sort_threshold in (1 << 16, 1024):
+        for rows in (16384, 1):
+            srv, cross = _serve(scenes, reqs, streams=2, graphs=True, sort_threshold=sort_threshold, max_batch_rows=rows)
+            for e, g in zip(eager, cross):
+                assert g.error is None and np.array_equal(e.result, g.result), g.scene
+            assert srv.captured_batches >= 1
+            if rows == 1:
+                assert srv.replayed_batches >= 4, (srv.captured_batches, srv.replayed_batches)
+    dev_reqs = [(n, torch.tensor(p, device="cuda:0"), h, k) for n, p, h, k in reqs]
+    srv, dev = _serve(scenes, dev_reqs, streams=2, graphs=True, sort_threshold=1024, max_batch_rows=1)
+    for e, g in zip(eager, dev):
+        assert np.array_equal(e.result, g.result.cpu().numpy()), g.scene
+    assert srv.replayed_batches >= 4
+
+
+def test_captured_batches_follow_a_new_scene():
+    """a scene added between batches renews the descriptor table: the captured batches of the old table are retired
+    and the new scene's requests (and the old ones) still equal the eager launches"""
+    scenes = [s for s in _scenes(12, seed=32) if s.dim == 2][:4]
+    assert len(scenes) == 4
+    reqs = _requests(scenes[:3], steps=3)
+    late = _requests(scenes, steps=2)
+    eager_srv = BoundaryServer(streams=2)
+    graph_srv = BoundaryServer(streams=2, graphs=True, sort_threshold=1024)
+    out = []
+    for srv in (eager_srv, graph_srv):
+        for sc in scenes[:3]:
+            srv.add_scene(sc.name, sc.loops, sc.h)
+        first = [srv.submit(n, p, h, k) for n, p, h, k in reqs]
+        srv.run()
+        srv.add_scene(scenes[3].name, scenes[3].loops, scenes[3].h)
+        second = [srv.submit(n, p, h, k) for n, p, h, k in late]
+        srv.run()
+        out.append(first + second)
+    for e, g in zip(*out):
+        assert np.array_equal(e.result, g.result), g.scene
+    assert graph_srv.captured_batches >= 2
 
 
 def test_hbm_admission_queues_and_rejects():

@@ -53,13 +53,14 @@ SIGNATURES = {
     "wsb_geom2d_desc_bytes": (c_size_t, []),
     "wsb_geom2d_desc": (c_int, [c_void_p, c_void_p]),
     "wsb_batch2d_adjacency": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
-                                      c_void_p, c_void_p]),
-    "wsb_batch2d_sort_scratch_bytes": (c_size_t, [c_int]),
-    "wsb_batch2d_sort": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+                                      c_void_p, c_void_p, c_void_p]),
+    "wsb_desc2d_cell_bases": (c_int, [c_void_p, c_int, c_void_p]),
+    "wsb_batch2d_cellsort_scratch_bytes": (c_size_t, [c_int, c_int]),
+    "wsb_batch2d_cellsort": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
     "wsb_batch2d_moments": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
-                                    c_int, c_int, c_void_p, c_void_p]),
+                                    c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "wsb_batch2d_plain_moments": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
-                                          c_int, c_int, c_void_p, c_void_p]),
+                                          c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "wsb_row_segment_bytes": (c_size_t, []),
     "wsb_gather_rows": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "wsb_scatter_rows": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),

@@ -17,7 +17,8 @@
 // device copy of one geometry's grid parameters and array pointers
 struct GeoDesc2 {
     double clo0, clo1, cell, y0, dy, bg;
-    int32_t nx, ny, nb, K, ne, pad;
+    int32_t nx, ny, nb, K, ne;
+    int32_t cbase;              // the first bin of this geometry's cells in a batch cell sort (wsb_desc2d_cell_bases)
     const int32_t* cstart;
     const int32_t* citems;
     const double4* cxy;
@@ -47,13 +48,14 @@ struct BatchAdjArgs {
     int32_t* slots;
     int32_t* counts;
     double* ind;
+    const int32_t* nrows;       // nullable (device): only the first min(n, *nrows) rows are served
 };
 
 // one thread per row, the fixed-capacity slot fill of the row's own geometry (reference scene/fixedadj.py:67-147)
 __global__ __launch_bounds__(256) void batch_adjacency2d_kernel(BatchAdjArgs a) {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
+    if (t >= (a.nrows ? min(a.n, *a.nrows) : a.n)) return;
     const int r = a.order ? a.order[t] : t;
     const int di = a.row_desc[r];
     if (di < 0) {                      // a row without a geometry in this pass
@@ -118,6 +120,7 @@ struct BatchTileArgs {
     int n;
     int flags;
     double* out;
+    const int32_t* nrows;       // nullable (device): only the first min(n, *nrows) rows are served
 };
 
 // the tile path of the per-scene moments with the row's geometry taken from the descriptor table
@@ -131,7 +134,9 @@ __global__ __launch_bounds__(TILEB) void batch_tile12_kernel(BatchTileArgs a) {
     __shared__ int s_desc[TILEB];
     const int tid = threadIdx.x;
     const int row = blockIdx.x * TILEB + tid;
-    const bool live = row < a.n;
+    const int n = a.nrows ? min(a.n, *a.nrows) : a.n;
+    if ((int64_t)blockIdx.x * TILEB >= n) return;          // a whole workgroup past the rows
+    const bool live = row < n;
     const int di = live ? a.row_desc[row] : -1;
     const int cnt = di >= 0 ? min(max(a.counts[row], 0), a.Kstride) : 0;
     double h = 1.0;
@@ -200,28 +205,6 @@ int dispatch_batch_tile(int kid, const BatchTileArgs& a, hipStream_t s) {
     return WSB_OK;
 }
 
-// ------------------------------------------------------------------------------------------------ row order
-// key = geometry index << 32 | cell (rows outside their grid: cell 0xffffffff; rows without a geometry: all ones),
-// radix-sorted with the row index as value: rows of one geometry and cell become adjacent (the thread schedule of the
-// slot kernel only, never a result)
-__global__ __launch_bounds__(256) void batch_keys_kernel(const GeoDesc2* descs, const int32_t* row_desc, const double* lpos, int n,
-                                                         int ndesc, uint64_t* key, int32_t* val) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int di = row_desc[r];
-    uint64_t k = ((uint64_t)(uint32_t)ndesc << 32) | 0xffffffffu;
-    if (di >= 0) {
-        const GeoDesc2& d = descs[di];
-        const double fx = floor((lpos[2 * r] - d.clo0) / d.cell);
-        const double fy = floor((lpos[2 * r + 1] - d.clo1) / d.cell);
-        const bool inside = fx >= 0.0 && fx < (double)d.nx && fy >= 0.0 && fy < (double)d.ny;
-        const uint32_t cid = inside ? (uint32_t)((int)fx * d.ny + (int)fy) : 0xffffffffu;
-        k = ((uint64_t)(uint32_t)di << 32) | cid;
-    }
-    key[r] = k;
-    val[r] = r;
-}
-
 // ------------------------------------------------------------------------------------------------ plain FP64 path
 // the plan interpreter of the per-scene plain path, one thread per row: the row's pairs are evaluated and summed in
 // slot order (the order of the per-scene row sums), so each row is bitwise the per-scene plain result; no pair
@@ -240,11 +223,12 @@ struct BatchPlainArgs {
     double plane[12];
     int flags;
     double* out;
+    const int32_t* nrows;
 };
 
 __global__ __launch_bounds__(256) void batch_plain_rows_kernel(BatchPlainArgs a) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.n) return;
+    if (r >= (a.nrows ? min(a.n, *a.nrows) : a.n)) return;
     const int di = a.row_desc[r];
     if (di < 0) return;
     const GeoDesc2& d = a.descs[di];
@@ -321,13 +305,15 @@ extern "C" size_t wsb_geom2d_desc_bytes(void) { return sizeof(GeoDesc2); }
 
 extern "C" int wsb_batch2d_plain_moments(const void* descs, const int32_t* row_desc, const wsb_plan* plan, const int32_t* slots,
                                          const int32_t* counts, int Kstride, const double* ind, const double* lpos,
-                                         const double* lsup, int n, int flags, double* out, void* stream) {
+                                         const double* lsup, int n, int flags, double* out, const int32_t* nrows,
+                                         void* stream) {
     WSB_REQUIRE(n >= 0 && Kstride >= 1, WSB_EINVAL, "bad batch shape (n=%d, Kstride=%d)", n, Kstride);
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(descs && row_desc && plan && slots && counts && ind && lpos && lsup && out, WSB_EINVAL, "null buffer");
     WSB_REQUIRE(!(flags & WSB_OUT_WORLD), WSB_ENOTSUP, "batched moments are evaluated in the local frame");
     BatchPlainArgs a{plain::Plan{plan->radii, plan->cn, plan->cc, plan->E, plan->V, plan->mR, plan->nE, plan->nV},
-                     static_cast<const GeoDesc2*>(descs), row_desc, slots, counts, Kstride, ind, lpos, lsup, n, {0}, flags, out};
+                     static_cast<const GeoDesc2*>(descs), row_desc, slots, counts, Kstride, ind, lpos, lsup, n, {0}, flags, out,
+                     nrows};
     const int kid = plan->kernel_id;
     const double* pl = kid == WSB_CUBIC ? KTab<WSB_CUBIC>::PLANE
                      : kid == WSB_W2    ? KTab<WSB_W2>::PLANE
@@ -393,61 +379,196 @@ extern "C" int wsb_geom2d_desc(const wsb_geom2d* g, void* out) {
 
 namespace {
 inline size_t al256b(size_t x) { return (x + 255) & ~(size_t)255; }
-
-struct BSortLayout {
-    size_t kin, kout, vin, vout, tmp, tmp_bytes, total;
-};
-BSortLayout bsort_layout(int n) {
-    BSortLayout L{};
-    size_t tmp = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint64_t*)nullptr, (uint64_t*)nullptr, (const int32_t*)nullptr,
-                                             (int32_t*)nullptr, n > 0 ? n : 1, 0, 64);
-    L.kin = 0;
-    L.kout = al256b(L.kin + 8 * (size_t)n);
-    L.vin = al256b(L.kout + 8 * (size_t)n);
-    L.vout = al256b(L.vin + 4 * (size_t)n);
-    L.tmp = al256b(L.vout + 4 * (size_t)n);
-    L.tmp_bytes = tmp;
-    L.total = al256b(L.tmp + tmp);
-    return L;
-}
 }  // namespace
 
-extern "C" size_t wsb_batch2d_sort_scratch_bytes(int n) {
-    if (n <= 0) return 0;
-    return bsort_layout(n).total;
+// ------------------------------------------------------------------------------------------------ counting sort
+// The rows of a batch grouped by (geometry, cell) with a counting sort over the bins of every geometry's grid
+// (bin = cbase + cell of the row's own geometry; rows outside their grid or without a geometry share the last bin):
+// a kernel clears the counters, one pass counts, a three-level exclusive scan (1,024 bins per workgroup, one
+// workgroup over the workgroup totals, one pass adding them back) gives each bin's first position, a last pass places
+// every row at its bin's next position.  Fixed launch shapes for a given (n, bins), no memset and no library call:
+// the sequence can be captured in a graph.  The order of the rows inside one bin follows the atomics' timing; it only
+// decides which thread serves a row, never a result.
+namespace {
+
+constexpr int CS_T = 256;
+constexpr int CS_PER = 4;                   // bins per thread of a scan workgroup
+constexpr int CS_CHUNK = CS_T * CS_PER;     // bins per scan workgroup
+
+__global__ __launch_bounds__(CS_T) void cs_clear_kernel(int32_t* hist, int64_t nb) {
+    for (int64_t i = blockIdx.x * (int64_t)CS_T + threadIdx.x; i < nb; i += (int64_t)gridDim.x * CS_T) hist[i] = 0;
 }
 
-extern "C" int wsb_batch2d_sort(const void* descs, const int32_t* row_desc, const double* lpos, int n, int ndesc, int32_t* order,
-                                void* scratch, size_t scratch_bytes, void* stream) {
-    WSB_REQUIRE(n >= 0 && ndesc >= 1, WSB_EINVAL, "bad batch shape (n=%d, ndesc=%d)", n, ndesc);
+__global__ __launch_bounds__(CS_T) void cs_count_kernel(const GeoDesc2* descs, const int32_t* row_desc, const double* lpos,
+                                                        int n, const int32_t* nrows, int32_t nbins, int32_t* key,
+                                                        int32_t* hist) {
+    const int r = blockIdx.x * CS_T + threadIdx.x;
+    if (nrows) n = min(n, *nrows);
+    if (r >= n) return;
+    const int di = row_desc[r];
+    int32_t k = nbins;
+    if (di >= 0) {
+        const GeoDesc2& d = descs[di];
+        const double fx = floor((lpos[2 * r] - d.clo0) / d.cell);
+        const double fy = floor((lpos[2 * r + 1] - d.clo1) / d.cell);
+        if (fx >= 0.0 && fx < (double)d.nx && fy >= 0.0 && fy < (double)d.ny) k = d.cbase + (int)fx * d.ny + (int)fy;
+    }
+    key[r] = k;
+    // the last bin (rows outside their grid, often most of a batch) is counted once per wavefront: one atomic
+    // per 64 rows instead of one contended atomic per row
+    const uint64_t last = __ballot(k == nbins);
+    if (k != nbins) atomicAdd(&hist[k], 1);
+    else if (__lane_id() == (unsigned)(__ffsll((unsigned long long)last) - 1)) atomicAdd(&hist[nbins], __popcll(last));
+}
+
+// exclusive scan of CS_CHUNK bins per workgroup in place; the workgroup's total to tot[blockIdx.x]
+__global__ __launch_bounds__(CS_T) void cs_scan_local_kernel(int32_t* hist, int64_t nb, int32_t* tot) {
+    __shared__ int32_t s[CS_T];
+    const int64_t b0 = blockIdx.x * (int64_t)CS_CHUNK + threadIdx.x * CS_PER;
+    int32_t v[CS_PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < CS_PER; ++i) {
+        v[i] = b0 + i < nb ? hist[b0 + i] : 0;
+        sum += v[i];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < CS_T; o <<= 1) {               // inclusive Hillis-Steele over the thread sums
+        const int32_t add = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int32_t run = s[threadIdx.x] - sum;
+#pragma unroll
+    for (int i = 0; i < CS_PER; ++i) {
+        if (b0 + i < nb) hist[b0 + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == CS_T - 1) tot[blockIdx.x] = s[CS_T - 1];
+}
+
+// one workgroup: exclusive scan of the workgroup totals, in chunks of CS_T with a running carry
+__global__ __launch_bounds__(CS_T) void cs_scan_top_kernel(int32_t* tot, int nblk) {
+    __shared__ int32_t s[CS_T];
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nblk; c0 += CS_T) {
+        const int i = c0 + threadIdx.x;
+        const int32_t v = i < nblk ? tot[i] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < CS_T; o <<= 1) {
+            const int32_t add = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < nblk) tot[i] = carry + s[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += s[CS_T - 1];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(CS_T) void cs_scan_add_kernel(int32_t* hist, int64_t nb, const int32_t* tot) {
+    const int64_t i = blockIdx.x * (int64_t)CS_T + threadIdx.x;
+    if (i < nb) hist[i] += tot[i / CS_CHUNK];
+}
+
+__global__ __launch_bounds__(CS_T) void cs_place_kernel(const int32_t* key, int n, const int32_t* nrows, int32_t nbins,
+                                                        int32_t* start, int32_t* order) {
+    const int r = blockIdx.x * CS_T + threadIdx.x;
+    if (nrows) n = min(n, *nrows);
+    const int k = r < n ? key[r] : -1;
+    // the last bin's rows of a wavefront take one block of positions (one atomic) and their rank inside it
+    const uint64_t last = __ballot(k == nbins);
+    int first = 0;
+    const int leader = last ? __ffsll((unsigned long long)last) - 1 : 0;
+    if (last && (int)__lane_id() == leader) first = atomicAdd(&start[nbins], __popcll(last));
+    first = __shfl(first, leader);
+    if (r >= n) return;
+    if (k != nbins) {
+        order[atomicAdd(&start[k], 1)] = r;
+    } else {
+        const uint64_t below = last & ((1ull << __lane_id()) - 1ull);
+        order[first + __popcll(below)] = r;
+    }
+}
+
+struct CSLayout {
+    size_t key, hist, tot, total;
+    int64_t nb;
+    int nblk;
+};
+CSLayout cs_layout(int n, int32_t nbins) {
+    CSLayout L{};
+    L.nb = (int64_t)nbins + 1;
+    L.nblk = (int)((L.nb + CS_CHUNK - 1) / CS_CHUNK);
+    L.key = 0;
+    L.hist = al256b(4 * (size_t)n);
+    L.tot = al256b(L.hist + 4 * (size_t)L.nb);
+    L.total = al256b(L.tot + 4 * (size_t)L.nblk);
+    return L;
+}
+
+}  // namespace
+
+extern "C" int wsb_desc2d_cell_bases(void* table, int ndesc, int32_t* nbins) {
+    WSB_REQUIRE(table && nbins && ndesc >= 0, WSB_EINVAL, "null argument");
+    int64_t base = 0;
+    for (int i = 0; i < ndesc; ++i) {
+        GeoDesc2 d;
+        char* p = static_cast<char*>(table) + (size_t)i * sizeof(GeoDesc2);
+        std::memcpy(&d, p, sizeof d);
+        WSB_REQUIRE(base + (int64_t)d.nx * d.ny < ((int64_t)1 << 30), WSB_EINVAL, "the geometries' cells exceed 2^30 bins");
+        d.cbase = (int32_t)base;
+        base += (int64_t)d.nx * d.ny;
+        std::memcpy(p, &d, sizeof d);
+    }
+    *nbins = (int32_t)base;
+    return WSB_OK;
+}
+
+extern "C" size_t wsb_batch2d_cellsort_scratch_bytes(int n, int nbins) {
+    if (n <= 0 || nbins < 0) return 0;
+    return cs_layout(n, nbins).total;
+}
+
+extern "C" int wsb_batch2d_cellsort(const void* descs, const int32_t* row_desc, const double* lpos, int n, int nbins,
+                                    int32_t* order, void* scratch, size_t scratch_bytes, const int32_t* nrows, void* stream) {
+    WSB_REQUIRE(n >= 0 && nbins >= 0, WSB_EINVAL, "bad batch shape (n=%d, nbins=%d)", n, nbins);
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(descs && row_desc && lpos && order, WSB_EINVAL, "null buffer");
-    const BSortLayout L = bsort_layout(n);
-    WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", L.total, scratch_bytes);
+    const CSLayout L = cs_layout(n, nbins);
+    WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "cell sort needs %zu scratch bytes (got %zu)", L.total,
+                scratch_bytes);
     char* base = static_cast<char*>(scratch);
-    uint64_t* kin = reinterpret_cast<uint64_t*>(base + L.kin);
-    uint64_t* kout = reinterpret_cast<uint64_t*>(base + L.kout);
-    int32_t* vin = reinterpret_cast<int32_t*>(base + L.vin);
+    int32_t* key = reinterpret_cast<int32_t*>(base + L.key);
+    int32_t* hist = reinterpret_cast<int32_t*>(base + L.hist);
+    int32_t* tot = reinterpret_cast<int32_t*>(base + L.tot);
     hipStream_t s = wsb::as_stream(stream);
-    hipLaunchKernelGGL(batch_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, static_cast<const GeoDesc2*>(descs), row_desc,
-                       lpos, n, ndesc, kin, vin);
+    const unsigned cb = (unsigned)std::min<int64_t>((L.nb + CS_T - 1) / CS_T, 4096);
+    hipLaunchKernelGGL(cs_clear_kernel, dim3(cb), dim3(CS_T), 0, s, hist, L.nb);
+    hipLaunchKernelGGL(cs_count_kernel, dim3((n + CS_T - 1) / CS_T), dim3(CS_T), 0, s, static_cast<const GeoDesc2*>(descs),
+                       row_desc, lpos, n, nrows, (int32_t)nbins, key, hist);
+    hipLaunchKernelGGL(cs_scan_local_kernel, dim3(L.nblk), dim3(CS_T), 0, s, hist, L.nb, tot);
+    hipLaunchKernelGGL(cs_scan_top_kernel, dim3(1), dim3(CS_T), 0, s, tot, L.nblk);
+    hipLaunchKernelGGL(cs_scan_add_kernel, dim3((unsigned)((L.nb + CS_T - 1) / CS_T)), dim3(CS_T), 0, s, hist, L.nb, tot);
+    hipLaunchKernelGGL(cs_place_kernel, dim3((n + CS_T - 1) / CS_T), dim3(CS_T), 0, s, key, n, nrows, (int32_t)nbins, hist,
+                       order);
     WSB_HIP(hipGetLastError());
-    // significant key bits: the cell (32) and the geometry index (ndesc for rows without a geometry) above them
-    int gbits = 1;
-    while (gbits < 31 && ((int64_t)1 << gbits) <= ndesc) ++gbits;
-    size_t tmp = L.tmp_bytes;
-    WSB_HIP(hipcub::DeviceRadixSort::SortPairs(base + L.tmp, tmp, kin, kout, vin, order, n, 0, 32 + gbits, s));
     return WSB_OK;
 }
 
 extern "C" int wsb_batch2d_adjacency(const void* descs, const int32_t* row_desc, const int32_t* order, const double* lpos,
                                      const double* lsup, int n, int Kstride, int32_t* slots, int32_t* counts, double* ind,
-                                     void* stream) {
+                                     const int32_t* nrows, void* stream) {
     WSB_REQUIRE(n >= 0 && Kstride >= 1, WSB_EINVAL, "bad batch shape (n=%d, Kstride=%d)", n, Kstride);
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(descs && row_desc && lpos && lsup && slots && counts && ind, WSB_EINVAL, "null buffer");
-    BatchAdjArgs a{static_cast<const GeoDesc2*>(descs), row_desc, order, lpos, lsup, n, Kstride, slots, counts, ind};
+    BatchAdjArgs a{static_cast<const GeoDesc2*>(descs), row_desc, order, lpos, lsup, n, Kstride, slots, counts, ind, nrows};
     hipLaunchKernelGGL(batch_adjacency2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
@@ -455,12 +576,12 @@ extern "C" int wsb_batch2d_adjacency(const void* descs, const int32_t* row_desc,
 
 extern "C" int wsb_batch2d_moments(const void* descs, const int32_t* row_desc, int kernel_id, int form, const int32_t* slots,
                                    const int32_t* counts, int Kstride, const double* ind, const double* lpos,
-                                   const double* lsup, int n, int flags, double* out, void* stream) {
+                                   const double* lsup, int n, int flags, double* out, const int32_t* nrows, void* stream) {
     WSB_REQUIRE(n >= 0 && Kstride >= 1, WSB_EINVAL, "bad batch shape (n=%d, Kstride=%d)", n, Kstride);
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(descs && row_desc && slots && counts && ind && lpos && lsup && out, WSB_EINVAL, "null buffer");
     WSB_REQUIRE(!(flags & WSB_OUT_WORLD), WSB_ENOTSUP, "batched moments are evaluated in the local frame");
-    BatchTileArgs a{static_cast<const GeoDesc2*>(descs), row_desc, slots, counts, Kstride, ind, lpos, lsup, n, flags, out};
+    BatchTileArgs a{static_cast<const GeoDesc2*>(descs), row_desc, slots, counts, Kstride, ind, lpos, lsup, n, flags, out, nrows};
     hipStream_t s = wsb::as_stream(stream);
     if (form == WSB_FORM_FP64) return dispatch_batch_tile<closed::FP64>(kernel_id, a, s);
     if (form == WSB_FORM_MIXED) return dispatch_batch_tile<closed::MIXED>(kernel_id, a, s);

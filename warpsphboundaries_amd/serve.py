@@ -26,9 +26,14 @@ counterpart: the reference evaluates one position set per call).
   previous pair counts; the count is read back with the batch's results, and a batch whose count exceeded the list is
   launched again with the exact size before its requests complete (no host synchronisation on the launch path).  The
   3-D path has the fp64 and mixed forms; a server of the plain form runs its 3-D scenes in fp64.
-* Captured launches: with ``graphs=True`` (per-scene batches) the adjacency + moment launches of a (scene, kernel,
-  stream, row bucket) are captured once in a HIP graph over fixed buffers and replayed (rows beyond the request count
-  are parked outside the geometry's grid, where they cost one cell lookup).
+* Captured launches (``graphs=True``): a cross-scene batch's whole sequence (segment-table copy, row gathers, per
+  representation index the row-geometry fill, the counting cell sort, adjacency and moments, the result scatter) is
+  captured once per (stream, row bucket, kernel, representation count, request kind, sort) over fixed buffers and
+  replayed: the segment tables have a fixed width (``GRAPH_SEGMENTS`` requests, unused segments empty) and the batch's
+  row count travels in a header in front of them, which the kernels read (launch shapes follow the bucket, the rows
+  beyond the batch are never touched), so a batch costs the host one table write and one graph launch.  A new scene
+  changes the descriptor table and retires the captured graphs.
+  Per-scene batches (``batched=False``) capture their adjacency + moment launches per (scene, kernel, stream, bucket).
 * Latency = arrival (``submit``) to retirement (the poll that sees the batch's event complete), host clock.
 Results are bitwise independent of batching: every row's channels are a fixed-order sum over its own slots.
 """
@@ -49,6 +54,8 @@ from .scene import SurfaceRep, kernelName
 
 F64 = torch.float64
 PARK = 1.0e30          # coordinate of padded rows: outside every grid
+GRAPH_SEGMENTS = 256   # requests per captured cross-scene batch
+_HEADER = 256          # bytes before a batch's segment tables: its row count (int32), read by the counted launches
 
 
 @dataclass
@@ -144,6 +151,7 @@ class _BSlot:
     host_seg: Optional[torch.Tensor] = None      # pinned segment tables (rewritten only after the slot's batch retired)
     dev_seg: Optional[torch.Tensor] = None
     ready: Optional[torch.cuda.Event] = None
+    graphs: dict = field(default_factory=dict)   # captured batches by (kernel, nrep, on_dev, sort, table version)
 
 
 @dataclass
@@ -208,12 +216,14 @@ class BoundaryServer:
         self.max_batch_rows = int(max_batch_rows)
         self.sort_threshold = int(sort_threshold)
         self.use_graphs = bool(graphs)
-        self.batched = (not graphs) if batched is None else bool(batched)
-        if self.batched and graphs:
-            raise NotImplementedError("cross-scene batches are launched eagerly (graph capture is per scene)")
+        self.batched = True if batched is None else bool(batched)
         self._desc_host = bytearray()
         self._desc_dev = None
         self._desc_bytes = int(L.load().wsb_geom2d_desc_bytes())
+        self._nbins = 0                     # bins of the batch cell sort over every geometry of the table
+        self._table_version = 0             # bumped by every change of the descriptor table
+        self.captured_batches = 0
+        self.replayed_batches = 0
         self._Kmax = 1
         self.bslots: List[Dict[int, _BSlot]] = []
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(int(streams))]
@@ -255,7 +265,16 @@ class BoundaryServer:
             L.check(lib.wsb_geom2d_desc(g.handle, buf))
             sc.desc.append(len(self._desc_host) // self._desc_bytes)
             self._desc_host += buf.raw
+        table = (ctypes.c_char * len(self._desc_host)).from_buffer(self._desc_host)
+        nbins = ctypes.c_int32()
+        L.check(lib.wsb_desc2d_cell_bases(table, len(self._desc_host) // self._desc_bytes, ctypes.byref(nbins)))
+        del table
+        self._nbins = int(nbins.value)
         self._desc_dev = torch.frombuffer(bytearray(self._desc_host), dtype=torch.uint8).to(self.device)
+        self._table_version += 1
+        for pool in self.bslots:             # captured batches hold the previous table's address
+            for bs in pool.values():
+                bs.graphs.clear()
         self._Kmax = max(self._Kmax, sc.K)
         self.scenes[name] = sc
         return sc
@@ -400,6 +419,8 @@ class BoundaryServer:
                 cross = self.batched and dim == 2 and not sc.conditioned and not rs.conditioned
                 same = (r.kernel == head.kernel and r.on_device == head.on_device and rs.dim == dim
                         and (cross or r.scene == head.scene))
+                if same and len(group) >= self._group_limit(dim):
+                    break
                 if same and (not group or rows + r.n <= self.max_batch_rows):
                     group.append(r)
                     rows += r.n
@@ -419,8 +440,13 @@ class BoundaryServer:
                 # stream affinity: a free stream whose buffers already hold this batch's captured graph
                 cap = _bucket(rows)
                 for i in free:
-                    sl = self.slots[i].get(cap)
-                    if sl is not None and (sc.name, head.kernel) in sl.graphs:
+                    if dim == 2 and self.batched and not sc.conditioned:
+                        sl = self.bslots[i].get(cap)
+                        hit = sl is not None and any(k[0] == head.kernel and k[-1] == self._table_version for k in sl.graphs)
+                    else:
+                        sl = self.slots[i].get(cap)
+                        hit = sl is not None and (sc.name, head.kernel) in sl.graphs
+                    if hit:
                         si = i
                         break
             t1 = time.perf_counter()
@@ -433,6 +459,9 @@ class BoundaryServer:
             self.host_seconds["launch"] += time.perf_counter() - t1
         self.completed.extend(now_done)
         return now_done
+
+    def _group_limit(self, dim):
+        return GRAPH_SEGMENTS if (self.use_graphs and self.batched and dim == 2) else 1 << 30
 
     def run(self, until_idle=True, sleep=0.0):
         while self.pending or any(b is not None for b in self.busy):
@@ -538,16 +567,10 @@ class BoundaryServer:
             pool[cap] = _mark_ready(s, dev)
         return s
 
-    def _launch_batched(self, si: int, kernel: str, group: List[Request], rows: int, fp: int):
-        """one cross-scene batch: rows in (pinned copies for host requests, a gather kernel for device requests), per
-        surface-representation index one fill of the row geometries, one adjacency and one moment launch, rows out."""
-        s = self._bslot(si, rows)
-        st = self.streams[si]
-        nrep = max(len(self.scenes[r.scene].geoms) for r in group)
-        on_dev = group[0].on_device
-        ng = len(group)
-        # segment tables: [pos gather | sup gather | out scatter | row geometry of pass 0 .. nrep-1]
-        seg = np.zeros((3 + nrep, ng), dtype=_SEG)
+    def _batch_tables(self, s: _BSlot, group: List[Request], nrep: int, on_dev: bool, width: int):
+        """the segment tables of one batch: [pos gather | sup gather | out scatter | row geometry of pass 0 .. nrep-1],
+        ``width`` segments each (the segments beyond the group are empty)."""
+        seg = np.zeros((3 + nrep, width), dtype=_SEG)
         off = 0
         hp = hs = None
         if not on_dev:
@@ -566,63 +589,111 @@ class BoundaryServer:
                 hp[off:off + r.n] = r.positions
                 hs[off:off + r.n] = r.supports
             off += r.n
+        return seg
+
+    def _batch_sequence(self, s: _BSlot, kernel: str, nrep: int, on_dev: bool, n: int, width: int, sort: bool, nbytes: int,
+                        counted: bool):
+        """the launches of one cross-scene batch over the slot's buffers, on the current stream: launch shapes for
+        ``n`` rows and ``width`` segments per table; ``counted``: the kernels serve the row count the batch header
+        (the first bytes of the copied tables) carries, so a captured sequence serves any batch up to n rows."""
         lib = L.load()
         from .kernels import KERNEL_ID
         from .device import surface_device_plan
         kid, fid = KERNEL_ID[kernel], L.form_id(self.form)
-        if fid == 0:
-            plan = surface_device_plan(kernel, self.device.index or 0).handle
+        plan = surface_device_plan(kernel, self.device.index or 0).handle if fid == 0 else None
+        sb = _SEG.itemsize
+        s.dev_seg[:nbytes].copy_(s.host_seg[:nbytes], non_blocking=True)
+        sp = L.stream_ptr(self.device)
+        base = s.dev_seg.data_ptr()
+        nrows = ctypes.c_void_p(base) if counted else None
+        base += _HEADER
+        tab = [ctypes.c_void_p(base + t * width * sb) for t in range(3 + nrep)]
+        if on_dev:
+            L.check(lib.wsb_gather_rows(tab[0], width, 2, L.ptr(s.lpos), sp))
+            L.check(lib.wsb_gather_rows(tab[1], width, 1, L.ptr(s.lsup), sp))
+        else:
+            s.lpos[:n].copy_(s.host_pos[:n], non_blocking=True)
+            s.lsup[:n].copy_(s.host_sup[:n], non_blocking=True)
+        for j in range(nrep):
+            L.check(lib.wsb_fill_rows_i32(tab[3 + j], width, L.ptr(s.rdesc), sp))
+            order = None
+            if sort:
+                L.check(lib.wsb_batch2d_cellsort(L.ptr(self._desc_dev), L.ptr(s.rdesc), L.ptr(s.lpos), n, self._nbins,
+                                                 L.ptr(s.order), L.ptr(s.sort_scratch), s.sort_scratch.numel(), nrows, sp))
+                order = s.order
+            L.check(lib.wsb_batch2d_adjacency(L.ptr(self._desc_dev), L.ptr(s.rdesc), L.ptr(order), L.ptr(s.lpos), L.ptr(s.lsup),
+                                              n, s.Kstride, L.ptr(s.slots), L.ptr(s.counts), L.ptr(s.ind), nrows, sp))
+            flags = L.OUT_PHYSICAL | (L.OUT_ACCUMULATE if j else 0)
+            if fid == 0:
+                L.check(lib.wsb_batch2d_plain_moments(L.ptr(self._desc_dev), L.ptr(s.rdesc), plan, L.ptr(s.slots),
+                                                      L.ptr(s.counts), s.Kstride, L.ptr(s.ind), L.ptr(s.lpos),
+                                                      L.ptr(s.lsup), n, flags, L.ptr(s.out), nrows, sp))
+            else:
+                L.check(lib.wsb_batch2d_moments(L.ptr(self._desc_dev), L.ptr(s.rdesc), kid, fid, L.ptr(s.slots),
+                                                L.ptr(s.counts), s.Kstride, L.ptr(s.ind), L.ptr(s.lpos), L.ptr(s.lsup), n,
+                                                flags, L.ptr(s.out), nrows, sp))
+        if on_dev:
+            L.check(lib.wsb_scatter_rows(tab[2], width, 12, L.ptr(s.out), sp))
+        else:
+            s.host_out[:n].copy_(s.out[:n], non_blocking=True)
+
+    def _launch_batched(self, si: int, kernel: str, group: List[Request], rows: int, fp: int):
+        """one cross-scene batch: rows in (pinned copies for host requests, a gather kernel for device requests), per
+        surface-representation index one fill of the row geometries, one cell sort (large batches), one adjacency and
+        one moment launch, rows out; captured once and replayed with ``graphs=True``."""
+        s = self._bslot(si, rows)
+        st = self.streams[si]
+        nrep = max(len(self.scenes[r.scene].geoms) for r in group)
+        on_dev = group[0].on_device
+        captured = self.use_graphs
+        width = GRAPH_SEGMENTS if captured else len(group)
+        n = s.cap if captured else rows
+        sort = (s.cap if captured else rows) >= self.sort_threshold
+        seg = self._batch_tables(s, group, nrep, on_dev, width)
         raw = seg.view(np.uint8).reshape(-1)
-        if s.host_seg is None or s.host_seg.numel() < raw.size:
-            nb = max(raw.size, 1 << 16)
+        need = _HEADER + (3 + nrep) * GRAPH_SEGMENTS * _SEG.itemsize if captured else _HEADER + raw.size
+        if s.host_seg is None or s.host_seg.numel() < need:
+            nb = max(need, 1 << 16)
             s.host_seg = torch.empty(nb, dtype=torch.uint8).pin_memory()
             s.dev_seg = torch.empty(nb, dtype=torch.uint8, device=self.device)
-        s.host_seg.numpy()[:raw.size] = raw
-        sb = seg.dtype.itemsize
+            s.graphs.clear()                 # they copy from the previous buffers
+        hseg = s.host_seg.numpy()
+        hseg[:4] = np.array([rows], dtype=np.int32).view(np.uint8)     # the batch header: its row count
+        hseg[_HEADER:_HEADER + raw.size] = raw
+        nbytes = _HEADER + raw.size
+        if sort:
+            lib = L.load()
+            sbytes = max(int(lib.wsb_batch2d_cellsort_scratch_bytes(s.cap, self._nbins)), 1)
+            if s.order is None or s.sort_scratch is None or s.sort_scratch.numel() < sbytes:
+                s.order = torch.empty(s.cap, dtype=torch.int32, device=self.device)
+                s.sort_scratch = torch.empty(sbytes, dtype=torch.uint8, device=self.device)
+                s.graphs.clear()
         with torch.cuda.stream(st):
             _wait_ready(s, st)
             if on_dev:
                 for r in group:
                     st.wait_event(r.ready)
-            dev_seg = s.dev_seg
-            dev_seg[:raw.size].copy_(s.host_seg[:raw.size], non_blocking=True)
-            sp = L.stream_ptr(self.device)
-            base = dev_seg.data_ptr()
-            tab = [ctypes.c_void_p(base + t * ng * sb) for t in range(3 + nrep)]
-            if on_dev:
-                L.check(lib.wsb_gather_rows(tab[0], ng, 2, L.ptr(s.lpos), sp))
-                L.check(lib.wsb_gather_rows(tab[1], ng, 1, L.ptr(s.lsup), sp))
-            else:
-                s.lpos[:rows].copy_(s.host_pos[:rows], non_blocking=True)
-                s.lsup[:rows].copy_(s.host_sup[:rows], non_blocking=True)
-            ndesc = len(self._desc_host) // self._desc_bytes
-            sort = rows >= self.sort_threshold
-            if sort and (s.order is None or s.order.numel() < s.cap):
-                s.order = torch.empty(s.cap, dtype=torch.int32, device=self.device)
-                s.sort_scratch = torch.empty(max(int(lib.wsb_batch2d_sort_scratch_bytes(s.cap)), 1), dtype=torch.uint8,
-                                             device=self.device)
-            for j in range(nrep):
-                L.check(lib.wsb_fill_rows_i32(tab[3 + j], ng, L.ptr(s.rdesc), sp))
-                order = None
-                if sort:
-                    L.check(lib.wsb_batch2d_sort(L.ptr(self._desc_dev), L.ptr(s.rdesc), L.ptr(s.lpos), rows, ndesc, L.ptr(s.order),
-                                                 L.ptr(s.sort_scratch), s.sort_scratch.numel(), sp))
-                    order = s.order
-                L.check(lib.wsb_batch2d_adjacency(L.ptr(self._desc_dev), L.ptr(s.rdesc), L.ptr(order), L.ptr(s.lpos), L.ptr(s.lsup),
-                                                  rows, s.Kstride, L.ptr(s.slots), L.ptr(s.counts), L.ptr(s.ind), sp))
-                flags = L.OUT_PHYSICAL | (L.OUT_ACCUMULATE if j else 0)
-                if fid == 0:
-                    L.check(lib.wsb_batch2d_plain_moments(L.ptr(self._desc_dev), L.ptr(s.rdesc), plan, L.ptr(s.slots),
-                                                          L.ptr(s.counts), s.Kstride, L.ptr(s.ind), L.ptr(s.lpos),
-                                                          L.ptr(s.lsup), rows, flags, L.ptr(s.out), sp))
+            if captured:
+                key = (kernel, nrep, on_dev, sort, self._table_version)
+                gr = s.graphs.get(key)
+                if gr is None:
+                    # this batch runs eagerly (it also warms every launch up); the same sequence is captured for the next
+                    self._batch_sequence(s, kernel, nrep, on_dev, n, width, sort, nbytes, True)
+                    # captured on this stream alone (no device-wide synchronisation, unlike torch.cuda.graph's entry):
+                    # the sequence allocates nothing and makes no other call while the capture is open
+                    gr = torch.cuda.CUDAGraph()
+                    gr.capture_begin(capture_error_mode="thread_local")
+                    try:
+                        self._batch_sequence(s, kernel, nrep, on_dev, n, width, sort, nbytes, True)
+                    finally:
+                        gr.capture_end()
+                    s.graphs[key] = gr
+                    self.captured_batches += 1
                 else:
-                    L.check(lib.wsb_batch2d_moments(L.ptr(self._desc_dev), L.ptr(s.rdesc), kid, fid, L.ptr(s.slots),
-                                                    L.ptr(s.counts), s.Kstride, L.ptr(s.ind), L.ptr(s.lpos), L.ptr(s.lsup), rows,
-                                                    flags, L.ptr(s.out), sp))
-            if on_dev:
-                L.check(lib.wsb_scatter_rows(tab[2], ng, 12, L.ptr(s.out), sp))
+                    gr.replay()
+                    self.replayed_batches += 1
             else:
-                s.host_out[:rows].copy_(s.out[:rows], non_blocking=True)
+                self._batch_sequence(s, kernel, nrep, on_dev, n, width, sort, nbytes, False)
             ev = torch.cuda.Event()
             ev.record(st)
         self.busy[si] = _Batch(si, None, kernel, group, rows, s, ev, fp)
