@@ -19,6 +19,7 @@ from . import _lib as L
 from . import triangle
 from .device import Disks2D, Geom2D
 from .kernels import KERNEL_ID, plane_channels
+from .periodic import Periodic, min_image
 
 F64 = torch.float64
 
@@ -512,6 +513,29 @@ class DiskArrayRep:
                                        dev.index or 0)
         return self._geoms[key]
 
+    def tiled(self, periodic: Periodic, supportMax):
+        """this bundle with its periodic images (reference scene/scene.py:834-850): every disk at every box shift k,
+        |k| <= 2 per periodic axis, whose image reaches the window [-L/2 - H, L/2 + H] around the body-frame origin
+        (H = ``supportMax``).  A query min-imaged into the box then meets each disk it can see once (support < L / 2).
+        The images are listed shift by shift, (kx, ky) in lexicographic order, each the bundle's disks in order; the
+        body frame must be the world frame (no rotation) for the offsets to be box lengths."""
+        L = periodic.length(self.centres)
+        Lh = L.cpu().tolist()
+        half = 0.5 * L
+        keep_c, keep_r = [], []
+        for kx in range(-2, 3):
+            for ky in range(-2, 3):
+                if (kx and Lh[0] == 0.0) or (ky and Lh[1] == 0.0):
+                    continue
+                c = self.centres + torch.stack([kx * L[0], ky * L[1]])
+                over = c.abs() - half - float(supportMax) - self.radii[:, None]
+                keep = ((over <= 0) | (L == 0)).all(dim=1)
+                keep_c.append(c[keep])
+                keep_r.append(self.radii[keep])
+        out = DiskArrayRep(torch.cat(keep_c), torch.cat(keep_r)).to(self.centres.device)
+        out._tileOf = (len(self.radii), len(out.radii))
+        return out
+
     def signed(self, pos):
         """(d, n): the smallest |x - c| - R over the disks (positive in the fluid; the first disk on a tie) and the
         unit vector from that disk's centre, body frame (reference scene/scene.py:853-867).  The points go in blocks
@@ -577,6 +601,8 @@ class Body:
     linearVelocity: tuple = (0.0, 0.0)
     angularVelocity: float = 0.0
     reps: list = field(default_factory=list)
+    periodic: Optional[Periodic] = None     # set by Scene.setPeriodic: particles are seen at their nearest image
+    periodicSupport: float = 0.0            # the largest kernel support of the particles (the reach of the images)
 
     def __post_init__(self):
         self.center = torch.as_tensor(self.center, dtype=F64)
@@ -586,8 +612,62 @@ class Body:
     def pose(self):
         return Pose(self.center, float(self.angle))
 
+    def image(self, world):
+        """the world points at their nearest periodic image of the body centre, c + min_image(x - c) (the points
+        themselves without a periodic box; the positions are never written)."""
+        if self.periodic is None:
+            return world
+        return self.center + min_image(world - self.center, self.periodic)
+
+    def relative(self, world):
+        """x - c at the nearest image (the lever arms of the loads)."""
+        return min_image(world - self.center, self.periodic)
+
     def toLocal(self, world):
-        return self.pose.toLocal(world)
+        """body-frame coordinates of world points (nearest image of the centre in a periodic box)."""
+        return self.pose.toLocal(self.image(world))
+
+    def tiledRep(self, rep):
+        """``rep`` with its periodic images for the reach ``periodicSupport`` (cached); ``rep`` itself without a box
+        (reference scene/scene.py:998-1006)."""
+        if self.periodic is None:
+            return rep
+        return self._tile(rep, self.periodicSupport)
+
+    def _tile(self, rep, supportMax):
+        """a disk bundle of more than one disk in a periodic box: its tiled images for the reach max(supportMax,
+        periodicSupport); any other representation: checked to fit or span the box and returned as it is
+        (reference scene/scene.py:1008-1018).  Images need a body that does not rotate."""
+        if self.periodic is None:
+            return rep
+        H = max(float(supportMax), float(self.periodicSupport))
+        if not isinstance(rep, DiskArrayRep) or len(rep.radii) < 2:
+            self._checkCompact(rep, H)
+            return rep
+        cache = self.__dict__.setdefault("_tileCache", {})
+        key = (id(rep), H, self.periodic)
+        if key not in cache:
+            t = rep.tiled(self.periodic, H)
+            if len(t.radii) > len(rep.radii) and float(self.angle) != 0.0:
+                raise NotImplementedError("a bundle with periodic images cannot rotate (its image offsets are fixed "
+                                          "in the body frame)")
+            cache[key] = t
+        return cache[key]
+
+    def _checkCompact(self, rep, supportMax):
+        """on every periodic axis the representation fits its cell (extent from the centre + support below L / 2: the
+        nearest image of the centre serves every row) or spans the window [-L/2 - H, L/2 + H] (a wall that is itself
+        periodic, taken as translation invariant by L); anything else is a ValueError (reference
+        scene/scene.py:1020-1031)."""
+        lo, hi = rep.bounds()
+        L = self.periodic.length(lo)
+        H = float(supportMax)
+        ext = torch.maximum(lo.abs(), hi.abs())
+        compact = ext + H < 0.5 * L
+        spanning = (lo <= -0.5 * L - H) & (hi >= 0.5 * L + H)
+        if bool(((L > 0) & ~(compact | spanning)).any()):
+            raise ValueError("a periodic body must fit its cell (extent + support below half the box length) or span "
+                             "it (cover the box plus a support) on every periodic axis")
 
     def unbounded(self):
         """whether the solid of the body reaches infinity (its broadphase keeps every query): a surface loop with
@@ -617,13 +697,24 @@ class Body:
     def fusedReps(self, hmin, device):
         """the representations this body is evaluated with: a SurfaceRep or DiskArrayRep as it is, a BoxRep its
         polygon, a solid ImplicitRep disk a one-disk DiskArrayRep, a cavity its tier-2 polygon for the smallest support
-        ``hmin`` (reference scene/scene.py:970-996)."""
-        return [r if isinstance(r, SurfaceRep) else r.lowered(hmin, device) for r in self.reps]
+        ``hmin`` (reference scene/scene.py:970-996); in a periodic box a bundle's tiled images, the others checked to
+        fit or span the box."""
+        out = []
+        for r in self.reps:
+            low = r if isinstance(r, SurfaceRep) else r.lowered(hmin, device)
+            out.append(self._tile(low, hmin) if self.periodic is not None else low)
+        return out
 
     def obb(self):
         los, his = zip(*[r.bounds() for r in self.reps])
         dev = self.center.device
-        return torch.stack([a.to(dev) for a in los]).amin(0), torch.stack([a.to(dev) for a in his]).amax(0)
+        lo = torch.stack([a.to(dev) for a in los]).amin(0)
+        hi = torch.stack([a.to(dev) for a in his]).amax(0)
+        if self.periodic is not None and any(isinstance(r, DiskArrayRep) and len(r.radii) > 1 for r in self.reps):
+            # a bundle sees its tiled images: every query of the box around the centre is a candidate
+            L = self.periodic.length(lo)
+            lo, hi = torch.where(L > 0, -0.5 * L, lo), torch.where(L > 0, 0.5 * L, hi)
+        return lo, hi
 
     def velocityAt(self, world):
         w = float(self.angularVelocity)
@@ -659,11 +750,22 @@ class Scene:
     def __init__(self, bodies: List[Body], device="cuda:0"):
         self.bodies = bodies
         self.device = device
+        self.periodic = None
         for b in bodies:
             b.center = b.center.to(device)
             b.linearVelocity = b.linearVelocity.to(device)
             for r in b.reps:
                 r.to(device)
+
+    def setPeriodic(self, periodic: Optional[Periodic], support: float = 0.0):
+        """a periodic box for all bodies (reference scene/scene.py:1243-1250): every body sees the particles at the
+        nearest image of its centre; ``support`` is the particles' largest support (the reach of a bundle's images)."""
+        if periodic is not None:
+            periodic.checkSupport(support)
+        self.periodic = periodic
+        for b in self.bodies:
+            b.periodic, b.periodicSupport = periodic, float(support)
+            b.__dict__.pop("_tileCache", None)
 
     def adjacency(self, queryParticles, operationProperties=None, supportMax: float = None):
         """`SceneAdjacency` over the broadphase candidates of every body (reference scene/scene.py:1400-1416): the
@@ -709,7 +811,7 @@ class Scene:
         above 1/2 for a surface loop, a negative signed distance for the other representations (reference
         scene/scene.py:1279-1310)."""
         pts = points.to(self.device, F64)
-        flags = [_rep_inside(rep, bd.toLocal(pts)) for _, bd in self._chosen(body) for rep in bd.reps]
+        flags = [_rep_inside(bd.tiledRep(rep), bd.toLocal(pts)) for _, bd in self._chosen(body) for rep in bd.reps]
         if not flags:
             return torch.zeros(len(pts), dtype=torch.bool, device=pts.device)
         return torch.stack(flags).any(dim=0)
@@ -725,7 +827,7 @@ class Scene:
         for bi, bd in self._chosen(body):
             lp = bd.toLocal(pts)
             for rep in bd.reps:
-                d, n_loc = _rep_signed(rep, lp, supportMax)
+                d, n_loc = _rep_signed(bd.tiledRep(rep), lp, supportMax)
                 dist.append(torch.nan_to_num(d, nan=float("inf")))
                 normals.append(bd.pose.vecToWorld(n_loc))
                 owner.append(torch.full((M,), bi, dtype=torch.long, device=pts.device))
