@@ -209,6 +209,40 @@ def serve_measure(form, qps, n_scenes=256, steps=1, graphs=False, streams=4, see
     return st
 
 
+def search3d_compare(torch, dev, reps=3):
+    """the two 3-D face searches (cell lists, face tree) on the 64 meshes of the served workload: face lists and
+    winding numbers compared, per-scene adjacency launches timed with HIP events and summed."""
+    from warpsphboundaries_amd import workloads as WL
+    from warpsphboundaries_amd.mesh3d import Mesh3D
+    scenes = [s for s in WL.config5(n_scenes=256, seed=5) if s.dim == 3]
+    tot = {"grid": 0.0, "bvh": 0.0}
+    pairs, rows = 0, 0
+    for s in scenes:
+        V, F = s.mesh
+        m = Mesh3D(V, F, s.h, device=dev)
+        P = torch.tensor(s.positions, dtype=torch.float64, device=dev)
+        H = torch.full((len(P),), s.h, dtype=torch.float64, device=dev)
+        o = m.sort(P)
+        a = m.adjacency(P, H, order=o, search="grid")
+        b = m.adjacency(P, H, order=o, search="bvh")
+        if not (a.count == b.count and torch.equal(a.faces[:a.count], b.faces[:b.count]) and torch.equal(a.chi, b.chi)):
+            raise RuntimeError(f"face searches differ on {s.name}")
+        pairs += a.count
+        rows += len(P)
+        for search in ("grid", "bvh"):
+            m.adjacency(P, H, order=o, search=search, capacity=a.faces.numel())
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                m.adjacency(P, H, order=o, search=search, capacity=a.faces.numel())
+            e1.record()
+            torch.cuda.synchronize()
+            tot[search] += e0.elapsed_time(e1) / reps
+        m.close()
+    return {"scenes": len(scenes), "rows": rows, "pairs": pairs, "adjacency_ms_summed": tot,
+            "equal": True}
+
+
 def mesh3d_measure(torch, dev, steps, warmup):
     """the 3-D mesh path on the 81,920-face sphere cavity workload: adjacency rebuild + the 22 stored moments (w4)."""
     from warpsphboundaries_amd import workloads as WL
@@ -245,6 +279,7 @@ def mesh3d_measure(torch, dev, steps, warmup):
     stream = torch.cuda.current_stream()
     per = {}
     for name, fn in (("adjacency", lambda: mesh.adjacency(pos, sup, capacity=cap)),
+                     ("adjacency_face_tree", lambda: mesh.adjacency(pos, sup, capacity=cap, search="bvh")),
                      ("moments_fp64", lambda: mesh.moments(adj, pos, sup, kern, "fp64", out=out)),
                      ("moments_mixed", lambda: mesh.moments(adj, pos, sup, kern, "mixed", out=out))):
         fn()
@@ -414,6 +449,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--only-3d", action="store_true", help="run the 3-D section alone and print it")
+    ap.add_argument("--search-3d", action="store_true",
+                    help="compare the two 3-D face searches on the served workload's meshes, print it and exit")
     ap.add_argument("--no-disks", action="store_true")
     ap.add_argument("--only-disks", action="store_true", help="run the fibre-bundle disk section alone and print it")
     ap.add_argument("--probe", action="store_true",
@@ -430,6 +467,9 @@ def main():
     from warpsphboundaries_amd import _lib as L
 
     dev = torch.device("cuda:0")
+    if args.search_3d:
+        print(json.dumps({"search_3d": search3d_compare(torch, dev)}), flush=True)
+        return
     if args.only_3d:
         print(json.dumps(mesh3d_measure(torch, dev, args.steps, args.warmup)), flush=True)
         return

@@ -296,6 +296,15 @@ int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, c
                     int64_t* offsets, int32_t* faces, int64_t capacity, double* chi, void* scratch, size_t scratch_bytes,
                     void* stream);
 
+/* ---- the same face lists (ascending) and winding numbers with the faces found through a tree of the face boxes
+ * (reference fluxbridge/device_search.py:125-212: Morton-ordered LBVH over outward-rounded FP32 boxes, stack
+ * traversal, the FP64 element test): per particle a stack walk of the tree, leaves through the cell lists' FP32 filter
+ * and element test (so the same set), rows of at most 32 faces sorted in registers, longer rows sorted through an LDS
+ * bitmap of face ids.  Meshes of at most 262,144 faces (more: WSB_ENOTSUP).  Arguments as wsb_adjacency3d. */
+int wsb_adjacency3d_bvh(const wsb_mesh3d* m, const double* pos, const double* sup, const int32_t* order, int n,
+                        int64_t* offsets, int32_t* faces, int64_t capacity, double* chi, void* scratch,
+                        size_t scratch_bytes, void* stream);
+
 /* ---- particles grouped by grid cell (device counting sort of the cell keys; no counterpart in the reference):
  * order[n] is a permutation of 0..n-1: first the particles whose cell has candidate faces or faces of its own,
  * grouped by cell, then every other particle in input order.  Passed to wsb_adjacency3d it changes which thread

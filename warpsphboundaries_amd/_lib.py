@@ -83,6 +83,8 @@ SIGNATURES = {
     "wsb_adjacency3d_scratch_bytes": (c_size_t, [c_int]),
     "wsb_adjacency3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                 c_size_t, c_void_p]),
+    "wsb_adjacency3d_bvh": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                c_size_t, c_void_p]),
     "wsb_sort3d_scratch_bytes": (c_size_t, [c_void_p, c_int]),
     "wsb_sort3d": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_moments3d_scratch_bytes": (c_size_t, [c_int]),

@@ -33,6 +33,13 @@ struct wsb_mesh3d {
     std::vector<double> frec, fbox, cchi;
     std::vector<int32_t> cstart, citems, cwalk, wdir, wstart, witems;
     std::vector<float4> crec, wrec;
+    // bounding-volume tree of the faces (wsb_adjacency3d_bvh): per node two float4, (lo.xyz, left) and (hi.xyz,
+    // right) for an inner node, (lo.xyz, -(first + 1)) and (hi.xyz, count) for a leaf of items [first, first + count);
+    // per item the face's FP32 box rounded outward, (lo.xyz, id) and (hi.xyz, 0), in Morton order of the face centres
+    std::vector<float4> bnode, bitem;
+    int bvh_depth = 0;
+    float4* d_bnode = nullptr;
+    float4* d_bitem = nullptr;
     double* d_frec = nullptr;
     double* d_fbox = nullptr;
     double* d_cchi = nullptr;
@@ -379,6 +386,116 @@ int upload(T** dst, const std::vector<T>& src, int64_t* bytes) {
     return WSB_OK;
 }
 
+// ------------------------------------------------------------------------------------------------ face tree
+// Morton order of the face-box centres (10 bits per axis over the centres' box), then a binary tree whose split of
+// a range is the first position where the highest bit in which its first and last codes differ is set (a range of
+// equal codes is split in half), leaves of at most BVH_LEAF faces; node boxes are unions of the faces' FP32 boxes
+// rounded outward, so a node whose box the FP32 filter rejects holds no face the element test keeps.
+constexpr int BVH_LEAF = 4;
+constexpr int BVH_STACK = 64;
+
+static uint32_t spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+struct BvhBuild {
+    const std::vector<uint32_t>& code;
+    const std::vector<float4>& item;
+    std::vector<float4>& node;
+    int depth = 0;
+
+    int build(int first, int last, int d) {           // items [first, last]
+        const int me = (int)(node.size() / 2);
+        node.push_back(float4{});
+        node.push_back(float4{});
+        depth = std::max(depth, d);
+        float4 lo{INFINITY, INFINITY, INFINITY, 0.0f}, hi{-INFINITY, -INFINITY, -INFINITY, 0.0f};
+        auto grow = [&](const float4& l, const float4& u) {
+            lo.x = std::min(lo.x, l.x); lo.y = std::min(lo.y, l.y); lo.z = std::min(lo.z, l.z);
+            hi.x = std::max(hi.x, u.x); hi.y = std::max(hi.y, u.y); hi.z = std::max(hi.z, u.z);
+        };
+        if (last - first + 1 <= BVH_LEAF) {
+            for (int i = first; i <= last; ++i) grow(item[2 * i], item[2 * i + 1]);
+            const int tag = -(first + 1), cnt = last - first + 1;
+            std::memcpy(&lo.w, &tag, sizeof(float));
+            std::memcpy(&hi.w, &cnt, sizeof(float));
+        } else {
+            int split = (first + last) / 2;
+            const uint32_t a = code[first], b = code[last];
+            if (a != b) {
+                const int top = 31 - __builtin_clz(a ^ b);
+                int lo_i = first, hi_i = last;                   // first index whose code has bit `top` set
+                while (lo_i < hi_i) {
+                    const int mid = (lo_i + hi_i) / 2;
+                    if ((code[mid] >> top) & 1u) hi_i = mid;
+                    else lo_i = mid + 1;
+                }
+                split = lo_i - 1;
+            }
+            const int l = build(first, split, d + 1);
+            const int r = build(split + 1, last, d + 1);
+            grow(node[2 * l], node[2 * l + 1]);
+            grow(node[2 * r], node[2 * r + 1]);
+            std::memcpy(&lo.w, &l, sizeof(float));
+            std::memcpy(&hi.w, &r, sizeof(float));
+        }
+        node[2 * me] = lo;
+        node[2 * me + 1] = hi;
+        return me;
+    }
+};
+
+int build_bvh(wsb_mesh3d* m) {
+    const int nf = m->nf;
+    auto down = [](double v) { float f = (float)v; if ((double)f > v) f = std::nextafter(f, -INFINITY); return f; };
+    auto up = [](double v) { float f = (float)v; if ((double)f < v) f = std::nextafter(f, INFINITY); return f; };
+    std::vector<double> cx(3 * (size_t)nf);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int f = 0; f < nf; ++f)
+        for (int a = 0; a < 3; ++a) {
+            const double c = 0.5 * (m->fbox[6 * (size_t)f + a] + m->fbox[6 * (size_t)f + 3 + a]);
+            cx[3 * (size_t)f + a] = c;
+            lo[a] = std::min(lo[a], c);
+            hi[a] = std::max(hi[a], c);
+        }
+    std::vector<uint32_t> key(nf);
+    for (int f = 0; f < nf; ++f) {
+        uint32_t k = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double ext = hi[a] - lo[a];
+            const double u = ext > 0 ? (cx[3 * (size_t)f + a] - lo[a]) / ext : 0.0;
+            k |= spread10((uint32_t)std::min(1023.0, std::max(0.0, std::floor(u * 1024.0)))) << (2 - a);
+        }
+        key[f] = k;
+    }
+    std::vector<int32_t> ord(nf);
+    for (int f = 0; f < nf; ++f) ord[f] = f;
+    std::sort(ord.begin(), ord.end(), [&](int32_t p, int32_t q) { return key[p] < key[q] || (key[p] == key[q] && p < q); });
+    std::vector<uint32_t> code(nf);
+    m->bitem.assign(2 * (size_t)nf, float4{});
+    for (int i = 0; i < nf; ++i) {
+        const int f = ord[i];
+        code[i] = key[f];
+        const double* bx = m->fbox.data() + 6 * (size_t)f;
+        float id;
+        std::memcpy(&id, &f, sizeof(float));
+        m->bitem[2 * (size_t)i] = float4{down(bx[0]), down(bx[1]), down(bx[2]), id};
+        m->bitem[2 * (size_t)i + 1] = float4{up(bx[3]), up(bx[4]), up(bx[5]), 0.0f};
+    }
+    m->bnode.clear();
+    m->bnode.reserve(4 * (size_t)nf / BVH_LEAF + 4);
+    BvhBuild b{code, m->bitem, m->bnode};
+    b.build(0, nf - 1, 1);
+    m->bvh_depth = b.depth;
+    WSB_REQUIRE(b.depth < BVH_STACK, WSB_EINVAL, "face tree depth %d exceeds %d", b.depth, BVH_STACK);
+    return WSB_OK;
+}
+
 // ------------------------------------------------------------------------------------------------ device kernels
 struct Adj3Args {
     m3::Grid3 g;
@@ -398,6 +515,11 @@ struct Adj3Args {
     int wide_min;               // candidate lists / axis walks longer than this go to the wavefront kernels
     int32_t* exact_count;       // scratch: the number of particles whose walk hit a rounding bound, then their indices
     int32_t* exact_list;
+    bool no_faces;              // the face lists come from the face tree: adj3_kernel<0> evaluates the winding only
+    const float4* bnode;        // the face tree (wsb_adjacency3d_bvh)
+    const float4* bitem;
+    int32_t* long_count;        // scratch: the rows of more than BVH_SHORT faces, then their indices
+    int32_t* long_list;
 };
 
 constexpr int ADJ_UNROLL = 8;
@@ -487,7 +609,7 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
         it0 = a.g.cstart[cid];
         it1 = a.g.cstart[cid + 1];
     }
-    const bool wide = it1 - it0 > a.wide_min;
+    const bool wide = !a.no_faces && it1 - it0 > a.wide_min;
     // the winding number where it is a lookup (outside the grid, a face-free cell); NaN: the walk decides
     double w0 = 0.0;
     bool long_walk = false;
@@ -522,7 +644,7 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
             if (long_walk) a.walk_list[first + __popcll(lm & ((1ull << lane) - 1))] = i;
         }
     }
-    if (MODE != 2 && in && !wide) {
+    if (MODE != 2 && in && !wide && !a.no_faces) {
         const Near32 nq = near32(px, py, pz, h);
         const int64_t base = FILL ? a.offsets[i] : 0;
         // ADJ_UNROLL candidates per step: their 32-byte records are loaded together (independent loads in flight),
@@ -550,7 +672,7 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     if (MODE == 1) return;
     if (MODE == 0) {
         if (live) {
-            if (!wide) a.counts[i] = cnt;
+            if (!wide && !a.no_faces) a.counts[i] = cnt;
             if (a.chi) a.chi[i] = w0;
         }
         return;
@@ -567,6 +689,135 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
         if (need) a.exact_list[first + __popcll(mask & ((1ull << lane) - 1))] = i;
     }
     if (todo && !need) a.chi[i] = chi;          // the listed ones keep NaN until adj3_exact_kernel
+}
+
+// face lists from the face tree, one thread per particle: a stack walk of the tree (a node whose FP32 box the
+// particle's FP32 filter rejects is skipped whole), the faces of the leaves reached through the same decision as the
+// cell lists (FP32 filter, then the element test), so the set is the cell lists' set.  MODE 0: counts; MODE 1: the
+// list in ascending face order -- a row of at most BVH_SHORT faces sorted in registers, a longer one written in tree
+// order and listed for bvh3_sort_kernel.
+constexpr int BVH_SHORT = 32;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void bvh3_kernel(Adj3Args a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < a.n;
+    const int i = live ? (a.order ? a.order[t] : t) : 0;
+    int64_t base = 0, total = 0;
+    if (MODE == 1 && live) {
+        base = a.offsets[i];
+        total = a.offsets[i + 1] - base;
+    }
+    const bool shortrow = total <= BVH_SHORT;
+    const bool fits = base + total <= a.capacity;
+    if (MODE == 1) {            // the long rows go to bvh3_sort_kernel (one atomic per wavefront)
+        const bool lng = live && !shortrow && fits;
+        const uint64_t lm = __ballot(lng);
+        if (lm) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((unsigned long long)lm) - 1;
+            int first = 0;
+            if (lane == leader) first = atomicAdd(a.long_count, __popcll(lm));
+            first = __shfl(first, leader);
+            if (lng) a.long_list[first + __popcll(lm & ((1ull << lane) - 1))] = i;
+        }
+    }
+    if (!live) return;
+    const double px = a.pos[3 * (int64_t)i], py = a.pos[3 * (int64_t)i + 1], pz = a.pos[3 * (int64_t)i + 2];
+    const double h = a.sup[i];
+    const Near32 nq = near32(px, py, pz, h);
+    int32_t buf[BVH_SHORT];
+    int64_t cnt = 0;
+    int stack[BVH_STACK];
+    int sp = 0, node = 0;
+    for (;;) {
+        const float4 lo = a.bnode[2 * node], hi = a.bnode[2 * node + 1];
+        bool go = classify32(nq, lo, hi) != 0;
+        if (go) {
+            const int l = m3::bits_int(lo.w), r = m3::bits_int(hi.w);
+            if (l >= 0) {
+                stack[sp++] = r;
+                node = l;
+                continue;
+            }
+            const int first = -l - 1;
+            for (int it = first; it < first + r; ++it) {
+                const float4 il = a.bitem[2 * (int64_t)it], iu = a.bitem[2 * (int64_t)it + 1];
+                const int k32 = classify32(nq, il, iu);
+                if (k32 == 0) continue;
+                const int f = m3::bits_int(il.w);
+                if (k32 == 2 && !keep_face_fast(a.g.fbox + 6 * (int64_t)f, px, py, pz, h)) continue;
+                if (MODE == 1 && fits) {
+                    if (shortrow) {
+                        if (cnt < BVH_SHORT) buf[cnt] = f;
+                    } else {
+                        a.faces[base + cnt] = f;
+                    }
+                }
+                ++cnt;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    if (MODE == 0) {
+        a.counts[i] = cnt;
+        return;
+    }
+    if (shortrow && fits) {
+        const int c = (int)min(cnt, (int64_t)BVH_SHORT);
+        for (int p = 1; p < c; ++p) {                  // insertion sort: ascending face ids
+            const int32_t v = buf[p];
+            int q = p - 1;
+            while (q >= 0 && buf[q] > v) {
+                buf[q + 1] = buf[q];
+                --q;
+            }
+            buf[q + 1] = v;
+        }
+        for (int p = 0; p < c; ++p) a.faces[base + p] = buf[p];
+    }
+}
+
+// the long rows listed by bvh3_kernel<1>, one workgroup each: the row's face ids marked in an LDS bitmap over all
+// face ids, then written back in ascending order (the bits of each thread's words, placed after a scan of the
+// threads' counts).  Faces of a mesh: at most BVH_BITS.
+constexpr int BVH_WORDS = 8192;                        // 32 KiB of LDS
+constexpr int BVH_BITS = BVH_WORDS * 32;
+
+__global__ __launch_bounds__(256) void bvh3_sort_kernel(Adj3Args a) {
+    using BlockScan = hipcub::BlockScan<int, 256>;
+    __shared__ typename BlockScan::TempStorage scan_tmp;
+    __shared__ uint32_t bits[BVH_WORDS];
+    const int nlong = *a.long_count;
+    const int nw = (a.g.nf + 31) / 32;
+    constexpr int PER = BVH_WORDS / 256;
+    for (int q = blockIdx.x; q < nlong; q += gridDim.x) {
+        const int i = a.long_list[q];
+        const int64_t base = a.offsets[i];
+        const int total = (int)(a.offsets[i + 1] - base);
+        for (int w = threadIdx.x; w < nw; w += 256) bits[w] = 0u;
+        __syncthreads();
+        for (int j = threadIdx.x; j < total; j += 256) {
+            const int f = a.faces[base + j];
+            atomicOr(&bits[f >> 5], 1u << (f & 31));
+        }
+        __syncthreads();
+        const int w0 = threadIdx.x * PER;
+        int mine = 0;
+        for (int w = w0; w < w0 + PER && w < nw; ++w) mine += __popc(bits[w]);
+        int at = 0;
+        BlockScan(scan_tmp).ExclusiveSum(mine, at);
+        for (int w = w0; w < w0 + PER && w < nw; ++w) {
+            uint32_t v = bits[w];
+            while (v) {
+                const int b = __ffs((int)v) - 1;
+                a.faces[base + at++] = w * 32 + b;
+                v &= v - 1u;
+            }
+        }
+        __syncthreads();                            // bits and scan_tmp are reused by the next row
+    }
 }
 
 // the particles listed by adj3_kernel<0> for their long axis walk, one wavefront each: lane k takes walk items k,
@@ -930,6 +1181,7 @@ __global__ void zero3_counts_kernel(int32_t* c) {
     c[0] = 0;
     c[1] = 0;
     c[2] = 0;
+    c[3] = 0;
 }
 
 template <int KID, int FORM, int MODE>
@@ -1033,6 +1285,9 @@ extern "C" int wsb_mesh3d_create(const double* verts, int nv, const int32_t* fac
         if (!st) st = upload(&m->d_wstart, m->wstart, &m->bytes);
         if (!st) st = upload(&m->d_witems, m->witems, &m->bytes);
         if (!st) st = upload(&m->d_wrec, m->wrec, &m->bytes);
+        if (!st) st = build_bvh(m);
+        if (!st) st = upload(&m->d_bnode, m->bnode, &m->bytes);
+        if (!st) st = upload(&m->d_bitem, m->bitem, &m->bytes);
         if (st) {
             wsb_mesh3d_destroy(m);
             return st;
@@ -1058,7 +1313,8 @@ extern "C" int wsb_mesh3d_destroy(wsb_mesh3d* m) {
     if (!m) return WSB_OK;
     for (void* p : {(void*)m->d_frec, (void*)m->d_fbox, (void*)m->d_cstart, (void*)m->d_citems, (void*)m->d_crec, (void*)m->d_cchi,
                     (void*)m->d_cwalk,
-                    (void*)m->d_wdir, (void*)m->d_wstart, (void*)m->d_witems, (void*)m->d_wrec})
+                    (void*)m->d_wdir, (void*)m->d_wstart, (void*)m->d_witems, (void*)m->d_wrec, (void*)m->d_bnode,
+                    (void*)m->d_bitem})
         if (p) (void)hipFree(p);
     delete m;
     return WSB_OK;
@@ -1174,7 +1430,7 @@ extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const dou
     size_t tmp_bytes = scratch_bytes - lay.tmp;
     const bool split = n < WIDE3_MAX_N;
     Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, list_counts, wide_list, list_counts + 1,
-               walk_list, split ? WIDE3 : INT_MAX, list_counts + 2, exact_list};
+               walk_list, split ? WIDE3 : INT_MAX, list_counts + 2, exact_list, false, nullptr, nullptr, nullptr, nullptr};
     const dim3 grid((n + 255) / 256), block(256);
     // up to 8,192 wavefronts (32 per CU) for the wide particles, one per particle at a time
     const dim3 wgrid((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 2048));
@@ -1202,6 +1458,60 @@ extern "C" int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const dou
         hipLaunchKernelGGL(adj3_kernel<2>, grid, block, 0, s, a);
         WSB_HIP(hipGetLastError());
         // rare (particles within rounding of a face plane, rays through edges): 1,024 wavefronts at most
+        hipLaunchKernelGGL(adj3_exact_kernel, dim3((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 256)), block, 0, s, a);
+        WSB_HIP(hipGetLastError());
+    }
+    return WSB_OK;
+}
+
+extern "C" int wsb_adjacency3d_bvh(const wsb_mesh3d* m, const double* pos, const double* sup, const int32_t* order, int n,
+                                   int64_t* offsets, int32_t* faces, int64_t capacity, double* chi, void* scratch,
+                                   size_t scratch_bytes, void* stream) {
+    WSB_REQUIRE(m, WSB_EINVAL, "null mesh");
+    WSB_REQUIRE(!m->host_only, WSB_EINVAL, "host-only mesh has no device tables");
+    WSB_REQUIRE(m->d_bnode, WSB_EINVAL, "mesh has no face tree");
+    WSB_REQUIRE(m->nf <= BVH_BITS, WSB_ENOTSUP, "face-tree adjacency takes meshes of at most %d faces (%d)", BVH_BITS, m->nf);
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    WSB_REQUIRE(offsets, WSB_EINVAL, "null offsets");
+    hipStream_t s = wsb::as_stream(stream);
+    hipLaunchKernelGGL(offsets_head_kernel, dim3(1), dim3(1), 0, s, offsets);
+    WSB_HIP(hipGetLastError());
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(pos && sup, WSB_EINVAL, "null buffer");
+    const size_t need = wsb_adjacency3d_scratch_bytes(n);
+    WSB_REQUIRE(scratch && scratch_bytes >= need, WSB_ENOMEM, "adjacency needs %zu scratch bytes (got %zu)", need, scratch_bytes);
+    const Adj3Layout lay = adj3_layout(n);
+    char* sb = static_cast<char*>(scratch);
+    int64_t* counts = reinterpret_cast<int64_t*>(sb + lay.counts);
+    int32_t* list_counts = reinterpret_cast<int32_t*>(sb + lay.list_counts);     // wide (unused), walk, exact, long
+    int32_t* long_list = reinterpret_cast<int32_t*>(sb + lay.wide_list);
+    int32_t* walk_list = reinterpret_cast<int32_t*>(sb + lay.walk_list);
+    int32_t* exact_list = reinterpret_cast<int32_t*>(sb + lay.exact_list);
+    char* tmp = sb + lay.tmp;
+    size_t tmp_bytes = scratch_bytes - lay.tmp;
+    const bool split = n < WIDE3_MAX_N;
+    Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, list_counts, long_list, list_counts + 1,
+               walk_list, split ? WIDE3 : INT_MAX, list_counts + 2, exact_list, true, m->d_bnode, m->d_bitem,
+               list_counts + 3, long_list};
+    const dim3 grid((n + 255) / 256), block(256);
+    const dim3 wgrid((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 2048));
+    hipLaunchKernelGGL(zero3_counts_kernel, dim3(1), dim3(1), 0, s, list_counts);
+    if (chi) hipLaunchKernelGGL(adj3_kernel<0>, grid, block, 0, s, a);      // winding lookups and walk lists only
+    hipLaunchKernelGGL(bvh3_kernel<0>, grid, block, 0, s, a);
+    WSB_HIP(hipGetLastError());
+    WSB_HIP(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, counts, offsets + 1, n, s));
+    if (faces && capacity > 0) {
+        hipLaunchKernelGGL(bvh3_kernel<1>, grid, block, 0, s, a);
+        hipLaunchKernelGGL(bvh3_sort_kernel, dim3((unsigned)std::min(n, 1024)), block, 0, s, a);
+        WSB_HIP(hipGetLastError());
+    }
+    if (chi) {
+        if (split) {
+            hipLaunchKernelGGL(adj3_walk_kernel, wgrid, block, 0, s, a);
+            WSB_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(adj3_kernel<2>, grid, block, 0, s, a);
+        WSB_HIP(hipGetLastError());
         hipLaunchKernelGGL(adj3_exact_kernel, dim3((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 256)), block, 0, s, a);
         WSB_HIP(hipGetLastError());
     }

@@ -88,6 +88,7 @@ class Mesh3D:
         self._sort_scratch = None
         self._mom_scratch = None
         self._cap = 0
+        self.search = "grid"         # the default face search of adjacency()
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -138,10 +139,14 @@ class Mesh3D:
         return order
 
     def adjacency(self, pos: torch.Tensor, sup: torch.Tensor, capacity: int | None = None, scratch=None,
-                  order: torch.Tensor | None = None, sort: bool = True) -> Adjacency3D:
+                  order: torch.Tensor | None = None, sort: bool = True, search: str | None = None) -> Adjacency3D:
         """face adjacency + winding numbers.  Particles are grouped by cell first (``sort``; or pass ``order``), which
         changes only the thread schedule.  ``capacity`` (pairs) defaults to a remembered bound; the call retries once
-        with the exact count when it was too small (one host read of the pair count)."""
+        with the exact count when it was too small (one host read of the pair count).  ``search``: "grid" (the cell
+        lists) or "bvh" (the face tree); both give the same face lists and winding numbers."""
+        search = self.search if search is None else search
+        if search not in ("grid", "bvh"):
+            raise ValueError(f"search {search!r}: expected 'grid' or 'bvh'")
         self._check_inputs(pos, sup)
         if float(sup.max()) > self.support_max * (1 + 1e-12) if len(sup) else False:
             raise ValueError("support exceeds the mesh's support_max")
@@ -157,14 +162,15 @@ class Mesh3D:
         cap = int(capacity if capacity is not None else max(self._cap, 4 * n, 1024))
         faces = torch.empty(cap, dtype=torch.int32, device=dev)
         st = L.stream_ptr(dev)
-        L.check(lib.wsb_adjacency3d(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap,
-                                    L.ptr(chi), L.ptr(scratch), scratch.numel(), st))
+        fn = lib.wsb_adjacency3d if search == "grid" else lib.wsb_adjacency3d_bvh
+        L.check(fn(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap, L.ptr(chi),
+                   L.ptr(scratch), scratch.numel(), st))
         total = int(offsets[n]) if n else 0
         if total > cap:
             cap = total
             faces = torch.empty(cap, dtype=torch.int32, device=dev)
-            L.check(lib.wsb_adjacency3d(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap,
-                                        L.ptr(chi), L.ptr(scratch), scratch.numel(), st))
+            L.check(fn(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap, L.ptr(chi),
+                       L.ptr(scratch), scratch.numel(), st))
         self._cap = max(self._cap, total)
         return Adjacency3D(offsets, faces, chi, total)
 
