@@ -508,19 +508,23 @@ __device__ __forceinline__ void serve_row(const MultiArgs& ma, int pos) {
 #pragma unroll
         for (int q = 0; q < wsb::NCH; ++q) acc[k][q] = 0.0;
     const int32_t* sl = a.slots + (int64_t)row * a.K;
+    // two-stage prefetch: the edge records of slot s + 1 and the edge id of slot s + 2 are requested while slot s is
+    // evaluated, so a record load never waits on an id load issued in the same iteration
     int e_next = cnt > 0 ? sl[0] : 0;
     bool ok_next = e_next >= 0 && e_next < a.ne;
     double4 ab_next = a.erec[ok_next ? e_next : 0];
     double4 tn_next = a.etan[ok_next ? e_next : 0];
+    int e_after = cnt > 1 ? sl[1] : 0;
     for (int s = 0; s < cnt; ++s) {
         const bool ok = ok_next;
         const double4 ab = ab_next, tn = tn_next;
         if (s + 1 < cnt) {
-            e_next = sl[s + 1];
+            e_next = e_after;
             ok_next = e_next >= 0 && e_next < a.ne;
             const int en = ok_next ? e_next : 0;
             ab_next = a.erec[en];
             tn_next = a.etan[en];
+            if (s + 2 < cnt) e_after = sl[s + 2];
         }
         const double ux = ab.x - x, uy = ab.y - y, vx = ab.z - x, vy = ab.w - y;
         const double tx = tn.x, ty = tn.y, iell = tn.z;
