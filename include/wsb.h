@@ -136,6 +136,19 @@ int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t* order, con
                         const int32_t* counts, int K, const double* ind, const double* lpos, const double* lsup, int n,
                         const double* rot4, int flags, double* const* outs, void* stream);
 
+/* ---- the sorted-position layout (no counterpart in the reference, whose arrays stay in input order; for callers that
+ * sort the rows by cell every step): with order[n] from wsb_sort2d, wsb_adjacency2d_sorted writes the adjacency of
+ * row order[i] at position i -- slots[i*K], counts[i] and prow[i*4] = (x, y, h, indicator) of that row -- so that
+ * wsb_moments2d_multi_sorted reads every row's record, count and slots contiguously in cell order; it writes each
+ * row's outputs at the row itself (outs[k][order[i]*12]).  The values are those of wsb_adjacency2d and
+ * wsb_moments2d_multi (same slots, counts, indicator and twelve channels per row); valid[n] (input order, nullable)
+ * and pad as wsb_adjacency2d; forms and flags as wsb_moments2d_multi. */
+int wsb_adjacency2d_sorted(const wsb_geom2d* g, const double* lpos, const double* lsup, const uint8_t* valid, int n,
+                           const int32_t* order, int32_t* slots, int32_t* counts, double* prow, int pad, void* stream);
+int wsb_moments2d_multi_sorted(const wsb_geom2d* g, int form, const int32_t* order, const int32_t* slots,
+                               const int32_t* counts, int K, const double* prow, int n, const double* rot4, int flags,
+                               double* const* outs, void* stream);
+
 /* ---- rows grouped by slot count, the largest count first (device counting sort; no counterpart in the reference,
  * which evaluates rows in input order): order[n] is a permutation of base[0..n) (of 0..n-1 when base is NULL) along
  * which the slot counts never increase (counts at or above 1023 share one bin), the rows with no slot last; *nwork

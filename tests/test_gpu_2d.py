@@ -103,6 +103,76 @@ def test_adjacency_bit_exact(name):
     np.testing.assert_allclose(got[frac], i_ref[frac], rtol=0, atol=1e-14)
 
 
+@pytest.mark.parametrize("case", ["star-two-pass", "box-fine-grid-one-pass"])
+def test_cell_sort_groups_rows_by_cell_at_scale(case):
+    """wsb_sort2d over 300,000 rows (73 chunks of the two-pass sort), a large share outside the grid: a permutation of
+    the rows whose cell keys ascend, the rows outside the grid last.  The 4-edge box at support 4e-4 has a grid of
+    2,502 x 2,502 cells, too fine for the two-pass sort's coarse bins: the one-pass sort serves it."""
+    if case.startswith("star"):
+        loop, solid, smax = WL.star_polygon(512, g=WL.rng(21)), "inside", 0.004
+    else:
+        loop, solid, smax = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]]), "outside", 4e-4
+    rep = _rep(loop, solid)
+    pts = WL.rng(22).uniform(-0.1, 1.1, (300000, 2))
+    pts[:64] = pts[64]                                    # one cell crowded by a whole wavefront of rows
+    cl = oadj.edge_cell_list(rep.vertices.numpy(), rep.edges.numpy().astype(np.int64), smax)
+    nx, ny = cl.dims
+    geo = rep.geom(smax, DEV)
+    assert (geo.info["nx"], geo.info["ny"]) == (nx, ny)
+    order = geo.sort(torch.tensor(pts, dtype=torch.float64, device=DEV)).cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(len(pts)))
+    fx = np.floor((pts[:, 0] - cl.lo[0]) / cl.cell)
+    fy = np.floor((pts[:, 1] - cl.lo[1]) / cl.cell)
+    inside = (fx >= 0) & (fx < nx) & (fy >= 0) & (fy < ny)
+    key = np.where(inside, fx * ny + fy, nx * ny)
+    print(case, "cells", nx * ny, "rows outside the grid", int((~inside).sum()))
+    assert 0 < (~inside).sum() < len(pts)
+    assert np.array_equal(key[order], np.sort(key))
+
+
+@pytest.mark.parametrize("name", ["star256", "notch"])
+def test_sorted_layout_equals_row_layout(name):
+    """wsb_adjacency2d_sorted / wsb_moments2d_multi_sorted: position i holds row order[i]'s slots, count and (x, y, h,
+    indicator), bitwise those of wsb_adjacency2d (with valid rows masked, padded and not), and the fused moments of
+    both forms, written at the rows, bitwise those of wsb_moments2d_multi; the slots also against the oracle."""
+    loop, solid = _geometries()[name]
+    rep = _rep(loop, solid)
+    h = 0.01
+    pts = _particles(rep.vertices.numpy(), h, 3000, 31)
+    g = WL.rng(32)
+    sup = np.where(g.uniform(0, 1, len(pts)) < 0.3, 0.7 * h, h)
+    st = oadj.Static(rep.vertices.numpy(), rep.edges.numpy(), rep.background, h)
+    geo = rep.geom(h, DEV)
+    K = geo.K
+    lp = torch.tensor(pts, dtype=torch.float64, device=DEV)
+    ls = torch.tensor(sup, dtype=torch.float64, device=DEV)
+    valid = torch.tensor(g.uniform(0, 1, len(pts)) < 0.9, device=DEV)
+    e_ref, c_ref, i_ref = oadj.fixed_topology(st, pts, sup, valid.cpu().numpy())
+    slots, counts, ind = geo.adjacency(lp, ls, valid=valid, pad=True)
+    order = geo.sort(lp)
+    o = order.long()
+    for pad in (False, True):
+        sp, cp, prow = geo.adjacency_sorted(lp, ls, order, valid=valid, pad=pad)
+        assert torch.equal(cp, counts[o])
+        assert torch.equal(prow[:, 3], ind[o]) and torch.equal(prow[:, :2], lp[o]) and torch.equal(prow[:, 2], ls[o])
+        got = sp.view(-1, K)
+        want = slots.view(-1, K)[o]
+        if pad:
+            assert torch.equal(got, want)
+        else:
+            live = torch.arange(K, device=DEV)[None, :] < cp[:, None]
+            assert torch.equal(torch.where(live, got, 0), torch.where(live, want, 0))
+    inv = np.empty(len(pts), dtype=np.int64)
+    inv[o.cpu().numpy()] = np.arange(len(pts))
+    assert np.array_equal(sp.view(-1, K).cpu().numpy()[inv].reshape(-1), e_ref)
+    kset = ("cubic", "w2", "w4", "w6")
+    for form in ("fp64", "mixed"):
+        a = geo.moments_multi(kset, slots, counts, ind, lp, ls, form=form, order=order)
+        b = geo.moments_multi_sorted(kset, order, sp, cp, prow, form=form)
+        for k in kset:
+            assert torch.equal(a[k], b[k]), (form, k)
+
+
 @pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("kname", KERNELS)
 def test_moments_match_oracle(form, kname):

@@ -4,7 +4,11 @@ tables for the optimized HIP kernels (one template specialisation per kernel, fu
 Run by the build (``__graft_entry__.build`` / ``warpsphboundaries_amd.build``); output next to this file.
 """
 import os
+import struct
 import sys
+from math import gcd
+
+import mpmath
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
@@ -15,6 +19,12 @@ from fractions import Fraction as F                   # noqa: E402
 from warpsphboundaries_amd.kernels import ALL_NAMES, CLOSED_NAMES, NAMES, plane_channels   # noqa: E402
 
 NMAX = 16          # fixed array extent (>= max power + 1 of every profile)
+
+
+def _literal(x):
+    """True when the double nearest x is exact with a zero low word (an FP64 instruction's 32-bit literal)"""
+    v = float(x)
+    return F(v) == x and struct.pack("<d", v)[:4] == b"\0\0\0\0"
 
 
 def _arr(vals):
@@ -46,8 +56,29 @@ def render():
         out.append(f"  static constexpr double MRP[NB] = {{{', '.join(repr(float(b.mR_P)) for b in bl)}}};")
         out.append(f"  static constexpr double MRF[NB] = {{{', '.join(repr(float(b.mR_Phi)) for b in bl)}}};")
         out.append(f"  static constexpr double PLANE[12] = {{{', '.join(repr(v) for v in plane_channels(name))}}};")
+        # the fused pass's integer form: P and Phi times the least common denominator SC of their coefficients, so
+        # that every coefficient is an integer that an FP64 instruction takes as its 32-bit literal operand (the
+        # high word of the double); the value sums use the shared bases dI_m / (m + 2) and dR_m / m instead of
+        # P / (m + 2), and the channel sums are divided by SC with the factor 1/pi at the end of the row
+        sc = 1
+        for b in bl:
+            for c in list(b.P) + list(b.Phi):
+                sc = sc * c.denominator // gcd(sc, c.denominator)
+        PS = [[c * sc for c in b.P] for b in bl]
+        FS = [[c * sc for c in b.Phi] for b in bl]
+        assert all(x.denominator == 1 for r in PS + FS for x in r)
+        out.append(f"  static constexpr double SC = {float(sc)!r};")
+        out.append(f"  static constexpr bool LITERAL = {str(all(_literal(x) for r in PS + FS for x in r)).lower()};")
+        out.append(f"  static constexpr double PS[NB][{NMAX}] = {{{', '.join(_arr(r) for r in PS)}}};")
+        out.append(f"  static constexpr double FS[NB][{NMAX}] = {{{', '.join(_arr(r) for r in FS)}}};")
+        out.append(f"  static constexpr double MRPS[NB] = {{{', '.join(repr(float(b.mR_P * sc)) for b in bl)}}};")
+        out.append(f"  static constexpr double MRFS[NB] = {{{', '.join(repr(float(b.mR_Phi * sc)) for b in bl)}}};")
+        out.append(f"  static constexpr double IPS = {float(mpmath.mpf(1) / (mpmath.pi * sc))!r};   // 1 / (pi SC)")
         out.append("};")
         out.append("")
+    out.append(f"// 1/k, k = 0 .. {NMAX + 1} (entries 0 and 1: 1)")
+    out.append(f"constexpr double RK[{NMAX + 2}] = {{{', '.join(repr(1.0 / max(k, 1)) for k in range(NMAX + 2))}}};")
+    out.append("")
     # 3-D sphere: per block the moment polynomial coefficients c_n/(n+k), k = 2, 3, 4
     from warpsphboundaries_amd.sphere import device_tables
     out.append("template <int KID> struct STab;")

@@ -235,6 +235,7 @@ struct AdjArgs {
     int32_t* counts;
     double* ind;
     int pad;
+    double4* prow;              // non-null: the sorted-position layout (slots, counts, prow indexed by position)
 };
 
 // one thread per row (reference scene/fixedadj.py:67-147); with `order` (rows sorted by cell) thread i serves
@@ -248,11 +249,12 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
     const int i_ = blockIdx.x * blockDim.x + threadIdx.x;
     if (i_ >= a.n) return;
     const int r = a.order ? a.order[i_] : i_;
+    const int64_t w = a.prow ? i_ : r;          // the index the outputs are written at
     int n = 0;
     double ind = 0.0;
+    const double px = a.lpos[2 * r], py = a.lpos[2 * r + 1];
+    const double h = a.lsup[r];
     if (a.valid == nullptr || a.valid[r] != 0) {
-        const double px = a.lpos[2 * r], py = a.lpos[2 * r + 1];
-        const double h = a.lsup[r];
         const double fx = floor((px - a.clo0) / a.cell);
         const double fy = floor((py - a.clo1) / a.cell);
         const bool inside = fx >= 0.0 && fx < (double)a.nx && fy >= 0.0 && fy < (double)a.ny;
@@ -265,8 +267,20 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
             const bool local = q.w != 0.0;
             int dw = 0;
             bool unsure = false;
-            for (int i = a.cstart[cid]; i < a.cstart[cid + 1]; ++i) {
-                const double4 rec = a.cxy[i];
+            // the records of the next two items are requested while one is tested (the loop waits on load latency)
+            const int i0 = a.cstart[cid], i1 = a.cstart[cid + 1];
+            double4 rec_n = i0 < i1 ? a.cxy[i0] : double4{0.0, 0.0, 0.0, 0.0};
+            double4 rec_nn = i0 + 1 < i1 ? a.cxy[i0 + 1] : double4{0.0, 0.0, 0.0, 0.0};
+            int item_n = i0 < i1 ? a.citems[i0] : 0, item_nn = i0 + 1 < i1 ? a.citems[i0 + 1] : 0;
+            for (int i = i0; i < i1; ++i) {
+                const double4 rec = rec_n;
+                const int item = item_n;
+                rec_n = rec_nn;
+                item_n = item_nn;
+                if (i + 2 < i1) {
+                    rec_nn = a.cxy[i + 2];
+                    item_nn = a.citems[i + 2];
+                }
                 const double ax = rec.x, ay = rec.y;
                 const double bx = rec.z, by = rec.w;
                 if (local) local_winding_edge(ax, ay, bx, by, px, py, q.x, q.y, dw, unsure);
@@ -280,7 +294,7 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
                 const double ex = px - (ax + t * dx);
                 const double ey = py - (ay + t * dyy);
                 if (sqrt(ex * ex + ey * ey) < h) {
-                    a.slots[(int64_t)r * a.K + n] = a.citems[i];
+                    a.slots[w * a.K + n] = item;
                     ++n;
                 }
             }
@@ -296,9 +310,10 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
         }
     }
     if (a.pad)
-        for (int s = n; s < a.K; ++s) a.slots[(int64_t)r * a.K + s] = -1;
-    if (a.counts) a.counts[r] = n;
-    a.ind[r] = ind;
+        for (int s = n; s < a.K; ++s) a.slots[w * a.K + s] = -1;
+    if (a.counts) a.counts[w] = n;
+    if (a.prow) a.prow[w] = double4{px, py, h, ind};
+    else a.ind[r] = ind;
 }
 
 struct SdArgs {
@@ -436,14 +451,12 @@ __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
     if (a.edge) a.edge[r] = k;
 }
 
-// counting sort of the rows by static cell: histogram, exclusive scan, scatter.  The order of the rows inside one
-// cell depends on the atomics' timing; it only decides which thread serves a row, never a result.
-// Rows inside the grid are placed by per-cell atomics (a cell holds a few dozen rows); the rows outside the grid,
-// which share one key and may be most of the particles, are placed after them by an exclusive scan of their flags.
-// counting sort by cell: the cell key of every row and its rank among the rows of its key, then each row to its
-// key's offset plus its rank.  A row inside the grid takes its rank from an atomic count of its cell; the rows
-// outside it (often most of a workload) take the key nbins + their 64-lane wavefront and their rank among that
-// wavefront's outside rows from a ballot, so they need no atomic (and land last, in input order).
+// One-pass counting sort by cell (grids too fine for the two-pass sort below): the cell key of every row and its
+// rank among the rows of its key, then each row to its key's offset plus its rank.  A row inside the grid takes its
+// rank from an atomic count of its cell; the rows outside it (often most of a workload) take the key nbins + their
+// 64-lane wavefront and their rank among that wavefront's outside rows from a ballot, so they need no atomic (and
+// land last, in input order).  The order of the rows inside one cell depends on the atomics' timing; it only decides
+// which thread serves a row, never a result.
 __global__ __launch_bounds__(256) void cell_rank_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
                                                         int ny, int32_t* key, int32_t* rank, int32_t* hist) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,18 +477,181 @@ __global__ __launch_bounds__(256) void cell_rank_kernel(const double* lpos, int 
         if (below == 0) hist[k] = __popcll(out);
     }
 }
+constexpr int PLACE_RPT = 4;
 __global__ __launch_bounds__(256) void cell_place_kernel(const int32_t* key, const int32_t* rank, const int32_t* start, int n,
                                                          int32_t* order) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) order[start[key[r]] + rank[r]] = r;
+    const int64_t r0 = (int64_t)blockIdx.x * 256 * PLACE_RPT + threadIdx.x;
+    int k[PLACE_RPT], rk[PLACE_RPT], st[PLACE_RPT];
+#pragma unroll
+    for (int j = 0; j < PLACE_RPT; ++j) {
+        const int64_t r = r0 + 256 * j;
+        k[j] = r < n ? key[r] : 0;
+        rk[j] = r < n ? rank[r] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < PLACE_RPT; ++j) st[j] = r0 + 256 * j < n ? start[k[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < PLACE_RPT; ++j)
+        if (r0 + 256 * j < n) order[st[j] + rk[j]] = (int)(r0 + 256 * j);
 }
+// Two-pass counting sort by cell without global atomics (grids of at most CS_NC - 1 coarse bins of CS_FMAX cells).
+// Coarse bin = cell >> shift, plus one bin for the rows outside the grid.  Pass 1: per chunk of CS_CH rows an LDS
+// histogram of the coarse bins, written column by column (bin-major) for one exclusive scan.  Pass 2: per chunk, each
+// row takes the next place of its coarse bin from an LDS cursor; a row outside the grid goes straight to its final
+// place (the rows outside a 64-row group stay in input order, each group's block placed by one LDS atomic), a row
+// inside to a staging list grouped by coarse bin.  Pass 3: one workgroup per coarse bin counts its rows per cell in
+// LDS, scans and places them.  Which thread serves a row inside a cell depends on the LDS atomics' timing, never a
+// result.
+constexpr int CS_T = 256;
+constexpr int CS_RPT = 16;
+constexpr int CS_CH = CS_T * CS_RPT;
+constexpr int CS_NC = 1024;
+constexpr int CS_FMAX = 4096;
+
+struct CoarseGrid {
+    int shift, ncb;             // ncb coarse bins inside the grid; the rows outside it take bin ncb
+};
+
+inline CoarseGrid coarse_grid(int nbins) {
+    int sh = 0;
+    while (((int64_t)nbins + (1 << sh) - 1) >> sh > CS_NC - 1) ++sh;
+    return {sh, (int)(((int64_t)nbins + (1 << sh) - 1) >> sh)};
+}
+
+inline bool two_pass_sort(int nbins) { return (1 << coarse_grid(nbins).shift) <= CS_FMAX; }
+
+__device__ __forceinline__ int lane_rank(uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__global__ __launch_bounds__(CS_T) void cs_hist_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
+                                                       int ny, int shift, int nc, int nblk, int32_t* key, int32_t* ghist) {
+    __shared__ int h[CS_NC];
+    for (int i = threadIdx.x; i < nc; i += CS_T) h[i] = 0;
+    const int64_t c0 = (int64_t)blockIdx.x * CS_CH;
+    // every position of the chunk is loaded before the first atomic, so the loads overlap
+    double2 p[CS_RPT];
+#pragma unroll
+    for (int j = 0; j < CS_RPT; ++j) {
+        const int64_t r = c0 + j * CS_T + threadIdx.x;
+        p[j] = r < n ? reinterpret_cast<const double2*>(lpos)[r] : double2{0.0, 0.0};
+    }
+    int k[CS_RPT];
+#pragma unroll
+    for (int j = 0; j < CS_RPT; ++j) {
+        const int64_t r = c0 + j * CS_T + threadIdx.x;
+        const double fx = floor((p[j].x - clo0) / cell);
+        const double fy = floor((p[j].y - clo1) / cell);
+        k[j] = fx >= 0.0 && fx < (double)nx && fy >= 0.0 && fy < (double)ny ? (int)fx * ny + (int)fy : -1;
+        if (r < n) key[r] = k[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CS_RPT; ++j) {
+        const bool live = c0 + j * CS_T + threadIdx.x < n;
+        const uint64_t out = __ballot(live && k[j] < 0);
+        if (live && k[j] >= 0) atomicAdd(&h[k[j] >> shift], 1);
+        else if (live && lane_rank(out) == 0) atomicAdd(&h[nc - 1], __popcll(out));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += CS_T) ghist[(int64_t)i * nblk + blockIdx.x] = h[i];
+}
+
+__global__ __launch_bounds__(CS_T) void cs_scatter_kernel(const int32_t* key, int n, int shift, int nc, int nblk,
+                                                          const int32_t* goff, int32_t* stage, int32_t* skey,
+                                                          int32_t* order) {
+    __shared__ int cur[CS_NC];
+    for (int i = threadIdx.x; i < nc; i += CS_T) cur[i] = goff[(int64_t)i * nblk + blockIdx.x];
+    const int64_t c0 = (int64_t)blockIdx.x * CS_CH;
+    const int lane = threadIdx.x & 63;
+    int k[CS_RPT];
+#pragma unroll
+    for (int j = 0; j < CS_RPT; ++j) {
+        const int64_t r = c0 + j * CS_T + threadIdx.x;
+        k[j] = r < n ? key[r] : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CS_RPT; ++j) {
+        const int64_t r = c0 + j * CS_T + threadIdx.x;
+        const bool live = r < n;
+        const uint64_t out = __ballot(live && k[j] < 0);
+        if (out) {
+            const int leader = __builtin_ctzll(out);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&cur[nc - 1], __popcll(out));
+            base = __shfl(base, leader, 64);
+            if (live && k[j] < 0) order[base + lane_rank(out)] = (int)r;
+        }
+        if (live && k[j] >= 0) {
+            const int pos = atomicAdd(&cur[k[j] >> shift], 1);
+            stage[pos] = (int)r;
+            skey[pos] = k[j];
+        }
+    }
+}
+
+__global__ __launch_bounds__(CS_T) void cs_fine_kernel(const int32_t* stage, const int32_t* skey, const int32_t* goff, int nblk,
+                                                       int shift, int32_t* order) {
+    __shared__ int fh[CS_FMAX];
+    __shared__ int part[CS_T];
+    const int cb = blockIdx.x;
+    const int s0 = goff[(int64_t)cb * nblk], s1 = goff[(int64_t)(cb + 1) * nblk];
+    if (s0 == s1) return;
+    const int F = 1 << shift, mask = F - 1;
+    for (int i = threadIdx.x; i < F; i += CS_T) fh[i] = 0;
+    __syncthreads();
+    for (int i0 = s0 + threadIdx.x; i0 < s1; i0 += 4 * CS_T) {
+        int f[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) f[u] = i0 + u * CS_T < s1 ? skey[i0 + u * CS_T] & mask : -1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (f[u] >= 0) atomicAdd(&fh[f[u]], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the F counts: each thread a run of consecutive bins, then the runs' sums
+    const int per = (F + CS_T - 1) / CS_T;
+    const int b0 = min(F, (int)threadIdx.x * per), b1 = min(F, b0 + per);
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) sum += fh[b];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 1; d < CS_T; d <<= 1) {
+        const int v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = s0 + part[threadIdx.x] - sum;
+    for (int b = b0; b < b1; ++b) {
+        const int c = fh[b];
+        fh[b] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (int i0 = s0 + threadIdx.x; i0 < s1; i0 += 4 * CS_T) {
+        int f[4], row[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool in = i0 + u * CS_T < s1;
+            f[u] = in ? skey[i0 + u * CS_T] & mask : -1;
+            row[u] = in ? stage[i0 + u * CS_T] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (f[u] >= 0) order[atomicAdd(&fh[f[u]], 1)] = row[u];
+    }
+}
+
 __global__ void zero_kernel(int32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
+// one-pass sort: key | rank | hist | start | scan temp; two-pass sort: key | stage | skey | ghist | goff | scan temp
 struct SortLayout {
-    size_t key, rank, hist, start, tmp, tmp_bytes, total;
+    size_t key, rank, skey, hist, start, tmp, tmp_bytes, total;
 };
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -486,6 +662,20 @@ inline int sort_keys(int n, int nbins) { return nbins + (n + 63) / 64; }
 SortLayout sort_layout(int n, int nbins) {
     SortLayout L{};
     size_t tmp = 0;
+    if (two_pass_sort(nbins)) {
+        const int nc = coarse_grid(nbins).ncb + 1, nblk = (n + CS_CH - 1) / CS_CH;
+        const size_t nh = (size_t)nc * (size_t)nblk;
+        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, (int)nh);
+        L.key = 0;
+        L.rank = al256(L.key + sizeof(int32_t) * (size_t)n);
+        L.skey = al256(L.rank + sizeof(int32_t) * (size_t)n);
+        L.hist = al256(L.skey + sizeof(int32_t) * (size_t)n);
+        L.start = al256(L.hist + sizeof(int32_t) * nh);
+        L.tmp = al256(L.start + sizeof(int32_t) * nh);
+        L.tmp_bytes = tmp;
+        L.total = al256(L.tmp + tmp);
+        return L;
+    }
     const int nk = sort_keys(n, nbins);
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, nk);
     L.key = 0;
@@ -668,6 +858,21 @@ extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_
     int32_t* hist = reinterpret_cast<int32_t*>(base + L.hist);
     int32_t* start = reinterpret_cast<int32_t*>(base + L.start);
     hipStream_t s = wsb::as_stream(stream);
+    if (two_pass_sort(nbins)) {
+        const CoarseGrid cg = coarse_grid(nbins);
+        const int nc = cg.ncb + 1, nblk = (n + CS_CH - 1) / CS_CH;
+        int32_t* skey = reinterpret_cast<int32_t*>(base + L.skey);
+        hipLaunchKernelGGL(cs_hist_kernel, dim3(nblk), dim3(CS_T), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx, g->ny,
+                           cg.shift, nc, nblk, key, hist);
+        WSB_HIP(hipGetLastError());
+        size_t tmp = L.tmp_bytes;
+        WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, hist, start, nc * nblk, s));
+        hipLaunchKernelGGL(cs_scatter_kernel, dim3(nblk), dim3(CS_T), 0, s, key, n, cg.shift, nc, nblk, start, rank, skey,
+                           order);
+        hipLaunchKernelGGL(cs_fine_kernel, dim3(cg.ncb), dim3(CS_T), 0, s, rank, skey, start, nblk, cg.shift, order);
+        WSB_HIP(hipGetLastError());
+        return WSB_OK;
+    }
     const int nk = sort_keys(n, nbins);
     hipLaunchKernelGGL(zero_kernel, dim3((nk + 255) / 256), dim3(256), 0, s, hist, nk);
     hipLaunchKernelGGL(cell_rank_kernel, dim3((n + 255) / 256), dim3(256), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx,
@@ -675,7 +880,8 @@ extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_
     WSB_HIP(hipGetLastError());
     size_t tmp = L.tmp_bytes;
     WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, hist, start, nk, s));
-    hipLaunchKernelGGL(cell_place_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key, rank, start, n, order);
+    hipLaunchKernelGGL(cell_place_kernel, dim3((n + 256 * PLACE_RPT - 1) / (256 * PLACE_RPT)), dim3(256), 0, s, key, rank,
+                       start, n, order);
     WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
@@ -688,7 +894,23 @@ extern "C" int wsb_adjacency2d(const wsb_geom2d* g, const double* lpos, const do
     WSB_REQUIRE(lpos && lsup && slots && ind, WSB_EINVAL, "null buffer");
     AdjArgs a{lpos, lsup, valid, order, n, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny,
               g->cstart, g->citems, g->K, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb,
-              g->bstart, g->bxy, g->cxy, g->local ? g->cref : nullptr, slots, counts, ind, pad};
+              g->bstart, g->bxy, g->cxy, g->local ? g->cref : nullptr, slots, counts, ind, pad, nullptr};
+    hipLaunchKernelGGL(adjacency2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+extern "C" int wsb_adjacency2d_sorted(const wsb_geom2d* g, const double* lpos, const double* lsup, const uint8_t* valid,
+                                      int n, const int32_t* order, int32_t* slots, int32_t* counts, double* prow, int pad,
+                                      void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
+    WSB_REQUIRE(n >= 0, WSB_EINVAL, "negative row count");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(lpos && lsup && order && slots && counts && prow, WSB_EINVAL, "null buffer");
+    AdjArgs a{lpos, lsup, valid, order, n, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny,
+              g->cstart, g->citems, g->K, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb,
+              g->bstart, g->bxy, g->cxy, g->local ? g->cref : nullptr, slots, counts, nullptr, pad,
+              reinterpret_cast<double4*>(prow)};
     hipLaunchKernelGGL(adjacency2d_kernel, dim3((n + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;

@@ -119,6 +119,7 @@ struct TileArgs {
     const double* rot4;
     int flags;
     double* out;
+    const double4* prow;        // non-null: the sorted-position layout (slots, counts, prow by position; out by row)
 };
 
 // A workgroup owns TILE consecutive rows.  Their pairs are packed densely over the lanes (pair j of the tile
@@ -245,24 +246,56 @@ constexpr int mask_nm() {
     return nm;
 }
 
-// one term m of block b of kernel KID: dIm = dI_m (m <= NMAXP of the pass), dR = rh^m - rl^m (m >= 2)
-template <int KID, int B>
-__device__ __forceinline__ void block_term(int m, double dIm, double dR, bool have_dI, BlockSums& s) {
+// one term m of block b of kernel KID.  INTF (the mixed form): the integer form, every sum SC times its value, with
+// dIm = dI_m and dIq = dI_m / (m + 2) (m <= NMAXP of the pass) and dRq = (rh^m - rl^m) / m (m >= 2); the coefficients
+// PS, FS are integers, so each product takes its coefficient as a literal operand instead of a register (the FP64
+// form keeps the profiles P, QP, F, QF with dR = rh^m - rl^m: its workgroup tile kernel has no registers to spare for
+// the two extra bases)
+template <int KID, int B, bool INTF>
+__device__ __forceinline__ void block_term(int m, double dIm, double dIq, double dR, bool have_dI, BlockSums& s) {
     using T = KTab<KID>;
-    if (have_dI && m < 16) {
-        if (T::P[B][m] != 0.0) s.AP += T::P[B][m] * dIm;
-        if (T::QP[B][m] != 0.0) s.VP += T::QP[B][m] * dIm;
-        if (T::F[B][m] != 0.0) s.AF += T::F[B][m] * dIm;
-        if (T::QF[B][m] != 0.0) s.VF += T::QF[B][m] * dIm;
-    }
-    if (m >= 2 && m - 2 < 16) {
-        if (T::QP[B][m - 2] != 0.0) s.BP += T::QP[B][m - 2] * dR;
-        if (T::QF[B][m - 2] != 0.0) s.BF += T::QF[B][m - 2] * dR;
+    if (INTF) {
+        if (have_dI && m < 16) {
+            if (T::PS[B][m] != 0.0) {
+                s.AP += T::PS[B][m] * dIm;
+                s.VP += T::PS[B][m] * dIq;
+            }
+            if (T::FS[B][m] != 0.0) {
+                s.AF += T::FS[B][m] * dIm;
+                s.VF += T::FS[B][m] * dIq;
+            }
+        }
+        if (m >= 2 && m - 2 < 16) {
+            if (T::PS[B][m - 2] != 0.0) s.BP += T::PS[B][m - 2] * dR;
+            if (T::FS[B][m - 2] != 0.0) s.BF += T::FS[B][m - 2] * dR;
+        }
+    } else {
+        if (have_dI && m < 16) {
+            if (T::P[B][m] != 0.0) s.AP += T::P[B][m] * dIm;
+            if (T::QP[B][m] != 0.0) s.VP += T::QP[B][m] * dIm;
+            if (T::F[B][m] != 0.0) s.AF += T::F[B][m] * dIm;
+            if (T::QF[B][m] != 0.0) s.VF += T::QF[B][m] * dIm;
+        }
+        if (m >= 2 && m - 2 < 16) {
+            if (T::QP[B][m - 2] != 0.0) s.BP += T::QP[B][m - 2] * dR;
+            if (T::QF[B][m - 2] != 0.0) s.BF += T::QF[B][m - 2] * dR;
+        }
     }
 }
 
+// the angle weights of block b of kernel KID and the factor of its row sums (1 / (pi SC) in the integer form)
+template <int KID, bool INTF>
+__device__ __forceinline__ constexpr double mrp(int b) { return INTF ? KTab<KID>::MRPS[b] : KTab<KID>::MRP[b]; }
+template <int KID, bool INTF>
+__device__ __forceinline__ constexpr double mrf(int b) { return INTF ? KTab<KID>::MRFS[b] : KTab<KID>::MRF[b]; }
+template <bool INTF>
+__device__ __forceinline__ constexpr double multi_ips(int k) {
+    if (!INTF) return WSB_INV_PI;
+    return k == 0 ? KTab<WSB_CUBIC>::IPS : k == 1 ? KTab<WSB_W2>::IPS : k == 2 ? KTab<WSB_W4>::IPS : KTab<WSB_W6>::IPS;
+}
+
 // the twelve channels of one kernel from its block sums (as closed::pair_channels): mp = sum over blocks of
-// MRP[b] th_b, mf = sum of MRF[b] th_b (the angle terms; the cubic's two blocks share the sums' accumulators)
+// MRPS[b] th_b, mf = sum of MRFS[b] th_b (the angle terms, integer form; the cubic's two blocks share the sums' accumulators)
 __device__ __forceinline__ void kernel_channels(const BlockSums& s, double mp, double mf, double z, double n0, double n1,
                                                 double tx, double ty, double c[wsb::NCH]) {
     const double lam = z * s.VP - mp;
@@ -322,15 +355,17 @@ __device__ __forceinline__ void radius_sums(double R, bool valid, double z, doub
             dIm2 = dIm1;
             dIm1 = dIm;
         }
-        const double dR = ph - pl;
+        constexpr bool INTF = FORM == closed::MIXED;
+        const double dR = INTF ? (m >= 2 ? (ph - pl) * RK[m] : 0.0) : ph - pl;
+        const double dIq = INTF && m <= NM ? dIm * RK[m + 2] : 0.0;
         const bool have = m <= NM;
         if (SMALL) {
-            block_term<WSB_CUBIC, 1>(m, dIm, dR, have, S[0]);
+            block_term<WSB_CUBIC, 1, INTF>(m, dIm, dIq, dR, have, S[0]);
         } else {
-            if (MASK & 1) block_term<WSB_CUBIC, 0>(m, dIm, dR, have && m <= KTab<WSB_CUBIC>::NMAXP, S[0]);
-            if (MASK & 2) block_term<WSB_W2, 0>(m, dIm, dR, have && m <= KTab<WSB_W2>::NMAXP, S[1]);
-            if (MASK & 4) block_term<WSB_W4, 0>(m, dIm, dR, have && m <= KTab<WSB_W4>::NMAXP, S[2]);
-            if (MASK & 8) block_term<WSB_W6, 0>(m, dIm, dR, have && m <= KTab<WSB_W6>::NMAXP, S[3]);
+            if (MASK & 1) block_term<WSB_CUBIC, 0, INTF>(m, dIm, dIq, dR, have && m <= KTab<WSB_CUBIC>::NMAXP, S[0]);
+            if (MASK & 2) block_term<WSB_W2, 0, INTF>(m, dIm, dIq, dR, have && m <= KTab<WSB_W2>::NMAXP, S[1]);
+            if (MASK & 4) block_term<WSB_W4, 0, INTF>(m, dIm, dIq, dR, have && m <= KTab<WSB_W4>::NMAXP, S[2]);
+            if (MASK & 8) block_term<WSB_W6, 0, INTF>(m, dIm, dIq, dR, have && m <= KTab<WSB_W6>::NMAXP, S[3]);
         }
     }
     const double t = FORM == closed::MIXED ? closed::dangle_mixed(z, lo, hi) : closed::dangle_fp64(z, lo, hi);
@@ -345,11 +380,12 @@ struct MultiArgs {
 
 template <int MASK, int FORM>
 __global__ __launch_bounds__(MTILE_T) __attribute__((amdgpu_waves_per_eu(FORM == closed::MIXED ? 4 : 3))) void tile12_multi_kernel(MultiArgs ma) {
+    constexpr bool INTF = FORM == closed::MIXED;
     const TileArgs& a = ma.t;
     using BlockScan = hipcub::BlockScan<int, MTILE_T>;
     __shared__ typename BlockScan::TempStorage scan_tmp;
     __shared__ int s_off[MTILE_T + 1];
-    __shared__ int s_row[MTILE_R];
+    __shared__ int s_row[MTILE_R], s_src[MTILE_R];
     __shared__ double s_px[MTILE_R], s_py[MTILE_R], s_h[MTILE_R], s_ih[MTILE_R];
     __shared__ double s_res[wsb::NCH][MTILE_T];          // one kernel's pair channels; at the end the row sums
     static_assert(4 * wsb::NCH * MTILE_R <= wsb::NCH * MTILE_T && wsb::NCH == 12 && MTILE_T == 4 * MTILE_R,
@@ -358,12 +394,22 @@ __global__ __launch_bounds__(MTILE_T) __attribute__((amdgpu_waves_per_eu(FORM ==
     const int pos = blockIdx.x * MTILE_R + tid;
     const bool live = tid < MTILE_R && pos < a.n;
     const int row = live ? (a.order ? a.order[pos] : pos) : 0;
-    const int cnt = live ? min(max(a.counts[row], 0), a.K) : 0;
+    const int src = a.prow ? pos : row;              // where the row's adjacency (and position) is read
+    const int cnt = live ? min(max(a.counts[src], 0), a.K) : 0;
     if (live) {
         s_row[tid] = row;
-        s_px[tid] = a.lpos[2 * row];
-        s_py[tid] = a.lpos[2 * row + 1];
-        const double h = a.lsup[row];
+        s_src[tid] = src;
+        double h;
+        if (a.prow) {
+            const double4 pr = a.prow[pos];
+            s_px[tid] = pr.x;
+            s_py[tid] = pr.y;
+            h = pr.z;
+        } else {
+            s_px[tid] = a.lpos[2 * row];
+            s_py[tid] = a.lpos[2 * row + 1];
+            h = a.lsup[row];
+        }
         s_h[tid] = h;
         s_ih[tid] = 1.0 / h;
     }
@@ -395,7 +441,7 @@ __global__ __launch_bounds__(MTILE_T) __attribute__((amdgpu_waves_per_eu(FORM ==
                 else hi = mid;
             }
             const int r = lo;
-            const int e0 = a.slots[(int64_t)s_row[r] * a.K + (jc - s_off[r])];
+            const int e0 = a.slots[(int64_t)s_src[r] * a.K + (jc - s_off[r])];
             const bool ok = j < total && e0 >= 0 && e0 < a.ne;
             const int e = ok ? e0 : 0;
             const double x = s_px[r], y = s_py[r], h = s_h[r], ih = s_ih[r];
@@ -426,11 +472,11 @@ __global__ __launch_bounds__(MTILE_T) __attribute__((amdgpu_waves_per_eu(FORM ==
             if (!(MASK & (1 << k))) continue;
             double c[wsb::NCH];
             if (k == 0)
-                kernel_channels(S[0], KTab<WSB_CUBIC>::MRP[0] * th1 + KTab<WSB_CUBIC>::MRP[1] * th05,
-                                KTab<WSB_CUBIC>::MRF[0] * th1 + KTab<WSB_CUBIC>::MRF[1] * th05, z, n0, n1, tx, ty, c);
-            if (k == 1) kernel_channels(S[1], KTab<WSB_W2>::MRP[0] * th1, KTab<WSB_W2>::MRF[0] * th1, z, n0, n1, tx, ty, c);
-            if (k == 2) kernel_channels(S[2], KTab<WSB_W4>::MRP[0] * th1, KTab<WSB_W4>::MRF[0] * th1, z, n0, n1, tx, ty, c);
-            if (k == 3) kernel_channels(S[3], KTab<WSB_W6>::MRP[0] * th1, KTab<WSB_W6>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+                kernel_channels(S[0], mrp<WSB_CUBIC, INTF>(0) * th1 + mrp<WSB_CUBIC, INTF>(1) * th05,
+                                mrf<WSB_CUBIC, INTF>(0) * th1 + mrf<WSB_CUBIC, INTF>(1) * th05, z, n0, n1, tx, ty, c);
+            if (k == 1) kernel_channels(S[1], mrp<WSB_W2, INTF>(0) * th1, mrf<WSB_W2, INTF>(0) * th1, z, n0, n1, tx, ty, c);
+            if (k == 2) kernel_channels(S[2], mrp<WSB_W4, INTF>(0) * th1, mrf<WSB_W4, INTF>(0) * th1, z, n0, n1, tx, ty, c);
+            if (k == 3) kernel_channels(S[3], mrp<WSB_W6, INTF>(0) * th1, mrf<WSB_W6, INTF>(0) * th1, z, n0, n1, tx, ty, c);
 #pragma unroll
             for (int q = 0; q < wsb::NCH; ++q) s_res[q][tid] = c[q];
             __syncthreads();
@@ -478,10 +524,11 @@ __global__ __launch_bounds__(MTILE_T) __attribute__((amdgpu_waves_per_eu(FORM ==
         const int rr = s_row[r];
         double acc[wsb::NCH];
 #pragma unroll
-        for (int q = 0; q < wsb::NCH; ++q) acc[q] = stage[(k * wsb::NCH + q) * MTILE_R + r] * WSB_INV_PI;
+        for (int q = 0; q < wsb::NCH; ++q) acc[q] = stage[(k * wsb::NCH + q) * MTILE_R + r] * multi_ips<INTF>(k);
         const double* plane = k == 0 ? KTab<WSB_CUBIC>::PLANE : k == 1 ? KTab<WSB_W2>::PLANE : k == 2 ? KTab<WSB_W4>::PLANE
                                                                                                       : KTab<WSB_W6>::PLANE;
-        finalize_row(acc, a.ind[rr], s_h[r], plane, a.rot4, a.flags, ma.out[k] + (int64_t)rr * wsb::NCH);
+        finalize_row(acc, a.prow ? a.prow[s_src[r]].w : a.ind[rr], s_h[r], plane, a.rot4, a.flags,
+                     ma.out[k] + (int64_t)rr * wsb::NCH);
     }
 }
 
@@ -497,17 +544,29 @@ constexpr int RHEAVY_T = 256;           // four wavefronts: two workgroups share
 
 template <int MASK, int FORM>
 __device__ __forceinline__ void serve_row(const MultiArgs& ma, int pos) {
+    constexpr bool INTF = FORM == closed::MIXED;
     const TileArgs& a = ma.t;
     const int row = a.order ? a.order[pos] : pos;
-    const int cnt = min(max(a.counts[row], 0), a.K);
-    const double x = a.lpos[2 * row], y = a.lpos[2 * row + 1], h = a.lsup[row];
+    const int64_t src = a.prow ? pos : row;          // where the row's adjacency (and position) is read
+    const int cnt = min(max(a.counts[src], 0), a.K);
+    double x, y, h;
+    if (a.prow) {
+        const double4 pr = a.prow[pos];
+        x = pr.x;
+        y = pr.y;
+        h = pr.z;
+    } else {
+        x = a.lpos[2 * row];
+        y = a.lpos[2 * row + 1];
+        h = a.lsup[row];
+    }
     const double ih = 1.0 / h;
     double acc[4][wsb::NCH];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int q = 0; q < wsb::NCH; ++q) acc[k][q] = 0.0;
-    const int32_t* sl = a.slots + (int64_t)row * a.K;
+    const int32_t* sl = a.slots + src * a.K;
     // two-stage prefetch: the edge records of slot s + 1 and the edge id of slot s + 2 are requested while slot s is
     // evaluated, so a record load never waits on an id load issued in the same iteration
     int e_next = cnt > 0 ? sl[0] : 0;
@@ -545,34 +604,34 @@ __device__ __forceinline__ void serve_row(const MultiArgs& ma, int pos) {
         double c[wsb::NCH];
         // the single-radius kernels first, so that only the cubic's sums live through its second radius
         if (MASK & 2) {
-            kernel_channels(S[1], KTab<WSB_W2>::MRP[0] * th1, KTab<WSB_W2>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+            kernel_channels(S[1], mrp<WSB_W2, INTF>(0) * th1, mrf<WSB_W2, INTF>(0) * th1, z, n0, n1, tx, ty, c);
 #pragma unroll
             for (int q = 0; q < wsb::NCH; ++q) acc[1][q] += c[q];
         }
         if (MASK & 4) {
-            kernel_channels(S[2], KTab<WSB_W4>::MRP[0] * th1, KTab<WSB_W4>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+            kernel_channels(S[2], mrp<WSB_W4, INTF>(0) * th1, mrf<WSB_W4, INTF>(0) * th1, z, n0, n1, tx, ty, c);
 #pragma unroll
             for (int q = 0; q < wsb::NCH; ++q) acc[2][q] += c[q];
         }
         if (MASK & 8) {
-            kernel_channels(S[3], KTab<WSB_W6>::MRP[0] * th1, KTab<WSB_W6>::MRF[0] * th1, z, n0, n1, tx, ty, c);
+            kernel_channels(S[3], mrp<WSB_W6, INTF>(0) * th1, mrf<WSB_W6, INTF>(0) * th1, z, n0, n1, tx, ty, c);
 #pragma unroll
             for (int q = 0; q < wsb::NCH; ++q) acc[3][q] += c[q];
         }
         if (MASK & 1) {
             radius_sums<MASK, FORM, true>(0.5, valid, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th05, S);
-            kernel_channels(S[0], KTab<WSB_CUBIC>::MRP[0] * th1 + KTab<WSB_CUBIC>::MRP[1] * th05,
-                            KTab<WSB_CUBIC>::MRF[0] * th1 + KTab<WSB_CUBIC>::MRF[1] * th05, z, n0, n1, tx, ty, c);
+            kernel_channels(S[0], mrp<WSB_CUBIC, INTF>(0) * th1 + mrp<WSB_CUBIC, INTF>(1) * th05,
+                            mrf<WSB_CUBIC, INTF>(0) * th1 + mrf<WSB_CUBIC, INTF>(1) * th05, z, n0, n1, tx, ty, c);
 #pragma unroll
             for (int q = 0; q < wsb::NCH; ++q) acc[0][q] += c[q];
         }
     }
-    const double ind = a.ind[row];
+    const double ind = a.prow ? a.prow[pos].w : a.ind[row];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (!(MASK & (1 << k))) continue;
 #pragma unroll
-        for (int q = 0; q < wsb::NCH; ++q) acc[k][q] *= WSB_INV_PI;
+        for (int q = 0; q < wsb::NCH; ++q) acc[k][q] *= multi_ips<INTF>(k);
         const double* plane = k == 0 ? KTab<WSB_CUBIC>::PLANE : k == 1 ? KTab<WSB_W2>::PLANE : k == 2 ? KTab<WSB_W4>::PLANE
                                                                                                       : KTab<WSB_W6>::PLANE;
         finalize_row(acc[k], ind, h, plane, a.rot4, a.flags, ma.out[k] + (int64_t)row * wsb::NCH);
@@ -821,8 +880,31 @@ extern "C" int wsb_moments2d_multi(const wsb_geom2d* g, int form, const int32_t*
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(slots && counts && ind && lpos && lsup, WSB_EINVAL, "null buffer");
     WSB_REQUIRE(!nwork || order, WSB_EINVAL, "nwork needs the order it counts (wsb_count_order2d)");
-    a.t = TileArgs{order, slots, counts, K, ind, lpos, lsup, n, g->erec, g->etan, g->ne, rot4, flags, nullptr};
+    a.t = TileArgs{order, slots, counts, K, ind, lpos, lsup, n, g->erec, g->etan, g->ne, rot4, flags, nullptr, nullptr};
     a.nwork = nwork;
+    hipStream_t s = wsb::as_stream(stream);
+    return form == WSB_FORM_FP64 ? dispatch_multi<closed::FP64>(mask, a, s) : dispatch_multi<closed::MIXED>(mask, a, s);
+}
+
+extern "C" int wsb_moments2d_multi_sorted(const wsb_geom2d* g, int form, const int32_t* order, const int32_t* slots,
+                                          const int32_t* counts, int K, const double* prow, int n, const double* rot4,
+                                          int flags, double* const* outs, void* stream) {
+    WSB_REQUIRE(g && outs, WSB_EINVAL, "null argument");
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes n=%d K=%d", n, K);
+    WSB_REQUIRE(!(flags & WSB_OUT_WORLD) || rot4, WSB_EINVAL, "WSB_OUT_WORLD needs rot4");
+    WSB_REQUIRE(form == WSB_FORM_FP64 || form == WSB_FORM_MIXED, WSB_ENOTSUP, "several kernels: the fp64 and mixed forms");
+    int mask = 0;
+    MultiArgs a{};
+    for (int k = 0; k < 4; ++k) {
+        a.out[k] = outs[k];
+        if (outs[k]) mask |= 1 << k;
+    }
+    WSB_REQUIRE(mask != 0, WSB_EINVAL, "no kernel requested");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(order && slots && counts && prow, WSB_EINVAL, "null buffer");
+    a.t = TileArgs{order, slots, counts, K, nullptr, nullptr, nullptr, n, g->erec, g->etan, g->ne, rot4, flags, nullptr,
+                   reinterpret_cast<const double4*>(prow)};
+    a.nwork = nullptr;
     hipStream_t s = wsb::as_stream(stream);
     return form == WSB_FORM_FP64 ? dispatch_multi<closed::FP64>(mask, a, s) : dispatch_multi<closed::MIXED>(mask, a, s);
 }

@@ -167,6 +167,57 @@ class Geom2D:
                                              L.stream_ptr(dev)))
         return res
 
+    def adjacency_sorted(self, lpos, lsup, order, valid=None, pad=False, slots=None, counts=None, prow=None):
+        """(slots [N*K] int32, counts [N] int32, prow [N, 4] f64) in the sorted-position layout: position i holds the
+        adjacency of row order[i] (``order`` from ``sort``) and prow[i] = (x, y, h, indicator) of that row; the same
+        values as ``adjacency`` (``wsb_adjacency2d_sorted``)."""
+        L.require_device(lpos, lsup, order)
+        n = lpos.shape[0]
+        dev = lpos.device
+        lpos = lpos.to(torch.float64).contiguous()
+        lsup = lsup.to(torch.float64).contiguous()
+        _require_buffer(order, "order", torch.int32, n, dev)
+        if slots is None:
+            slots = torch.empty(n * self.K, dtype=torch.int32, device=dev)
+        if counts is None:
+            counts = torch.empty(n, dtype=torch.int32, device=dev)
+        if prow is None:
+            prow = torch.empty((n, 4), dtype=torch.float64, device=dev)
+        _require_buffer(slots, "slots", torch.int32, n * self.K, dev)
+        _require_buffer(counts, "counts", torch.int32, n, dev)
+        _require_buffer(prow, "prow", torch.float64, 4 * n, dev)
+        v = None if valid is None else valid.to(torch.uint8).contiguous()
+        L.check(L.load().wsb_adjacency2d_sorted(self.handle, L.ptr(lpos), L.ptr(lsup), L.ptr(v), n, L.ptr(order),
+                                                L.ptr(slots), L.ptr(counts), L.ptr(prow), 1 if pad else 0,
+                                                L.stream_ptr(dev)))
+        return slots, counts, prow
+
+    def moments_multi_sorted(self, kernels, order, slots, counts, prow, form="fp64", rot4=None, flags=0, outs=None):
+        """{kernel: [N, 12]} in input order from the sorted-position layout of ``adjacency_sorted``: the values of
+        ``moments_multi`` (``wsb_moments2d_multi_sorted``), each row's record read contiguously in cell order."""
+        L.require_device(order, slots, counts, prow)
+        n = order.shape[0]
+        dev = order.device
+        _require_buffer(slots, "slots", torch.int32, n * self.K, dev)
+        _require_buffer(counts, "counts", torch.int32, n, dev)
+        _require_buffer(prow, "prow", torch.float64, 4 * n, dev)
+        ptrs = (ctypes.c_void_p * 4)()
+        res = {}
+        for k in kernels:
+            kid = KERNEL_ID[k]
+            if kid > 3:
+                raise NotImplementedError(f"{k}: several kernels in one pass are the scheme kernels cubic, w2, w4, w6")
+            o = outs[k] if outs is not None and k in outs else (
+                torch.zeros((n, 12), dtype=torch.float64, device=dev) if flags & L.OUT_ACCUMULATE
+                else torch.empty((n, 12), dtype=torch.float64, device=dev))
+            _require_buffer(o, f"outs[{k}]", torch.float64, 12 * n, dev)
+            res[k] = o
+            ptrs[kid] = o.data_ptr()
+        L.check(L.load().wsb_moments2d_multi_sorted(self.handle, L.form_id(form), L.ptr(order), L.ptr(slots),
+                                                    L.ptr(counts), self.K, L.ptr(prow), n, L.ptr(rot4), int(flags), ptrs,
+                                                    L.stream_ptr(dev)))
+        return res
+
     def signed_distance(self, lpos, support_max=None):
         L.require_device(lpos)
         m = lpos.shape[0]
