@@ -120,7 +120,9 @@ int wsb_edge_channels2d(const wsb_plan* plan, int kernel_id, int form, const int
                         const double* verts, const int32_t* edges, int ne, double* out, void* stream);
 
 /* ---- signed distance to a surface representation (reference scene/scene.py:1312-1372): d > 0 in the
- * fluid, nrm the unit normal into the fluid, both in the local frame; edge index of the closest edge. */
+ * fluid, nrm the unit normal into the fluid, both in the local frame; edge index of the closest edge (the lowest id
+ * among equal distances, as the reference's argmin).  The first call on a geometry builds its nearest-edge candidate
+ * grid on the device (it allocates and waits for the build); later calls allocate nothing. */
 int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max,
                           double* d, double* nrm, int32_t* edge, void* stream);
 

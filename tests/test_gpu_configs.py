@@ -197,13 +197,14 @@ def _nearest_edges(V, E, pts, chunk=1024):
 
 def test_config2_signed_distance_edge_tree(star_subsample):
     """signed distance on the 4,096-edge star at h = 0.004 for the 65,536-particle subsample (70 % of it farther than
-    one support from every edge, where the edge tree finds the nearest edge) and 8,192 points over a box twice the
-    domain (inside the solid, and outside the cell grid): the distance within 1e-14 of a scan over all edges, the
-    scan's edge wherever its nearest edge is unique by more than 1e-12, the sign from the even-odd rule, and the
-    normal of the reference's rule."""
+    one support from every edge, where the nearest-edge candidate grid finds the nearest edge), 8,192 points over a
+    box twice the domain (inside the solid, and outside the cell grid) and 2,048 points over a box seven times the
+    domain (most of them outside the candidate grid: the edge tree): the distance within 1e-14 of a scan over all
+    edges, the scan's edge wherever its nearest edge is unique by more than 1e-12, the sign from the even-odd rule,
+    and the normal of the reference's rule."""
     w, rep, _ = star_subsample
     g = WL.rng(31)
-    pts = np.concatenate([w.positions, g.uniform(-0.5, 1.5, (8192, 2))])
+    pts = np.concatenate([w.positions, g.uniform(-0.5, 1.5, (8192, 2)), g.uniform(-3.0, 4.0, (2048, 2))])
     geo = rep.geom(w.h, DEV)
     d, n, edge = geo.signed_distance(torch.tensor(pts, device=DEV), w.h)
     d, n, edge = d.cpu().numpy(), n.cpu().numpy(), edge.cpu().numpy()

@@ -333,6 +333,8 @@ struct SdArgs {
     int nb;
     const int32_t* bstart;
     const double4* bxy;
+    const double4* cxy;
+    const double4* cref;        // nullptr: no local winding
     const double4* tbox;
     const int2* tlink;
     const int32_t* titems;
@@ -340,7 +342,23 @@ struct SdArgs {
     double* d;
     double* nrm;
     int32_t* edge;
+    int ng;                     // nearest-edge candidate grid (0: none)
+    double nlo0, nlo1, ncs;
+    const int32_t* nstart;
+    const int32_t* nitems;
 };
+
+// distance from (px, py) to the segment (ax, ay) - (bx, by) and its closest point
+__device__ inline double seg_dist_xy(double px, double py, double ax, double ay, double bx, double by, double* cpx,
+                                     double* cpy) {
+    const double ex = bx - ax, ey = by - ay;
+    double t = ((px - ax) * ex + (py - ay) * ey) / (ex * ex + ey * ey);
+    t = fmin(fmax(t, 0.0), 1.0);
+    *cpx = ax + t * ex;
+    *cpy = ay + t * ey;
+    const double dx = px - *cpx, dy = py - *cpy;
+    return sqrt(dx * dx + dy * dy);
+}
 
 __device__ inline double seg_dist(double px, double py, const double* V, const int32_t* E, int e, double* cpx, double* cpy) {
     const double ax = V[2 * E[2 * e]], ay = V[2 * E[2 * e] + 1];
@@ -363,8 +381,21 @@ __device__ inline double box_dist(double px, double py, double4 b) {
 // the nearest edge over the tree (the lowest edge id among equal distances), starting from the candidate (best, k):
 // a subtree is skipped only when its box is farther than best by more than the rounding of seg_dist can account for
 // (slack), so every edge whose computed distance could equal or beat best is evaluated
-__device__ inline void tree_nearest(const SdArgs& a, double px, double py, double& best, int& k, double& bx, double& by) {
-    const double slack = 256.0 * 1.1102230246251565e-16 * (fabs(px) + fabs(py) + a.extent);
+struct TreeView {
+    const double* V;
+    const int32_t* E;
+    const double4* tbox;
+    const int2* tlink;
+    const int32_t* titems;
+    double extent;
+};
+
+__device__ inline double tree_slack(double px, double py, double extent) {
+    return 256.0 * 1.1102230246251565e-16 * (fabs(px) + fabs(py) + extent);
+}
+
+__device__ inline void tree_nearest(const TreeView& a, double px, double py, double& best, int& k, double& bx, double& by) {
+    const double slack = tree_slack(px, py, a.extent);
     int stack[kTreeStack];
     int sp = 0, node = 0;
     for (;;) {
@@ -401,6 +432,67 @@ __device__ inline void tree_nearest(const SdArgs& a, double px, double py, doubl
     }
 }
 
+// the edges whose computed distance from (px, py) is at most r: their number, the first `cap` of them written to out
+// (tree order; out nullable)
+__device__ inline int tree_within(const TreeView& a, double px, double py, double r, int32_t* out, int cap) {
+    const double slack = tree_slack(px, py, a.extent);
+    int stack[kTreeStack];
+    int sp = 0, node = 0, n = 0;
+    for (;;) {
+        const int2 l = a.tlink[node];
+        if (l.x < 0) {
+            const int first = -l.x - 1;
+            for (int i = first; i < first + l.y; ++i) {
+                const int e = a.titems[i];
+                double qx, qy;
+                if (seg_dist(px, py, a.V, a.E, e, &qx, &qy) <= r) {
+                    if (out && n < cap) out[n] = e;
+                    ++n;
+                }
+            }
+        } else {
+            const int lc = node + 1, rc = l.x;
+            const bool gl = box_dist(px, py, a.tbox[lc]) <= r + slack, gr = box_dist(px, py, a.tbox[rc]) <= r + slack;
+            if (gl && gr) {
+                stack[sp++] = rc;
+                node = lc;
+                continue;
+            }
+            if (gl || gr) {
+                node = gl ? lc : rc;
+                continue;
+            }
+        }
+        if (sp == 0) return n;
+        node = stack[--sp];
+    }
+}
+
+// Nearest-edge candidates of one grid cell (one thread per cell): from the cell's centre c, d_c = the distance of the
+// nearest edge; every point p of the cell (half-diagonal hd) has its nearest edge e* at |p - e*| <= |p - e_c| <= d_c
+// + hd, so |c - e*| <= d_c + 2 hd: the edges within d_c + 2 hd (+ the rounding of the distances, four times the tree's
+// slack) hold every edge that can be nearest, ties included.  Pass 1 (out == nullptr) counts them; pass 2 writes the
+// lists of the cells whose count fits NC_CAP (the others keep an empty list: their points search the tree).
+constexpr int NC_CAP = 256;
+constexpr int NC_G = 256;
+__global__ __launch_bounds__(64) void ncand_kernel(TreeView t, double lo0, double lo1, double cs, int G, int32_t* counts,
+                                                   const int32_t* start, int32_t* items) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G * G) return;
+    const double cx = lo0 + ((double)(i / G) + 0.5) * cs, cy = lo1 + ((double)(i % G) + 0.5) * cs;
+    const double hd = 0.7071067811865476 * cs * (1.0 + 1e-12);
+    double best = INFINITY, bx, by;
+    int k = -1;
+    tree_nearest(t, cx, cy, best, k, bx, by);
+    const double r = best + 2.0 * hd + 4.0 * tree_slack(fabs(cx) + hd, fabs(cy) + hd, t.extent);
+    if (!items) {
+        counts[i] = tree_within(t, cx, cy, r, nullptr, 0);
+    } else {
+        const int n = start[i + 1] - start[i];
+        if (n > 0) tree_within(t, cx, cy, r, items + start[i], n);
+    }
+}
+
 // nearest edge, sign from the winding number at support normalisation `smax` (reference scene/scene.py:1312-1372
 // with the sync-free indicator of scene/fixedadj.py:273-276); the cell list gives the exact nearest edge whenever
 // it is closer than smax, else the edge tree is searched from the cell list's candidate
@@ -414,21 +506,54 @@ __global__ __launch_bounds__(256) void signed_distance2d_kernel(SdArgs a) {
     const int cid = inside ? (int)fx * a.ny + (int)fy : 0;
     double best = INFINITY, bx = 0, by = 0;
     int k = -1;
+    double ind = a.bg;
     if (inside) {
+        // the cell's items (ascending edge id, the endpoints in their records): the nearest edge among them, and
+        // the winding number by the local crossings from the cell's reference point (as the adjacency)
+        const bool empty = a.cempty[cid] != 0;
+        double4 q = double4{0.0, 0.0, 0.0, 0.0};
+        if (!empty && a.cref) q = a.cref[cid];
+        const bool local = q.w != 0.0;
+        int dw = 0;
+        bool unsure = false;
         for (int i = a.cstart[cid]; i < a.cstart[cid + 1]; ++i) {
+            const double4 rec = a.cxy[i];
+            if (local) local_winding_edge(rec.x, rec.y, rec.z, rec.w, px, py, q.x, q.y, dw, unsure);
             double qx, qy;
-            const int e = a.citems[i];
-            const double dd = seg_dist(px, py, a.V, a.E, e, &qx, &qy);
-            if (dd < best) { best = dd; k = e; bx = qx; by = qy; }
+            const double dd = seg_dist_xy(px, py, rec.x, rec.y, rec.z, rec.w, &qx, &qy);
+            if (dd < best) {
+                best = dd;
+                k = a.citems[i];
+                bx = qx;
+                by = qy;
+            }
         }
+        if (empty) ind = a.cind[cid];
+        else if (local && !unsure) ind = q.z + (double)dw;
+        else ind = winding_point(px, py, a.smax, a.V, a.E, a.ne, a.bstart, a.bxy, a.y0, a.dy, a.nb, a.bg);
     }
-    if (!(best < a.cell)) tree_nearest(a, px, py, best, k, bx, by);     // cell = the support the list was inflated by
-    double ind;
-    if (inside) {
-        ind = a.cempty[cid] ? a.cind[cid]
-                            : winding_point(px, py, a.smax, a.V, a.E, a.ne, a.bstart, a.bxy, a.y0, a.dy, a.nb, a.bg);
-    } else {
-        ind = a.bg;
+    if (!(best < a.cell)) {                 // cell = the support the list was inflated by
+        // the candidates of the point's cell in the nearest-edge grid, else the edge tree
+        const double gx = floor((px - a.nlo0) / a.ncs), gy = floor((py - a.nlo1) / a.ncs);
+        const int G = a.ng;
+        const bool in_grid = G > 0 && gx >= 0.0 && gx < (double)G && gy >= 0.0 && gy < (double)G;
+        const int gc = in_grid ? (int)gx * G + (int)gy : 0;
+        const int j0 = in_grid ? a.nstart[gc] : 0, j1 = in_grid ? a.nstart[gc + 1] : 0;
+        if (j1 > j0) {
+            for (int j = j0; j < j1; ++j) {
+                const int e = a.nitems[j];
+                double qx, qy;
+                const double dd = seg_dist(px, py, a.V, a.E, e, &qx, &qy);
+                if (dd < best || (dd == best && e < k)) {
+                    best = dd;
+                    k = e;
+                    bx = qx;
+                    by = qy;
+                }
+            }
+        } else {
+            tree_nearest(TreeView{a.V, a.E, a.tbox, a.tlink, a.titems, a.extent}, px, py, best, k, bx, by);
+        }
     }
     const bool ins = ind > 0.5;
     a.d[r] = ins ? -best : best;
@@ -810,7 +935,8 @@ extern "C" int wsb_geom2d_destroy(wsb_geom2d* g) {
     if (!g) return WSB_OK;
     for (void* p : {(void*)g->verts, (void*)g->edges, (void*)g->cstart, (void*)g->citems, (void*)g->cind,
                     (void*)g->cempty, (void*)g->bstart, (void*)g->bitems, (void*)g->cxy, (void*)g->bxy, (void*)g->erec,
-                    (void*)g->etan, (void*)g->cref, (void*)g->tbox, (void*)g->tlink, (void*)g->titems})
+                    (void*)g->etan, (void*)g->cref, (void*)g->tbox, (void*)g->tlink, (void*)g->titems,
+                    (void*)g->nstart, (void*)g->nitems})
         if (p) (void)hipFree(p);
     delete g;
     return WSB_OK;
@@ -1085,6 +1211,61 @@ extern "C" int wsb_dir_extreme2d(const wsb_geom2d* g, const int32_t* slots, cons
     return WSB_OK;
 }
 
+// the nearest-edge candidate grid of a geometry (its first signed-distance call: allocates, launches two passes of
+// ncand_kernel on the null stream and waits for them)
+static int build_nearest_grid(wsb_geom2d* g) {
+    std::vector<double> V(2 * (size_t)g->nv);
+    WSB_HIP(hipMemcpy(V.data(), g->verts, V.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double lo0 = V[0], lo1 = V[1], hi0 = V[0], hi1 = V[1];
+    for (int i = 0; i < g->nv; ++i) {
+        lo0 = std::min(lo0, V[2 * i]);
+        hi0 = std::max(hi0, V[2 * i]);
+        lo1 = std::min(lo1, V[2 * i + 1]);
+        hi1 = std::max(hi1, V[2 * i + 1]);
+    }
+    double ext = std::max(hi0 - lo0, hi1 - lo1);
+    if (!(ext > 0.0)) ext = 1.0;
+    const int G = NC_G;
+    const double cs = 3.0 * ext / G;
+    const double nlo0 = 0.5 * (lo0 + hi0) - 1.5 * ext, nlo1 = 0.5 * (lo1 + hi1) - 1.5 * ext;
+    const TreeView t{g->verts, g->edges, g->tbox, g->tlink, g->titems, g->extent};
+    const int ncell = G * G;
+    int32_t* counts = nullptr;
+    WSB_HIP(hipMalloc(&counts, sizeof(int32_t) * (size_t)ncell));
+    hipLaunchKernelGGL(ncand_kernel, dim3((ncell + 63) / 64), dim3(64), 0, nullptr, t, nlo0, nlo1, cs, G, counts,
+                       (const int32_t*)nullptr, (int32_t*)nullptr);
+    std::vector<int32_t> cnt(ncell), start(ncell + 1, 0);
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpy(cnt.data(), counts, sizeof(int32_t) * (size_t)ncell, hipMemcpyDeviceToHost);
+    (void)hipFree(counts);
+    WSB_HIP(err);
+    for (int i = 0; i < ncell; ++i) start[i + 1] = start[i] + (cnt[i] <= NC_CAP ? cnt[i] : 0);
+    int32_t* nstart = nullptr;
+    int32_t* nitems = nullptr;
+    WSB_HIP(hipMalloc(&nstart, sizeof(int32_t) * (size_t)(ncell + 1)));
+    err = hipMalloc(&nitems, sizeof(int32_t) * (size_t)std::max(start[ncell], 1));
+    if (err == hipSuccess) err = hipMemcpy(nstart, start.data(), sizeof(int32_t) * (size_t)(ncell + 1), hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(ncand_kernel, dim3((ncell + 63) / 64), dim3(64), 0, nullptr, t, nlo0, nlo1, cs, G,
+                           (int32_t*)nullptr, (const int32_t*)nstart, nitems);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(nullptr);
+    if (err != hipSuccess) {
+        (void)hipFree(nstart);
+        if (nitems) (void)hipFree(nitems);
+        WSB_HIP(err);
+    }
+    g->ng = G;
+    g->nlo0 = nlo0;
+    g->nlo1 = nlo1;
+    g->ncs = cs;
+    g->nitems = nitems;
+    g->nstart = nstart;
+    g->bytes += (int64_t)(sizeof(int32_t) * ((size_t)ncell + 1 + (size_t)std::max(start[ncell], 1)));
+    return WSB_OK;
+}
+
 extern "C" int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, int m, double support_max, double* d,
                                      double* nrm, int32_t* edge, void* stream) {
     WSB_REQUIRE(g, WSB_EINVAL, "null geometry");
@@ -1092,9 +1273,14 @@ extern "C" int wsb_signed_distance2d(const wsb_geom2d* g, const double* lpos, in
     if (m == 0) return WSB_OK;
     WSB_REQUIRE(lpos && d && nrm, WSB_EINVAL, "null buffer");
     WSB_REQUIRE(support_max > 0, WSB_EINVAL, "support_max must be positive");
+    if (!g->nstart) {
+        const int st = build_nearest_grid(const_cast<wsb_geom2d*>(g));
+        if (st != WSB_OK) return st;
+    }
     SdArgs a{lpos, m, support_max, g->verts, g->edges, g->ne, g->clo0, g->clo1, g->cell, g->nx, g->ny, g->cstart,
              g->citems, g->cind, g->cempty, (double)g->background, g->y0, g->dy, g->nb, g->bstart, g->bxy,
-             g->tbox, g->tlink, g->titems, g->extent, d, nrm, edge};
+             g->cxy, g->local ? g->cref : nullptr, g->tbox, g->tlink, g->titems, g->extent, d, nrm, edge,
+             g->ng, g->nlo0, g->nlo1, g->ncs, g->nstart, g->nitems};
     hipLaunchKernelGGL(signed_distance2d_kernel, dim3((m + 255) / 256), dim3(256), 0, wsb::as_stream(stream), a);
     WSB_HIP(hipGetLastError());
     return WSB_OK;

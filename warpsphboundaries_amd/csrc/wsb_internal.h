@@ -79,6 +79,13 @@ struct wsb_geom2d {
     double extent = 0;
     int local = 0;
     int64_t noref = 0;            // cells with edges and no reference point
+    // nearest-edge candidate grid (built on the device by the first signed-distance call): ng x ng cells over the
+    // vertex box widened by its larger side on every side; per cell the edges that can be nearest to a point of the
+    // cell (CSR, edge ids); an empty list sends the cell's points to the edge tree
+    int ng = 0;
+    double nlo0 = 0, nlo1 = 0, ncs = 0;
+    int32_t* nstart = nullptr;    // [ng*ng+1]
+    int32_t* nitems = nullptr;    // [nstart[ng*ng]]
 };
 
 struct wsb_plan {
