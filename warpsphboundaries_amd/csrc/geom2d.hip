@@ -12,6 +12,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "cellsort.h"
 #include "geom2d_device.h"
 #include "wsb_internal.h"
 
@@ -619,36 +620,8 @@ __global__ __launch_bounds__(256) void cell_place_kernel(const int32_t* key, con
     for (int j = 0; j < PLACE_RPT; ++j)
         if (r0 + 256 * j < n) order[st[j] + rk[j]] = (int)(r0 + 256 * j);
 }
-// Two-pass counting sort by cell without global atomics (grids of at most CS_NC - 1 coarse bins of CS_FMAX cells).
-// Coarse bin = cell >> shift, plus one bin for the rows outside the grid.  Pass 1: per chunk of CS_CH rows an LDS
-// histogram of the coarse bins, written column by column (bin-major) for one exclusive scan.  Pass 2: per chunk, each
-// row takes the next place of its coarse bin from an LDS cursor; a row outside the grid goes straight to its final
-// place (the rows outside a 64-row group stay in input order, each group's block placed by one LDS atomic), a row
-// inside to a staging list grouped by coarse bin.  Pass 3: one workgroup per coarse bin counts its rows per cell in
-// LDS, scans and places them.  Which thread serves a row inside a cell depends on the LDS atomics' timing, never a
-// result.
-constexpr int CS_T = 256;
-constexpr int CS_RPT = 16;
-constexpr int CS_CH = CS_T * CS_RPT;
-constexpr int CS_NC = 1024;
-constexpr int CS_FMAX = 4096;
-
-struct CoarseGrid {
-    int shift, ncb;             // ncb coarse bins inside the grid; the rows outside it take bin ncb
-};
-
-inline CoarseGrid coarse_grid(int nbins) {
-    int sh = 0;
-    while (((int64_t)nbins + (1 << sh) - 1) >> sh > CS_NC - 1) ++sh;
-    return {sh, (int)(((int64_t)nbins + (1 << sh) - 1) >> sh)};
-}
-
-inline bool two_pass_sort(int nbins) { return (1 << coarse_grid(nbins).shift) <= CS_FMAX; }
-
-__device__ __forceinline__ int lane_rank(uint64_t mask) {
-    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
+// The 2-D key pass of the two-pass cell sort (cellsort.h): the cell key of every row computed from its position
+// and the chunk's LDS histogram of coarse bins in one pass.
 __global__ __launch_bounds__(CS_T) void cs_hist_kernel(const double* lpos, int n, double clo0, double clo1, double cell, int nx,
                                                        int ny, int shift, int nc, int nblk, int32_t* key, int32_t* ghist) {
     __shared__ int h[CS_NC];
@@ -682,101 +655,15 @@ __global__ __launch_bounds__(CS_T) void cs_hist_kernel(const double* lpos, int n
     for (int i = threadIdx.x; i < nc; i += CS_T) ghist[(int64_t)i * nblk + blockIdx.x] = h[i];
 }
 
-__global__ __launch_bounds__(CS_T) void cs_scatter_kernel(const int32_t* key, int n, int shift, int nc, int nblk,
-                                                          const int32_t* goff, int32_t* stage, int32_t* skey,
-                                                          int32_t* order) {
-    __shared__ int cur[CS_NC];
-    for (int i = threadIdx.x; i < nc; i += CS_T) cur[i] = goff[(int64_t)i * nblk + blockIdx.x];
-    const int64_t c0 = (int64_t)blockIdx.x * CS_CH;
-    const int lane = threadIdx.x & 63;
-    int k[CS_RPT];
-#pragma unroll
-    for (int j = 0; j < CS_RPT; ++j) {
-        const int64_t r = c0 + j * CS_T + threadIdx.x;
-        k[j] = r < n ? key[r] : 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < CS_RPT; ++j) {
-        const int64_t r = c0 + j * CS_T + threadIdx.x;
-        const bool live = r < n;
-        const uint64_t out = __ballot(live && k[j] < 0);
-        if (out) {
-            const int leader = __builtin_ctzll(out);
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&cur[nc - 1], __popcll(out));
-            base = __shfl(base, leader, 64);
-            if (live && k[j] < 0) order[base + lane_rank(out)] = (int)r;
-        }
-        if (live && k[j] >= 0) {
-            const int pos = atomicAdd(&cur[k[j] >> shift], 1);
-            stage[pos] = (int)r;
-            skey[pos] = k[j];
-        }
-    }
-}
-
-__global__ __launch_bounds__(CS_T) void cs_fine_kernel(const int32_t* stage, const int32_t* skey, const int32_t* goff, int nblk,
-                                                       int shift, int32_t* order) {
-    __shared__ int fh[CS_FMAX];
-    __shared__ int part[CS_T];
-    const int cb = blockIdx.x;
-    const int s0 = goff[(int64_t)cb * nblk], s1 = goff[(int64_t)(cb + 1) * nblk];
-    if (s0 == s1) return;
-    const int F = 1 << shift, mask = F - 1;
-    for (int i = threadIdx.x; i < F; i += CS_T) fh[i] = 0;
-    __syncthreads();
-    for (int i0 = s0 + threadIdx.x; i0 < s1; i0 += 4 * CS_T) {
-        int f[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) f[u] = i0 + u * CS_T < s1 ? skey[i0 + u * CS_T] & mask : -1;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (f[u] >= 0) atomicAdd(&fh[f[u]], 1);
-    }
-    __syncthreads();
-    // exclusive scan of the F counts: each thread a run of consecutive bins, then the runs' sums
-    const int per = (F + CS_T - 1) / CS_T;
-    const int b0 = min(F, (int)threadIdx.x * per), b1 = min(F, b0 + per);
-    int sum = 0;
-    for (int b = b0; b < b1; ++b) sum += fh[b];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (int d = 1; d < CS_T; d <<= 1) {
-        const int v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int run = s0 + part[threadIdx.x] - sum;
-    for (int b = b0; b < b1; ++b) {
-        const int c = fh[b];
-        fh[b] = run;
-        run += c;
-    }
-    __syncthreads();
-    for (int i0 = s0 + threadIdx.x; i0 < s1; i0 += 4 * CS_T) {
-        int f[4], row[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const bool in = i0 + u * CS_T < s1;
-            f[u] = in ? skey[i0 + u * CS_T] & mask : -1;
-            row[u] = in ? stage[i0 + u * CS_T] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (f[u] >= 0) order[atomicAdd(&fh[f[u]], 1)] = row[u];
-    }
-}
 
 __global__ void zero_kernel(int32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
-// one-pass sort: key | rank | hist | start | scan temp; two-pass sort: key | stage | skey | ghist | goff | scan temp
+// one-pass sort: key | rank | hist | start | scan temp (the two-pass sort: cellsort.h's layout)
 struct SortLayout {
-    size_t key, rank, skey, hist, start, tmp, tmp_bytes, total;
+    size_t key, rank, hist, start, tmp, tmp_bytes, total;
 };
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -787,20 +674,6 @@ inline int sort_keys(int n, int nbins) { return nbins + (n + 63) / 64; }
 SortLayout sort_layout(int n, int nbins) {
     SortLayout L{};
     size_t tmp = 0;
-    if (two_pass_sort(nbins)) {
-        const int nc = coarse_grid(nbins).ncb + 1, nblk = (n + CS_CH - 1) / CS_CH;
-        const size_t nh = (size_t)nc * (size_t)nblk;
-        (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, (int)nh);
-        L.key = 0;
-        L.rank = al256(L.key + sizeof(int32_t) * (size_t)n);
-        L.skey = al256(L.rank + sizeof(int32_t) * (size_t)n);
-        L.hist = al256(L.skey + sizeof(int32_t) * (size_t)n);
-        L.start = al256(L.hist + sizeof(int32_t) * nh);
-        L.tmp = al256(L.start + sizeof(int32_t) * nh);
-        L.tmp_bytes = tmp;
-        L.total = al256(L.tmp + tmp);
-        return L;
-    }
     const int nk = sort_keys(n, nbins);
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (const int32_t*)nullptr, (int32_t*)nullptr, nk);
     L.key = 0;
@@ -966,7 +839,8 @@ extern "C" int wsb_geom2d_arrays(const wsb_geom2d* g, const double** verts, cons
 
 extern "C" size_t wsb_sort2d_scratch_bytes(const wsb_geom2d* g, int n) {
     if (!g || n <= 0) return 0;
-    return sort_layout(n, g->nx * g->ny).total;
+    const int nbins = g->nx * g->ny;
+    return two_pass_sort(nbins) ? cs_layout(n, nbins).total : sort_layout(n, nbins).total;
 }
 
 extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_t* order, void* scratch, size_t scratch_bytes,
@@ -976,29 +850,26 @@ extern "C" int wsb_sort2d(const wsb_geom2d* g, const double* lpos, int n, int32_
     if (n == 0) return WSB_OK;
     WSB_REQUIRE(lpos && order, WSB_EINVAL, "null buffer");
     const int nbins = g->nx * g->ny;
+    char* base = static_cast<char*>(scratch);
+    hipStream_t s = wsb::as_stream(stream);
+    if (two_pass_sort(nbins)) {
+        const CsLayout C = cs_layout(n, nbins);
+        WSB_REQUIRE(scratch && scratch_bytes >= C.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", C.total,
+                    scratch_bytes);
+        const CoarseGrid cg = coarse_grid(nbins);
+        const int nblk = (n + CS_CH - 1) / CS_CH;
+        hipLaunchKernelGGL(cs_hist_kernel, dim3(nblk), dim3(CS_T), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx, g->ny,
+                           cg.shift, cg.ncb + 1, nblk, reinterpret_cast<int32_t*>(base + C.key),
+                           reinterpret_cast<int32_t*>(base + C.ghist));
+        WSB_HIP(hipGetLastError());
+        return cs_place(C, base, n, nbins, order, s);
+    }
     const SortLayout L = sort_layout(n, nbins);
     WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", L.total, scratch_bytes);
-    char* base = static_cast<char*>(scratch);
     int32_t* key = reinterpret_cast<int32_t*>(base + L.key);
     int32_t* rank = reinterpret_cast<int32_t*>(base + L.rank);
     int32_t* hist = reinterpret_cast<int32_t*>(base + L.hist);
     int32_t* start = reinterpret_cast<int32_t*>(base + L.start);
-    hipStream_t s = wsb::as_stream(stream);
-    if (two_pass_sort(nbins)) {
-        const CoarseGrid cg = coarse_grid(nbins);
-        const int nc = cg.ncb + 1, nblk = (n + CS_CH - 1) / CS_CH;
-        int32_t* skey = reinterpret_cast<int32_t*>(base + L.skey);
-        hipLaunchKernelGGL(cs_hist_kernel, dim3(nblk), dim3(CS_T), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx, g->ny,
-                           cg.shift, nc, nblk, key, hist);
-        WSB_HIP(hipGetLastError());
-        size_t tmp = L.tmp_bytes;
-        WSB_HIP(hipcub::DeviceScan::ExclusiveSum(base + L.tmp, tmp, hist, start, nc * nblk, s));
-        hipLaunchKernelGGL(cs_scatter_kernel, dim3(nblk), dim3(CS_T), 0, s, key, n, cg.shift, nc, nblk, start, rank, skey,
-                           order);
-        hipLaunchKernelGGL(cs_fine_kernel, dim3(cg.ncb), dim3(CS_T), 0, s, rank, skey, start, nblk, cg.shift, order);
-        WSB_HIP(hipGetLastError());
-        return WSB_OK;
-    }
     const int nk = sort_keys(n, nbins);
     hipLaunchKernelGGL(zero_kernel, dim3((nk + 255) / 256), dim3(256), 0, s, hist, nk);
     hipLaunchKernelGGL(cell_rank_kernel, dim3((n + 255) / 256), dim3(256), 0, s, lpos, n, g->clo0, g->clo1, g->cell, g->nx,

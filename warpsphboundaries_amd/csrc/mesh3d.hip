@@ -18,6 +18,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "cellsort.h"
 #include "mesh3d_device.h"
 #include "wsb_internal.h"
 
@@ -948,6 +949,21 @@ __global__ __launch_bounds__(256) void cell_scatter3_kernel(const int32_t* key, 
     }
 }
 
+// the cell key of every particle for the two-pass sort: its cell when the cell has candidate faces or a walk,
+// else -1 (placed after the others)
+__global__ __launch_bounds__(256) void cell_key3_kernel(m3::Grid3 g, const double* pos, int n, int32_t* key) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int c[3];
+    const bool in = m3::cell_of(g, pos[3 * (int64_t)r], pos[3 * (int64_t)r + 1], pos[3 * (int64_t)r + 2], c);
+    int k = -1;
+    if (in) {
+        const int64_t cid = m3::cell_id(g, c[0], c[1], c[2]);
+        if (g.cstart[cid + 1] > g.cstart[cid] || (g.cwalk && g.cwalk[cid] >= 0)) k = (int)cid;
+    }
+    key[r] = k;
+}
+
 __global__ void zero3_kernel(int32_t* p, int64_t n) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -1348,7 +1364,8 @@ extern "C" int wsb_mesh3d_grid(const wsb_mesh3d* m, double* lo_cell) {
 
 extern "C" size_t wsb_sort3d_scratch_bytes(const wsb_mesh3d* m, int n) {
     if (!m || n <= 0) return 0;
-    return sort3_layout(n, (int64_t)m->hg.n[0] * m->hg.n[1] * m->hg.n[2]).total;
+    const int64_t nbins = (int64_t)m->hg.n[0] * m->hg.n[1] * m->hg.n[2];
+    return two_pass_sort(nbins) ? cs_layout(n, nbins).total : sort3_layout(n, nbins).total;
 }
 
 extern "C" int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t* order, void* scratch, size_t scratch_bytes,
@@ -1360,9 +1377,24 @@ extern "C" int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t
     WSB_REQUIRE(pos && order, WSB_EINVAL, "null buffer");
     const int64_t nbins = (int64_t)m->hg.n[0] * m->hg.n[1] * m->hg.n[2];
     WSB_REQUIRE(nbins < (int64_t)1 << 31, WSB_ENOTSUP, "grid too large for the cell sort");
+    char* base = static_cast<char*>(scratch);
+    if (two_pass_sort(nbins)) {
+        // the cell keys, then the two-pass counting sort without global atomics (cellsort.h)
+        const CsLayout C = cs_layout(n, nbins);
+        WSB_REQUIRE(scratch && scratch_bytes >= C.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", C.total,
+                    scratch_bytes);
+        hipStream_t s = wsb::as_stream(stream);
+        int32_t* key = reinterpret_cast<int32_t*>(base + C.key);
+        const CoarseGrid cg = coarse_grid(nbins);
+        const int nblk = (n + CS_CH - 1) / CS_CH;
+        hipLaunchKernelGGL(cell_key3_kernel, dim3((n + 255) / 256), dim3(256), 0, s, m->dg, pos, n, key);
+        hipLaunchKernelGGL(cs_hist_keys_kernel, dim3(nblk), dim3(CS_T), 0, s, key, n, cg.shift, cg.ncb + 1, nblk,
+                           reinterpret_cast<int32_t*>(base + C.ghist));
+        WSB_HIP(hipGetLastError());
+        return cs_place(C, base, n, nbins, order, s);
+    }
     const Sort3Layout L = sort3_layout(n, nbins);
     WSB_REQUIRE(scratch && scratch_bytes >= L.total, WSB_ENOMEM, "sort needs %zu scratch bytes (got %zu)", L.total, scratch_bytes);
-    char* base = static_cast<char*>(scratch);
     int32_t* key = reinterpret_cast<int32_t*>(base + L.key);
     int32_t* flag = reinterpret_cast<int32_t*>(base + L.flag);
     int32_t* opos = reinterpret_cast<int32_t*>(base + L.opos);
