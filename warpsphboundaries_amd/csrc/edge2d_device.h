@@ -204,6 +204,31 @@ __device__ __forceinline__ double dangle_mixed(double z, double lo, double hi) {
     return (double)atan2f((float)y, (float)x);
 }
 
+// the same two functions without a branch (the fused multi-kernel pass, where a branch splits the slot's code and keeps
+// the scheduler from interleaving the chord's transcendental chains with the kernels' sums): the angle's special
+// cases by a select; the square root of a >= 0 from an FP32 reciprocal-square-root seed of a scaled by an even power
+// of two into [0.5, 2) (exact), one FP64 Newton step, scaled back (relative error ~1e-14 for every finite a > 0; 0
+// at a = 0)
+__device__ __forceinline__ double dangle_mixed_nb(double z, double lo, double hi) {
+    const double y = z * (hi - lo);
+    const double x = z * z + lo * hi;
+    int e;
+    (void)frexp(fmax(fabs(x), fabs(y)), &e);
+    const double t = (double)atan2f((float)ldexp(y, -e), (float)ldexp(x, -e));
+    return (z == 0.0 || (y == 0.0 && x == 0.0)) ? 0.0 : t;
+}
+
+__device__ __forceinline__ double xsqrt_mixed_nb(double a) {
+    int e;
+    (void)frexp(a, &e);
+    const int k = e >> 1;
+    const double s = ldexp(a, -2 * k);
+    const double g = (double)__builtin_amdgcn_rsqf((float)s);
+    const double y = s * g;
+    const double r = ldexp(fma(fma(-y, y, s), 0.5 * g, y), k);
+    return a > 0.0 ? r : 0.0;
+}
+
 template <int FORM>
 __device__ __forceinline__ double xsqrt(double a) {
     if (FORM == MIXED) {

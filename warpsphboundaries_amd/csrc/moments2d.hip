@@ -246,6 +246,12 @@ constexpr int mask_nm() {
     return nm;
 }
 
+// the square root of the fused pass: branch-free in the mixed form
+template <int FORM>
+__device__ __forceinline__ double msqrt(double a) {
+    return FORM == closed::MIXED ? closed::xsqrt_mixed_nb(a) : closed::xsqrt<FORM>(a);
+}
+
 // one term m of block b of kernel KID.  INTF (the mixed form): the integer form, every sum SC times its value, with
 // dIm = dI_m and dIq = dI_m / (m + 2) (m <= NMAXP of the pass) and dRq = (rh^m - rl^m) / m (m >= 2); the coefficients
 // PS, FS are integers, so each product takes its coefficient as a literal operand instead of a register (the FP64
@@ -326,7 +332,7 @@ __device__ __forceinline__ void radius_sums(double R, bool valid, double z, doub
                                             double r0sq, double r1sq, double r0, double r1, double& th, BlockSums* S) {
     const bool clo = s0 < 0.0 && r0sq > R * R;
     const bool chi = s1 > 0.0 && r1sq > R * R;
-    const double L = closed::xsqrt<FORM>(fmax((R - az) * (R + az), 0.0));
+    const double L = msqrt<FORM>(fmax((R - az) * (R + az), 0.0));
     double lo = clo ? -L : s0, hi = chi ? L : s1;
     const bool on = valid && az < R && lo < hi;
     double rl = clo ? R : r0, rh = chi ? R : r1;
@@ -368,7 +374,7 @@ __device__ __forceinline__ void radius_sums(double R, bool valid, double z, doub
             if (MASK & 8) block_term<WSB_W6, 0, INTF>(m, dIm, dIq, dR, have && m <= KTab<WSB_W6>::NMAXP, S[3]);
         }
     }
-    const double t = FORM == closed::MIXED ? closed::dangle_mixed(z, lo, hi) : closed::dangle_fp64(z, lo, hi);
+    const double t = FORM == closed::MIXED ? closed::dangle_mixed_nb(z, lo, hi) : closed::dangle_fp64(z, lo, hi);
     th = on ? t : 0.0;
 }
 
@@ -463,7 +469,7 @@ __global__ __launch_bounds__(MTILE_T) __attribute__((amdgpu_waves_per_eu(FORM ==
             n1 = -tx;
             const double az = fabs(z), zz = z * z;
             const double r0sq = px * px + py * py, r1sq = qx * qx + qy * qy;
-            const double r0 = closed::xsqrt<FORM>(r0sq), r1 = closed::xsqrt<FORM>(r1sq);
+            const double r0 = msqrt<FORM>(r0sq), r1 = msqrt<FORM>(r1sq);
             radius_sums<MASK, FORM, false>(1.0, valid, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th1, S);
             if (MASK & 1) radius_sums<MASK, FORM, true>(0.5, valid, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th05, S);
         }
@@ -597,7 +603,7 @@ __device__ __forceinline__ void serve_row(const MultiArgs& ma, int pos) {
         const double n0 = ty, n1 = -tx;
         const double az = fabs(z), zz = z * z;
         const double r0sq = px * px + py * py, r1sq = qx * qx + qy * qy;
-        const double r0 = closed::xsqrt<FORM>(r0sq), r1 = closed::xsqrt<FORM>(r1sq);
+        const double r0 = msqrt<FORM>(r0sq), r1 = msqrt<FORM>(r1sq);
         BlockSums S[4] = {};
         double th1 = 0.0, th05 = 0.0;
         radius_sums<MASK, FORM, false>(1.0, valid, z, az, zz, s0, s1, r0sq, r1sq, r0, r1, th1, S);
