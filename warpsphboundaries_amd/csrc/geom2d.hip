@@ -268,35 +268,37 @@ __global__ __launch_bounds__(256) void adjacency2d_kernel(AdjArgs a) {
             const bool local = q.w != 0.0;
             int dw = 0;
             bool unsure = false;
-            // the records of the next two items are requested while one is tested (the loop waits on load latency)
+            // four items' records and ids are requested together, then tested in list order (the loop waits on load
+            // latency, not on arithmetic)
             const int i0 = a.cstart[cid], i1 = a.cstart[cid + 1];
-            double4 rec_n = i0 < i1 ? a.cxy[i0] : double4{0.0, 0.0, 0.0, 0.0};
-            double4 rec_nn = i0 + 1 < i1 ? a.cxy[i0 + 1] : double4{0.0, 0.0, 0.0, 0.0};
-            int item_n = i0 < i1 ? a.citems[i0] : 0, item_nn = i0 + 1 < i1 ? a.citems[i0 + 1] : 0;
-            for (int i = i0; i < i1; ++i) {
-                const double4 rec = rec_n;
-                const int item = item_n;
-                rec_n = rec_nn;
-                item_n = item_nn;
-                if (i + 2 < i1) {
-                    rec_nn = a.cxy[i + 2];
-                    item_nn = a.citems[i + 2];
+            for (int it = i0; it < i1; it += 4) {
+                double4 rec[4];
+                int item[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = min(it + u, i1 - 1);
+                    rec[u] = a.cxy[j];
+                    item[u] = a.citems[j];
                 }
-                const double ax = rec.x, ay = rec.y;
-                const double bx = rec.z, by = rec.w;
-                if (local) local_winding_edge(ax, ay, bx, by, px, py, q.x, q.y, dw, unsure);
-                if (fmin(ax, bx) - px > reach || px - fmax(ax, bx) > reach || fmin(ay, by) - py > reach ||
-                    py - fmax(ay, by) > reach)
-                    continue;
-                const double dx = bx - ax;
-                const double dyy = by - ay;
-                double t = ((px - ax) * dx + (py - ay) * dyy) / fmax(dx * dx + dyy * dyy, 1.0e-300);
-                t = fmin(fmax(t, 0.0), 1.0);
-                const double ex = px - (ax + t * dx);
-                const double ey = py - (ay + t * dyy);
-                if (sqrt(ex * ex + ey * ey) < h) {
-                    a.slots[w * a.K + n] = item;
-                    ++n;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (it + u >= i1) break;
+                    const double ax = rec[u].x, ay = rec[u].y;
+                    const double bx = rec[u].z, by = rec[u].w;
+                    if (local) local_winding_edge(ax, ay, bx, by, px, py, q.x, q.y, dw, unsure);
+                    if (fmin(ax, bx) - px > reach || px - fmax(ax, bx) > reach || fmin(ay, by) - py > reach ||
+                        py - fmax(ay, by) > reach)
+                        continue;
+                    const double dx = bx - ax;
+                    const double dyy = by - ay;
+                    double t = ((px - ax) * dx + (py - ay) * dyy) / fmax(dx * dx + dyy * dyy, 1.0e-300);
+                    t = fmin(fmax(t, 0.0), 1.0);
+                    const double ex = px - (ax + t * dx);
+                    const double ey = py - (ay + t * dyy);
+                    if (sqrt(ex * ex + ey * ey) < h) {
+                        a.slots[w * a.K + n] = item[u];
+                        ++n;
+                    }
                 }
             }
             if (empty) {
