@@ -257,11 +257,13 @@ def mesh3d_measure(torch, dev, steps, warmup):
     sup = torch.tensor(w.supports, dtype=torch.float64, device=dev)
     adj = mesh.adjacency(pos, sup)
     cap = adj.faces.numel()
+    adj_s = mesh.adjacency(pos, sup, capacity=cap, layout="sorted")
     out = torch.empty((N, 22), dtype=torch.float64, device=dev)
     kern = w.kernels[0]
 
-    def step(form):
-        a = mesh.adjacency(pos, sup, capacity=cap)
+    def step(form, layout="sorted"):
+        # cell sort, face adjacency + winding numbers (entries at the cell-order positions), 22 moments at the rows
+        a = mesh.adjacency(pos, sup, capacity=cap, layout=layout)
         mesh.moments(a, pos, sup, kern, form, out=out)
 
     res = {}
@@ -276,12 +278,21 @@ def mesh3d_measure(torch, dev, steps, warmup):
         res[form] = (time.perf_counter() - t0) / steps
         res[form + "_out"] = out.clone()
         log(f"3-D {form}: {res[form] * 1e3:.3f} ms/step")
+    # the particle (rows) layout of the same step: the same values bitwise
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step("mixed", "rows")
+    torch.cuda.synchronize()
+    res["rows_mixed"] = (time.perf_counter() - t0) / steps
+    assert torch.equal(out, res["mixed_out"])
     stream = torch.cuda.current_stream()
     per = {}
-    for name, fn in (("adjacency", lambda: mesh.adjacency(pos, sup, capacity=cap)),
+    for name, fn in (("adjacency", lambda: mesh.adjacency(pos, sup, capacity=cap, layout="sorted")),
+                     ("adjacency_rows_layout", lambda: mesh.adjacency(pos, sup, capacity=cap)),
                      ("adjacency_face_tree", lambda: mesh.adjacency(pos, sup, capacity=cap, search="bvh")),
-                     ("moments_fp64", lambda: mesh.moments(adj, pos, sup, kern, "fp64", out=out)),
-                     ("moments_mixed", lambda: mesh.moments(adj, pos, sup, kern, "mixed", out=out))):
+                     ("moments_fp64", lambda: mesh.moments(adj_s, pos, sup, kern, "fp64", out=out)),
+                     ("moments_mixed", lambda: mesh.moments(adj_s, pos, sup, kern, "mixed", out=out)),
+                     ("moments_mixed_rows_layout", lambda: mesh.moments(adj, pos, sup, kern, "mixed", out=out))):
         fn()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
@@ -300,11 +311,12 @@ def mesh3d_measure(torch, dev, steps, warmup):
     bytes_mom = N * (24 + 8 + 8 + 8 + 22 * 8) + adj.count * (4 + 36 * 8)
     return {
         "workload": f"{w.name} (box [0,1]^3 minus a level-6 icosphere of radius 0.25), {N:,} particles, w4, h=0.01; "
-                    "step = face adjacency + winding rebuild + 22 stored moments",
+                    "step = cell sort + face adjacency + winding rebuild (sorted-position layout) + 22 stored moments",
         "N": N, "faces": int(len(w.faces)), "pairs": int(adj.count), "pairs_per_particle": adj.count / N,
         "winding_values": sorted({float(v) for v in torch.unique(chi).cpu().tolist()}),
         "static_build_s": t_build, "grid": list(mesh.grid_shape), "K": mesh.K,
-        "ms_per_step": {"fp64": res["fp64"] * 1e3, "mixed": res["mixed"] * 1e3},
+        "ms_per_step": {"fp64": res["fp64"] * 1e3, "mixed": res["mixed"] * 1e3,
+                        "mixed_rows_layout": res["rows_mixed"] * 1e3},
         "evals_per_s": {"fp64": evals / res["fp64"], "mixed": evals / res["mixed"]},
         "launch_ms": {k: v * 1e3 for k, v in per.items()},
         "moments_hbm_GBps_algorithmic": bytes_mom / per["moments_fp64"] / 1e9,

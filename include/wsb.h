@@ -310,6 +310,14 @@ size_t wsb_adjacency3d_scratch_bytes(int n);
 int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, const int32_t* order, int n,
                     int64_t* offsets, int32_t* faces, int64_t capacity, double* chi, void* scratch, size_t scratch_bytes,
                     void* stream);
+/* the sorted-position layout (no counterpart in the reference; for callers that sort the particles every step): with
+ * order[n] from wsb_sort3d, the adjacency of particle order[i] is written at position i -- offsets (CSR over
+ * positions), faces, chi[i] and prow[i*4] = (x, y, z, h) of that particle -- so that wsb_moments3d_sorted reads every
+ * position's record, pair range and winding number contiguously; the same face lists and winding numbers per particle,
+ * bitwise (the cell-list search only). */
+int wsb_adjacency3d_sorted(const wsb_mesh3d* m, const double* pos, const double* sup, const int32_t* order, int n,
+                           int64_t* offsets, int32_t* faces, int64_t capacity, double* chi, double* prow, void* scratch,
+                           size_t scratch_bytes, void* stream);
 
 /* ---- the same face lists (ascending) and winding numbers with the faces found through a tree of the face boxes
  * (reference fluxbridge/device_search.py:125-212: Morton-ordered LBVH over outward-rounded FP32 boxes, stack
@@ -338,6 +346,12 @@ size_t wsb_moments3d_scratch_bytes(int n);
 int wsb_moments3d(const wsb_mesh3d* m, int kernel_id, int form, const int64_t* offsets, const int32_t* faces,
                   int64_t capacity, const double* chi, const double* pos, const double* sup, int n, int flags,
                   double* out, void* scratch, size_t scratch_bytes, void* stream);
+/* the 22 moments from the sorted-position layout of wsb_adjacency3d_sorted: a workgroup serves 128 consecutive positions
+ * (particles of neighbouring cells), each particle's row stored at the particle (out[order[i]*22]); the values of
+ * wsb_moments3d, bitwise.  Arguments otherwise as wsb_moments3d. */
+int wsb_moments3d_sorted(const wsb_mesh3d* m, int kernel_id, int form, const int32_t* order, const double* prow,
+                         const int64_t* offsets, const int32_t* faces, int64_t capacity, const double* chi, int n,
+                         int flags, double* out, void* scratch, size_t scratch_bytes, void* stream);
 
 /* ---- the six Hessian entries of baseline(3)[22:28] (reference fluxbridge/contract.py:41-46, the request (0, (i, j)),
  * i <= j; mesh3d/reference.py:296-305 kernel_hessian at alpha = 0): out[n*6] = int_V d_i d_j W dV in the order (0,0)

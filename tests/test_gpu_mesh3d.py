@@ -410,3 +410,44 @@ def test_face_tree_search_equals_the_cell_lists(case):
         assert (counts > 32).sum() > 100
     c = m.adjacency(Q, HH, capacity=3, search="bvh")
     assert c.count == b.count and torch.equal(c.faces[: c.count], b.faces[: b.count])
+
+
+@pytest.mark.parametrize("case", ["cavity-300k", "fine-sphere-heavy"])
+def test_sorted_layout_equals_rows_layout(case):
+    """wsb_adjacency3d_sorted / wsb_moments3d_sorted: the entry at position i of the cell order is particle order[i]'s
+    face list, winding number and (x, y, z, h), bitwise those of wsb_adjacency3d, and the 22 moments of both forms,
+    stored at the particles, bitwise those of wsb_moments3d.  The cavity at 300,000 particles keeps one lane per
+    particle; the fine sphere (1,000 particles, hundreds of faces each) takes the wide-candidate, long-walk and
+    heavy-row kernels."""
+    if case == "cavity-300k":
+        V, F = _cavity(3)
+        h = 0.05
+        P = _points(100000, seed=5)
+    else:
+        V, F = MM.box_surface((0, 0, 0), (1, 1, 1))
+        Vs, Fs = MM.icosphere(4, 0.25, (0.5, 0.5, 0.5))
+        V, F = np.vstack([V, Vs]), np.vstack([F, (Fs + 8)[:, ::-1]])
+        h = 0.12
+        rng = np.random.default_rng(6)
+        d = rng.normal(size=(700, 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        P = np.vstack([0.5 + d * (0.25 + rng.uniform(0.0, 0.1, 700))[:, None], rng.uniform(0, 1, (300, 3))])
+    m = MM.Mesh3D(V, F, h, device=DEV)
+    Q = torch.tensor(P, device=DEV)
+    H = torch.full((len(P),), h, dtype=torch.float64, device=DEV)
+    a = m.adjacency(Q, H)
+    b = m.adjacency(Q, H, layout="sorted")
+    assert b.count == a.count and b.prow is not None
+    if case != "cavity-300k":
+        assert (np.diff(a.offsets.cpu().numpy()) > 64).sum() > 100
+    off, faces, chi = b.by_particle()
+    assert torch.equal(off, a.offsets) and torch.equal(faces, a.faces[: a.count])
+    assert torch.equal(chi, a.chi)
+    o = b.order.long()
+    assert torch.equal(b.prow[:, :3], Q[o]) and torch.equal(b.prow[:, 3], H[o])
+    for x, y in zip(a.numpy(), b.numpy()):
+        assert np.array_equal(x, y)
+    for form in ("fp64", "mixed"):
+        assert torch.equal(m.moments(a, Q, H, "w4", form), m.moments(b, Q, H, "w4", form)), form
+    with pytest.raises(NotImplementedError):
+        m.hessian(b, Q, H, "w4")

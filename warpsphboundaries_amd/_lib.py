@@ -87,6 +87,8 @@ SIGNATURES = {
     "wsb_adjacency3d_scratch_bytes": (c_size_t, [c_int]),
     "wsb_adjacency3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                 c_size_t, c_void_p]),
+    "wsb_adjacency3d_sorted": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p,
+                                       c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_adjacency3d_bvh": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                 c_size_t, c_void_p]),
     "wsb_sort3d_scratch_bytes": (c_size_t, [c_void_p, c_int]),
@@ -94,6 +96,8 @@ SIGNATURES = {
     "wsb_moments3d_scratch_bytes": (c_size_t, [c_int]),
     "wsb_moments3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_int,
                               c_void_p, c_void_p, c_size_t, c_void_p]),
+    "wsb_moments3d_sorted": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int,
+                                     c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_hessian3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int, c_int,
                               c_void_p, c_void_p, c_size_t, c_void_p]),
     "wsb_moments3d_static_max_rows": (c_int64, []),

@@ -700,6 +700,8 @@ struct Grid3 {
     const float4* crec;         // [2*items] per cell item: face box rounded outward to FP32 (lo.xyz, id), (hi.xyz, 0)
     const double* cchi;         // [ncells] winding number of face-free cells
     const int32_t* cwalk;       // [ncells] -1 for a face-free cell, else the index d of the cell's axis walk
+    const uint32_t* cwork;      // [ceil(ncells / 32)] bit c % 32 of word c / 32 set when cell c lists candidate faces
+                                // or has an axis walk (the cell sort's test, one small table instead of two large ones)
     const int32_t* wdir;        // [ndirty] dir | steps << 3 (dir 0..5 = +x -x +y -y +z -z)
     const int32_t* wstart;      // [ndirty+1] faces a ray of the walk can cross (CSR), ascending face ids
     const int32_t* witems;

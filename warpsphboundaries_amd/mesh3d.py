@@ -43,18 +43,48 @@ def _kernel_name(kernel) -> str:
 @dataclass
 class Adjacency3D:
     """CSR face adjacency of a particle snapshot: ``offsets[n+1]`` (int64), ``faces[offsets[n]]`` (int32, ascending per
-    particle), ``chi[n]`` winding numbers; device tensors."""
+    particle), ``chi[n]`` winding numbers; device tensors.  The sorted-position layout (``prow`` set): entry i belongs
+    to particle ``order[i]`` (cell order) and ``prow[i]`` holds its (x, y, z, h)."""
     offsets: torch.Tensor
     faces: torch.Tensor
     chi: torch.Tensor
     count: int
+    order: torch.Tensor | None = None
+    prow: torch.Tensor | None = None
 
     def numpy(self):
         """(query, element, offsets) as numpy arrays, the layout of the reference's Adjacency."""
         off = self.offsets.cpu().numpy()
         el = self.faces[: self.count].cpu().numpy()
         q = np.repeat(np.arange(len(off) - 1, dtype=np.int32), np.diff(off))
+        if self.prow is not None:
+            # positions -> particles, ascending by particle (a stable sort keeps each particle's ascending faces)
+            order = self.order.cpu().numpy()
+            q = order[q].astype(np.int32)
+            k = np.argsort(q, kind="stable")
+            q, el = q[k], el[k]
+            off = np.zeros(len(order) + 1, dtype=np.int64)
+            np.cumsum(np.bincount(q, minlength=len(order)), out=off[1:])
         return q, el, off
+
+    def by_particle(self):
+        """(offsets, faces, chi) in the particle (row) layout, whatever the layout of this adjacency (device tensors)"""
+        if self.prow is None:
+            return self.offsets, self.faces[: self.count], self.chi
+        o = self.order.long()
+        n = o.numel()
+        cnt_pos = self.offsets[1:] - self.offsets[:-1]
+        cnt = torch.empty_like(cnt_pos)
+        cnt[o] = cnt_pos
+        off = torch.zeros(n + 1, dtype=torch.int64, device=cnt.device)
+        torch.cumsum(cnt, 0, out=off[1:])
+        p = torch.repeat_interleave(torch.arange(n, device=o.device), cnt_pos)        # the position of every slot
+        rank = torch.arange(self.count, device=o.device) - self.offsets[p]
+        faces = torch.empty(self.count, dtype=self.faces.dtype, device=o.device)
+        faces[off[o[p]] + rank] = self.faces[: self.count]
+        chi = torch.empty_like(self.chi)
+        chi[o] = self.chi
+        return off, faces, chi
 
 
 class Mesh3D:
@@ -139,14 +169,21 @@ class Mesh3D:
         return order
 
     def adjacency(self, pos: torch.Tensor, sup: torch.Tensor, capacity: int | None = None, scratch=None,
-                  order: torch.Tensor | None = None, sort: bool = True, search: str | None = None) -> Adjacency3D:
+                  order: torch.Tensor | None = None, sort: bool = True, search: str | None = None,
+                  layout: str = "rows") -> Adjacency3D:
         """face adjacency + winding numbers.  Particles are grouped by cell first (``sort``; or pass ``order``), which
         changes only the thread schedule.  ``capacity`` (pairs) defaults to a remembered bound; the call retries once
         with the exact count when it was too small (one host read of the pair count).  ``search``: "grid" (the cell
-        lists) or "bvh" (the face tree); both give the same face lists and winding numbers."""
+        lists) or "bvh" (the face tree); both give the same face lists and winding numbers.  ``layout="sorted"``
+        (cell lists, sorted particles): the entries at the positions of the cell order (``wsb_adjacency3d_sorted``),
+        which ``moments`` reads contiguously."""
         search = self.search if search is None else search
         if search not in ("grid", "bvh"):
             raise ValueError(f"search {search!r}: expected 'grid' or 'bvh'")
+        if layout not in ("rows", "sorted"):
+            raise ValueError(f"layout {layout!r}: expected 'rows' or 'sorted'")
+        if layout == "sorted" and (search != "grid" or not (sort or order is not None)):
+            raise ValueError("the sorted layout takes the cell-list search and a cell order")
         self._check_inputs(pos, sup)
         if float(sup.max()) > self.support_max * (1 + 1e-12) if len(sup) else False:
             raise ValueError("support exceeds the mesh's support_max")
@@ -162,17 +199,26 @@ class Mesh3D:
         cap = int(capacity if capacity is not None else max(self._cap, 4 * n, 1024))
         faces = torch.empty(cap, dtype=torch.int32, device=dev)
         st = L.stream_ptr(dev)
-        fn = lib.wsb_adjacency3d if search == "grid" else lib.wsb_adjacency3d_bvh
-        L.check(fn(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap, L.ptr(chi),
-                   L.ptr(scratch), scratch.numel(), st))
+        prow = torch.empty((n, 4), dtype=torch.float64, device=dev) if layout == "sorted" else None
+        if prow is not None:
+            def run():
+                L.check(lib.wsb_adjacency3d_sorted(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets),
+                                                   L.ptr(faces), cap, L.ptr(chi), L.ptr(prow), L.ptr(scratch),
+                                                   scratch.numel(), st))
+        else:
+            fn = lib.wsb_adjacency3d if search == "grid" else lib.wsb_adjacency3d_bvh
+
+            def run():
+                L.check(fn(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap,
+                           L.ptr(chi), L.ptr(scratch), scratch.numel(), st))
+        run()
         total = int(offsets[n]) if n else 0
         if total > cap:
             cap = total
             faces = torch.empty(cap, dtype=torch.int32, device=dev)
-            L.check(fn(self._h, L.ptr(pos), L.ptr(sup), L.ptr(order), n, L.ptr(offsets), L.ptr(faces), cap, L.ptr(chi),
-                       L.ptr(scratch), scratch.numel(), st))
+            run()
         self._cap = max(self._cap, total)
-        return Adjacency3D(offsets, faces, chi, total)
+        return Adjacency3D(offsets, faces, chi, total, order if prow is not None else None, prow)
 
     def moments(self, adj: Adjacency3D, pos: torch.Tensor, sup: torch.Tensor, kernel="w2", form="fp64",
                 physical: bool = True, indicator: bool = True, out: torch.Tensor | None = None) -> torch.Tensor:
@@ -187,6 +233,14 @@ class Mesh3D:
             raise NotImplementedError("the 3-D path provides the fp64 and mixed forms")
         lib = L.load()
         scratch = self._buf("_mom_scratch", lib.wsb_moments3d_scratch_bytes(n), pos.device)
+        if getattr(adj, "prow", None) is not None:
+            if adj.prow.shape[0] != n:
+                raise ValueError("the adjacency's particle count differs from the positions'")
+            L.check(lib.wsb_moments3d_sorted(self._h, KERNEL_ID[_kernel_name(kernel)], fid, L.ptr(adj.order),
+                                             L.ptr(adj.prow), L.ptr(adj.offsets), L.ptr(adj.faces), adj.faces.numel(),
+                                             L.ptr(adj.chi), n, flags, L.ptr(out), L.ptr(scratch), scratch.numel(),
+                                             L.stream_ptr(pos.device)))
+            return out
         L.check(lib.wsb_moments3d(self._h, KERNEL_ID[_kernel_name(kernel)], fid, L.ptr(adj.offsets), L.ptr(adj.faces),
                                   adj.faces.numel(), L.ptr(adj.chi), L.ptr(pos), L.ptr(sup), n, flags, L.ptr(out),
                                   L.ptr(scratch), scratch.numel(), L.stream_ptr(pos.device)))
@@ -207,6 +261,8 @@ class Mesh3D:
             raise NotImplementedError("the 3-D path provides the fp64 and mixed forms")
         lib = L.load()
         scratch = self._buf("_mom_scratch", lib.wsb_moments3d_scratch_bytes(n), pos.device)
+        if getattr(adj, "prow", None) is not None:
+            raise NotImplementedError("the Hessian entries take the particle (rows) layout of the adjacency")
         L.check(lib.wsb_hessian3d(self._h, KERNEL_ID[_kernel_name(kernel)], fid, L.ptr(adj.offsets), L.ptr(adj.faces),
                                   adj.faces.numel(), L.ptr(pos), L.ptr(sup), n, flags, L.ptr(out), L.ptr(scratch),
                                   scratch.numel(), L.stream_ptr(pos.device)))
