@@ -305,7 +305,9 @@ int wsb_mesh3d_grid(const wsb_mesh3d* m, double* lo_cell);
  * each triangle an element; winding number = the solid angle sum of mesh3d/reference.py:255-258 / 4 pi).
  * pos[n*3], sup[n] (<= support_max); order[n] nullable (wsb_sort3d).  offsets[n+1] (CSR, int64) is always written; faces[capacity] receives the
  * kept faces of each particle in ascending order (entries past `capacity` are dropped: the caller compares
- * offsets[n] with it); chi[n] (nullable) the winding number.  scratch: wsb_adjacency3d_scratch_bytes(n) bytes. */
+ * offsets[n] with it); chi[n] (nullable) the winding number.  A support above support_max (1 + 1e-12), or NaN, is
+ * found on the device: offsets[n] then reads -1 and the lists are not valid (one host read of offsets[n] checks both
+ * the capacity and the supports).  scratch: wsb_adjacency3d_scratch_bytes(n) bytes. */
 size_t wsb_adjacency3d_scratch_bytes(int n);
 int wsb_adjacency3d(const wsb_mesh3d* m, const double* pos, const double* sup, const int32_t* order, int n,
                     int64_t* offsets, int32_t* faces, int64_t capacity, double* chi, void* scratch, size_t scratch_bytes,

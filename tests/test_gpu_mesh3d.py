@@ -106,6 +106,27 @@ def test_capacity_retry():
     assert torch.equal(a.faces[: a.count], b.faces[: b.count])
 
 
+@pytest.mark.parametrize("search,layout", [("grid", "rows"), ("grid", "sorted"), ("bvh", "rows")])
+def test_support_above_support_max_is_refused(search, layout):
+    # the device flags the supports in the count pass; offsets[n] = -1 makes the one host read raise, and the flag
+    # starts clear again on the next call
+    V, F = _cavity(2)
+    m = MM.Mesh3D(V, F, 0.08, device=DEV)
+    P = torch.tensor(_points(300), device=DEV)
+    H = torch.full((len(P),), 0.08, dtype=torch.float64, device=DEV)
+    ok = m.adjacency(P, H, search=search, layout=layout)
+    for bad in (0.08 * (1 + 1e-9), float("nan")):
+        Hb = H.clone()
+        Hb[137] = bad
+        with pytest.raises(ValueError, match="support_max"):
+            m.adjacency(P, Hb, search=search, layout=layout)
+    m.adjacency(P, H * (1 + 1e-13), search=search, layout=layout)        # within the 1e-12 slack
+    again = m.adjacency(P, H, search=search, layout=layout)
+    assert again.count == ok.count
+    for x, y in zip(again.numpy()[1:], ok.numpy()[1:]):      # per particle (the sorted layout's order inside a
+        assert np.array_equal(x, y)                          # cell may differ between calls)
+
+
 @pytest.mark.parametrize("k", KERNELS)
 def test_moments_match_host_and_oracle(k):
     V, F = _cavity(2)

@@ -525,6 +525,8 @@ struct Adj3Args {
     int32_t* long_list;
     double4* prow;              // non-null: the sorted-position layout (counts, offsets, faces, chi and the particle's
                                 // (x, y, z, h) at its position in order; lists hold positions)
+    double hlim;                // a support above this (or NaN) sets *bad: the mesh's cells reach support_max only
+    int32_t* bad;
 };
 
 // the particle served at position p of a list (a position in the sorted-position layout, else the particle itself)
@@ -616,6 +618,7 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
             py = a.pos[3 * (int64_t)i + 1];
             pz = a.pos[3 * (int64_t)i + 2];
             h = a.sup[i];
+            if (MODE == 0 && !(h <= a.hlim)) atomicOr(a.bad, 1);
         }
     }
     int c[3] = {0, 0, 0};
@@ -744,6 +747,7 @@ __global__ __launch_bounds__(256) void bvh3_kernel(Adj3Args a) {
     if (!live) return;
     const double px = a.pos[3 * (int64_t)i], py = a.pos[3 * (int64_t)i + 1], pz = a.pos[3 * (int64_t)i + 2];
     const double h = a.sup[i];
+    if (MODE == 0 && !(h <= a.hlim)) atomicOr(a.bad, 1);
     const Near32 nq = near32(px, py, pz, h);
     int32_t buf[BVH_SHORT];
     int64_t cnt = 0;
@@ -1243,6 +1247,12 @@ __global__ void zero3_counts_kernel(int32_t* c) {
     c[1] = 0;
     c[2] = 0;
     c[3] = 0;
+    c[4] = 0;
+}
+
+// a support above the mesh's support_max: the pair total offsets[n] reads -1 (the lists are not valid)
+__global__ void adj3_mark_kernel(const int32_t* bad, int64_t* total) {
+    if (*bad) *total = -1;
 }
 
 template <int KID, int FORM, int MODE>
@@ -1519,7 +1529,7 @@ static int adjacency3d_grid(const wsb_mesh3d* m, const double* pos, const double
     const bool split = n < WIDE3_MAX_N;
     Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, list_counts, wide_list, list_counts + 1,
                walk_list, split ? WIDE3 : INT_MAX, list_counts + 2, exact_list, false, nullptr, nullptr, nullptr, nullptr,
-               reinterpret_cast<double4*>(prow)};
+               reinterpret_cast<double4*>(prow), m->support_max * (1.0 + 1e-12), list_counts + 4};
     const dim3 grid((n + 255) / 256), block(256);
     // up to 8,192 wavefronts (32 per CU) for the wide particles, one per particle at a time
     const dim3 wgrid((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 2048));
@@ -1550,6 +1560,8 @@ static int adjacency3d_grid(const wsb_mesh3d* m, const double* pos, const double
         hipLaunchKernelGGL(adj3_exact_kernel, dim3((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 256)), block, 0, s, a);
         WSB_HIP(hipGetLastError());
     }
+    hipLaunchKernelGGL(adj3_mark_kernel, dim3(1), dim3(1), 0, s, a.bad, offsets + n);
+    WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
 
@@ -1594,7 +1606,7 @@ extern "C" int wsb_adjacency3d_bvh(const wsb_mesh3d* m, const double* pos, const
     const bool split = n < WIDE3_MAX_N;
     Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, list_counts, long_list, list_counts + 1,
                walk_list, split ? WIDE3 : INT_MAX, list_counts + 2, exact_list, true, m->d_bnode, m->d_bitem,
-               list_counts + 3, long_list};
+               list_counts + 3, long_list, nullptr, m->support_max * (1.0 + 1e-12), list_counts + 4};
     const dim3 grid((n + 255) / 256), block(256);
     const dim3 wgrid((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 2048));
     hipLaunchKernelGGL(zero3_counts_kernel, dim3(1), dim3(1), 0, s, list_counts);
@@ -1617,6 +1629,8 @@ extern "C" int wsb_adjacency3d_bvh(const wsb_mesh3d* m, const double* pos, const
         hipLaunchKernelGGL(adj3_exact_kernel, dim3((unsigned)std::min<int64_t>(((int64_t)n + 3) / 4, 256)), block, 0, s, a);
         WSB_HIP(hipGetLastError());
     }
+    hipLaunchKernelGGL(adj3_mark_kernel, dim3(1), dim3(1), 0, s, a.bad, offsets + n);
+    WSB_HIP(hipGetLastError());
     return WSB_OK;
 }
 

@@ -185,8 +185,6 @@ class Mesh3D:
         if layout == "sorted" and (search != "grid" or not (sort or order is not None)):
             raise ValueError("the sorted layout takes the cell-list search and a cell order")
         self._check_inputs(pos, sup)
-        if float(sup.max()) > self.support_max * (1 + 1e-12) if len(sup) else False:
-            raise ValueError("support exceeds the mesh's support_max")
         n = pos.shape[0]
         dev = pos.device
         lib = L.load()
@@ -213,6 +211,8 @@ class Mesh3D:
                            L.ptr(chi), L.ptr(scratch), scratch.numel(), st))
         run()
         total = int(offsets[n]) if n else 0
+        if total < 0:           # the device's check of the supports
+            raise ValueError("support exceeds the mesh's support_max")
         if total > cap:
             cap = total
             faces = torch.empty(cap, dtype=torch.int32, device=dev)
