@@ -217,6 +217,14 @@ int wsb_disk_adjacency2d(const wsb_disks2d* g, const double* lpos, const double*
 int wsb_disk_moments2d(const wsb_disks2d* g, int kernel_id, const int32_t* slots, const int32_t* counts, int K,
                        const double* lpos, const double* lsup, int n, const double* rot4, int flags, double* out,
                        void* stream);
+/* several of the scheme kernels in one pass (as wsb_moments2d_multi for surfaces): outs[4] (a host array, indexed by
+ * kernel id WSB_CUBIC..WSB_W6; nullptr = not requested) receive what wsb_disk_moments2d gives for each kernel, every
+ * Gauss node's geometry (alpha, its sine and cosine, the node's radius and weight) computed once for all of them.
+ * Without the cubic the kernels share their panels; with it every kernel takes the cubic's panels (also cut at its
+ * knot r = h / 2: more nodes, the same accuracy).  The values equal wsb_disk_moments2d's to rounding (the compiler
+ * fuses the shared products into FMAs differently).  Flags as wsb_disk_moments2d. */
+int wsb_disk_moments2d_multi(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K, const double* lpos,
+                             const double* lsup, int n, const double* rot4, int flags, double* const* outs, void* stream);
 
 /* the geometric queries of the solid disks within one support, as wsb_cone_area2d / wsb_dir_extreme2d for a surface
  * (replace _disk_cone_area_kernel / _disk_dir_extreme_kernel, scene/fused.py:312-470): per row the swept areas of its

@@ -121,6 +121,39 @@ def test_channels_match_the_oracle(name):
     assert err.max() < 1e-12, (name, err)
 
 
+@pytest.mark.parametrize("kernels", [("cubic", "w2", "w4", "w6"), ("w2", "w4", "w6"), ("w4",), ("cubic", "w6"),
+                                     ("w2", "w6")])
+def test_several_kernels_in_one_pass(kernels):
+    # one pass over the scheme kernels: the single-kernel values to rounding (with the cubic the others on the cubic's
+    # finer panels), and the oracle at the single-kernel tolerance
+    cs, rs = BUNDLE
+    rng = np.random.default_rng(9)
+    k = rng.integers(0, len(rs), 500)
+    ang = rng.uniform(0, 2 * math.pi, 500)
+    off = np.concatenate([10.0 ** rng.uniform(-12, -1, 200) * rng.choice([-1, 1], 200), rng.uniform(-1.2, 1.2, 300)]) * H
+    pos = np.asarray(cs)[k] + np.maximum(np.asarray(rs)[k] + off, 0.0)[:, None] * np.stack([np.cos(ang), np.sin(ang)], 1)
+    pos = np.concatenate([pos, _queries(300, seed=10), np.asarray(cs) + 1e-10])
+    sup = H * rng.uniform(0.5, 1.0, len(pos))
+    rep = DiskArrayRep(cs, rs).to(DEV)
+    g = rep.geom(H, DEV)
+    lp = torch.as_tensor(pos, dtype=F64, device=DEV)
+    ls = torch.as_tensor(sup, dtype=F64, device=DEV)
+    print("disk multi:", kernels, "n", len(pos), flush=True)
+    slots, counts = g.adjacency(lp, ls)
+    for flags in (0, L.OUT_PHYSICAL):
+        multi = g.moments_multi(kernels, slots, counts, lp, ls, flags=flags)
+        for name in kernels:
+            single = g.moments(name, slots, counts, lp, ls, flags=flags).cpu().numpy()
+            d = (np.abs(multi[name].cpu().numpy() - single) / _family_scale(single, 1e-3)).max()
+            assert d <= (1e-13 if "cubic" in kernels else 1e-14), (name, d)
+    multi = g.moments_multi(kernels, slots, counts, lp, ls)
+    sl = slots.reshape(-1, g.K).cpu().numpy()
+    for name in kernels:
+        ref = _oracle_rows(name, cs, rs, pos, sup, sl)
+        err = (np.abs(multi[name].cpu().numpy() - ref) / _family_scale(ref, 1e-3)).max(0)
+        assert err.max() < 1e-12, (name, err)
+
+
 def test_physical_world_units_of_a_rotated_bundle():
     cs, rs = BUNDLE
     pose = dict(center=(0.15, -0.1), angle=0.5)

@@ -75,6 +75,8 @@ SIGNATURES = {
     "wsb_disk_adjacency2d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "wsb_disk_moments2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int,
                                    c_void_p, c_void_p]),
+    "wsb_disk_moments2d_multi": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int,
+                                         c_void_p, c_void_p]),
     "wsb_disk_channels_host": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "wsb_disk_cone_area2d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_double,
                                      c_double, c_double, c_void_p, c_void_p]),

@@ -69,26 +69,18 @@ __host__ __device__ __forceinline__ void disk_profile(double r, double& P, doubl
     }
 }
 
-// v: lam, m1 (along e), g0 (along e), g1 (e e), g1 (t t), m2 (e e), m2 (t t) of one disk, units of h, from the
-// Gauss nodes j0, j0 + jstep, ... of every panel (the host takes them all, a 16-lane group one node per lane)
-template <int KID>
-__host__ __device__ void disk_channels_part(double D, double R, int j0, int jstep, double v[7]) {
-    using T = DTab<KID>;
+// The Gauss nodes of one disk's radial integrals, panels cut for kernel PKID (its knot, its widest panel), nodes j0,
+// j0 + jstep, ... of every panel (the host takes them all, a 16-lane group one node per lane).  Calls inner(r, wg) for
+// a node inside the disk where the whole circle counts (alpha = pi) and arc(r, wg, al, sa, sc) on the arc range, with
+// al = alpha, sa = sin(alpha), sc = sin(alpha) cos(alpha).  The caller handles D >= 1 + R (nothing) and D <= R - 1
+// (the whole support).
+template <int PKID, class Inner, class Arc>
+__host__ __device__ __forceinline__ void disk_nodes(double D, double R, int j0, int jstep, Inner&& inner, Arc&& arc) {
+    using T = DTab<PKID>;
     // the W^5 kernels (degree 25 / 40, every power of u = 1 - r up to 30) take panels of at most 1/16
-    constexpr double MAXW = (KID == WSB_W2P5 || KID == WSB_W4P5) ? 0.0625 : 2.0;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) v[k] = 0.0;
-    if (!(D < 1.0 + R)) return;
-    if (D <= R - 1.0) {
-        if (j0 == 0) {
-            v[0] = v[3] = v[4] = T::MASS * 3.1415926535897932384626433832795;
-            v[5] = v[6] = T::SECOND * 3.1415926535897932384626433832795;
-        }
-        return;                                      // (the caller's 1/pi gives the whole-support values)
-    }
+    constexpr double MAXW = (PKID == WSB_W2P5 || PKID == WSB_W4P5) ? 0.0625 : 2.0;
     const double a = fabs(D - R), b = D + R;
     const bool inside = D < R;
-    const double pi = 3.1415926535897932384626433832795;
     // r < a inside the disk: the whole circle (alpha = pi); polynomial in r, exact with 16 nodes between knots
     if (inside) {
         const double end = fmin(a, 1.0);
@@ -97,18 +89,7 @@ __host__ __device__ void disk_channels_part(double D, double R, int j0, int jste
             double hi = fmin(end, lo + MAXW);
             if (T::NP == 2 && lo < T::HI[0]) hi = fmin(hi, T::HI[0]);
             const double w = hi - lo;
-            for (int j = j0; j < DISK_NODES; j += jstep) {
-                const double r = lo + w * GLTab::X16[j], wg = w * GLTab::W16[j];
-                double P, dP;
-                disk_profile<KID>(r, P, dP);
-                v[0] += wg * P * 2.0 * pi * r;
-                const double g = -wg * dP * r * r * pi;
-                v[3] += g;
-                v[4] += g;
-                const double s2 = wg * P * r * r * r * pi;
-                v[5] += s2;
-                v[6] += s2;
-            }
+            for (int j = j0; j < DISK_NODES; j += jstep) inner(lo + w * GLTab::X16[j], w * GLTab::W16[j]);
             lo = hi;
         }
     }
@@ -171,20 +152,108 @@ __host__ __device__ void disk_channels_part(double D, double R, int j0, int jste
                 const double al = 2.0 * atan2(sqrt(omc), sqrt(opc));
                 const double sc = sqrt(omc * opc) * 0.5 * (opc - omc);
                 const double sa = sqrt(omc * opc);
-                double P, dP;
-                disk_profile<KID>(r, P, dP);
-                const double wP = wg * P, wdP = wg * dP, rr = r * r;
-                v[0] += wP * 2.0 * al * r;
-                v[1] += wP * 2.0 * sa * rr;
-                v[2] -= wdP * 2.0 * sa * r;
-                v[3] -= wdP * rr * (al + sc);
-                v[4] -= wdP * rr * (al - sc);
-                v[5] += wP * rr * r * (al + sc);
-                v[6] += wP * rr * r * (al - sc);
+                arc(r, wg, al, sa, sc);
             }
             cur = nxt;
         }
     }
+}
+
+// a node's weights of the seven radial sums below, shared by every kernel: the sums of P take a0, a1, a5, a6, those
+// of P' take b2, b3, b4 (a1 = b2 = 0 inside the disk, where the whole circle counts)
+struct DiskNode {
+    double r, a0, a1, a5, a6, b2, b3, b4;
+};
+
+__host__ __device__ __forceinline__ DiskNode disk_inner_node(double r, double wg) {
+    const double pi = 3.1415926535897932384626433832795;
+    const double g = wg * r * r * pi;
+    return DiskNode{r, wg * 2.0 * pi * r, 0.0, g * r, g * r, 0.0, g, g};
+}
+
+__host__ __device__ __forceinline__ DiskNode disk_arc_node(double r, double wg, double al, double sa, double sc) {
+    const double rr = r * r;
+    const double b3 = wg * rr * (al + sc), b4 = wg * rr * (al - sc);
+    return DiskNode{r, wg * 2.0 * al * r, wg * 2.0 * sa * rr, b3 * r, b4 * r, wg * 2.0 * sa * r, b3, b4};
+}
+
+// one node's terms of kernel KID added to v (ARC: the node lies on the arc range, else inside the disk)
+template <int KID, bool ARC>
+__host__ __device__ __forceinline__ void disk_node_terms(const DiskNode& q, double v[7]) {
+    double P, dP;
+    disk_profile<KID>(q.r, P, dP);
+    v[0] += P * q.a0;
+    if (ARC) {
+        v[1] += P * q.a1;
+        v[2] -= dP * q.b2;
+    }
+    v[3] -= dP * q.b3;
+    v[4] -= dP * q.b4;
+    v[5] += P * q.a5;
+    v[6] += P * q.a6;
+}
+
+// the whole support (D <= R - 1): the values before the factor 1/pi, on the first node only
+template <int KID>
+__host__ __device__ __forceinline__ void disk_whole_terms(double v[7]) {
+    using T = DTab<KID>;
+    v[0] = v[3] = v[4] = T::MASS * 3.1415926535897932384626433832795;
+    v[5] = v[6] = T::SECOND * 3.1415926535897932384626433832795;
+}
+
+// v: lam, m1 (along e), g0 (along e), g1 (e e), g1 (t t), m2 (e e), m2 (t t) of one disk, units of h, from the
+// Gauss nodes j0, j0 + jstep, ... of every panel
+template <int KID>
+__host__ __device__ void disk_channels_part(double D, double R, int j0, int jstep, double v[7]) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] = 0.0;
+    if (!(D < 1.0 + R)) return;
+    if (D <= R - 1.0) {
+        if (j0 == 0) disk_whole_terms<KID>(v);
+        return;                                      // (the caller's 1/pi gives the whole-support values)
+    }
+    disk_nodes<KID>(
+        D, R, j0, jstep, [&](double r, double wg) { disk_node_terms<KID, false>(disk_inner_node(r, wg), v); },
+        [&](double r, double wg, double al, double sa, double sc) {
+            disk_node_terms<KID, true>(disk_arc_node(r, wg, al, sa, sc), v);
+        });
+}
+
+// the scheme kernels of MASK (bit k = kernel id k, cubic .. w6) at the same nodes: the panels of the cubic (cut at
+// its knot r = 1/2) when it is in MASK, else the panels that w2, w4 and w6 share; v[k] as disk_channels_part
+template <int MASK>
+__host__ __device__ void disk_channels_multi_part(double D, double R, int j0, int jstep, double v[4][7]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 7; ++c) v[k][c] = 0.0;
+    if (!(D < 1.0 + R)) return;
+    if (D <= R - 1.0) {
+        if (j0 == 0) {
+            if (MASK & 1) disk_whole_terms<WSB_CUBIC>(v[0]);
+            if (MASK & 2) disk_whole_terms<WSB_W2>(v[1]);
+            if (MASK & 4) disk_whole_terms<WSB_W4>(v[2]);
+            if (MASK & 8) disk_whole_terms<WSB_W6>(v[3]);
+        }
+        return;
+    }
+    constexpr int PKID = (MASK & 1) ? WSB_CUBIC : WSB_W2;
+    disk_nodes<PKID>(
+        D, R, j0, jstep,
+        [&](double r, double wg) {
+            const DiskNode q = disk_inner_node(r, wg);
+            if (MASK & 1) disk_node_terms<WSB_CUBIC, false>(q, v[0]);
+            if (MASK & 2) disk_node_terms<WSB_W2, false>(q, v[1]);
+            if (MASK & 4) disk_node_terms<WSB_W4, false>(q, v[2]);
+            if (MASK & 8) disk_node_terms<WSB_W6, false>(q, v[3]);
+        },
+        [&](double r, double wg, double al, double sa, double sc) {
+            const DiskNode q = disk_arc_node(r, wg, al, sa, sc);
+            if (MASK & 1) disk_node_terms<WSB_CUBIC, true>(q, v[0]);
+            if (MASK & 2) disk_node_terms<WSB_W2, true>(q, v[1]);
+            if (MASK & 4) disk_node_terms<WSB_W4, true>(q, v[2]);
+            if (MASK & 8) disk_node_terms<WSB_W6, true>(q, v[3]);
+        });
 }
 
 template <int KID>
@@ -236,6 +305,51 @@ __global__ void disk_slots_kernel(const double4* __restrict__ rec, const int32_t
 // a group of 16 lanes per row: the row's disks in slot order, each disk's panels with one Gauss node per lane and a
 // fixed butterfly sum over the group, the disk's channels turned into the body frame, then the physical scaling, the
 // world rotation and the store of finalize_row by the group's first lane (the disk is complete: no indicator term)
+// x plus its value in the lane that the DPP control CTRL names within the lane's row of 16 (the two 32-bit halves moved
+// by the VALU, no LDS permute)
+template <int CTRL>
+__device__ __forceinline__ double dpp_pair_sum(double x) {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    const int ylo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    const int yhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return x + __hiloint2double(yhi, ylo);
+}
+
+// the sum over the 16 lanes of a row's group, which is one DPP row (groups are aligned to 16 lanes and all their lanes
+// are active together): lane ^ 1, lane ^ 2 (quad permutes), then the mirror within each half-row and within the row.
+// Each step adds the same two values in both lanes of a pair, so every lane ends with the same sum.
+__device__ __forceinline__ double group_sum(double x) {
+    x = dpp_pair_sum<0xB1>(x);      // quad_perm [1, 0, 3, 2]
+    x = dpp_pair_sum<0x4E>(x);      // quad_perm [2, 3, 0, 1]
+    x = dpp_pair_sum<0x141>(x);     // row_half_mirror: lane i <-> 7 - i of its eight
+    x = dpp_pair_sum<0x140>(x);     // row_mirror: lane i <-> 15 - i of its row
+    return x;
+}
+
+// the twelve channels of one disk from its seven radial sums v (already times 1/pi), centre at (yx, yy) = D e from the
+// row, added to c
+__device__ __forceinline__ void disk_add_channels(const double v[7], double yx, double yy, double D, double c[wsb::NCH]) {
+    double ex = 1.0, ey = 0.0;
+    if (D > 1e-12) {
+        ex = yx / D;
+        ey = yy / D;
+    }
+    const double tx = -ey, ty = ex;
+    c[0] += v[0];
+    c[1] += v[1] * ex;
+    c[2] += v[1] * ey;
+    c[3] += v[2] * ex;
+    c[4] += v[2] * ey;
+    const double gxy = v[3] * ex * ey + v[4] * tx * ty;
+    c[5] += v[3] * ex * ex + v[4] * tx * tx;
+    c[6] += gxy;
+    c[7] += gxy;
+    c[8] += v[3] * ey * ey + v[4] * ty * ty;
+    c[9] += v[5] * ex * ex + v[6] * tx * tx;
+    c[10] += v[5] * ex * ey + v[6] * tx * ty;
+    c[11] += v[5] * ey * ey + v[6] * ty * ty;
+}
+
 template <int KID>
 __global__ void __launch_bounds__(256) disk_moments_kernel(const double4* __restrict__ rec,
                                                            const int32_t* __restrict__ slots,
@@ -261,32 +375,55 @@ __global__ void __launch_bounds__(256) disk_moments_kernel(const double4* __rest
         double v[7];
         disk_channels_part<KID>(D, d.z * ih, lane, DISK_NODES, v);
 #pragma unroll
-        for (int k = 0; k < 7; ++k) {
-#pragma unroll
-            for (int o = DISK_NODES / 2; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, DISK_NODES);
-            v[k] *= WSB_INV_PI;
-        }
-        double ex = 1.0, ey = 0.0;
-        if (D > 1e-12) {
-            ex = yx / D;
-            ey = yy / D;
-        }
-        const double tx = -ey, ty = ex;
-        c[0] += v[0];
-        c[1] += v[1] * ex;
-        c[2] += v[1] * ey;
-        c[3] += v[2] * ex;
-        c[4] += v[2] * ey;
-        const double gxy = v[3] * ex * ey + v[4] * tx * ty;
-        c[5] += v[3] * ex * ex + v[4] * tx * tx;
-        c[6] += gxy;
-        c[7] += gxy;
-        c[8] += v[3] * ey * ey + v[4] * ty * ty;
-        c[9] += v[5] * ex * ex + v[6] * tx * tx;
-        c[10] += v[5] * ex * ey + v[6] * tx * ty;
-        c[11] += v[5] * ey * ey + v[6] * ty * ty;
+        for (int k = 0; k < 7; ++k) v[k] = group_sum(v[k]) * WSB_INV_PI;
+        disk_add_channels(v, yx, yy, D, c);
     }
     if (lane == 0) finalize_row(c, 0.0, h, nullptr, rot4, flags | WSB_OUT_NO_INDICATOR, out + (size_t)r * wsb::NCH);
+}
+
+struct DiskOuts {
+    double* p[4];               // by kernel id WSB_CUBIC .. WSB_W6 (nullptr: not requested)
+};
+
+// the scheme kernels of MASK in one pass (wsb_disk_moments2d_multi): per pair every node's geometry (alpha, its sine
+// and cosine) once for all of them; after the group sums lane k of the row's group (k < 4) keeps the channel sums of
+// kernel k and stores them
+template <int MASK>
+__global__ void __launch_bounds__(256) disk_moments_multi_kernel(const double4* __restrict__ rec,
+                                                                 const int32_t* __restrict__ slots,
+                                                                 const int32_t* __restrict__ counts, int K,
+                                                                 const double* __restrict__ lpos,
+                                                                 const double* __restrict__ lsup, int n,
+                                                                 const double* __restrict__ rot4, int flags, DiskOuts outs) {
+    const int lane = threadIdx.x & (DISK_NODES - 1);
+    const int r = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / DISK_NODES);
+    if (r >= n) return;                              // whole groups leave together
+    const int kk = lane & 3;
+    double c[wsb::NCH];
+#pragma unroll
+    for (int k = 0; k < wsb::NCH; ++k) c[k] = 0.0;
+    const double px = lpos[2 * r], py = lpos[2 * r + 1], h = lsup[r], ih = 1.0 / h;
+    const int cnt = counts ? min(counts[r], K) : K;
+    for (int s = 0; s < cnt; ++s) {
+        const int m = slots[(size_t)r * K + s];
+        if (m < 0) continue;
+        const double4 d = rec[m];
+        const double yx = (d.x - px) * ih, yy = (d.y - py) * ih;
+        const double D = sqrt(yx * yx + yy * yy);
+        double v[4][7];
+        disk_channels_multi_part<MASK>(D, d.z * ih, lane, DISK_NODES, v);
+        double w[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if ((MASK >> q) & 1) v[q][k] = group_sum(v[q][k]) * WSB_INV_PI;
+            w[k] = kk == 0 ? v[0][k] : kk == 1 ? v[1][k] : kk == 2 ? v[2][k] : v[3][k];
+        }
+        disk_add_channels(w, yx, yy, D, c);
+    }
+    if (lane < 4 && ((MASK >> lane) & 1))
+        finalize_row(c, 0.0, h, nullptr, rot4, flags | WSB_OUT_NO_INDICATOR, outs.p[lane] + (size_t)r * wsb::NCH);
 }
 
 // ------------------------------------------------------------------------------------ cone area, direction extreme
@@ -444,6 +581,28 @@ void launch_moments(const wsb_disks2d* g, const int32_t* slots, const int32_t* c
                        slots, counts, K, lpos, lsup, n, rot4, flags, out);
 }
 
+template <int MASK>
+void launch_moments_multi(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K, const double* lpos,
+                          const double* lsup, int n, const double* rot4, int flags, const DiskOuts& outs, hipStream_t st) {
+    const size_t threads = (size_t)n * DISK_NODES;
+    hipLaunchKernelGGL(disk_moments_multi_kernel<MASK>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, g->rec,
+                       slots, counts, K, lpos, lsup, n, rot4, flags, outs);
+}
+
+template <int MASK>
+int dispatch_multi(int mask, const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K, const double* lpos,
+                   const double* lsup, int n, const double* rot4, int flags, const DiskOuts& outs, hipStream_t st) {
+    if constexpr (MASK > 15) {
+        return wsb::set_error(WSB_EINVAL, "no kernel requested");
+    } else {
+        if (mask == MASK) {
+            launch_moments_multi<MASK>(g, slots, counts, K, lpos, lsup, n, rot4, flags, outs, st);
+            return WSB_OK;
+        }
+        return dispatch_multi<MASK + 1>(mask, g, slots, counts, K, lpos, lsup, n, rot4, flags, outs, st);
+    }
+}
+
 template <int KID>
 void host_channels(const double* D, const double* R, int n, double* out) {
     for (int i = 0; i < n; ++i) disk_channels<KID>(D[i], R[i], out + 7 * (size_t)i);
@@ -589,6 +748,31 @@ extern "C" int wsb_disk_moments2d(const wsb_disks2d* g, int kernel_id, const int
         case WSB_W4P5: launch_moments<WSB_W4P5>(g, slots, counts, K, lpos, lsup, n, rot4, flags, out, st); break;
         default: return wsb::set_error(WSB_EINVAL, "unknown kernel id %d", kernel_id);
     }
+    WSB_HIP(hipGetLastError());
+    return WSB_OK;
+}
+
+extern "C" int wsb_disk_moments2d_multi(const wsb_disks2d* g, const int32_t* slots, const int32_t* counts, int K,
+                                        const double* lpos, const double* lsup, int n, const double* rot4, int flags,
+                                        double* const* outs, void* stream) {
+    WSB_REQUIRE(g, WSB_EINVAL, "null disk array");
+    WSB_REQUIRE(outs, WSB_EINVAL, "null output table");
+    WSB_REQUIRE(n >= 0 && K >= 1, WSB_EINVAL, "bad sizes (n=%d K=%d)", n, K);
+    WSB_REQUIRE((size_t)n * DISK_NODES / 256 < (size_t)1 << 31, WSB_EINVAL, "too many rows (%d)", n);
+    WSB_REQUIRE(!(flags & ~(WSB_OUT_ACCUMULATE | WSB_OUT_PHYSICAL | WSB_OUT_WORLD | WSB_OUT_NO_INDICATOR)), WSB_EINVAL,
+                "unknown flags %d", flags);
+    WSB_REQUIRE(!(flags & WSB_OUT_WORLD) || rot4, WSB_EINVAL, "WSB_OUT_WORLD needs rot4");
+    DiskOuts o{};
+    int mask = 0;
+    for (int k = 0; k < 4; ++k) {
+        o.p[k] = outs[k];
+        if (outs[k]) mask |= 1 << k;
+    }
+    WSB_REQUIRE(mask, WSB_EINVAL, "no kernel requested");
+    if (n == 0) return WSB_OK;
+    WSB_REQUIRE(slots && lpos && lsup, WSB_EINVAL, "null buffer");
+    const int st = dispatch_multi<1>(mask, g, slots, counts, K, lpos, lsup, n, rot4, flags, o, wsb::as_stream(stream));
+    if (st) return st;
     WSB_HIP(hipGetLastError());
     return WSB_OK;
 }

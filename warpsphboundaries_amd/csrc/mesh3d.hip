@@ -1086,7 +1086,8 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
     __shared__ int s_off[TILE3 + 1];
     __shared__ int s_first[TILE3];
     __shared__ double s_p[4][TILE3];
-    __shared__ double s_res[M::NA][TILE3];
+    __shared__ double s_res[M::NA][TILE3 + 1];     // one spare column: the store loop's lanes read one row's values
+                                                   // (consecutive k), TILE3 + 1 doubles apart, in distinct banks
     const int tid = threadIdx.x;
     const int64_t r0 = blockIdx.x * (int64_t)a.rows_per_tile;
     const int64_t row = r0 + tid;

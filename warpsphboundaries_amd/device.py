@@ -325,6 +325,35 @@ class Disks2D:
                                             L.stream_ptr(dev)))
         return out
 
+    def moments_multi(self, kernels, slots, counts, lpos, lsup, rot4=None, flags=0, outs=None):
+        """{kernel: [N, 12]} for several of the scheme kernels (cubic, w2, w4, w6) in one pass: every Gauss node's
+        geometry is shared by the kernels (``wsb_disk_moments2d_multi``); the values of ``moments`` to rounding (with
+        the cubic every kernel takes the cubic's panels, cut at r = h / 2 as well)."""
+        L.require_device(lpos, lsup, slots)
+        n = _rows(lpos, lsup)
+        dev = lpos.device
+        _require_buffer(slots, "slots", torch.int32, n * self.K, dev)
+        _require_buffer(counts, "counts", torch.int32, n, dev)
+        if rot4 is not None:
+            _require_buffer(rot4, "rot4", torch.float64, 4, dev)
+        ptrs = (ctypes.c_void_p * 4)()
+        res = {}
+        for k in kernels:
+            kid = KERNEL_ID[k]
+            if kid > 3:
+                raise NotImplementedError(f"{k}: several kernels in one pass are the scheme kernels cubic, w2, w4, w6")
+            o = outs[k] if outs is not None and k in outs else (
+                torch.zeros((n, 12), dtype=torch.float64, device=dev) if flags & L.OUT_ACCUMULATE
+                else torch.empty((n, 12), dtype=torch.float64, device=dev))
+            _require_buffer(o, f"outs[{k!r}]", torch.float64, n * 12, dev)
+            res[k] = o
+            ptrs[kid] = o.data_ptr()
+        lpos = lpos.to(torch.float64).contiguous()
+        lsup = lsup.to(torch.float64).contiguous()
+        L.check(L.load().wsb_disk_moments2d_multi(self.handle, L.ptr(slots), L.ptr(counts), self.K, L.ptr(lpos),
+                                                  L.ptr(lsup), n, L.ptr(rot4), int(flags), ptrs, L.stream_ptr(dev)))
+        return res
+
 
 def count_order(counts, K, base=None, order=None, nwork=None, scratch=None):
     """(order [N] int32, nwork [1] int32): the rows (of ``base`` [N] when given, else 0..N-1) grouped by slot count
