@@ -190,6 +190,26 @@ def test_mesh_batch_with_short_face_list_is_relaunched():
         assert np.array_equal(a.result, b.result)
 
 
+def test_mesh_batch_with_support_beyond_the_mesh_fails_its_requests():
+    # NaN supports are refused at submission; a batch whose supports pass the scene's bound but not the mesh's
+    # (forced here by widening the scene's bound) is failed by the adjacency's device check, not served wrong
+    scenes = _scenes3d(1, seed=32)
+    s = scenes[0]
+    srv = BoundaryServer(streams=1)
+    srv.add_scene(s.name, s.loops, s.h, mesh=s.mesh)
+    sup = torch.full((len(s.positions),), s.h, dtype=torch.float64, device="cuda:0")
+    sup[3] = float("nan")
+    with pytest.raises(ValueError):
+        srv.submit(s.name, torch.tensor(s.positions, device="cuda:0"), sup, s.kernel)
+    srv.scenes[s.name].support_max = 2 * s.h
+    bad = srv.submit(s.name, s.positions, 1.5 * s.h, s.kernel)
+    srv.run()
+    assert isinstance(bad.error, ValueError) and bad.result is None
+    good = srv.submit(s.name, s.positions, s.h, s.kernel)       # the next batch of the same slot serves again
+    srv.run()
+    assert good.error is None and good.result.shape == (len(s.positions), 22)
+
+
 def test_mixed_dimension_device_requests():
     scenes = _scenes(8, seed=31)
     assert {s.dim for s in scenes} == {2, 3}

@@ -327,7 +327,7 @@ class BoundaryServer:
             pos = positions.to(device=self.device, dtype=F64).reshape(-1, dim).contiguous()
             if isinstance(supports, torch.Tensor):
                 sup = supports.to(device=self.device, dtype=F64).reshape(-1).contiguous()
-                bad = len(sup) != len(pos) or (len(sup) and bool(((sup <= 0) | (sup > sc.support_max)).any()))
+                bad = len(sup) != len(pos) or (len(sup) and bool((~((sup > 0) & (sup <= sc.support_max))).any()))
             else:
                 h = float(supports)
                 sup = torch.full((len(pos),), h, dtype=F64, device=self.device)
@@ -386,6 +386,14 @@ class BoundaryServer:
             if b is not None and b.event.query():
                 if b.scene is not None and b.scene.dim == 3:
                     total = int(b.slot.host_count[0])
+                    if total < 0:       # the adjacency's device check: a support beyond the mesh's support_max
+                        now = time.perf_counter()
+                        for r in b.requests:
+                            r.error = ValueError("support exceeds the mesh's support_max")
+                            r.done = now
+                            now_done.append(r)
+                        self.busy[i] = None
+                        continue
                     b.scene.pairs_per_row = max(b.scene.pairs_per_row, total / max(b.rows, 1))
                     if total > b.pair_capacity:           # face list too short: launch again, exactly sized
                         self.relaunched_batches += 1
