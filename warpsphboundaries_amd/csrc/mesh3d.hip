@@ -1085,6 +1085,7 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
     __shared__ typename BlockScan::TempStorage scan_tmp;
     __shared__ int s_off[TILE3 + 1];
     __shared__ int s_first[TILE3];
+    __shared__ int s_ord[TILE3];          // the sorted layout: the row each position of the tile fills
     __shared__ double s_p[4][TILE3];
     __shared__ double s_res[M::NA][TILE3 + 1];     // one spare column: the store loop's lanes read one row's values
                                                    // (consecutive k), TILE3 + 1 doubles apart, in distinct banks
@@ -1102,6 +1103,7 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
             s_p[1][tid] = pr.y;
             s_p[2][tid] = pr.z;
             h = pr.w;
+            s_ord[tid] = a.order[row];
         } else {
             s_p[0][tid] = a.pos[3 * row];
             s_p[1][tid] = a.pos[3 * row + 1];
@@ -1169,7 +1171,7 @@ __global__ __launch_bounds__(TILE3) void tile22_kernel(Mom3Args a) {
         if (s_p[3][rr] < 0.0) continue;                // stored by heavy22_kernel
         const double v = s_res[k][rr];
         // the sorted layout's rows lie apart, each stored by consecutive lanes (its NO values contiguous)
-        double* q = a.out + (a.order ? (int64_t)a.order[r0 + rr] * M::NO + k : r0 * M::NO + idx);
+        double* q = a.out + (a.order ? (int64_t)s_ord[rr] * M::NO + k : r0 * M::NO + idx);
         if (accumulate) *q += v;
         else *q = v;
     }
@@ -1261,8 +1263,8 @@ int launch22(const Mom3Args& a, hipStream_t s) {
     hipLaunchKernelGGL(zero_count3_kernel, dim3(1), dim3(1), 0, s, a.heavy_count);
     // rows per tile: TILE3, halved (down to 16) while the launch has fewer than 2,048 workgroups
     Mom3Args t = a;
-    // sorted layout: 128 particles per workgroup (grouped by cell, their pairs are dense: measured 395 us against 501
-    // us at 256 on the sphere cavity)
+    // sorted layout: 128 particles per workgroup (grouped by cell, their pairs are dense; 64 or 256 measured slower on
+    // the sphere cavity)
     t.rows_per_tile = a.prow ? TILE3 / 2 : TILE3;
     while (t.rows_per_tile > 16 && ((int64_t)a.n + t.rows_per_tile - 1) / t.rows_per_tile < 2048) t.rows_per_tile >>= 1;
     hipLaunchKernelGGL((tile22_kernel<KID, FORM, MODE>), dim3((unsigned)(((int64_t)a.n + t.rows_per_tile - 1) / t.rows_per_tile)),
