@@ -176,6 +176,7 @@ class _MSlot:
     dev_seg: Optional[torch.Tensor] = None
     ready: Optional[torch.cuda.Event] = None
     mom_scratch: Optional[torch.Tensor] = None
+    prow: Optional[torch.Tensor] = None           # [cap, 4] the sorted-position layout's (x, y, z, h) records
 
 
 def _mark_ready(slot, device):
@@ -363,7 +364,7 @@ class BoundaryServer:
         cap = _bucket(rows)
         if sc.dim == 3:
             pairs = int(sc.pairs_per_row * cap * 1.25) + 4096
-            return cap * (24 + 8 + 4 + 8 + 8 + 176 + 16) + 4 * pairs + int(L.load().wsb_adjacency3d_scratch_bytes(cap)) + (1 << 20)
+            return cap * (24 + 8 + 4 + 8 + 8 + 176 + 16 + 32) + 4 * pairs + int(L.load().wsb_adjacency3d_scratch_bytes(cap)) + (1 << 20)
         K = self._Kmax if self.batched else sc.K
         per_row = 16 + 8 + 4 * K + 4 + 8 + 4 + 96 + 16 + 16 + 96
         return cap * per_row + (1 << 20)
@@ -722,8 +723,9 @@ class BoundaryServer:
                        torch.empty(ab, dtype=torch.uint8, device=dev),
                        torch.empty((cap, 3), dtype=F64).pin_memory(), torch.empty(cap, dtype=F64).pin_memory(),
                        torch.empty((cap, 22), dtype=F64).pin_memory(), torch.empty(1, dtype=torch.int64).pin_memory(),
-                       cap * (24 + 8 + 4 + 8 + 8 + 176) + ab + mb)
+                       cap * (24 + 8 + 4 + 8 + 8 + 176 + 32) + ab + mb)
             s.mom_scratch = torch.empty(mb, dtype=torch.uint8, device=dev)
+            s.prow = torch.empty((cap, 4), dtype=F64, device=dev)
             pool[cap] = _mark_ready(s, dev)
         return s
 
@@ -786,12 +788,15 @@ class BoundaryServer:
                 s.sup[:rows].copy_(s.host_sup[:rows], non_blocking=True)
             L.check(lib.wsb_sort3d(m.handle, L.ptr(s.pos), rows, L.ptr(s.order), L.ptr(s.sort_scratch),
                                    s.sort_scratch.numel(), sp))
-            L.check(lib.wsb_adjacency3d(m.handle, L.ptr(s.pos), L.ptr(s.sup), L.ptr(s.order), rows, L.ptr(s.offsets),
-                                        L.ptr(s.faces), cap_pairs, L.ptr(s.chi), L.ptr(s.adj_scratch),
-                                        s.adj_scratch.numel(), sp))
-            L.check(lib.wsb_moments3d(m.handle, KERNEL_ID[kernel], fid, L.ptr(s.offsets), L.ptr(s.faces), cap_pairs,
-                                      L.ptr(s.chi), L.ptr(s.pos), L.ptr(s.sup), rows, L.OUT_PHYSICAL, L.ptr(s.out),
-                                      L.ptr(s.mom_scratch), s.mom_scratch.numel(), sp))
+            # the sorted-position layout: the adjacency at the cell-order positions, the moments read it contiguously
+            # and store each row's 22 values at the row (the values of the particle layout, bitwise)
+            L.check(lib.wsb_adjacency3d_sorted(m.handle, L.ptr(s.pos), L.ptr(s.sup), L.ptr(s.order), rows,
+                                               L.ptr(s.offsets), L.ptr(s.faces), cap_pairs, L.ptr(s.chi), L.ptr(s.prow),
+                                               L.ptr(s.adj_scratch), s.adj_scratch.numel(), sp))
+            L.check(lib.wsb_moments3d_sorted(m.handle, KERNEL_ID[kernel], fid, L.ptr(s.order), L.ptr(s.prow),
+                                             L.ptr(s.offsets), L.ptr(s.faces), cap_pairs, L.ptr(s.chi), rows,
+                                             L.OUT_PHYSICAL, L.ptr(s.out), L.ptr(s.mom_scratch), s.mom_scratch.numel(),
+                                             sp))
             s.host_count.copy_(s.offsets[rows:rows + 1], non_blocking=True)
             if on_dev:
                 L.check(lib.wsb_scatter_rows(tab[2], ng, 22, L.ptr(s.out), sp))
