@@ -595,15 +595,18 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     constexpr bool FILL = MODE == 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = t < a.n;
-    const int i = live ? (a.order ? a.order[t] : t) : 0;
+    // (the later passes of the sorted layout read everything at the position: no index through the order)
+    const int i = live ? ((a.order && !(MODE != 0 && a.prow)) ? a.order[t] : t) : 0;
     const int wi = a.prow ? t : i;      // where the particle's adjacency is written (its position, sorted layout)
-    // MODE 2 serves only the particles the count pass left undecided (NaN winding number)
+    // MODE 2 serves only the particles the count pass left undecided (NaN winding number); MODE 1 only those with
+    // faces
     double chi = 0.0;
     bool todo = live;
     if (MODE == 2 && live) {
         chi = a.chi[wi];
         todo = chi != chi;
     }
+    if (FILL && live) todo = a.offsets[wi + 1] > a.offsets[wi];
     double px = 0.0, py = 0.0, pz = 0.0, h = 1.0;
     if (todo) {
         if (MODE != 0 && a.prow) {
