@@ -1485,6 +1485,77 @@ extern "C" int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t
     return WSB_OK;
 }
 
+// inclusive scan of the pair counts into offsets[1..n] (int64): per chunk of SCAN_CH counts its sum, one workgroup
+// scanning the chunk sums (exclusive, in place), then per chunk the scan through LDS -- loaded and stored coalesced, a
+// thread scanning its run of SCAN_RPT counts, the element e at e + e / SCAN_RPT so that the runs fall in distinct banks
+constexpr int SCAN_T = 256, SCAN_RPT = 16, SCAN_CH = SCAN_T * SCAN_RPT;
+
+__global__ __launch_bounds__(SCAN_T) void scan_sum_kernel(const int64_t* in, int n, int64_t* bsum) {
+    using BR = hipcub::BlockReduce<int64_t, SCAN_T>;
+    __shared__ typename BR::TempStorage tmp;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CH;
+    int64_t v = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_RPT; ++j) {
+        const int64_t e = base + j * SCAN_T + threadIdx.x;
+        if (e < n) v += in[e];
+    }
+    const int64_t tot = BR(tmp).Sum(v);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void scan_chunks_kernel(int64_t* bsum, int nb) {
+    using BS = hipcub::BlockScan<int64_t, 1024>;
+    __shared__ typename BS::TempStorage tmp;
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        const int b = b0 + threadIdx.x;
+        const int64_t v = b < nb ? bsum[b] : 0;
+        int64_t ex, agg;
+        BS(tmp).ExclusiveSum(v, ex, agg);
+        const int64_t c = carry;
+        if (b < nb) bsum[b] = c + ex;
+        __syncthreads();                // every thread has read carry (and the scan's storage is free again)
+        if (threadIdx.x == 0) carry = c + agg;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_apply_kernel(const int64_t* in, int n, const int64_t* boff, int64_t* out) {
+    using BS = hipcub::BlockScan<int64_t, SCAN_T>;
+    __shared__ typename BS::TempStorage tmp;
+    __shared__ int64_t sv[SCAN_CH + SCAN_CH / SCAN_RPT];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CH;
+#pragma unroll
+    for (int j = 0; j < SCAN_RPT; ++j) {
+        const int e = j * SCAN_T + threadIdx.x;
+        sv[e + e / SCAN_RPT] = base + e < n ? in[base + e] : 0;
+    }
+    __syncthreads();
+    // the thread's run: elements 16 t .. 16 t + 15 at 17 t .. 17 t + 15
+    const int r0 = threadIdx.x * (SCAN_RPT + 1);
+    int64_t run[SCAN_RPT];
+    int64_t acc = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_RPT; ++q) {
+        acc += sv[r0 + q];
+        run[q] = acc;
+    }
+    int64_t ex;
+    BS(tmp).ExclusiveSum(acc, ex);
+    const int64_t off = boff[blockIdx.x] + ex;
+#pragma unroll
+    for (int q = 0; q < SCAN_RPT; ++q) sv[r0 + q] = off + run[q];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SCAN_RPT; ++j) {
+        const int e = j * SCAN_T + threadIdx.x;
+        if (base + e < n) out[base + e] = sv[e + e / SCAN_RPT];
+    }
+}
+
 // adjacency scratch: pair counts [n] i64 | the three list counts (256 bytes) | wide list [n] i32 | walk list [n] i32 |
 // exact list [n] i32 | scan temporary
 struct Adj3Layout {
@@ -1494,6 +1565,7 @@ Adj3Layout adj3_layout(int n) {
     Adj3Layout L{};
     size_t tmp = 0;
     (void)hipcub::DeviceScan::InclusiveSum(nullptr, tmp, (const int64_t*)nullptr, (int64_t*)nullptr, n);
+    tmp = std::max(tmp, sizeof(int64_t) * (size_t)((n + SCAN_CH - 1) / SCAN_CH));   // or the chunk sums
     L.counts = 0;
     L.list_counts = al256(sizeof(int64_t) * (size_t)n);
     L.wide_list = L.list_counts + 256;
@@ -1530,8 +1602,7 @@ static int adjacency3d_grid(const wsb_mesh3d* m, const double* pos, const double
     int32_t* wide_list = reinterpret_cast<int32_t*>(sb + lay.wide_list);
     int32_t* walk_list = reinterpret_cast<int32_t*>(sb + lay.walk_list);
     int32_t* exact_list = reinterpret_cast<int32_t*>(sb + lay.exact_list);
-    char* tmp = sb + lay.tmp;
-    size_t tmp_bytes = scratch_bytes - lay.tmp;
+    char* tmp = sb + lay.tmp;                    // the scan's chunk sums
     const bool split = n < WIDE3_MAX_N;
     Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, list_counts, wide_list, list_counts + 1,
                walk_list, split ? WIDE3 : INT_MAX, list_counts + 2, exact_list, false, nullptr, nullptr, nullptr, nullptr,
@@ -1546,7 +1617,14 @@ static int adjacency3d_grid(const wsb_mesh3d* m, const double* pos, const double
         hipLaunchKernelGGL(adj3_wide_kernel<false>, wgrid, block, 0, s, a);
         WSB_HIP(hipGetLastError());
     }
-    WSB_HIP(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, counts, offsets + 1, n, s));
+    {
+        const int nb = (int)(((int64_t)n + SCAN_CH - 1) / SCAN_CH);
+        int64_t* bsum = reinterpret_cast<int64_t*>(tmp);
+        hipLaunchKernelGGL(scan_sum_kernel, dim3(nb), dim3(SCAN_T), 0, s, counts, n, bsum);
+        hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(1024), 0, s, bsum, nb);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(SCAN_T), 0, s, counts, n, bsum, offsets + 1);
+        WSB_HIP(hipGetLastError());
+    }
     if (faces && capacity > 0) {
         hipLaunchKernelGGL(adj3_kernel<1>, grid, block, 0, s, a);
         WSB_HIP(hipGetLastError());
