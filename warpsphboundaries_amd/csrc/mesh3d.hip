@@ -540,10 +540,13 @@ constexpr int ADJ_UNROLL = 8;
 // thousands of candidates per particle and too few particles to fill the chip at one lane each.  Either way a
 // particle's face list is the same ascending list.  Likewise a particle whose cell's axis walk has more than WIDE3
 // items takes its winding number from adj3_walk_kernel, one wavefront per particle (the crossings are integers, so
-// their sum does not depend on the order).  Launches of WIDE3_MAX_N particles or more keep one lane per particle
-// throughout: they fill the chip as they are.
+// their sum does not depend on the order).  Launches of WIDE3_MAX_N particles or more fill the chip at one lane per
+// particle; there the wavefront kernels take the particles of more than WIDE3_LARGE candidates (their 32-byte
+// candidate records then load as contiguous runs instead of one cache line per lane; 0.323 -> 0.316 ms on the sphere
+// cavity, 16 and 64 within 0.004 ms of 32).
 constexpr int WIDE3 = 128;
 constexpr int WIDE3_MAX_N = 1 << 18;
+constexpr int WIDE3_LARGE = 32;
 
 // Conservative FP32 rejection of a candidate from its FP32 box rounded outward (l, u): farther than h (1 + 1e-6) on one
 // axis, or squared distance beyond h^2 (1 + 1e-5).  `tol` bounds the FP32 rounding of the particle's coordinates and of the
@@ -1603,9 +1606,9 @@ static int adjacency3d_grid(const wsb_mesh3d* m, const double* pos, const double
     int32_t* walk_list = reinterpret_cast<int32_t*>(sb + lay.walk_list);
     int32_t* exact_list = reinterpret_cast<int32_t*>(sb + lay.exact_list);
     char* tmp = sb + lay.tmp;                    // the scan's chunk sums
-    const bool split = n < WIDE3_MAX_N;
+    const bool split = true;
     Adj3Args a{m->dg, pos, sup, order, n, counts, offsets, faces, capacity, chi, list_counts, wide_list, list_counts + 1,
-               walk_list, split ? WIDE3 : INT_MAX, list_counts + 2, exact_list, false, nullptr, nullptr, nullptr, nullptr,
+               walk_list, n < WIDE3_MAX_N ? WIDE3 : WIDE3_LARGE, list_counts + 2, exact_list, false, nullptr, nullptr, nullptr, nullptr,
                reinterpret_cast<double4*>(prow), m->support_max * (1.0 + 1e-12), list_counts + 4};
     const dim3 grid((n + 255) / 256), block(256);
     // up to 8,192 wavefronts (32 per CU) for the wide particles, one per particle at a time
