@@ -631,10 +631,16 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     const bool in = todo && m3::cell_of(a.g, px, py, pz, c);
     int64_t cnt = 0;
     int it0 = 0, it1 = 0;
+    // a cell whose bit is clear in the work bitmap (small, cached) lists no candidate and has no walk: its particles
+    // skip the two larger tables
+    bool work = false;
     if (MODE != 2 && in) {
         const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
-        it0 = a.g.cstart[cid];
-        it1 = a.g.cstart[cid + 1];
+        work = (a.g.cwork[cid >> 5] >> (cid & 31)) & 1u;
+        if (work) {
+            it0 = a.g.cstart[cid];
+            it1 = a.g.cstart[cid + 1];
+        }
     }
     const bool wide = !a.no_faces && it1 - it0 > a.wide_min;
     // the winding number where it is a lookup (outside the grid, a face-free cell); NaN: the walk decides
@@ -642,7 +648,7 @@ __global__ __launch_bounds__(256) void adj3_kernel(Adj3Args a) {
     bool long_walk = false;
     if (MODE == 0 && a.chi && in) {
         const int64_t cid = m3::cell_id(a.g, c[0], c[1], c[2]);
-        const int d = a.g.cwalk ? a.g.cwalk[cid] : -1;
+        const int d = work && a.g.cwalk ? a.g.cwalk[cid] : -1;
         if (d >= 0) {
             w0 = __builtin_nan("");
             long_walk = a.g.wstart[d + 1] - a.g.wstart[d] > a.wide_min;
