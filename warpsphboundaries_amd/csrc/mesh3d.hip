@@ -988,19 +988,45 @@ __global__ __launch_bounds__(256) void cell_scatter3_kernel(const int32_t* key, 
     }
 }
 
-// the cell key of every particle for the two-pass sort: its cell when the cell has candidate faces or a walk,
-// else -1 (placed after the others)
-__global__ __launch_bounds__(256) void cell_key3_kernel(m3::Grid3 g, const double* pos, int n, int32_t* key) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    int c[3];
-    const bool in = m3::cell_of(g, pos[3 * (int64_t)r], pos[3 * (int64_t)r + 1], pos[3 * (int64_t)r + 2], c);
-    int k = -1;
-    if (in) {
-        const int64_t cid = m3::cell_id(g, c[0], c[1], c[2]);
-        if ((g.cwork[cid >> 5] >> (cid & 31)) & 1u) k = (int)cid;
+// the key pass of the two-pass sort in one launch: every particle's key -- its cell when the cell's bit is set in the
+// work bitmap (candidate faces or a walk), else -1 (placed after the others) -- from the positions of the chunk, all
+// loaded before any key is formed, and the chunk's LDS histogram of coarse bins as cs_hist_keys_kernel
+__global__ __launch_bounds__(CS_T) void cs_hist3_kernel(m3::Grid3 g, const double* pos, int n, int shift, int nc, int nblk,
+                                                        int32_t* key, int32_t* ghist) {
+    __shared__ int hs[CS_NC];
+    for (int i = threadIdx.x; i < nc; i += CS_T) hs[i] = 0;
+    const int64_t c0 = (int64_t)blockIdx.x * CS_CH;
+    double px[CS_RPT], py[CS_RPT], pz[CS_RPT];
+#pragma unroll
+    for (int j = 0; j < CS_RPT; ++j) {
+        const int64_t r = c0 + j * CS_T + threadIdx.x;
+        const bool live = r < n;
+        px[j] = live ? pos[3 * r] : 0.0;
+        py[j] = live ? pos[3 * r + 1] : 0.0;
+        pz[j] = live ? pos[3 * r + 2] : 0.0;
     }
-    key[r] = k;
+    int k[CS_RPT];
+#pragma unroll
+    for (int j = 0; j < CS_RPT; ++j) {
+        const int64_t r = c0 + j * CS_T + threadIdx.x;
+        int c[3];
+        k[j] = -1;
+        if (r < n && m3::cell_of(g, px[j], py[j], pz[j], c)) {
+            const int64_t cid = m3::cell_id(g, c[0], c[1], c[2]);
+            if ((g.cwork[cid >> 5] >> (cid & 31)) & 1u) k[j] = (int)cid;
+        }
+        if (r < n) key[r] = k[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CS_RPT; ++j) {
+        const bool live = c0 + j * CS_T + threadIdx.x < n;
+        const uint64_t out = __ballot(live && k[j] < 0);
+        if (live && k[j] >= 0) atomicAdd(&hs[k[j] >> shift], 1);
+        else if (live && lane_rank(out) == 0) atomicAdd(&hs[nc - 1], __popcll(out));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += CS_T) ghist[(int64_t)i * nblk + blockIdx.x] = hs[i];
 }
 
 __global__ void zero3_kernel(int32_t* p, int64_t n) {
@@ -1468,8 +1494,7 @@ extern "C" int wsb_sort3d(const wsb_mesh3d* m, const double* pos, int n, int32_t
         int32_t* key = reinterpret_cast<int32_t*>(base + C.key);
         const CoarseGrid cg = coarse_grid(nbins);
         const int nblk = (n + CS_CH - 1) / CS_CH;
-        hipLaunchKernelGGL(cell_key3_kernel, dim3((n + 255) / 256), dim3(256), 0, s, m->dg, pos, n, key);
-        hipLaunchKernelGGL(cs_hist_keys_kernel, dim3(nblk), dim3(CS_T), 0, s, key, n, cg.shift, cg.ncb + 1, nblk,
+        hipLaunchKernelGGL(cs_hist3_kernel, dim3(nblk), dim3(CS_T), 0, s, m->dg, pos, n, cg.shift, cg.ncb + 1, nblk, key,
                            reinterpret_cast<int32_t*>(base + C.ghist));
         WSB_HIP(hipGetLastError());
         return cs_place(C, base, n, nbins, order, s);
